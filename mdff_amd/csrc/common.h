@@ -1,0 +1,200 @@
+// common.h -- shared device helpers and context layout of the gfx950 engforce library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mdff_gpu.h"
+
+#define MDFF_PI 3.14159265358979323846264338327950
+
+// ---------------------------------------------------------------------------
+// scalar device math shared by every kernel
+// ---------------------------------------------------------------------------
+
+// Fortran NINT (round half away from zero).  x - trunc(x) is exact for
+// |x| < 2^52, so the 0.5 comparisons are exact and the result equals the
+// reference's nint for every double.
+__host__ __device__ __forceinline__ double nint_d(double x) {
+  double t = trunc(x);
+  double r = x - t;
+  if (r >= 0.5) t += 1.0;
+  else if (r <= -0.5) t -= 1.0;
+  return t;
+}
+
+// Numerical Recipes erfcc, as src/functions.f90:28-40 (x >= 0 only here:
+// every caller passes alpha*d >= 0).
+template <typename T>
+__device__ __forceinline__ T errfc_pos(T z) {
+  T t = T(1) / (T(1) + T(0.5) * z);
+  T p = t * (T(1.48851587) + t * (T(-.82215223) + t * T(.17087277)));
+  p = t * (T(-1.13520398) + p);
+  p = t * (T(.27886807) + p);
+  p = t * (T(-.18628806) + p);
+  p = t * (T(.09678418) + p);
+  p = t * (T(.37409196) + p);
+  p = t * (T(1.00002368) + p);
+  return t * exp(-z * z - T(1.26551223) + p);
+}
+
+// Tang-Toennies f_n and f'_n (src/tt_damp.f90:68-97); c_k = 1/k!.
+template <typename T>
+__device__ __forceinline__ void tt_damp(T b, T c, T r, int order, T& f, T& fd) {
+  const T E[9] = {T(1), T(1), T(0.5), T(1.0 / 6.0), T(1.0 / 24.0), T(1.0 / 120.0), T(1.0 / 720.0),
+                  T(1.0 / 5040.0), T(1.0 / 40320.0)};
+  T br = b * r;
+  T ex = exp(-br) * c;
+  T ff = E[order];
+  for (int k = order - 1; k >= 1; --k) ff = ff * br + E[k];
+  ff = ff * br + E[0];
+  f = T(1) - ff * ex;
+  T brn = T(1), a = br;
+  for (int k = order;;) {
+    if (k & 1) brn *= a;
+    k >>= 1;
+    if (k == 0) break;
+    a *= a;
+  }
+  fd = E[order] * brn * ex * b;
+}
+
+// ---------------------------------------------------------------------------
+// per type-pair parameters (built on the host from the descriptor)
+// ---------------------------------------------------------------------------
+struct PairParam {
+  double rcutsq;                          // short-range cutoff^2 (strict <)
+  double A, B, C, D, BD;                  // BMHFT(D) (src/non_bonded.f90:103-107)
+  double epsp, sigsq, pl, ql, uc, uc1, uc2, fc;  // LJ derived (src/non_bonded.f90:563-600)
+  double epsmor, rhomor, rs;              // Morse
+  double db_i, dc_i, db_j, dc_j;          // TT damping of the field at i (from j) / at j
+  int32_t dk_i, dk_j, ldamp, pad;
+};
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct mdff_ctx {
+  mdff_desc d;
+  int n = 0, ntype = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[12] = {};
+  std::string err;
+
+  // cell
+  double A[9], B[9], omega, W[3];
+  double rc_vdw, rc_coul, alpha;
+  int task_qq = 0, task_qd = 0, task_dd = 0, linduced = 0, ldip_static = 0;
+  int sr_model = 0;                       // 0 none, 1 LJ, 2 BMH, 3 Morse
+  double u_tail = 0, p_tail = 0;
+  std::vector<PairParam> pp;              // ntype x ntype, (it + ntype*jt)
+
+  // per-atom device arrays (SoA FP64 unless noted)
+  double *rx = nullptr, *ry = nullptr, *rz = nullptr;
+  double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  double *fx = nullptr, *fy = nullptr, *fz = nullptr;
+  double *fsx = nullptr, *fsy = nullptr, *fsz = nullptr;   // saved forces (VV)
+  double *sx = nullptr, *sy = nullptr, *sz = nullptr;      // fractional coordinates
+  double *xs = nullptr, *ys = nullptr, *zs = nullptr;      // saved fractional (vnlistcheck)
+  double *bx = nullptr, *by = nullptr, *bz = nullptr;      // fractional at the last build
+  int32_t* itype = nullptr;               // 0-based
+  double* q = nullptr;
+  double* mass = nullptr;
+  double* pol = nullptr;                  // 9 per atom (column-major 3x3), 0 if not polar
+  double* invpol = nullptr;
+  double* dipia = nullptr;                // static dipoles, 3 per atom
+  double* mu = nullptr;                   // total dipoles (3n AoS)
+  double* mu_ind = nullptr;
+  double* efield = nullptr;               // output field ef_t (3n)
+  double* e_stat = nullptr;               // SCF static field (3n)
+  double* e_tot = nullptr;                // SCF total field (3n)
+  double* e_ind = nullptr;                // SCF induced field (3n)
+  double* hist = nullptr;                 // ASPC history 5 x 3n
+  double* frec = nullptr;                 // reciprocal force scratch (3n)
+  double* peratom = nullptr;              // per-atom energy/virial/stress partials (NPA x n)
+  double* red = nullptr;                  // reduction outputs
+  double* red_part = nullptr;             // reduction block partials
+  PairParam* d_pp = nullptr;
+  int itime = 0;
+  int32_t* polar = nullptr;               // 1 for polarisable ions
+  int npol = 0;                           // number of polarisable ions
+  double qsq = 0.0;                       // sum of q^2 (self term)
+
+  // neighbour list (full rows, cut = max(rc_vdw, rc_coul) + skin)
+  int32_t* cell_of = nullptr;             // cell index per atom
+  int32_t* cell_key = nullptr;            // sorted keys
+  int32_t* cell_atoms = nullptr;          // atoms sorted by cell
+  int32_t* cell_atoms_in = nullptr;
+  int32_t* cell_start = nullptr;          // ncell + 1
+  int32_t* nl_count = nullptr;            // n
+  int32_t* nl_start = nullptr;            // n + 1
+  int32_t* nl_j = nullptr;                // capacity entries
+  uint8_t* nl_flag = nullptr;             // bit0: r2 <= (rc_vdw+skin)^2, bit1: <= (rc_coul+skin)^2
+  int64_t nl_cap = 0, nl_total = 0;
+  int ncell[3] = {1, 1, 1};
+  int ncell_alloc = 0;
+  void* sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  int lists_valid = 0;
+  int32_t* d_flag = nullptr;              // device flags: [0] rebuild, [1] overflow
+  double* d_disp = nullptr;               // per-block max/second-max displacement partials
+  int32_t* h_flag = nullptr;              // pinned host mirror
+  double* h_red = nullptr;                // pinned host reduction mirror
+
+  // reciprocal space
+  int K[3] = {0, 0, 0};
+  int NXk = 0, NYk = 0, NZk = 0;          // box dims: Kx+1, 2Ky+1, 2Kz+1
+  int nk_box = 0, nk = 0;
+  double* kA = nullptr;                   // A(n) on the box, 0 where excluded (nk_box)
+  double* kvec = nullptr;                 // kx,ky,kz,kcoe per box entry (4 x nk_box)
+  void* tab = nullptr;                    // per-atom phase tables (precision dependent)
+  void* sk_part = nullptr;                // structure-factor partials
+  double* sk = nullptr;                   // S(n) (2 x nk_box) FP64
+  double* wk = nullptr;                   // W(n) = A(n) conj(S(n)) (2 x nk_box)
+  void* wk_t = nullptr;                   // W in the working precision
+  double* rec_part = nullptr;             // per (atom, split) partial moments
+  int rec_splits = 1, sk_chunks = 1;
+
+  double last_ms[8] = {0};
+  int n_last = 0;
+};
+
+// ---------------------------------------------------------------------------
+// error helpers
+// ---------------------------------------------------------------------------
+#define HIP_TRY(ctx, call)                                                              \
+  do {                                                                                  \
+    hipError_t _e = (call);                                                             \
+    if (_e != hipSuccess) {                                                             \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(_e);                   \
+      return MDFF_ERR_HIP;                                                              \
+    }                                                                                   \
+  } while (0)
+
+#define MDFF_TRY(call)                    \
+  do {                                    \
+    int _rc = (call);                     \
+    if (_rc != MDFF_OK) return _rc;       \
+  } while (0)
+
+static inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// launches implemented in the .hip files
+int nl_build(mdff_ctx* c);
+int nl_check(mdff_ctx* c, int* rebuilt);
+int nl_get_pairs(mdff_ctx* c, int which, int32_t* point, int32_t* list, int64_t cap, int64_t* n);
+void launch_kardir(mdff_ctx* c);
+int realspace_field(mdff_ctx* c, int mode, const double* mu_in, double* ef_out);
+int recip_setup(mdff_ctx* c);
+int recip_tables(mdff_ctx* c);
+int recip_eval(mdff_ctx* c, const double* q, const double* mu, bool use_q, bool use_mu,
+               bool forces, bool field, double* ef_out, double* f_out, double* e_out);
+void recip_free(mdff_ctx* c);

@@ -1,0 +1,987 @@
+// mdff_gpu.hip -- context, parameter tables and the engforce driver (C-ABI).
+//
+// Drop-in for SUBROUTINE engforce (src/engforce_driver.f90:102-190) and the
+// setup chain that feeds it (src/read_pos.f90:160-234,
+// src/non_bonded.f90:455-644, src/coulomb.f90:363-493, src/pim.f90:147-240).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "common.h"
+
+int realspace_force_impl(mdff_ctx* c, double* fsr, double* fcl, double* ef, bool with_field);
+
+namespace {
+
+constexpr int NPA = 15;  // per-atom columns of the pair kernel
+
+// -------------------------------------------------------------- small kernels
+
+__global__ void k_fill(size_t n, double v, double* p) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// column sums over atoms: in [col][n] -> per-block partials [block][ncol]
+__global__ void __launch_bounds__(256) k_colsum(int n, int ncol, const double* __restrict__ in,
+                                                double* __restrict__ part) {
+  __shared__ double sh[256];
+  for (int c = 0; c < ncol; ++c) {
+    double v = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+      v += in[(size_t)c * n + i];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x * ncol + c] = sh[0];
+    __syncthreads();
+  }
+}
+
+// total force f = f_sr + (f_rec + f_dir) * coul_unit (src/ewald.f90:237-246);
+// inputs AoS, output SoA
+__global__ void k_combine_forces(int n, const double* __restrict__ fsr, const double* __restrict__ fdir,
+                                 const double* __restrict__ frec, double coul, int has_sr, int has_cl,
+                                 double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double f[3];
+  for (int a = 0; a < 3; ++a) {
+    double v = has_sr ? fsr[3 * i + a] : 0.0;
+    if (has_cl) v = v + (frec[3 * i + a] + fdir[3 * i + a]) * coul;
+    f[a] = v;
+  }
+  fx[i] = f[0];
+  fy[i] = f[1];
+  fz[i] = f[2];
+}
+
+// ef = ef_dir + ef_rec + ef_surf + ef_self (src/ewald.f90:224-235);
+// ef_surf = -(sum q r + sum mu) 4pi/3V when lsurf; ef_self = 4 alpha^3/(3 sqrt(pi)) mu
+__global__ void k_combine_field(int n, const double* __restrict__ edir, const double* __restrict__ erec,
+                                const double* __restrict__ mu, double self2, int lsurf, double sx,
+                                double sy, double sz, double* __restrict__ ef) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s[3] = {sx, sy, sz};
+  for (int a = 0; a < 3; ++a) {
+    double e = edir[3 * i + a] + erec[3 * i + a];
+    if (lsurf) e += s[a];
+    if (mu) e += self2 * mu[3 * i + a];
+    ef[3 * i + a] = e;
+  }
+}
+
+// per-atom columns for the self and surface terms: mu^2, mu, q r
+__global__ void k_moments(int n, const double* __restrict__ q, const double* __restrict__ mu,
+                          const double* __restrict__ rx, const double* __restrict__ ry,
+                          const double* __restrict__ rz, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double m0 = mu ? mu[3 * i] : 0.0, m1 = mu ? mu[3 * i + 1] : 0.0, m2 = mu ? mu[3 * i + 2] : 0.0;
+  out[i] = m0 * m0 + m1 * m1 + m2 * m2;
+  out[(size_t)n + i] = m0;
+  out[(size_t)2 * n + i] = m1;
+  out[(size_t)3 * n + i] = m2;
+  out[(size_t)4 * n + i] = q[i] * rx[i];
+  out[(size_t)5 * n + i] = q[i] * ry[i];
+  out[(size_t)6 * n + i] = q[i] * rz[i];
+}
+
+// induced_moment (src/moment_from_pola.f90:955-993): mu = alpha E (alpha = 0
+// for non-polarisable ions), plus the per-atom u_pol term mu.alpha^-1.mu
+__global__ void k_induced(int n, const double* __restrict__ pol, const double* __restrict__ invpol,
+                          const double* __restrict__ E, double* __restrict__ mu, double* __restrict__ upol) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* p = pol + 9 * (size_t)i;
+  double m[3];
+  for (int a = 0; a < 3; ++a) {
+    double v = 0.0;
+    for (int b = 0; b < 3; ++b) v = v + p[a + 3 * b] * E[3 * i + b];
+    m[a] = v;
+    mu[3 * i + a] = v;
+  }
+  if (upol) {
+    const double* ip = invpol + 9 * (size_t)i;
+    double u = 0.0;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) u = u + m[a] * ip[a + 3 * b] * m[b];
+    upol[i] = u;
+  }
+}
+
+__global__ void k_upol(int n, const double* __restrict__ invpol, const double* __restrict__ mu,
+                       double* __restrict__ upol) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* ip = invpol + 9 * (size_t)i;
+  double u = 0.0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) u = u + mu[3 * i + a] * ip[a + 3 * b] * mu[3 * i + b];
+  upol[i] = u;
+}
+
+// ASPC predictor (src/moment_from_pola.f90:566-575) and history shift
+__global__ void k_aspc_predict(size_t n3, int ext, double b0, double b1, double b2, double b3, double b4,
+                               double* __restrict__ hist, double* __restrict__ mu) {
+  size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n3) return;
+  double B[5] = {b0, b1, b2, b3, b4};
+  double m = 0.0;
+  for (int k = 0; k <= ext; ++k) m = m + B[k] * hist[k * n3 + x];
+  mu[x] = m;
+  for (int k = ext; k >= 1; --k) hist[k * n3 + x] = hist[(k - 1) * n3 + x];
+}
+
+// ASPC corrector (:579-583): mu = w alpha E + (1 - w) mu_p
+__global__ void k_aspc_correct(int n, double w, const double* __restrict__ pol, const double* __restrict__ E,
+                               double* __restrict__ mu) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* p = pol + 9 * (size_t)i;
+  for (int a = 0; a < 3; ++a) {
+    double v = 0.0;
+    for (int b = 0; b < 3; ++b) v = v + p[a + 3 * b] * E[3 * i + b];
+    mu[3 * i + a] = w * v + (1.0 - w) * mu[3 * i + a];
+  }
+}
+
+// get_rmsd_mu (:998-1027) per-atom terms; columns: residual^2, polar flag,
+// max mu component (catastrophe check, :176-190)
+__global__ void k_rmsd_terms(int n, const int32_t* __restrict__ polar, const double* __restrict__ pol,
+                             const double* __restrict__ E, const double* __restrict__ mu,
+                             double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double r = 0.0;
+  if (polar[i]) {
+    for (int a = 0; a < 3; ++a) {
+      double p = pol[9 * (size_t)i + a + 3 * a];
+      if (p == 0.0) continue;
+      double x = mu[3 * i + a] / p - E[3 * i + a];
+      r = r + x * x;
+    }
+  }
+  out[i] = r;
+}
+
+__global__ void k_add3(size_t n3, const double* __restrict__ a, const double* __restrict__ b,
+                       double* __restrict__ o) {
+  size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < n3) o[x] = a[x] + b[x];
+}
+
+__global__ void k_maxcomp(int n, const double* __restrict__ mu, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double m = mu[3 * i];
+  m = fmax(m, mu[3 * i + 1]);
+  m = fmax(m, mu[3 * i + 2]);
+  out[i] = m > 10.0 ? 1.0 : 0.0;
+}
+
+// prop_velocity_verlet position update (src/integration.f90:165-169)
+__global__ void k_vv_pos(int n, double dt, double dtsq2, const double* __restrict__ m, double* rx, double* ry,
+                         double* rz, const double* vx, const double* vy, const double* vz, const double* fx,
+                         const double* fy, const double* fz, double* fsx, double* fsy, double* fsz) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fsx[i] = fx[i];
+  fsy[i] = fy[i];
+  fsz[i] = fz[i];
+  rx[i] = rx[i] + vx[i] * dt + (fx[i] * dtsq2) / m[i];
+  ry[i] = ry[i] + vy[i] * dt + (fy[i] * dtsq2) / m[i];
+  rz[i] = rz[i] + vz[i] * dt + (fz[i] * dtsq2) / m[i];
+}
+
+// velocity update (:179-183) + per-atom kinetic energy column
+__global__ void k_vv_vel(int n, double dt2, const double* __restrict__ m, double* vx, double* vy, double* vz,
+                         const double* fsx, const double* fsy, const double* fsz, const double* fx,
+                         const double* fy, const double* fz, double* __restrict__ ke) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a = vx[i] + (fsx[i] + fx[i]) * dt2 / m[i];
+  double b = vy[i] + (fsy[i] + fy[i]) * dt2 / m[i];
+  double cc = vz[i] + (fsz[i] + fz[i]) * dt2 / m[i];
+  vx[i] = a;
+  vy[i] = b;
+  vz[i] = cc;
+  ke[i] = (a * a + b * b + cc * cc) * m[i];
+}
+
+// prop_pos_vel_verlet (:420-427): half kick + drift
+__global__ void k_kick_drift(int n, double dt2, double dt, const double* __restrict__ m, double* rx, double* ry,
+                             double* rz, double* vx, double* vy, double* vz, const double* fx, const double* fy,
+                             const double* fz) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  vx[i] = vx[i] + fx[i] * dt2 / m[i];
+  vy[i] = vy[i] + fy[i] * dt2 / m[i];
+  vz[i] = vz[i] + fz[i] * dt2 / m[i];
+  rx[i] = rx[i] + vx[i] * dt;
+  ry[i] = ry[i] + vy[i] * dt;
+  rz[i] = rz[i] + vz[i] * dt;
+}
+
+__global__ void k_kick(int n, double dt2, const double* __restrict__ m, double* vx, double* vy, double* vz,
+                       const double* fx, const double* fy, const double* fz) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  vx[i] = vx[i] + fx[i] * dt2 / m[i];
+  vy[i] = vy[i] + fy[i] * dt2 / m[i];
+  vz[i] = vz[i] + fz[i] * dt2 / m[i];
+}
+
+__global__ void k_scale_v(int n, double s, double sx, double sy, double sz, double* vx, double* vy, double* vz) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  vx[i] = vx[i] * s - sx;
+  vy[i] = vy[i] * s - sy;
+  vz[i] = vz[i] * s - sz;
+}
+
+// kinetic energy column and velocity sums
+__global__ void k_kin(int n, const double* __restrict__ m, const double* vx, const double* vy, const double* vz,
+                      double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]) * m[i];
+  out[(size_t)n + i] = vx[i];
+  out[(size_t)2 * n + i] = vy[i];
+  out[(size_t)3 * n + i] = vz[i];
+}
+
+inline int nblk(int n) { return div_up(n, 256); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ helpers
+
+// deterministic column sums of a [ncol][n] device array; result on the host
+int reduce_cols(mdff_ctx* c, const double* in, int ncol, double* host_out) {
+  const int nb = std::min(1024, std::max(1, div_up(c->n, 256)));
+  hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, c->stream, c->n, ncol, in, c->red_part);
+  HIP_TRY(c, hipMemcpyAsync(c->h_red, c->red_part, sizeof(double) * nb * ncol, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < ncol; ++k) host_out[k] = 0.0;
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < ncol; ++k) host_out[k] += c->h_red[(size_t)b * ncol + k];
+  return MDFF_OK;
+}
+
+static int fail(mdff_ctx* c, int code, const char* msg) {
+  c->err = msg;
+  return code;
+}
+
+// 3x3 inverse (replaces LAPACK DGETRF/DGETRI of src/read_pos.f90:203-212)
+static void inv3(const double* m, double* o) {
+  double a = m[0], b = m[3], cc = m[6], d = m[1], e = m[4], f = m[7], g = m[2], h = m[5], k = m[8];
+  double det = a * (e * k - f * h) - b * (d * k - f * g) + cc * (d * h - e * g);
+  o[0] = (e * k - f * h) / det; o[3] = (cc * h - b * k) / det; o[6] = (b * f - cc * e) / det;
+  o[1] = (f * g - d * k) / det; o[4] = (a * k - cc * g) / det; o[7] = (cc * d - a * f) / det;
+  o[2] = (d * h - e * g) / det; o[5] = (b * g - a * h) / det; o[8] = (a * e - b * d) / det;
+}
+
+// lattice (src/lattice.f90:60-112)
+static void lattice(const double* A, double* B, double* omega, double* W) {
+  const double *a1 = A, *a2 = A + 3, *a3 = A + 6;
+  double* b[3] = {B, B + 3, B + 6};
+  const double* u[3][2] = {{a2, a3}, {a3, a1}, {a1, a2}};
+  for (int j = 0; j < 3; ++j) {
+    const double *U1 = u[j][0], *U2 = u[j][1];
+    b[j][0] = U1[1] * U2[2] - U1[2] * U2[1];
+    b[j][1] = U1[2] * U2[0] - U1[0] * U2[2];
+    b[j][2] = U1[0] * U2[1] - U1[1] * U2[0];
+  }
+  double om = B[0] * A[0] + B[1] * A[1] + B[2] * A[2];
+  *omega = om;
+  for (int j = 0; j < 3; ++j) W[j] = om / sqrt(b[j][0] * b[j][0] + b[j][1] * b[j][1] + b[j][2] * b[j][2]);
+  for (int k = 0; k < 9; ++k) B[k] = B[k] / om;
+}
+
+// pair tables (src/non_bonded.f90:455-644, src/pim.f90:78-81)
+static void build_pair_params(mdff_ctx* c, const int32_t* natmi) {
+  const mdff_desc& d = c->d;
+  const int nt = c->ntype;
+  const double two16 = pow(2.0, 1.0 / 6.0), tpi = 2.0 * MDFF_PI;
+  c->pp.assign(nt * nt, PairParam{});
+  c->u_tail = 0.0;
+  double ptail = 0.0;
+  for (int it = 0; it < nt; ++it)
+    for (int jt = 0; jt < nt; ++jt) {
+      const int p = it + 16 * jt;
+      PairParam& P = c->pp[it + nt * jt];
+      double pl = d.plj[p], ql = d.qlj[p];
+      double rc = d.cutshortrange, rcsq = rc * rc, rc3 = rc * rcsq, ppqq = pl * ql;
+      P.rcutsq = rcsq;
+      P.A = d.Abmhftd[p]; P.B = d.Bbmhftd[p]; P.C = d.Cbmhftd[p]; P.D = d.Dbmhftd[p]; P.BD = d.BDbmhftd[p];
+      P.pl = pl; P.ql = ql;
+      P.epsp = d.epslj[p] / (ql - pl);
+      P.sigsq = two16 * two16 * d.sigmalj[p] * d.sigmalj[p];
+      double sr2 = P.sigsq / rcsq, sr = sqrt(sr2), srp = pow(sr, pl), srq = pow(sr, ql);
+      P.uc = P.epsp * (pl * srq - ql * srp);
+      P.uc1 = P.epsp * ppqq / (2.0 * rcsq) * (srq - srp);
+      P.uc2 = 0.5 * P.epsp * ((2.0 * pl + ppqq) * srq - (2.0 * ql + ppqq) * srp);
+      P.fc = ppqq * P.epsp / P.sigsq;
+      P.epsmor = d.epsmor[p]; P.rhomor = d.rhomor[p];
+      P.rs = exp(d.rhomor[p] * d.sigmamor[p]);
+      double ut = P.epsp * (pl * srq / (ql - 3.0) - ql * srp / (pl - 3.0)) * rc3 * tpi;
+      double pt = ppqq * P.epsp * (srq / (ql - 3.0) - srp / (pl - 3.0)) * rc3 * tpi;
+      if (d.lnmlj && natmi[it] != 0 && natmi[jt] != 0) {
+        c->u_tail += ut * natmi[it] * natmi[jt] / c->omega;
+        ptail += pt * natmi[it] * natmi[jt] / c->omega;
+      }
+      // TT damping of the field at i (type it) from j (type jt) and the converse
+      const int ki = it + 16 * it + 256 * jt, kj = jt + 16 * it + 256 * jt;
+      P.ldamp = (d.ldip_damping[ki] || d.ldip_damping[kj]) ? 1 : 0;
+      P.db_i = d.pol_damp_b[ki]; P.dc_i = d.pol_damp_c[ki]; P.dk_i = d.pol_damp_k[ki];
+      P.db_j = d.pol_damp_b[kj]; P.dc_j = d.pol_damp_c[kj]; P.dk_j = d.pol_damp_k[kj];
+    }
+  c->p_tail = d.lnmlj ? ptail / c->omega / 3.0 / d.press_unit : 0.0;
+  if (!d.lnmlj) c->u_tail = 0.0;
+}
+
+#define DMALLOC(ptr, count)                                                            \
+  do {                                                                                 \
+    HIP_TRY(c, hipMalloc((void**)&(ptr), sizeof(*(ptr)) * (size_t)(count)));           \
+    HIP_TRY(c, hipMemset((ptr), 0, sizeof(*(ptr)) * (size_t)(count)));                 \
+  } while (0)
+
+static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
+  c->d = *d;
+  c->n = d->natm;
+  c->ntype = d->ntype;
+  const int n = c->n;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(c, MDFF_ERR_NO_DEVICE, "no HIP device");
+  HIP_TRY(c, hipGetDevice(&c->device));
+  HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto& e : c->ev) HIP_TRY(c, hipEventCreate(&e));
+  memcpy(c->A, d->cell, sizeof c->A);
+  lattice(c->A, c->B, &c->omega, c->W);
+  const double half = 0.5 * std::min(c->W[0], std::min(c->W[1], c->W[2]));
+  c->rc_coul = std::min(d->cutlongrange, half);  // ewald_param (src/coulomb.f90:377-383)
+  c->rc_vdw = d->cutshortrange;
+  c->alpha = d->alphaES;
+  // tasks (src/coulomb.f90:188-238, src/pim.f90:147-183)
+  int lqch = 0;
+  for (int i = 0; i < 16; ++i) lqch |= d->qch[i] != 0.0;
+  for (int i = 0; i < 48; ++i) c->ldip_static |= d->dip[i] != 0.0;
+  for (int i = 0; i < d->ntype; ++i) c->linduced |= d->ldip_polar[i] != 0;
+  c->task_qq = lqch;
+  c->task_qd = c->ldip_static && lqch;
+  c->task_dd = c->ldip_static;
+  if (c->linduced) { c->task_qd |= lqch; c->task_dd = 1; }
+  if (c->linduced && c->ldip_static)
+    return fail(c, MDFF_ERR_UNSUPPORTED, "static dipoles together with polarisable ions are not supported");
+  if (d->lnmlj) c->sr_model = 1;
+  if (d->lbmhft || d->lbmhftd) c->sr_model = 2;
+  if (d->lmorse) c->sr_model = 3;
+  // per-atom type data (src/read_pos.f90:160-234)
+  std::vector<int32_t> it0(n);
+  int32_t natmi[16] = {0};
+  std::vector<double> q(n), m(n), pol(9 * (size_t)n, 0.0), ipol(9 * (size_t)n, 0.0), dip(3 * (size_t)n);
+  std::vector<int32_t> polar(n);
+  for (int i = 0; i < n; ++i) {
+    int t = itype[i] - 1;
+    it0[i] = t;
+    natmi[t]++;
+    q[i] = d->qch[t];
+    m[i] = d->mass[t];
+    for (int a = 0; a < 3; ++a) dip[3 * i + a] = d->dip[t + 16 * a];
+    polar[i] = d->ldip_polar[t];
+    if (d->ldip_polar[t]) {
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) pol[9 * (size_t)i + a + 3 * b] = d->poldip[t + 16 * a + 48 * b];
+      inv3(&pol[9 * (size_t)i], &ipol[9 * (size_t)i]);
+    }
+  }
+  build_pair_params(c, natmi);
+  DMALLOC(c->rx, n); DMALLOC(c->ry, n); DMALLOC(c->rz, n);
+  DMALLOC(c->vx, n); DMALLOC(c->vy, n); DMALLOC(c->vz, n);
+  DMALLOC(c->fx, n); DMALLOC(c->fy, n); DMALLOC(c->fz, n);
+  DMALLOC(c->fsx, n); DMALLOC(c->fsy, n); DMALLOC(c->fsz, n);
+  DMALLOC(c->sx, n); DMALLOC(c->sy, n); DMALLOC(c->sz, n);
+  DMALLOC(c->xs, n); DMALLOC(c->ys, n); DMALLOC(c->zs, n);
+  DMALLOC(c->bx, n); DMALLOC(c->by, n); DMALLOC(c->bz, n);
+  DMALLOC(c->itype, n);
+  DMALLOC(c->q, n);
+  DMALLOC(c->mass, n);
+  DMALLOC(c->pol, 9 * (size_t)n);
+  DMALLOC(c->invpol, 9 * (size_t)n);
+  DMALLOC(c->dipia, 3 * (size_t)n);
+  DMALLOC(c->mu, 3 * (size_t)n);
+  DMALLOC(c->mu_ind, 3 * (size_t)n);
+  DMALLOC(c->efield, 3 * (size_t)n);
+  DMALLOC(c->e_stat, 3 * (size_t)n);
+  DMALLOC(c->e_tot, 3 * (size_t)n);
+  DMALLOC(c->e_ind, 3 * (size_t)n);
+  DMALLOC(c->hist, 15 * (size_t)n);
+  DMALLOC(c->frec, 3 * (size_t)n);
+  // per-atom scratch: NPA columns + f_sr, f_dir, ef_dir, ef_rec (3n each) + 8 extra columns
+  DMALLOC(c->peratom, (size_t)(NPA + 12 + 8) * n);
+  DMALLOC(c->d_pp, c->pp.size());
+  HIP_TRY(c, hipMemcpy(c->d_pp, c->pp.data(), sizeof(PairParam) * c->pp.size(), hipMemcpyHostToDevice));
+  DMALLOC(c->cell_of, n);
+  DMALLOC(c->cell_key, n);
+  DMALLOC(c->cell_atoms, n);
+  DMALLOC(c->cell_atoms_in, n);
+  DMALLOC(c->nl_count, n + 1);
+  DMALLOC(c->nl_start, n + 1);
+  DMALLOC(c->d_flag, 4);
+  DMALLOC(c->d_disp, 2 * (size_t)nblk(n) + 2);
+  HIP_TRY(c, hipHostMalloc((void**)&c->h_flag, sizeof(int32_t) * 4));
+  const size_t nred = std::max<size_t>(64 * 1024, 2 * (size_t)nblk(n) + 2);
+  HIP_TRY(c, hipHostMalloc((void**)&c->h_red, sizeof(double) * nred));
+  DMALLOC(c->red_part, nred);
+  DMALLOC(c->red, 64);
+  HIP_TRY(c, hipMemcpy(c->itype, it0.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->q, q.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->mass, m.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->pol, pol.data(), sizeof(double) * 9 * n, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->invpol, ipol.data(), sizeof(double) * 9 * n, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->dipia, dip.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMalloc((void**)&c->polar, sizeof(int32_t) * n));
+  HIP_TRY(c, hipMemcpy(c->polar, polar.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  c->npol = 0;
+  c->qsq = 0.0;
+  for (int i = 0; i < n; ++i) {
+    c->npol += polar[i] != 0;
+    c->qsq = c->qsq + q[i] * q[i];
+  }
+  if (d->lcoulomb) MDFF_TRY(recip_setup(c));
+  return MDFF_OK;
+}
+
+// scratch views inside c->peratom
+static double* col(mdff_ctx* c, int k) { return c->peratom + (size_t)k * c->n; }
+static double* f_sr(mdff_ctx* c) { return col(c, NPA); }          // 3n
+static double* f_dir(mdff_ctx* c) { return col(c, NPA + 3); }     // 3n
+static double* ef_dir(mdff_ctx* c) { return col(c, NPA + 6); }    // 3n
+static double* ef_rec(mdff_ctx* c) { return col(c, NPA + 9); }    // 3n
+static double* extra(mdff_ctx* c) { return col(c, NPA + 12); }    // 8 columns
+
+// ------------------------------------------------------------------ Ewald pieces
+
+// self / surface sums (src/ewald.f90:112-143)
+struct Moments {
+  double musq, mutot[3], qtot[3];
+};
+
+static int moments(mdff_ctx* c, const double* q, const double* mu, Moments* m) {
+  hipLaunchKernelGGL(k_moments, dim3(nblk(c->n)), dim3(256), 0, c->stream, c->n, q, mu, c->rx, c->ry, c->rz,
+                     extra(c));
+  double h[7];
+  MDFF_TRY(reduce_cols(c, extra(c), 7, h));
+  m->musq = h[0];
+  for (int a = 0; a < 3; ++a) {
+    m->mutot[a] = h[1 + a];
+    m->qtot[a] = h[4 + a];
+  }
+  return MDFF_OK;
+}
+
+// full Ewald field of a charge set (damped, static SCF field) or a dipole set
+// (undamped induced field): ef_dir + ef_rec + ef_self (+ ef_surf)
+static int ewald_field(mdff_ctx* c, bool charges, const double* mu, double* out) {
+  const double alpha2 = c->alpha * c->alpha;
+  const double selfa = c->alpha / sqrt(MDFF_PI), selfa2 = 2.0 * selfa * alpha2 / 3.0;
+  const double fpi_3V = 2.0 * (2.0 * MDFF_PI / c->omega / 3.0);
+  MDFF_TRY(realspace_field(c, charges ? 0 : 1, mu, ef_dir(c)));
+  MDFF_TRY(recip_eval(c, c->q, mu, charges, !charges, false, true, ef_rec(c), nullptr, nullptr));
+  double s[3] = {0, 0, 0};
+  if (c->d.lsurf) {
+    Moments m;
+    MDFF_TRY(moments(c, c->q, charges ? nullptr : mu, &m));
+    // the SCF zeroes the charges for the induced field (src/moment_from_pola.f90:347-351)
+    for (int a = 0; a < 3; ++a) s[a] = -(charges ? m.qtot[a] : m.mutot[a]) * fpi_3V;
+  }
+  hipLaunchKernelGGL(k_combine_field, dim3(nblk(c->n)), dim3(256), 0, c->stream, c->n, ef_dir(c), ef_rec(c),
+                     charges ? nullptr : mu, 2.0 * selfa2, c->d.lsurf, s[0], s[1], s[2], out);
+  HIP_TRY(c, hipGetLastError());
+  return MDFF_OK;
+}
+
+// moment_from_pola_scf (src/moment_from_pola.f90:240-485)
+static int scf(mdff_ctx* c, mdff_terms* t) {
+  const int n = c->n;
+  const size_t n3 = 3 * (size_t)n;
+  const mdff_desc& d = c->d;
+  MDFF_TRY(ewald_field(c, true, nullptr, c->e_stat));
+  HIP_TRY(c, hipMemcpyAsync(c->e_tot, c->e_stat, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  int iscf = 0;
+  double rmsd = 1.79769313486231571e308;
+  const int npol = c->npol;
+  int32_t* polar = c->polar;
+  // ASPC order (src/moment_from_pola.f90:505-522)
+  int ext;
+  if (c->itime >= d.extrapolate_order + 1) ext = d.extrapolate_order;
+  else if (c->itime == 1 || c->itime == 0) ext = 0;
+  else ext = c->itime - 1;
+  double Bc[5] = {0, 0, 0, 0, 0}, w = 0.0;
+  switch (ext) {
+    case 0: Bc[0] = 1.0; w = 0.0; break;
+    case 1: Bc[0] = 2.0; Bc[1] = -1.0; w = 2.0 / 3.0; break;
+    case 2: Bc[0] = 2.5; Bc[1] = -2.0; Bc[2] = 0.5; w = 0.6; break;
+    case 3: Bc[0] = 2.8; Bc[1] = -2.8; Bc[2] = 1.2; Bc[3] = -0.2; w = 4.0 / 7.0; break;
+    default: Bc[0] = 3.0; Bc[1] = -24.0 / 7.0; Bc[2] = 27.0 / 14.0; Bc[3] = -4.0 / 7.0; Bc[4] = 1.0 / 14.0; w = 5.0 / 9.0; break;
+  }
+  double upol = 0.0;
+  while ((iscf < d.max_scf_pol_iter && rmsd > d.conv_tol_ind) || iscf < d.min_scf_pol_iter) {
+    iscf++;
+    if (iscf != 1 || !d.calc_md) {
+      hipLaunchKernelGGL(k_induced, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->pol, c->invpol, c->e_tot,
+                         c->mu_ind, extra(c));
+    } else {
+      hipLaunchKernelGGL(k_aspc_predict, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, ext, Bc[0], Bc[1],
+                         Bc[2], Bc[3], Bc[4], c->hist, c->mu_ind);
+      hipLaunchKernelGGL(k_upol, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->invpol, c->mu_ind, extra(c));
+    }
+    double h[1];
+    MDFF_TRY(reduce_cols(c, extra(c), 1, h));
+    upol = 0.5 * h[0];
+    MDFF_TRY(ewald_field(c, false, c->mu_ind, c->e_ind));
+    hipLaunchKernelGGL(k_add3, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, c->e_stat, c->e_ind, c->e_tot);
+    if (iscf == 1 && d.calc_md)
+      hipLaunchKernelGGL(k_aspc_correct, dim3(nblk(n)), dim3(256), 0, c->stream, n, w, c->pol, c->e_tot, c->mu_ind);
+    hipLaunchKernelGGL(k_rmsd_terms, dim3(nblk(n)), dim3(256), 0, c->stream, n, polar, c->pol, c->e_tot, c->mu_ind,
+                       extra(c));
+    MDFF_TRY(reduce_cols(c, extra(c), 1, h));
+    rmsd = sqrt(h[0] / (double)npol);
+  }
+  HIP_TRY(c, hipMemcpyAsync(c->hist, c->mu_ind, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  t->u_pol = upol;
+  t->n_scf = iscf;
+  t->rmsd = rmsd;
+  return MDFF_OK;
+}
+
+static void ev_rec(mdff_ctx* c, int k) { (void)hipEventRecord(c->ev[k], c->stream); }
+
+// SUBROUTINE engforce (src/engforce_driver.f90:102-190)
+static int engforce(mdff_ctx* c, mdff_terms* t) {
+  const mdff_desc& d = c->d;
+  const int n = c->n;
+  memset(t, 0, sizeof *t);
+  ev_rec(c, 0);
+  if (!d.lvnlist || !c->lists_valid) {
+    MDFF_TRY(nl_build(c));
+    t->list_rebuilt = 1;
+  } else {
+    launch_kardir(c);
+  }
+  ev_rec(c, 1);
+  const bool coul = d.lcoulomb != 0;
+  if (coul) MDFF_TRY(recip_tables(c));
+  // induced dipoles (get_dipole_moments, src/moment_from_pola.f90:84-224)
+  const size_t n3 = 3 * (size_t)n;
+  if (coul && c->linduced) {
+    MDFF_TRY(scf(c, t));
+    hipLaunchKernelGGL(k_add3, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, c->dipia, c->mu_ind, c->mu);
+  } else {
+    HIP_TRY(c, hipMemcpyAsync(c->mu, c->dipia, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  }
+  ev_rec(c, 2);
+  if (coul && c->linduced) {
+    hipLaunchKernelGGL(k_maxcomp, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->mu, extra(c));
+    double h[1];
+    MDFF_TRY(reduce_cols(c, extra(c), 1, h));
+    if (h[0] > 0.0) return fail(c, MDFF_ERR_POLAR_CATASTROPHE, "ERROR : POLARIZATION CATASTROPH");
+  }
+  // short range + real-space Ewald
+  const bool want_field = coul && d.doefield;
+  MDFF_TRY(realspace_force_impl(c, f_sr(c), f_dir(c), ef_dir(c), want_field));
+  ev_rec(c, 3);
+  double erec[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (coul) {
+    const bool dip = c->task_qd || c->task_dd;
+    MDFF_TRY(recip_eval(c, c->q, c->mu, true, dip, true, want_field, ef_rec(c), c->frec, erec));
+  }
+  ev_rec(c, 4);
+  hipLaunchKernelGGL(k_combine_forces, dim3(nblk(n)), dim3(256), 0, c->stream, n, f_sr(c), f_dir(c), c->frec,
+                     d.coul_unit, c->sr_model != 0 ? 1 : 0, coul ? 1 : 0, c->fx, c->fy, c->fz);
+  double pa[NPA];
+  MDFF_TRY(reduce_cols(c, c->peratom, NPA, pa));
+  // short range (full rows count every pair twice)
+  const double inv_vp = 1.0 / c->omega / d.press_unit;
+  double tnb[9];
+  {
+    const double* s = pa + 2;
+    double v[9] = {s[0], s[3], s[4], s[3], s[1], s[5], s[4], s[5], s[2]};
+    for (int k = 0; k < 9; ++k) tnb[k] = 0.5 * v[k] * inv_vp;
+  }
+  double usr = 0.5 * pa[0], vir = 0.5 * pa[1] / 3.0;
+  if (c->sr_model == 1) { t->u_lj = usr; t->vir_lj = vir; t->u_tail = c->u_tail; t->p_tail = c->p_tail; }
+  if (c->sr_model == 2) { t->u_bmhft = usr; t->vir_bmhft = vir; }
+  if (c->sr_model == 3) { t->u_morse = usr; t->vir_morse = vir; }
+  if (c->sr_model) for (int k = 0; k < 9; ++k) { t->tau_nonb[k] = tnb[k]; t->tau[k] += tnb[k]; }
+  if (coul) {
+    Moments m;
+    MDFF_TRY(moments(c, c->q, c->mu, &m));
+    const double qsq = c->qsq;
+    const double tpi_V = 2.0 * MDFF_PI / c->omega, tpi_3V = tpi_V / 3.0, fpi_3V = tpi_3V * 2.0;
+    const double alpha2 = c->alpha * c->alpha;
+    const double selfa = c->alpha / sqrt(MDFF_PI), selfa2 = 2.0 * selfa * alpha2 / 3.0;
+    t->u_dir = 0.5 * pa[8];
+    t->u_rec = 2.0 * erec[0] * tpi_V;
+    t->u_self = -selfa * qsq + (-selfa2 * m.musq);
+    double uq = 0, ud = 0, uqd = 0;
+    for (int a = 0; a < 3; ++a) {
+      uq += m.qtot[a] * m.qtot[a];
+      ud += m.mutot[a] * m.mutot[a];
+      uqd += m.qtot[a] * m.mutot[a];
+    }
+    t->u_surf = d.lsurf ? (uq + 2.0 * uqd + ud) * tpi_3V : 0.0;
+    t->u_coul = (t->u_dir + t->u_rec + t->u_surf + t->u_self + t->u_pol) * d.coul_unit;
+    // tau_dir: per-atom -(r f + f r) summed over full rows: x 0.5 (double count) x 0.5 / omega
+    double tdir[9], trec[9];
+    {
+      const double* s = pa + 9;
+      double v[9] = {s[0], s[3], s[4], s[3], s[1], s[5], s[4], s[5], s[2]};
+      for (int k = 0; k < 9; ++k) tdir[k] = 0.25 * v[k] / c->omega;
+      double r[9] = {erec[1], erec[4], erec[5], erec[4], erec[2], erec[6], erec[5], erec[6], erec[3]};
+      for (int k = 0; k < 9; ++k) trec[k] = 2.0 * r[k] * tpi_V / c->omega;
+    }
+    for (int k = 0; k < 9; ++k) {
+      t->tau_coul[k] = (tdir[k] + trec[k]) * d.coul_unit / d.press_unit;
+      t->tau[k] += 2.0 * t->tau_coul[k];
+    }
+    t->pvirial_coul = 1.0 / 3.0 * (t->tau_coul[0] + t->tau_coul[4] + t->tau_coul[8]) * d.press_unit;
+    // ef_t = (ef_dir + ef_rec if doefield) + ef_self (+ ef_surf)
+    double s[3] = {0, 0, 0};
+    if (d.lsurf)
+      for (int a = 0; a < 3; ++a) s[a] = -(m.qtot[a] + m.mutot[a]) * fpi_3V;
+    if (!want_field) {
+      hipLaunchKernelGGL(k_fill, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, 0.0, ef_dir(c));
+      hipLaunchKernelGGL(k_fill, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, 0.0, ef_rec(c));
+    }
+    hipLaunchKernelGGL(k_combine_field, dim3(nblk(n)), dim3(256), 0, c->stream, n, ef_dir(c), ef_rec(c), c->mu,
+                       2.0 * selfa2, d.lsurf, s[0], s[1], s[2], c->efield);
+  }
+  ev_rec(c, 5);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[5]); c->last_ms[0] = ms;
+  (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]); c->last_ms[1] = ms;
+  (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]); c->last_ms[2] = ms;
+  (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]); c->last_ms[3] = ms;
+  (void)hipEventElapsedTime(&ms, c->ev[1], c->ev[2]); c->last_ms[4] = ms;
+  c->n_last = 5;
+  HIP_TRY(c, hipGetLastError());
+  return MDFF_OK;
+}
+
+// ====================================================================== C-ABI
+
+extern "C" {
+
+void mdff_desc_defaults(mdff_desc* d) {
+  memset(d, 0, sizeof *d);
+  d->abi_version = MDFF_ABI_VERSION;
+  d->lvnlist = 1;
+  d->calc_md = 1;
+  d->skindiff = 0.15;
+  d->coul_unit = 14.3996441494161;
+  d->press_unit = 0.00624150964712042;
+  for (int i = 0; i < 256; ++i) { d->plj[i] = 6.0; d->qlj[i] = 12.0; }
+  d->alphaES = 1.0;
+  d->kES[0] = d->kES[1] = d->kES[2] = 10;
+  d->conv_tol_ind = 1e-6;
+  d->min_scf_pol_iter = 3;
+  d->max_scf_pol_iter = 100;
+}
+
+const char* mdff_version(void) { return "mdff_gpu 1 (gfx950)"; }
+
+const char* mdff_last_error(const mdff_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mdff_ctx_create(const mdff_desc* d, const int32_t* itype, mdff_ctx** out) {
+  if (!out) return MDFF_ERR_BAD_ARG;
+  *out = nullptr;
+  if (!d || !itype || d->abi_version != MDFF_ABI_VERSION || d->natm <= 0 || d->ntype <= 0 || d->ntype > 8)
+    return MDFF_ERR_BAD_ARG;
+  if (d->lnmlj && (d->lbmhft || d->lbmhftd)) return MDFF_ERR_UNSUPPORTED;
+  if (d->lmorse && (d->lnmlj || d->lbmhft || d->lbmhftd)) return MDFF_ERR_UNSUPPORTED;
+  for (int i = 0; i < d->natm; ++i)
+    if (itype[i] < 1 || itype[i] > d->ntype) return MDFF_ERR_BAD_ARG;
+  for (int i = 0; i < 4096; ++i)
+    if (d->pol_damp_k[i] < 0 || d->pol_damp_k[i] > 8) return MDFF_ERR_BAD_ARG;  // src/pim.f90:201-204
+  mdff_ctx* c = new mdff_ctx();
+  int rc = ctx_init(c, d, itype);
+  if (rc != MDFF_OK) {
+    fprintf(stderr, "mdff_ctx_create: %s\n", c->err.c_str());
+    mdff_ctx_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return MDFF_OK;
+}
+
+int mdff_ctx_destroy(mdff_ctx* c) {
+  if (!c) return MDFF_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void* ps[] = {c->rx, c->ry, c->rz, c->vx, c->vy, c->vz, c->fx, c->fy, c->fz, c->fsx, c->fsy, c->fsz,
+                c->sx, c->sy, c->sz, c->xs, c->ys, c->zs, c->bx, c->by, c->bz, c->itype, c->q, c->mass,
+                c->pol, c->invpol, c->dipia, c->mu, c->mu_ind, c->efield, c->e_stat, c->e_tot, c->e_ind,
+                c->hist, c->frec, c->peratom, c->red, c->red_part, c->d_pp, c->cell_of, c->cell_key,
+                c->cell_atoms, c->cell_atoms_in, c->cell_start, c->nl_count, c->nl_start, c->nl_j,
+                c->nl_flag, c->sort_tmp, c->d_flag, c->d_disp};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  if (c->polar) (void)hipFree(c->polar);
+  recip_free(c);
+  if (c->h_flag) (void)hipHostFree(c->h_flag);
+  if (c->h_red) (void)hipHostFree(c->h_red);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return MDFF_OK;
+}
+
+static int copy3_in(mdff_ctx* c, double* dx, double* dy, double* dz, const double* x, const double* y,
+                    const double* z) {
+  if (!c || !x || !y || !z) return MDFF_ERR_BAD_ARG;
+  const size_t b = sizeof(double) * c->n;
+  HIP_TRY(c, hipMemcpyAsync(dx, x, b, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(dy, y, b, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(dz, z, b, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return MDFF_OK;
+}
+
+static int copy3_out(mdff_ctx* c, const double* dx, const double* dy, const double* dz, double* x, double* y,
+                     double* z) {
+  if (!c || !x || !y || !z) return MDFF_ERR_BAD_ARG;
+  const size_t b = sizeof(double) * c->n;
+  HIP_TRY(c, hipMemcpyAsync(x, dx, b, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(y, dy, b, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(z, dz, b, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return MDFF_OK;
+}
+
+int mdff_set_positions(mdff_ctx* c, const double* x, const double* y, const double* z) {
+  return copy3_in(c, c->rx, c->ry, c->rz, x, y, z);
+}
+int mdff_get_positions(mdff_ctx* c, double* x, double* y, double* z) {
+  return copy3_out(c, c->rx, c->ry, c->rz, x, y, z);
+}
+int mdff_set_velocities(mdff_ctx* c, const double* x, const double* y, const double* z) {
+  return copy3_in(c, c->vx, c->vy, c->vz, x, y, z);
+}
+int mdff_get_velocities(mdff_ctx* c, double* x, double* y, double* z) {
+  return copy3_out(c, c->vx, c->vy, c->vz, x, y, z);
+}
+int mdff_get_forces(mdff_ctx* c, double* x, double* y, double* z) {
+  return copy3_out(c, c->fx, c->fy, c->fz, x, y, z);
+}
+
+int mdff_set_cell(mdff_ctx* c, const double cell[9]) {
+  if (!c || !cell) return MDFF_ERR_BAD_ARG;
+  memcpy(c->A, cell, sizeof c->A);
+  lattice(c->A, c->B, &c->omega, c->W);
+  c->lists_valid = 0;
+  return MDFF_OK;
+}
+
+int mdff_set_itime(mdff_ctx* c, int32_t itime) {
+  if (!c) return MDFF_ERR_BAD_ARG;
+  c->itime = itime;
+  return MDFF_OK;
+}
+
+int mdff_vnlist_build(mdff_ctx* c) {
+  if (!c) return MDFF_ERR_BAD_ARG;
+  MDFF_TRY(nl_build(c));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return MDFF_OK;
+}
+
+int mdff_vnlistcheck(mdff_ctx* c, int32_t* rebuilt) {
+  if (!c) return MDFF_ERR_BAD_ARG;
+  int rb = 0;
+  MDFF_TRY(nl_check(c, &rb));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (rebuilt) *rebuilt = rb;
+  return MDFF_OK;
+}
+
+int mdff_get_pairs(mdff_ctx* c, int32_t which, int32_t* point, int32_t* list, int64_t cap, int64_t* n) {
+  if (!c || (which != 0 && which != 1)) return MDFF_ERR_BAD_ARG;
+  return nl_get_pairs(c, which, point, list, cap, n);
+}
+
+int mdff_engforce(mdff_ctx* c, double* fx, double* fy, double* fz, mdff_terms* terms) {
+  if (!c) return MDFF_ERR_BAD_ARG;
+  mdff_terms t;
+  MDFF_TRY(engforce(c, &t));
+  if (terms) *terms = t;
+  if (fx && fy && fz) MDFF_TRY(mdff_get_forces(c, fx, fy, fz));
+  return MDFF_OK;
+}
+
+int mdff_get_dipoles(mdff_ctx* c, double* mu, double* ef) {
+  if (!c) return MDFF_ERR_BAD_ARG;
+  const size_t b = 3 * sizeof(double) * c->n;
+  if (mu) HIP_TRY(c, hipMemcpyAsync(mu, c->mu, b, hipMemcpyDeviceToHost, c->stream));
+  if (ef) HIP_TRY(c, hipMemcpyAsync(ef, c->efield, b, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return MDFF_OK;
+}
+
+static double upot_of(const mdff_terms& t) { return t.u_lj + t.u_bmhft + t.u_morse + t.u_coul; }
+
+int mdff_md_vv(mdff_ctx* c, int32_t nsteps, double dt, double* ekin, double* upot) {
+  if (!c || nsteps < 0) return MDFF_ERR_BAD_ARG;
+  const int n = c->n;
+  for (int st = 0; st < nsteps; ++st) {
+    hipLaunchKernelGGL(k_vv_pos, dim3(nblk(n)), dim3(256), 0, c->stream, n, dt, dt * dt * 0.5, c->mass, c->rx,
+                       c->ry, c->rz, c->vx, c->vy, c->vz, c->fx, c->fy, c->fz, c->fsx, c->fsy, c->fsz);
+    mdff_terms t;
+    MDFF_TRY(engforce(c, &t));
+    hipLaunchKernelGGL(k_vv_vel, dim3(nblk(n)), dim3(256), 0, c->stream, n, dt * 0.5, c->mass, c->vx, c->vy,
+                       c->vz, c->fsx, c->fsy, c->fsz, c->fx, c->fy, c->fz, extra(c));
+    if (ekin) {
+      double h[1];
+      MDFF_TRY(reduce_cols(c, extra(c), 1, h));
+      ekin[st] = 0.5 * h[0];
+    }
+    if (upot) upot[st] = upot_of(t);
+    if (c->d.lvnlist) {
+      int rb = 0;
+      MDFF_TRY(nl_check(c, &rb));
+    }
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return MDFF_OK;
+}
+
+static int kinetic(mdff_ctx* c, double* kin, double* vsum) {
+  hipLaunchKernelGGL(k_kin, dim3(nblk(c->n)), dim3(256), 0, c->stream, c->n, c->mass, c->vx, c->vy, c->vz,
+                     extra(c));
+  double h[4];
+  MDFF_TRY(reduce_cols(c, extra(c), 4, h));
+  *kin = h[0] * 0.5;
+  if (vsum) { vsum[0] = h[1]; vsum[1] = h[2]; vsum[2] = h[3]; }
+  return MDFF_OK;
+}
+
+// chain_nhcn (src/integration.f90:619-701): thermostat variables on the host,
+// the velocity scale s applied on the device
+static int chain(mdff_ctx* c, double* kin, double* vxi, double* xi, const double* Q, double L, double temp,
+                 double dt, int nhc, int yosh, int mults) {
+  double w[9] = {1, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (yosh == 3) { w[0] = 1.0 / (2.0 - pow(2.0, 1.0 / 3.0)); w[1] = 1.0 - 2.0 * w[0]; w[2] = w[0]; }
+  else if (yosh == 5) { w[0] = 1.0 / (4.0 - pow(4.0, 1.0 / 3.0)); w[1] = w[2] = w[3] = w[0]; w[4] = 1.0 - 4.0 * w[0]; }
+  double G[16] = {0};
+  double s = 1.0;
+  G[0] = 2.0 * (*kin) - L * temp;
+  for (int k = 0; k < mults; ++k)
+    for (int j = 0; j < yosh; ++j) {
+      double dts = (w[j] * dt) / (double)mults, dts2 = dts * 0.5, dts4 = dts2 * 0.5, dts8 = dts4 * 0.5;
+      G[nhc - 1] = (vxi[nhc - 2] * vxi[nhc - 2] / Q[nhc - 2] - temp);
+      vxi[nhc - 1] = vxi[nhc - 1] + G[nhc - 1] * dts4;
+      for (int h = nhc - 2; h >= 0; --h) {
+        vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
+        vxi[h] = vxi[h] + G[h] * dts4;
+        vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
+      }
+      s = s * exp(-vxi[0] * dts2 / Q[0]);
+      for (int h = 0; h < nhc; ++h) xi[h] = xi[h] + vxi[h] * dts2 / Q[h];
+      G[0] = 2.0 * (*kin) * s * s - L * temp;
+      for (int h = 0; h < nhc - 1; ++h) {
+        vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
+        vxi[h] = vxi[h] + G[h] * dts4;
+        vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
+        G[h + 1] = (vxi[h] * vxi[h] / Q[h] - temp);
+      }
+      vxi[nhc - 1] = vxi[nhc - 1] + G[nhc - 1] * dts4;
+    }
+  hipLaunchKernelGGL(k_scale_v, dim3(nblk(c->n)), dim3(256), 0, c->stream, c->n, s, 0.0, 0.0, 0.0, c->vx, c->vy,
+                     c->vz);
+  return kinetic(c, kin, nullptr);
+}
+
+int mdff_md_nhcn(mdff_ctx* c, int32_t nsteps, double dt, double temp, double timesca, int32_t nhc, int32_t yosh,
+                 int32_t mults, double* xi, double* vxi, double* ekin, double* upot, double* e_nvt) {
+  if (!c || nhc < 2 || nhc > 16 || !xi || !vxi || (yosh != 1 && yosh != 3 && yosh != 5) || mults < 1)
+    return MDFF_ERR_BAD_ARG;
+  const int n = c->n;
+  const double L = 3.0 * (double)n;
+  double Q[16];
+  for (int h = 0; h < nhc; ++h) Q[h] = pow(timesca, 2.0) * temp;
+  Q[0] = Q[0] * L;
+  for (int st = 0; st < nsteps; ++st) {
+    double kin;
+    MDFF_TRY(kinetic(c, &kin, nullptr));
+    MDFF_TRY(chain(c, &kin, vxi, xi, Q, L, temp, dt, nhc, yosh, mults));
+    hipLaunchKernelGGL(k_kick_drift, dim3(nblk(n)), dim3(256), 0, c->stream, n, dt * 0.5, dt, c->mass, c->rx, c->ry,
+                       c->rz, c->vx, c->vy, c->vz, c->fx, c->fy, c->fz);
+    mdff_terms t;
+    MDFF_TRY(engforce(c, &t));
+    hipLaunchKernelGGL(k_kick, dim3(nblk(n)), dim3(256), 0, c->stream, n, dt * 0.5, c->mass, c->vx, c->vy, c->vz,
+                       c->fx, c->fy, c->fz);
+    MDFF_TRY(kinetic(c, &kin, nullptr));
+    MDFF_TRY(chain(c, &kin, vxi, xi, Q, L, temp, dt, nhc, yosh, mults));
+    if (ekin) ekin[st] = kin;
+    if (upot) upot[st] = upot_of(t);
+    if (e_nvt) {
+      double e3 = 0.0, e4 = 0.0;
+      for (int h = 1; h < nhc; ++h) { e3 = e3 + vxi[h] * vxi[h] * 0.5 / Q[h]; e4 = e4 + xi[h]; }
+      e_nvt[st] = L * temp * xi[0] + vxi[0] * vxi[0] * 0.5 / Q[0] + e3 + temp * e4;
+    }
+    if (c->d.lvnlist) {
+      int rb = 0;
+      MDFF_TRY(nl_check(c, &rb));
+    }
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return MDFF_OK;
+}
+
+int mdff_csvr_rescale(mdff_ctx* c, double temp, double taut, double r1, double sumnoise, double* conint) {
+  if (!c) return MDFF_ERR_BAD_ARG;
+  const int n = c->n;
+  double ekin;
+  MDFF_TRY(kinetic(c, &ekin, nullptr));
+  const int ndeg = 3 * n - 3;
+  const double sigma = ndeg * temp * 0.5;
+  const double factor = taut > 0.1 ? exp(-1.0 / taut) : 0.0;
+  const double knew = ekin + (1.0 - factor) * (sigma * (sumnoise + r1 * r1) / ndeg - ekin) +
+                      2.0 * r1 * sqrt(ekin * sigma / ndeg * (1.0 - factor) * factor);
+  if (conint) *conint = *conint - (knew - ekin);
+  const double lambda = sqrt(knew / ekin);
+  hipLaunchKernelGGL(k_scale_v, dim3(nblk(n)), dim3(256), 0, c->stream, n, lambda, 0.0, 0.0, 0.0, c->vx, c->vy,
+                     c->vz);
+  double k2, vs[3];
+  MDFF_TRY(kinetic(c, &k2, vs));
+  hipLaunchKernelGGL(k_scale_v, dim3(nblk(n)), dim3(256), 0, c->stream, n, 1.0, vs[0] / n, vs[1] / n, vs[2] / n,
+                     c->vx, c->vy, c->vz);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return MDFF_OK;
+}
+
+int mdff_calc_temp(mdff_ctx* c, double boltz_unit, double* temp, double* ekin) {
+  if (!c) return MDFF_ERR_BAD_ARG;
+  double e;
+  MDFF_TRY(kinetic(c, &e, nullptr));
+  if (ekin) *ekin = e;
+  if (temp) *temp = 2.0 * e / boltz_unit / (3.0 * c->n);
+  return MDFF_OK;
+}
+
+int mdff_last_timings(mdff_ctx* c, double* ms, int32_t* n) {
+  if (!c || !ms) return MDFF_ERR_BAD_ARG;
+  for (int k = 0; k < c->n_last; ++k) ms[k] = c->last_ms[k];
+  if (n) *n = c->n_last;
+  return MDFF_OK;
+}
+
+}  // extern "C"

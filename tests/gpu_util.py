@@ -1,0 +1,61 @@
+"""Helpers of the GPU parity tests: run the same system through the HIP
+library and through the oracle, and measure the differences."""
+from __future__ import annotations
+
+import copy
+import sys
+
+import numpy as np
+
+import oracle_lib as O
+from mdff_amd import capi
+
+TERMS = ["u_lj", "u_bmhft", "u_morse", "u_coul", "u_pol", "u_dir", "u_rec", "u_self", "u_surf",
+         "vir_lj", "vir_bmhft", "vir_morse"]
+
+
+def note(msg: str) -> None:
+    """Name the case before a GPU step (flushed, so a crash still shows it)."""
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def pair(system, precision=capi.PREC_FP64, **over):
+    d = copy.deepcopy(system.desc)
+    for k, v in over.items():
+        setattr(d, k, v)
+    d.precision = precision
+    g = capi.gpu_engine(d, system.itype)
+    d2 = copy.deepcopy(system.desc)
+    for k, v in over.items():
+        setattr(d2, k, v)
+    o = O.oracle(d2, system.itype)
+    for e in (g, o):
+        e.set_positions(system.pos)
+        e.set_velocities(system.vel)
+    return g, o
+
+
+def rel_force_err(f, fref):
+    return float(np.abs(f - fref).max() / max(np.abs(fref).max(), 1e-300))
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    scale = max(np.abs(b).max(), 1e-300)
+    return float(np.abs(a - b).max() / scale)
+
+
+def compare_terms(tg, to, tol, keys=TERMS):
+    bad = {}
+    for k in keys:
+        a, b = tg[k], to[k]
+        if abs(b) < 1e-300 and abs(a) < 1e-300:
+            continue
+        e = abs(a - b) / max(abs(b), 1e-12)
+        if e > tol:
+            bad[k] = (a, b, e)
+    for k in ("tau_nonb", "tau_coul"):
+        if np.abs(to[k]).max() > 0 and rel(tg[k], to[k]) > tol:
+            bad[k] = (tg[k], to[k], rel(tg[k], to[k]))
+    return bad

@@ -1,0 +1,294 @@
+"""Parity of the HIP library with the oracle (CPU restatement of the
+reference), through the C-ABI.  Bar: neighbour pair sets bit-exact; FP64
+form within 1e-10 relative on energies, forces, stress, fields and induced
+dipoles; mixed form within 1e-5 relative force error (max |dF| / max |F|)."""
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from gpu_util import compare_terms, note, pair, rel, rel_force_err
+from mdff_amd import capi, configs
+
+pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).resolve().parent / "golden"
+TOL64 = 1e-10
+TOL_MIXED = 1e-5
+
+
+def _lists_equal(g, o, which):
+    pg, lg = g.get_pairs(which)
+    po, lo = o.get_pairs(which)
+    assert np.array_equal(pg, po)
+    assert np.array_equal(lg, lo)
+    return len(lo)
+
+
+def _sio2(natm=600, seed=21, **kw):
+    kw.setdefault("rcut", 7.0)
+    kw.setdefault("kmax", 6)
+    kw.setdefault("skin", 0.5)
+    kw.setdefault("alpha", 0.4)
+    return configs.sio2(natm=natm, seed=seed, **kw)
+
+
+# ------------------------------------------------------------------ pair sets
+
+@pytest.mark.parametrize("case", ["nacl_perfect", "nacl", "sio2", "fcc_lj", "two_cut", "triclinic"])
+def test_pair_sets_bit_exact(case):
+    """vnlist_pbc (src/tools.f90:273-368): identical CSR half lists."""
+    if case == "nacl_perfect":
+        s = configs.nacl(ncell=3, sigma=0.0)       # exact +-0.5 fractional differences
+    elif case == "nacl":
+        s = configs.nacl(ncell=3)
+    elif case == "sio2":
+        s = _sio2()
+    elif case == "fcc_lj":
+        s = configs.fcc_lj(ncell=5, L=7.707, rc=3.0)  # two cells per direction
+    elif case == "two_cut":
+        s = _sio2(seed=22)
+        s.desc.cutshortrange = 5.0
+        s.desc.cutlongrange = 7.5
+    else:
+        s = _sio2(seed=23)
+        A = np.array(s.desc.cell[:]).reshape(3, 3, order="F")
+        shear = np.array([[1.0, 0.18, -0.07], [0.0, 1.0, 0.11], [0.0, 0.0, 1.0]])
+        A2 = shear @ A
+        s.desc.cell[:] = list(A2.reshape(-1, order="F"))
+        s.pos = s.pos @ shear.T
+    note(f"pair sets {case} n={s.natm}")
+    g, o = pair(s)
+    g.vnlist_build()
+    o.vnlist_build()
+    n0 = _lists_equal(g, o, 0)
+    n1 = _lists_equal(g, o, 1)
+    assert n0 > 0
+    assert n1 > 0 or s.desc.cutlongrange == 0.0  # LJ-only: the coul list is empty
+
+
+# ------------------------------------------------------------------ FP64 engforce
+
+def _check_engforce(s, tol=TOL64, precision=capi.PREC_FP64, **over):
+    g, o = pair(s, precision=precision, **over)
+    fg, tg = g.engforce()
+    fo, to = o.engforce()
+    err = rel_force_err(fg, fo)
+    bad = compare_terms(tg, to, tol) if precision == capi.PREC_FP64 else {}
+    return g, o, fg, fo, tg, to, err, bad
+
+
+@pytest.mark.parametrize("case", ["nacl_bmh", "sio2_buck", "fcc_lj", "morse", "bmhftd", "triclinic",
+                                  "lsurf", "nolist"])
+def test_engforce_fp64_parity(case):
+    """engforce (src/engforce_driver.f90:102-190), FP64 form vs the oracle."""
+    over = {}
+    if case == "nacl_bmh":
+        s = configs.nacl(ncell=3)
+    elif case == "sio2_buck":
+        s = _sio2()
+    elif case == "fcc_lj":
+        s = configs.fcc_lj(ncell=5, L=7.707, rc=3.0)
+        s.pos = s.pos + np.random.default_rng(3).normal(0, 0.03, s.pos.shape)
+    elif case == "morse":
+        s = _sio2(seed=24)
+        s.desc.lbmhft = 0
+        s.desc.lmorse = 1
+        for (i, j) in [(1, 1), (1, 2), (2, 2)]:
+            capi.set_pair(s.desc, "epsmor", i, j, 0.5 + 0.1 * i)
+            capi.set_pair(s.desc, "rhomor", i, j, 1.7)
+            capi.set_pair(s.desc, "sigmamor", i, j, 2.2 + 0.2 * j)
+    elif case == "bmhftd":
+        s = configs.nacl(ncell=3)
+        s.desc.lbmhft = 0
+        s.desc.lbmhftd = 1
+        for (i, j) in [(1, 1), (1, 2), (2, 2)]:
+            capi.set_pair(s.desc, "BDbmhftd", i, j, 1.9 + 0.3 * (i + j))
+    elif case == "triclinic":
+        s = _sio2(seed=25)
+        A = np.array(s.desc.cell[:]).reshape(3, 3, order="F")
+        shear = np.array([[1.0, 0.15, 0.05], [0.0, 1.0, -0.1], [0.0, 0.0, 1.0]])
+        s.desc.cell[:] = list((shear @ A).reshape(-1, order="F"))
+        s.pos = s.pos @ shear.T
+    elif case == "lsurf":
+        s = configs.nacl(ncell=2)
+        over["lsurf"] = 1
+    else:
+        s = configs.nacl(ncell=2)
+        over["lvnlist"] = 0
+    note(f"engforce fp64 {case} n={s.natm}")
+    *_, err, bad = _check_engforce(s, **over)
+    assert err < TOL64, err
+    assert not bad, bad
+
+
+def test_nymand_linse_on_gpu():
+    """examples/ex04 values reproduced by the GPU path itself."""
+    w = json.loads((GOLD / "ex04.json").read_text())
+    for name, builder in [("two_charges", configs.two_charges), ("two_dipoles", configs.two_dipoles)]:
+        s = builder()
+        s.desc.doefield = 1
+        note(f"nymand-linse {name}")
+        g, o = pair(s)
+        f, t = g.engforce()
+        mu, ef = g.get_dipoles()
+        assert abs(t["u_coul"] - w[name]["U"]) < 1e-6
+        assert abs(ef[0, 0] - w[name]["E1x"]) < 1e-6
+        assert abs(f[0, 0] - w[name]["f1x"]) < 1e-6
+        fo, to = o.engforce()
+        assert rel_force_err(f, fo) < TOL64
+        assert abs(t["u_coul"] - to["u_coul"]) < TOL64 * abs(to["u_coul"])
+        assert rel(ef, o.get_dipoles()[1]) < TOL64
+
+
+def test_field_output_parity():
+    """ef_t with doefield (src/engforce_driver.f90:164-170)."""
+    s = configs.nacl(ncell=2)
+    note("field output nacl")
+    g, o = pair(s, doefield=1)
+    g.engforce()
+    o.engforce()
+    assert rel(g.get_dipoles()[1], o.get_dipoles()[1]) < TOL64
+
+
+# ------------------------------------------------------------------ PIM
+
+def _pim(cells=(2, 2, 3), **kw):
+    kw.setdefault("alpha", 0.8)
+    kw.setdefault("rcut", 4.2)
+    kw.setdefault("kmax", 8)
+    kw.setdefault("tol", 1e-9)
+    return configs.mgo_pim(cells=cells, a=4.212, seed=9, sigma=0.1, **kw)
+
+
+@pytest.mark.parametrize("calc_md", [0, 1])
+def test_pim_scf_fp64_parity(calc_md):
+    """moment_from_pola_scf (src/moment_from_pola.f90:240-485): same
+    iteration count, dipoles, fields, forces and energies."""
+    s = _pim()
+    note(f"pim scf calc_md={calc_md} n={s.natm}")
+    g, o = pair(s, calc_md=calc_md, doefield=1)
+    for it in range(3):  # successive calls exercise the ASPC history
+        g.set_itime(it)
+        o.set_itime(it)
+        fg, tg = g.engforce()
+        fo, to = o.engforce()
+        assert tg["n_scf"] == to["n_scf"]
+        assert rel_force_err(fg, fo) < TOL64
+        bad = compare_terms(tg, to, TOL64)
+        assert not bad, bad
+        mg, eg = g.get_dipoles()
+        mo, eo = o.get_dipoles()
+        assert rel(mg, mo) < TOL64
+        assert rel(eg, eo) < TOL64
+
+
+def test_pim_extrapolation_order_parity():
+    s = _pim()
+    note("pim aspc order 3")
+    g, o = pair(s, calc_md=1, extrapolate_order=3)
+    for it in range(5):
+        g.set_itime(it)
+        o.set_itime(it)
+        fg, tg = g.engforce()
+        fo, to = o.engforce()
+        assert tg["n_scf"] == to["n_scf"]
+        assert rel_force_err(fg, fo) < TOL64
+
+
+# ------------------------------------------------------------------ mixed precision
+
+@pytest.mark.parametrize("case", ["sio2", "nacl", "pim"])
+def test_mixed_precision_force_error(case):
+    """FP32 pair / field arithmetic with FP64 accumulation: max |dF|/max|F|
+    against the FP64 oracle below 1e-5."""
+    if case == "sio2":
+        s = _sio2(natm=900, seed=26)
+    elif case == "nacl":
+        s = configs.nacl(ncell=3)
+    else:
+        s = _pim()
+    note(f"mixed {case} n={s.natm}")
+    g, o = pair(s, precision=capi.PREC_MIXED)
+    fg, tg = g.engforce()
+    fo, to = o.engforce()
+    assert rel_force_err(fg, fo) < TOL_MIXED
+    e_g = tg["u_coul"] + tg["u_bmhft"]
+    e_o = to["u_coul"] + to["u_bmhft"]
+    assert abs(e_g - e_o) < 1e-6 * abs(e_o)
+
+
+# ------------------------------------------------------------------ MD loop
+
+def test_vv_trajectory_and_skin_checks_match():
+    """prop_velocity_verlet + vnlistcheck over 30 steps: positions and
+    velocities track the oracle; list rebuilds happen at the same steps."""
+    s = configs.nacl(ncell=2, seed=5, temp_K=600.0)
+    s.desc.skindiff = 0.3
+    note("vv trajectory nacl")
+    g, o = pair(s)
+    g.engforce()
+    o.engforce()
+    for step in range(30):
+        eg, ug = g.md_vv(1, s.dt)
+        eo, uo = o.md_vv(1, s.dt)
+        assert abs(eg[0] - eo[0]) < 1e-9 * abs(eo[0])
+        assert abs(ug[0] - uo[0]) < 1e-9 * abs(uo[0])
+    assert rel(g.get_positions(), o.get_positions()) < 1e-10
+    assert rel(g.get_velocities(), o.get_velocities()) < 1e-8
+    for which in (0, 1):
+        _lists_equal(g, o, which)
+
+
+def test_nhcn_and_csvr_match():
+    """nose_hoover_chain_n (src/integration.f90:538-701) and the CSVR rescale
+    (src/kinetic.f90:122-198) with caller-supplied noise."""
+    s = configs.nacl(ncell=2, seed=6)
+    note("nhcn + csvr nacl")
+    g, o = pair(s)
+    g.engforce()
+    o.engforce()
+    xi_g, vxi_g = np.zeros(3), np.zeros(3)
+    xi_o, vxi_o = np.zeros(3), np.zeros(3)
+    T = 300 * capi.BOLTZ_UNIT
+    tau = 0.05 * capi.TIME_UNIT
+    r = g.md_nhcn(10, s.dt, T, tau, xi_g, vxi_g, yosh_order=3, nhc_mults=2)
+    q = o.md_nhcn(10, s.dt, T, tau, xi_o, vxi_o, yosh_order=3, nhc_mults=2)
+    assert rel(r[0], q[0]) < 1e-9
+    assert rel(r[2], q[2]) < 1e-9
+    assert rel(r[3], q[3]) < 1e-9
+    ci_g = g.csvr_rescale(T, 10.0, 0.37, 3.0 * s.natm - 4.2, 0.0)
+    ci_o = o.csvr_rescale(T, 10.0, 0.37, 3.0 * s.natm - 4.2, 0.0)
+    assert abs(ci_g - ci_o) < 1e-10 * max(1.0, abs(ci_o))
+    assert rel(g.get_velocities(), o.get_velocities()) < 1e-9
+    tg = g.calc_temp()
+    to = o.calc_temp()
+    assert abs(tg[0] - to[0]) < 1e-9 * to[0]
+
+
+# ------------------------------------------------------------------ invariants at size
+
+def test_sio2_full_size_invariants_mixed_vs_fp64():
+    """The 24,000-ion a-SiO2 workload (too large for the oracle in a test):
+    sum of forces vanishes, and the mixed form agrees with the FP64 form to
+    the 1e-5 bar."""
+    s = configs.sio2()
+    note(f"sio2 full size n={s.natm}")
+    d64 = s.desc
+    g64 = capi.gpu_engine(d64, s.itype)
+    g64.set_positions(s.pos)
+    f64, t64 = g64.engforce()
+    import copy
+    d32 = copy.deepcopy(s.desc)
+    d32.precision = capi.PREC_MIXED
+    g32 = capi.gpu_engine(d32, s.itype)
+    g32.set_positions(s.pos)
+    f32, t32 = g32.engforce()
+    assert np.abs(f64.sum(0)).max() < 1e-8 * np.abs(f64).max() * np.sqrt(s.natm)
+    assert rel_force_err(f32, f64) < TOL_MIXED
+    p64, l64 = g64.get_pairs(0)
+    p32, l32 = g32.get_pairs(0)
+    assert np.array_equal(l64, l32)
