@@ -220,6 +220,16 @@ int mdff_calc_temp(mdff_ctx *ctx, double boltz_unit, double *temp, double *ekin)
  * [4] SCF loop.  n = number of entries written (<= 8). */
 int mdff_last_timings(mdff_ctx *ctx, double *ms, int32_t *n);
 
+/* Kernel timing with HIP events on the context stream; replaces the
+ * reference's MPI_WTIME accumulators (src/time.f90:36-127, src/symbol.h:1-21).
+ * mdff_profile(ctx, 1) clears and starts recording one event pair around
+ * each timed launch; mdff_profile_read returns, per kernel class k <
+ * nkinds, the summed duration (ms) and launch count; mdff_profile_name(k)
+ * names the class (8 classes). */
+int mdff_profile(mdff_ctx *ctx, int32_t on);
+int mdff_profile_read(mdff_ctx *ctx, double *ms_total, int32_t *count, int32_t nkinds);
+const char *mdff_profile_name(int32_t kind);
+
 #ifdef __cplusplus
 }
 #endif

@@ -344,3 +344,23 @@ def scaled(builder, natm: int, **kw) -> System:
 
 __all__ = ["System", "nacl", "sio2", "lif", "mgo_pim", "two_charges", "two_dipoles", "lj_cluster",
            "fcc_lj", "maxwell_boltzmann", "random_packing", "set_pim", "COUL_UNIT", "PRESS_UNIT"]
+
+
+def relax_sd(engine, pos: np.ndarray, iters: int = 150, max_step: float = 0.05,
+             gain: float = 0.02) -> tuple[np.ndarray, float]:
+    """Steepest descent on the engine's engforce: every iteration moves all
+    ions along F by min(max_step, gain*max|F|) scaled by F/max|F|.  Used to
+    take a random packing off its overlapping contacts before dynamics (a raw
+    packing releases eV per ion and heats to ~1e5 K).  Returns the relaxed
+    positions and the final max |F|."""
+    p = np.array(pos, dtype=np.float64, copy=True)
+    fmax = float("inf")
+    for _ in range(iters):
+        engine.set_positions(p)
+        f, _ = engine.engforce()
+        fmax = float(np.abs(f).max())
+        if fmax == 0.0:
+            break
+        p += min(max_step, gain * fmax) * f / fmax
+    engine.set_positions(p)
+    return p, fmax

@@ -81,6 +81,13 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// kernel classes timed by the in-library event profiler
+enum ProfKind { PK_PAIR = 0, PK_FIELD, PK_TABLES, PK_SK, PK_SKRED, PK_GATHER, PK_GFINAL, PK_NLIST, PK_AMAT, PK_COUNT };
+struct ProfRec {
+  int kind;
+  hipEvent_t a, b;
+};
+
 struct mdff_ctx {
   mdff_desc d;
   int n = 0, ntype = 0;
@@ -162,10 +169,34 @@ struct mdff_ctx {
   void* wk_t = nullptr;                   // W in the working precision
   double* rec_part = nullptr;             // per (atom, split) partial moments
   int rec_splits = 1, sk_chunks = 1;
+  void* amat = nullptr;                   // matrix-core gather A operand
+  int mma_nks = 0, mma_mtiles = 0, mma_ct = 0, mma_splits = 1;
+  int use_mma = 1;                        // matrix-core reciprocal kernels
 
   double last_ms[8] = {0};
   int n_last = 0;
+
+  // event profiler (mdff_profile): one event pair per timed launch
+  bool prof_on = false;
+  std::vector<ProfRec> prof;
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
 };
+
+// Record a launch between two events on the context stream when profiling.
+hipEvent_t prof_event(mdff_ctx* c);
+#define PROF_LAUNCH(c, kind, ...)                                   \
+  do {                                                              \
+    if ((c)->prof_on) {                                             \
+      ProfRec _r{(kind), prof_event(c), prof_event(c)};             \
+      (void)hipEventRecord(_r.a, (c)->stream);                      \
+      __VA_ARGS__;                                                  \
+      (void)hipEventRecord(_r.b, (c)->stream);                      \
+      (c)->prof.push_back(_r);                                      \
+    } else {                                                        \
+      __VA_ARGS__;                                                  \
+    }                                                               \
+  } while (0)
 
 // ---------------------------------------------------------------------------
 // error helpers

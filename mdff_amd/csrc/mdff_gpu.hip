@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "common.h"
@@ -455,6 +456,11 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
     c->npol += polar[i] != 0;
     c->qsq = c->qsq + q[i] * q[i];
   }
+  // reciprocal kernels: matrix-core by default; MDFF_RECIP_VALU=1 selects the
+  // vector-ALU kernels (same sums, kept for comparison and for shapes the
+  // matrix-core kernels do not cover)
+  const char* ev = getenv("MDFF_RECIP_VALU");
+  c->use_mma = (ev && ev[0] == '1') ? 0 : 1;
   if (d->lcoulomb) MDFF_TRY(recip_setup(c));
   return MDFF_OK;
 }
@@ -677,6 +683,19 @@ static int engforce(mdff_ctx* c, mdff_terms* t) {
   return MDFF_OK;
 }
 
+hipEvent_t prof_event(mdff_ctx* c) {
+  if (c->evused == c->evpool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->evpool.push_back(e);
+  }
+  return c->evpool[c->evused++];
+}
+
+static const char* kProfNames[PK_COUNT] = {"k_pair_force", "k_pair_field", "k_tables",     "k_sk",
+                                           "k_sk_reduce",  "k_gather",     "k_gather_final", "k_nl_sweep",
+                                           "k_build_amat"};
+
 // ====================================================================== C-ABI
 
 extern "C" {
@@ -739,6 +758,8 @@ int mdff_ctx_destroy(mdff_ctx* c) {
   if (c->h_flag) (void)hipHostFree(c->h_flag);
   if (c->h_red) (void)hipHostFree(c->h_red);
   for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->evpool)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -976,6 +997,31 @@ int mdff_calc_temp(mdff_ctx* c, double boltz_unit, double* temp, double* ekin) {
   if (temp) *temp = 2.0 * e / boltz_unit / (3.0 * c->n);
   return MDFF_OK;
 }
+
+int mdff_profile(mdff_ctx* c, int32_t on) {
+  if (!c) return MDFF_ERR_BAD_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->prof.clear();
+  c->evused = 0;
+  c->prof_on = on != 0;
+  return MDFF_OK;
+}
+
+int mdff_profile_read(mdff_ctx* c, double* ms_total, int32_t* count, int32_t nkinds) {
+  if (!c || !ms_total || !count || nkinds <= 0) return MDFF_ERR_BAD_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < nkinds; ++k) { ms_total[k] = 0.0; count[k] = 0; }
+  for (const ProfRec& r : c->prof) {
+    if (r.kind >= nkinds || !r.a || !r.b) continue;
+    float ms = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, r.a, r.b));
+    ms_total[r.kind] += ms;
+    count[r.kind] += 1;
+  }
+  return MDFF_OK;
+}
+
+const char* mdff_profile_name(int32_t kind) { return (kind >= 0 && kind < PK_COUNT) ? kProfNames[kind] : ""; }
 
 int mdff_last_timings(mdff_ctx* c, double* ms, int32_t* n) {
   if (!c || !ms) return MDFF_ERR_BAD_ARG;

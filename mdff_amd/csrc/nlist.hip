@@ -237,7 +237,7 @@ int nl_build(mdff_ctx* c) {
   a.px = c->bx; a.py = c->by; a.pz = c->bz;
   dim3 grid(div_up((int64_t)n * 64, 256));
   HIP_TRY(c, hipMemsetAsync(c->nl_count + n, 0, sizeof(int32_t), c->stream));
-  hipLaunchKernelGGL(k_nl_sweep<false>, grid, dim3(256), 0, c->stream, a);
+  PROF_LAUNCH(c, PK_NLIST, hipLaunchKernelGGL(k_nl_sweep<false>, grid, dim3(256), 0, c->stream, a));
   HIP_TRY(c, rocprim::exclusive_scan(c->sort_tmp, tmp2, c->nl_count, c->nl_start, 0, (size_t)n + 1,
                                      rocprim::plus<int32_t>(), c->stream));
   int32_t total = 0;
@@ -262,7 +262,7 @@ int nl_build(mdff_ctx* c) {
     a.cap = cap;
   }
   HIP_TRY(c, hipMemsetAsync(c->d_flag + 1, 0, sizeof(int32_t), c->stream));
-  hipLaunchKernelGGL(k_nl_sweep<true>, grid, dim3(256), 0, c->stream, a);
+  PROF_LAUNCH(c, PK_NLIST, hipLaunchKernelGGL(k_nl_sweep<true>, grid, dim3(256), 0, c->stream, a));
   HIP_TRY(c, hipGetLastError());
   c->nl_total = total;
   c->lists_valid = 1;

@@ -375,14 +375,14 @@ template <typename T, bool DIP>
 void launch_force(mdff_ctx* c, const PairArgs& a) {
   constexpr int LPA = 16;
   dim3 grid(div_up((int64_t)c->n * LPA, 256));
-  hipLaunchKernelGGL((k_pair_force<T, DIP, LPA>), grid, dim3(256), 0, c->stream, a);
+  PROF_LAUNCH(c, PK_PAIR, hipLaunchKernelGGL((k_pair_force<T, DIP, LPA>), grid, dim3(256), 0, c->stream, a));
 }
 
 template <typename T, int MODE>
 void launch_field(mdff_ctx* c, const PairArgs& a) {
   constexpr int LPA = 16;
   dim3 grid(div_up((int64_t)c->n * LPA, 256));
-  hipLaunchKernelGGL((k_pair_field<T, MODE, LPA>), grid, dim3(256), 0, c->stream, a);
+  PROF_LAUNCH(c, PK_FIELD, hipLaunchKernelGGL((k_pair_field<T, MODE, LPA>), grid, dim3(256), 0, c->stream, a));
 }
 
 }  // namespace

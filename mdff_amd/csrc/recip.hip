@@ -162,8 +162,8 @@ __global__ void __launch_bounds__(256) k_sk(SkArgs a) {
 
 // S(n) = sum of chunk partials (FP64, fixed order); W(n) = A conj(S);
 // per-block sums of A|S|^2 and (delta_ab - kcoe k_a k_b) A|S|^2.
-template <typename T>
-__global__ void __launch_bounds__(256) k_sk_reduce(int nk_box, int nchunk, const T* __restrict__ part,
+template <typename T, typename TP>
+__global__ void __launch_bounds__(256) k_sk_reduce(int nk_box, int nchunk, const TP* __restrict__ part,
                                                    const double* __restrict__ kA,
                                                    const double* __restrict__ kvec,
                                                    double* __restrict__ sk, double* __restrict__ wk,
@@ -346,7 +346,367 @@ __global__ void k_gather_final(int n, int nsplit, int moments, const double* __r
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Matrix-core form of phase 2 (field / charge forces).
+//
+// For one 16-atom tile the contraction over nz is a real GEMM on MFMA
+// 16x16x4: rows R = (m_local, comp) of 4 values of m = (nx, ny) and four
+// components comp = (Re U0, Im U0, Re U1, Im U1), U_p(i, m) = sum_nz nz^p
+// W(m, nz) Z_i(nz); K = (nz, re/im of Z) padded to a multiple of 4; columns =
+// the 16 atoms.  The A operand (W, arranged per lane) is precomputed once per
+// evaluation (k_build_amat); the B operand (the atom's Z row) stays in
+// registers for the whole kernel.  With the row order below, lane l holds the
+// four components of atom (l % 16) for m_local = l / 16 in its four
+// accumulator registers, so the (nx, ny) epilogue (multiply by
+// X_i[nx] Y_i[ny], weight by nx, ny) is lane-local; the four m_local lane
+// groups are summed with two cross-lane shuffles at the very end.
+// Lane layouts (checked on gfx950): A(row = l%16, k = l/16), B(k = l/16,
+// col = l%16); D f32: row = 4*(l/16) + r; D f64: row = 4*r + l/16.
+// ---------------------------------------------------------------------------
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mma;
+template <>
+struct Mma<float> {
+  typedef f32x4 acc_t;
+  __device__ static __forceinline__ acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // A row R -> (m_local, comp) so that D register r = comp for m_local = l/16
+  __host__ __device__ static __forceinline__ void row(int R, int& ml, int& comp) { ml = R >> 2; comp = R & 3; }
+};
+template <>
+struct Mma<double> {
+  typedef f64x4 acc_t;
+  __device__ static __forceinline__ acc_t op(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __host__ __device__ static __forceinline__ void row(int R, int& ml, int& comp) { comp = R >> 2; ml = R & 3; }
+};
+
+// A operand of the gather GEMM: amat[(mtile * nks + s) * 64 + lane]
+template <typename T>
+__global__ void k_build_amat(int M, int NZ, int Kz, int nks, int mtiles, const double* __restrict__ wk,
+                             T* __restrict__ amat) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)mtiles * nks * 64) return;
+  const int lane = (int)(idx % 64);
+  const size_t rest = idx / 64;
+  const int s = (int)(rest % nks);
+  const int mt = (int)(rest / nks);
+  int ml, comp;
+  Mma<T>::row(lane % 16, ml, comp);
+  const int k = 4 * s + lane / 16;
+  const int t = k >> 1, cc = k & 1;
+  const int m = 4 * mt + ml;
+  double v = 0.0;
+  if (t < NZ && m < M) {
+    const double wr = wk[2 * ((size_t)m * NZ + t)], wi = wk[2 * ((size_t)m * NZ + t) + 1];
+    // (Re, Im) of W Z as a real dot product with (Re Z, Im Z)
+    v = ((comp & 1) == 0) ? (cc == 0 ? wr : -wi) : (cc == 0 ? wi : wr);
+    if (comp >= 2) v *= (double)(t - Kz);
+  }
+  amat[idx] = (T)v;
+}
+
+struct GmArgs {
+  int n, Kx, Ky, Kz, NY, NZ, M, nks, mt_per_split, mtiles;
+  const void* tab;
+  const void* amat;
+  double* out;                   // [split][3][n]
+};
+
+template <typename T, int CT, int NKS>
+__global__ void __launch_bounds__(256) k_gather_mma(GmArgs a) {
+  typedef typename Mma<T>::acc_t acc_t;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int g = lane >> 4, col = lane & 15;
+  const int atom0 = (blockIdx.x * (blockDim.x >> 6) + wave) * (16 * CT);
+  const cplx<T>* tab = reinterpret_cast<const cplx<T>*>(a.tab);
+  const T* amat = reinterpret_cast<const T*>(a.amat);
+  const int zcol = a.Kx + 1 + a.Ky + 1;
+  // B operand: the Z row of this lane's atoms, zero beyond NZ
+  T breg[CT][NKS];
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct) {
+    const int ia = atom0 + ct * 16 + col;
+    const bool ok = ia < a.n;
+#pragma unroll
+    for (int s = 0; s < NKS; ++s) {
+      const int k = 4 * s + g;
+      const int t = k >> 1;
+      const int nz = t - a.Kz;
+      const int az = nz < 0 ? -nz : nz;
+      T v = T(0);
+      if (ok && t < a.NZ) {
+        const cplx<T> z = tab[(size_t)(zcol + az) * a.n + ia];
+        v = (k & 1) ? (nz < 0 ? -z.im : z.im) : z.re;
+      }
+      breg[ct][s] = v;
+    }
+  }
+  double ax[CT], ay[CT], az3[CT];
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct) { ax[ct] = 0.0; ay[ct] = 0.0; az3[ct] = 0.0; }
+  const int mt0 = blockIdx.y * a.mt_per_split;
+  const int mt1 = min(a.mtiles, mt0 + a.mt_per_split);
+  // A operand, double buffered across m-tiles
+  T abuf[NKS];
+  if (mt0 < mt1) {
+    const T* ap = amat + (size_t)mt0 * NKS * 64 + lane;
+#pragma unroll
+    for (int s = 0; s < NKS; ++s) abuf[s] = ap[(size_t)s * 64];
+  }
+  for (int mt = mt0; mt < mt1; ++mt) {
+    T cur[NKS];
+#pragma unroll
+    for (int s = 0; s < NKS; ++s) cur[s] = abuf[s];
+    if (mt + 1 < mt1) {
+      const T* ap = amat + (size_t)(mt + 1) * NKS * 64 + lane;
+#pragma unroll
+      for (int s = 0; s < NKS; ++s) abuf[s] = ap[(size_t)s * 64];
+    }
+    // epilogue operands of this m-tile (independent of the MFMA chain)
+    const int m = 4 * mt + g;
+    const bool mok = m < a.M;
+    const int nx = mok ? m / a.NY : 0, ny = mok ? m % a.NY - a.Ky : 0;
+    cplx<T> XY[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+      const int ia = atom0 + ct * 16 + col;
+      XY[ct] = {T(0), T(0)};
+      if (mok && ia < a.n) {
+        const cplx<T> X = tab[(size_t)nx * a.n + ia];
+        cplx<T> Y = tab[(size_t)(a.Kx + 1 + (ny >= 0 ? ny : -ny)) * a.n + ia];
+        if (ny < 0) Y.im = -Y.im;
+        XY[ct] = cmul(X, Y);
+      }
+    }
+    acc_t acc[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) acc[ct] = acc_t{T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int s = 0; s < NKS; ++s) {
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) acc[ct] = Mma<T>::op(cur[s], breg[ct][s], acc[ct]);
+    }
+    const double fx = (double)nx, fy = (double)ny;
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+      const double im0 = (double)(XY[ct].re * acc[ct][1] + XY[ct].im * acc[ct][0]);
+      const double im1 = (double)(XY[ct].re * acc[ct][3] + XY[ct].im * acc[ct][2]);
+      ax[ct] += fx * im0;
+      ay[ct] += fy * im0;
+      az3[ct] += im1;
+    }
+  }
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct) {
+    ax[ct] += __shfl_xor(ax[ct], 16);
+    ax[ct] += __shfl_xor(ax[ct], 32);
+    ay[ct] += __shfl_xor(ay[ct], 16);
+    ay[ct] += __shfl_xor(ay[ct], 32);
+    az3[ct] += __shfl_xor(az3[ct], 16);
+    az3[ct] += __shfl_xor(az3[ct], 32);
+  }
+  if (g == 0) {
+    double* o = a.out + (size_t)blockIdx.y * 3 * a.n;
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+      const int ia = atom0 + ct * 16 + col;
+      if (ia < a.n) {
+        o[ia] = ax[ct];
+        o[(size_t)a.n + ia] = ay[ct];
+        o[(size_t)2 * a.n + ia] = az3[ct];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-core form of phase 1: S(m, nz) = sum_i P_i(m) Z_i(nz),
+// P_i(m) = a_i(m) X_i[nx] Y_i[ny].  Real GEMM on MFMA 16x16x4 with
+// rows R = (m_local, part): part = (Re, Im) of S1 [charges: a = q + i 2pi
+// (nx m_x + ny m_y)] and, with dipoles, (Re, Im) of S2 = sum mz XY Z, so that
+// S = S1 + i 2pi nz S2; columns = nz (four tiles of 16); K = (atom, Re/Im of
+// Z).  The A value of a lane is generated from the LDS-staged tables of the
+// atom and reused by the four column tiles; the B value is a table entry.
+// Each block (4 waves = 4 row tiles) sums over one chunk of atoms; chunk
+// partials are reduced in FP64 by k_sk_reduce.
+// ---------------------------------------------------------------------------
+struct SmArgs {
+  int n, Kx, Ky, Kz, NY, NZ, M, RPM, mtiles, chunk, TA, use_q, use_mu, nk_box;
+  const double* q;
+  const double* mu;
+  double b[9];
+  const void* tab;
+  double* part;                  // [chunk][2*nk_box] FP64
+};
+
+template <typename T, int RPM>
+__global__ void __launch_bounds__(256) k_sk_mma(SmArgs a) {
+  typedef typename Mma<T>::acc_t acc_t;
+  extern __shared__ unsigned char smem_raw[];
+  const int TA = a.TA;
+  const int ncol = a.Kx + a.Ky + a.Kz + 3;
+  cplx<T>* st = reinterpret_cast<cplx<T>*>(smem_raw);          // [TA][ncol], atom-major
+  T* sq = reinterpret_cast<T*>(st + (size_t)ncol * TA);         // q, mx, my, mz
+  const cplx<T>* tab = reinterpret_cast<const cplx<T>*>(a.tab);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, col = lane & 15;
+  constexpr int MPT = 16 / RPM;                                 // m values per row tile
+  const int mt = blockIdx.x * 4 + wave;                         // row tile of this wave
+  const bool wave_active = mt < a.mtiles;
+  const int m = mt * MPT + col / RPM;                           // A row = lane % 16
+  const int part = col % RPM;                                   // 0 S1.re 1 S1.im 2 S2.re 3 S2.im
+  const bool mok = wave_active && (m < a.M);
+  const int nx = mok ? m / a.NY : 0;
+  const int ny = mok ? m % a.NY - a.Ky : 0;
+  const int ycol = a.Kx + 1 + (ny >= 0 ? ny : -ny);
+  const int zcol = a.Kx + 1 + a.Ky + 1;
+  const int kk = g & 1;
+  const T tp = T(2.0 * MDFF_PI);
+  acc_t acc[4];
+  double dacc[4][4];
+#pragma unroll
+  for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) dacc[ct][r] = 0.0;
+  // B columns of this lane: nz index per column tile
+  int bz[4];
+  bool bok[4];
+#pragma unroll
+  for (int ct = 0; ct < 4; ++ct) {
+    const int nzi = ct * 16 + col;
+    bok[ct] = nzi < a.NZ;
+    const int nz = nzi - a.Kz;
+    bz[ct] = zcol + (nz < 0 ? -nz : nz);
+  }
+  const int abeg = blockIdx.y * a.chunk;
+  const int aend = min(a.n, abeg + a.chunk);
+  for (int t0 = abeg; t0 < aend; t0 += TA) {
+    const int nt = min(TA, aend - t0);
+    __syncthreads();
+    for (int x = threadIdx.x; x < ncol * TA; x += blockDim.x) {
+      const int cc = x / TA, ia = x % TA;
+      st[ia * ncol + cc] = (ia < nt) ? tab[(size_t)cc * a.n + t0 + ia] : cplx<T>{T(0), T(0)};
+    }
+    for (int ia = threadIdx.x; ia < TA; ia += blockDim.x) {
+      T qq = T(0), mx = T(0), my = T(0), mz = T(0);
+      if (ia < nt) {
+        const int i = t0 + ia;
+        if (a.use_q) qq = (T)a.q[i];
+        if (a.use_mu) {
+          const double u0 = a.mu[3 * i], u1 = a.mu[3 * i + 1], u2 = a.mu[3 * i + 2];
+          mx = (T)(u0 * a.b[0] + u1 * a.b[1] + u2 * a.b[2]);
+          my = (T)(u0 * a.b[3] + u1 * a.b[4] + u2 * a.b[5]);
+          mz = (T)(u0 * a.b[6] + u1 * a.b[7] + u2 * a.b[8]);
+        }
+      }
+      sq[4 * ia] = qq;
+      sq[4 * ia + 1] = mx;
+      sq[4 * ia + 2] = my;
+      sq[4 * ia + 3] = mz;
+    }
+    __syncthreads();
+    if (!wave_active) continue;          // whole wave: no MFMA issued
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) acc[ct] = acc_t{T(0), T(0), T(0), T(0)};
+    // K steps of 4: two atoms (Re Z, Im Z each) per step
+    for (int s = 0; s < (nt + 1) / 2; ++s) {
+      const int ia = 2 * s + (g >> 1);
+      const cplx<T>* row = st + (size_t)ia * ncol;
+      T av = T(0);
+      if (mok && ia < nt) {
+        const cplx<T> X = row[nx];
+        cplx<T> Y = row[ycol];
+        if (ny < 0) Y.im = -Y.im;
+        const cplx<T> XY = cmul(X, Y);
+        cplx<T> P;
+        if (part < 2) {
+          const T qq = sq[4 * ia];
+          if (RPM == 4) {
+            const T bb = tp * ((T)nx * sq[4 * ia + 1] + (T)ny * sq[4 * ia + 2]);
+            P = {qq * XY.re - bb * XY.im, qq * XY.im + bb * XY.re};
+          } else {
+            P = {qq * XY.re, qq * XY.im};
+          }
+        } else {
+          const T mzz = sq[4 * ia + 3];
+          P = {mzz * XY.re, mzz * XY.im};
+        }
+        // Re rows: (Pr, -Pi) . (Zr, Zi); Im rows: (Pi, Pr) . (Zr, Zi)
+        const bool re_row = (part & 1) == 0;
+        av = re_row ? (kk == 0 ? P.re : -P.im) : (kk == 0 ? P.im : P.re);
+      }
+      const bool aok = ia < nt;
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) {
+        T bv = T(0);
+        if (bok[ct] && aok) {
+          const cplx<T> z = row[bz[ct]];
+          const bool neg = (ct * 16 + col) < a.Kz;
+          bv = kk == 0 ? z.re : (neg ? -z.im : z.im);
+        }
+        acc[ct] = Mma<T>::op(av, bv, acc[ct]);
+      }
+    }
+    // FP64 accumulation across atom tiles
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) dacc[ct][r] += (double)acc[ct][r];
+  }
+  // write-out through LDS: every lane stores its 4 rows, then each (m, nz)
+  // combines its parts (all waves reach both barriers)
+  __syncthreads();
+  double* sd = reinterpret_cast<double*>(smem_raw) + (size_t)wave * 16 * 64;   // [R][nz col 0..63]
+#pragma unroll
+  for (int ct = 0; ct < 4; ++ct) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int Rd = (sizeof(T) == 4) ? (4 * g + r) : (4 * r + g);
+      sd[Rd * 64 + ct * 16 + col] = dacc[ct][r];
+    }
+  }
+  __syncthreads();
+  if (!wave_active) return;
+  double* out = a.part + (size_t)blockIdx.y * 2 * a.nk_box;
+  for (int x = lane; x < MPT * 64; x += 64) {
+    const int ml = x / 64, nzi = x % 64;
+    const int mm = mt * MPT + ml;
+    if (mm >= a.M || nzi >= a.NZ) continue;
+    double re = sd[(ml * RPM + 0) * 64 + nzi], im = sd[(ml * RPM + 1) * 64 + nzi];
+    if (RPM == 4) {
+      const double fz = 2.0 * MDFF_PI * (double)(nzi - a.Kz);
+      const double s2r = sd[(ml * RPM + 2) * 64 + nzi], s2i = sd[(ml * RPM + 3) * 64 + nzi];
+      re -= fz * s2i;          // S = S1 + i 2pi nz S2
+      im += fz * s2r;
+    }
+    const size_t idx = (size_t)mm * a.NZ + nzi;
+    out[2 * idx] = re;
+    out[2 * idx + 1] = im;
+  }
+}
+
 }  // namespace
+
+// K-step counts compiled for the matrix-core gather (Kz + 1 rounded up)
+static const int kMmaBuckets[] = {9, 13, 17, 19, 25, 33, 41, 57};
+static int mma_bucket(int nks) {
+  for (int b : kMmaBuckets)
+    if (nks <= b) return b;
+  return 0;
+}
+// atom tiles of 16 per wave: register budget of the B operand
+static int mma_ct(bool fp32, int nks) {
+  if (fp32) return nks <= 33 ? 4 : 2;
+  return nks <= 25 ? 2 : 1;
+}
 
 // k-mesh on the box with the reference's k vectors and A(n), kcoe
 // (src/kspace.f90:162-191); allocation of the reciprocal buffers.
@@ -391,20 +751,28 @@ int recip_setup(mdff_ctx* c) {
   chunks = std::min(chunks, std::max(1, c->n / 64));
   chunks = std::min(chunks, 64);
   c->sk_chunks = chunks;
-  HIP_TRY(c, hipMalloc(&c->sk_part, tsz * (size_t)chunks * nb));
+  HIP_TRY(c, hipMalloc(&c->sk_part, sizeof(double) * 2 * (size_t)chunks * nb));
   HIP_TRY(c, hipMalloc((void**)&c->sk, sizeof(double) * 2 * nb));
   HIP_TRY(c, hipMalloc((void**)&c->wk, sizeof(double) * 2 * nb));
   HIP_TRY(c, hipMalloc(&c->wk_t, tsz * (size_t)nb));
   // gather splits over (nx, ny): about 256k threads in flight
   const int M = c->NXk * c->NYk;
   int splits = std::max(1, std::min(M, div_up((int64_t)262144, c->n)));
+  // matrix-core gather: A operand and its own split count
+  c->mma_nks = mma_bucket(Kz + 1);
+  c->mma_mtiles = div_up(M, 4);
+  c->mma_ct = mma_ct(c->d.precision == MDFF_PREC_MIXED, c->mma_nks);
+  const int atom_waves = div_up(c->n, 16 * c->mma_ct);
+  c->mma_splits = std::max(1, std::min(c->mma_mtiles, div_up(2048, atom_waves)));
+  HIP_TRY(c, hipMalloc(&c->amat, tsz / 2 * (size_t)c->mma_mtiles * c->mma_nks * 64));
+  splits = std::max(splits, c->mma_splits);
   c->rec_splits = splits;
   HIP_TRY(c, hipMalloc((void**)&c->rec_part, sizeof(double) * 9 * (size_t)splits * c->n));
   return MDFF_OK;
 }
 
 void recip_free(mdff_ctx* c) {
-  void* ps[] = {c->kA, c->kvec, c->tab, c->sk_part, c->sk, c->wk, c->wk_t, c->rec_part};
+  void* ps[] = {c->kA, c->kvec, c->tab, c->sk_part, c->sk, c->wk, c->wk_t, c->rec_part, c->amat};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -412,14 +780,36 @@ void recip_free(mdff_ctx* c) {
 template <typename T>
 static void tables_t(mdff_ctx* c) {
   const int ncol = c->K[0] + c->K[1] + c->K[2] + 3;
-  hipLaunchKernelGGL((k_tables<T>), dim3(div_up(c->n, 256), ncol), dim3(256), 0, c->stream, c->n, c->K[0],
-                     c->K[1], c->K[2], c->sx, c->sy, c->sz, reinterpret_cast<cplx<T>*>(c->tab));
+  PROF_LAUNCH(c, PK_TABLES,
+              hipLaunchKernelGGL((k_tables<T>), dim3(div_up(c->n, 256), ncol), dim3(256), 0, c->stream, c->n,
+                                 c->K[0], c->K[1], c->K[2], c->sx, c->sy, c->sz,
+                                 reinterpret_cast<cplx<T>*>(c->tab)));
 }
 
 int recip_tables(mdff_ctx* c) {
   if (c->d.precision == MDFF_PREC_MIXED) tables_t<float>(c);
   else tables_t<double>(c);
   HIP_TRY(c, hipGetLastError());
+  return MDFF_OK;
+}
+
+#define GM_CASE(CTV, NKSV)                                                                      \
+  case (CTV) * 1000 + (NKSV):                                                                   \
+    PROF_LAUNCH(c, PK_GATHER,                                                                   \
+                hipLaunchKernelGGL((k_gather_mma<T, CTV, NKSV>), grid, dim3(256), 0, c->stream, gm)); \
+    break;
+
+template <typename T>
+static int launch_gather_mma(mdff_ctx* c, const GmArgs& gm, int ct, dim3 grid) {
+  switch (ct * 1000 + gm.nks) {
+    GM_CASE(4, 9) GM_CASE(4, 13) GM_CASE(4, 17) GM_CASE(4, 19) GM_CASE(4, 25) GM_CASE(4, 33)
+    GM_CASE(2, 41) GM_CASE(2, 57)
+    GM_CASE(2, 9) GM_CASE(2, 13) GM_CASE(2, 17) GM_CASE(2, 19) GM_CASE(2, 25)
+    GM_CASE(1, 33) GM_CASE(1, 41) GM_CASE(1, 57)
+    default:
+      c->err = "no matrix-core gather kernel for this kmax";
+      return MDFF_ERR_UNSUPPORTED;
+  }
   return MDFF_OK;
 }
 
@@ -451,13 +841,55 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     c->err = "kES too large for the structure-factor kernel";
     return MDFF_ERR_UNSUPPORTED;
   }
-  const int chunks = div_up(c->n, s.a0_step);
-  dim3 g1(div_up((int64_t)s.M * s.NG, 256), chunks);
-  hipLaunchKernelGGL((k_sk<T, GZ>), g1, dim3(256), shm, c->stream, s);
+  int chunks = div_up(c->n, s.a0_step);
+  bool sk_fp64 = false;
+  if (c->use_mma && c->NZk <= 64) {
+    sk_fp64 = true;
+    SmArgs sm{};
+    sm.n = c->n;
+    sm.Kx = c->K[0]; sm.Ky = c->K[1]; sm.Kz = c->K[2];
+    sm.NY = c->NYk; sm.NZ = c->NZk; sm.M = s.M;
+    sm.RPM = use_mu ? 4 : 2;
+    const int MPT = 16 / sm.RPM;
+    sm.mtiles = div_up(s.M, MPT);
+    const int rowblocks = div_up(sm.mtiles, 4);
+    // chunks over atoms so that about 2048 waves run
+    int ch = std::max(1, std::min(c->sk_chunks, div_up(512, rowblocks)));
+    sm.chunk = div_up(c->n, ch);
+    ch = div_up(c->n, sm.chunk);
+    chunks = ch;
+    sm.use_q = use_q; sm.use_mu = use_mu;
+    sm.nk_box = c->nk_box;
+    sm.q = q; sm.mu = mu;
+    for (int k = 0; k < 9; ++k) sm.b[k] = c->B[k];
+    sm.tab = c->tab;
+    sm.part = reinterpret_cast<double*>(c->sk_part);
+    int TA = 64;
+    auto lds = [&](int ta) {
+      return std::max(sizeof(cplx<T>) * (size_t)ncol * ta + sizeof(T) * 4 * ta, sizeof(double) * 4 * 16 * 64);
+    };
+    while (TA > 2 && lds(TA) > 65536) TA /= 2;
+    sm.TA = TA;
+    dim3 gs(rowblocks, ch);
+    if (use_mu)
+      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_mma<T, 4>), gs, dim3(256), lds(TA), c->stream, sm));
+    else
+      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_mma<T, 2>), gs, dim3(256), lds(TA), c->stream, sm));
+  } else {
+    dim3 g1(div_up((int64_t)s.M * s.NG, 256), chunks);
+    PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk<T, GZ>), g1, dim3(256), shm, c->stream, s));
+  }
   const int nb = div_up(c->nk_box, 256);
-  hipLaunchKernelGGL((k_sk_reduce<T>), dim3(nb), dim3(256), 0, c->stream, c->nk_box, chunks,
-                     reinterpret_cast<const T*>(c->sk_part), c->kA, c->kvec, c->sk, c->wk,
-                     reinterpret_cast<T*>(c->wk_t), c->red_part);
+  PROF_LAUNCH(c, PK_SKRED,
+              if (sk_fp64) {
+                hipLaunchKernelGGL((k_sk_reduce<T, double>), dim3(nb), dim3(256), 0, c->stream, c->nk_box, chunks,
+                                   reinterpret_cast<const double*>(c->sk_part), c->kA, c->kvec, c->sk, c->wk,
+                                   reinterpret_cast<T*>(c->wk_t), c->red_part);
+              } else {
+                hipLaunchKernelGGL((k_sk_reduce<T, T>), dim3(nb), dim3(256), 0, c->stream, c->nk_box, chunks,
+                                   reinterpret_cast<const T*>(c->sk_part), c->kA, c->kvec, c->sk, c->wk,
+                                   reinterpret_cast<T*>(c->wk_t), c->red_part);
+              });
   if (e_out) {
     // 7 sums of the k-blocks, reduced on the host in block order
     std::vector<double> blk(7 * (size_t)nb);
@@ -468,34 +900,59 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
       for (int k = 0; k < 7; ++k) e_out[k] += blk[7 * (size_t)b + k];
   }
   if (!forces && !field) return MDFF_OK;
-  GaArgs g{};
-  g.n = c->n;
-  g.Kx = c->K[0]; g.Ky = c->K[1]; g.Kz = c->K[2];
-  g.NY = c->NYk; g.NZ = c->NZk;
-  g.M = s.M;
-  g.m_per_split = div_up(s.M, c->rec_splits);
-  const int nsplit = div_up(s.M, g.m_per_split);
   const bool dipf = forces && use_mu;
-  g.moments = dipf ? 9 : 3;
-  g.tab = c->tab;
-  g.wk = c->wk_t;
-  g.out = c->rec_part;
-  // threads per block from the LDS budget (64 KiB)
-  int nthr = 256;
-  while (nthr > 64 && sizeof(cplx<T>) * ((size_t)(c->K[2] + 1) * nthr + 8 * (size_t)c->NZk) > 65536) nthr -= 64;
-  const size_t shm2 = sizeof(cplx<T>) * ((size_t)(c->K[2] + 1) * nthr + 8 * (size_t)c->NZk);
-  if (shm2 > 65536) {
-    c->err = "kES(3) too large for the reciprocal gather kernel";
-    return MDFF_ERR_UNSUPPORTED;
+  int nsplit = 0, moments = dipf ? 9 : 3;
+  if (!dipf && c->mma_nks > 0 && c->use_mma) {
+    // matrix-core path
+    const int total = c->mma_mtiles * c->mma_nks * 64;
+    PROF_LAUNCH(c, PK_AMAT,
+                hipLaunchKernelGGL((k_build_amat<T>), dim3(div_up(total, 256)), dim3(256), 0, c->stream, s.M,
+                                   c->NZk, c->K[2], c->mma_nks, c->mma_mtiles, c->wk,
+                                   reinterpret_cast<T*>(c->amat)));
+    GmArgs gm{};
+    gm.n = c->n;
+    gm.Kx = c->K[0]; gm.Ky = c->K[1]; gm.Kz = c->K[2];
+    gm.NY = c->NYk; gm.NZ = c->NZk; gm.M = s.M;
+    gm.nks = c->mma_nks;
+    gm.mtiles = c->mma_mtiles;
+    gm.mt_per_split = div_up(c->mma_mtiles, c->mma_splits);
+    nsplit = div_up(c->mma_mtiles, gm.mt_per_split);
+    gm.tab = c->tab;
+    gm.amat = c->amat;
+    gm.out = c->rec_part;
+    const int CT = c->mma_ct;
+    dim3 gg(div_up(c->n, 4 * 16 * CT), nsplit);
+    MDFF_TRY(launch_gather_mma<T>(c, gm, CT, gg));
+  } else {
+    GaArgs g{};
+    g.n = c->n;
+    g.Kx = c->K[0]; g.Ky = c->K[1]; g.Kz = c->K[2];
+    g.NY = c->NYk; g.NZ = c->NZk;
+    g.M = s.M;
+    g.m_per_split = div_up(s.M, c->rec_splits);
+    nsplit = div_up(s.M, g.m_per_split);
+    g.moments = moments;
+    g.tab = c->tab;
+    g.wk = c->wk_t;
+    g.out = c->rec_part;
+    // threads per block from the LDS budget (64 KiB)
+    int nthr = 256;
+    while (nthr > 64 && sizeof(cplx<T>) * ((size_t)(c->K[2] + 1) * nthr + 8 * (size_t)c->NZk) > 65536) nthr -= 64;
+    const size_t shm2 = sizeof(cplx<T>) * ((size_t)(c->K[2] + 1) * nthr + 8 * (size_t)c->NZk);
+    if (shm2 > 65536) {
+      c->err = "kES(3) too large for the reciprocal gather kernel";
+      return MDFF_ERR_UNSUPPORTED;
+    }
+    dim3 g2(div_up(c->n, nthr), nsplit);
+    if (dipf) PROF_LAUNCH(c, PK_GATHER, hipLaunchKernelGGL((k_gather<T, 9>), g2, dim3(nthr), shm2, c->stream, g));
+    else PROF_LAUNCH(c, PK_GATHER, hipLaunchKernelGGL((k_gather<T, 3>), g2, dim3(nthr), shm2, c->stream, g));
   }
-  dim3 g2(div_up(c->n, nthr), nsplit);
-  if (dipf) hipLaunchKernelGGL((k_gather<T, 9>), g2, dim3(nthr), shm2, c->stream, g);
-  else hipLaunchKernelGGL((k_gather<T, 3>), g2, dim3(nthr), shm2, c->stream, g);
   const double* B = c->B;
   const double fpi_V = 2.0 * (2.0 * MDFF_PI / c->omega);
-  hipLaunchKernelGGL(k_gather_final, dim3(div_up(c->n, 256)), dim3(256), 0, c->stream, c->n, nsplit, g.moments,
-                     c->rec_part, q, mu, B[0], B[1], B[2], B[3], B[4], B[5], B[6], B[7], B[8], fpi_V,
-                     use_q ? 1 : 0, field ? 1 : 0, forces ? 1 : 0, ef_out, f_out);
+  PROF_LAUNCH(c, PK_GFINAL,
+              hipLaunchKernelGGL(k_gather_final, dim3(div_up(c->n, 256)), dim3(256), 0, c->stream, c->n, nsplit,
+                                 moments, c->rec_part, q, mu, B[0], B[1], B[2], B[3], B[4], B[5], B[6], B[7],
+                                 B[8], fpi_V, use_q ? 1 : 0, field ? 1 : 0, forces ? 1 : 0, ef_out, f_out));
   HIP_TRY(c, hipGetLastError());
   return MDFF_OK;
 }
