@@ -134,6 +134,7 @@ struct mdff_ctx {
   int32_t* polar = nullptr;               // 1 for polarisable ions
   int npol = 0;                           // number of polarisable ions
   double qsq = 0.0;                       // sum of q^2 (self term)
+  double* zeros = nullptr;                // n zeros (charges switched off)
 
   // neighbour list (full rows, cut = max(rc_vdw, rc_coul) + skin)
   int32_t* cell_of = nullptr;             // cell index per atom

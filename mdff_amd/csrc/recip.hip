@@ -470,21 +470,17 @@ __global__ void __launch_bounds__(256) k_gather_mma(GmArgs a) {
 #pragma unroll
       for (int s = 0; s < NKS; ++s) abuf[s] = ap[(size_t)s * 64];
     }
-    // epilogue operands of this m-tile (independent of the MFMA chain)
+    // epilogue operands of this m-tile (independent of the MFMA chain):
+    // clamped addresses, so the loads issue without branches
     const int m = 4 * mt + g;
     const bool mok = m < a.M;
     const int nx = mok ? m / a.NY : 0, ny = mok ? m % a.NY - a.Ky : 0;
-    cplx<T> XY[CT];
+    cplx<T> Xv[CT], Yv[CT];
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct) {
-      const int ia = atom0 + ct * 16 + col;
-      XY[ct] = {T(0), T(0)};
-      if (mok && ia < a.n) {
-        const cplx<T> X = tab[(size_t)nx * a.n + ia];
-        cplx<T> Y = tab[(size_t)(a.Kx + 1 + (ny >= 0 ? ny : -ny)) * a.n + ia];
-        if (ny < 0) Y.im = -Y.im;
-        XY[ct] = cmul(X, Y);
-      }
+      const int ia = min(atom0 + ct * 16 + col, a.n - 1);
+      Xv[ct] = tab[(size_t)nx * a.n + ia];
+      Yv[ct] = tab[(size_t)(a.Kx + 1 + (ny >= 0 ? ny : -ny)) * a.n + ia];
     }
     acc_t acc[CT];
 #pragma unroll
@@ -497,8 +493,12 @@ __global__ void __launch_bounds__(256) k_gather_mma(GmArgs a) {
     const double fx = (double)nx, fy = (double)ny;
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct) {
-      const double im0 = (double)(XY[ct].re * acc[ct][1] + XY[ct].im * acc[ct][0]);
-      const double im1 = (double)(XY[ct].re * acc[ct][3] + XY[ct].im * acc[ct][2]);
+      cplx<T> Y = Yv[ct];
+      if (ny < 0) Y.im = -Y.im;
+      cplx<T> XY = cmul(Xv[ct], Y);
+      if (!mok || atom0 + ct * 16 + col >= a.n) XY = {T(0), T(0)};
+      const double im0 = (double)(XY.re * acc[ct][1] + XY.im * acc[ct][0]);
+      const double im1 = (double)(XY.re * acc[ct][3] + XY.im * acc[ct][2]);
       ax[ct] += fx * im0;
       ay[ct] += fy * im0;
       az3[ct] += im1;
@@ -693,6 +693,184 @@ __global__ void __launch_bounds__(256) k_sk_mma(SmArgs a) {
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Matrix-core structure factor, transposed form: S^T(nz, m) = sum_i
+// Z_i(nz) P_i(m).  Rows R = (nz, Re/Im) in 7 tiles of 16 (8 nz each), taken
+// straight from the Z table; columns = 16 values of m = (nx, ny); K = (atom,
+// Re/Im of P).  The generated operand P_i(m) = a_i X_i[nx] Y_i[ny] is made
+// once per lane and K step and feeds the 7 row tiles.
+//   mode 0: a = q            (charges)
+//   mode 1: a = q + i 2pi (nx m_x + ny m_y)   (charges and dipoles, first part)
+//   mode 2: a = m_z          (second part: S = S1 + i 2pi nz S2)
+// FP32/FP64 MFMA accumulation within one staged atom tile, FP64 across tiles;
+// per-chunk FP64 partials.
+// ---------------------------------------------------------------------------
+struct Sm2Args {
+  int n, Kx, Ky, Kz, NY, NZ, M, mtiles, chunk, TA, mode, nk_box;
+  const double* q;
+  const double* mu;
+  double b[9];
+  const void* tab;
+  double* part;                  // [chunk][2*nk_box] FP64
+};
+
+template <typename T, int CTM>
+__global__ void __launch_bounds__(256) k_sk_mma2(Sm2Args a) {
+  typedef typename Mma<T>::acc_t acc_t;
+  constexpr int RT = 7;                                          // row tiles: 56 nz values max
+  extern __shared__ unsigned char smem_raw[];
+  const int TA = a.TA;
+  const int ncol = a.Kx + a.Ky + a.Kz + 3;
+  cplx<T>* st = reinterpret_cast<cplx<T>*>(smem_raw);           // [TA][ncol]
+  T* sq = reinterpret_cast<T*>(st + (size_t)ncol * TA);          // a-coefficients per atom
+  const cplx<T>* tab = reinterpret_cast<const cplx<T>*>(a.tab);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, col = lane & 15;
+  const int kk = g & 1, ja = g >> 1;                             // K: atom 2s+ja, Re/Im kk
+  const int zcol = a.Kx + 1 + a.Ky + 1;
+  // A rows of this lane (R = col): nz index and part per row tile
+  int zc[RT];
+  T zs[RT];
+  bool zok[RT];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt) {
+    const int nzi = rt * 8 + (col >> 1);
+    const int part = col & 1;
+    const int nz = nzi - a.Kz;
+    zok[rt] = nzi < a.NZ;
+    zc[rt] = zcol + (nz < 0 ? -nz : nz);
+    // A = part 0: (Zr, -Zi); part 1: (Zi, Zr), with Z(-n) = conj Z(n)
+    const T sgn_im = nz < 0 ? T(-1) : T(1);
+    zs[rt] = part == 0 ? (kk == 0 ? T(1) : -sgn_im) : (kk == 0 ? sgn_im : T(1));
+  }
+  const bool take_im = (col & 1) ^ kk;                           // which Z component feeds A
+  // B columns of this lane: m per m-tile
+  const int mt0 = (blockIdx.x * (blockDim.x >> 6) + wave) * CTM;
+  int bnx[CTM], bnyc[CTM];
+  T bnyf[CTM], bnxf[CTM];
+  bool bneg[CTM], bok[CTM];
+#pragma unroll
+  for (int j = 0; j < CTM; ++j) {
+    const int m = (mt0 + j) * 16 + col;
+    bok[j] = m < a.M;
+    const int nx = bok[j] ? m / a.NY : 0, ny = bok[j] ? m % a.NY - a.Ky : 0;
+    bnx[j] = nx;
+    bnyc[j] = a.Kx + 1 + (ny >= 0 ? ny : -ny);
+    bneg[j] = ny < 0;
+    bnxf[j] = (T)nx;
+    bnyf[j] = (T)ny;
+  }
+  const T tp = T(2.0 * MDFF_PI);
+  acc_t acc[RT][CTM];
+  double dacc[RT][CTM][4];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int j = 0; j < CTM; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) dacc[rt][j][r] = 0.0;
+  const int abeg = blockIdx.y * a.chunk;
+  const int aend = min(a.n, abeg + a.chunk);
+  for (int t0 = abeg; t0 < aend; t0 += TA) {
+    const int nt = min(TA, aend - t0);
+    __syncthreads();
+    for (int x = threadIdx.x; x < ncol * TA; x += blockDim.x) {
+      const int cc = x / TA, ia = x % TA;
+      st[ia * ncol + cc] = (ia < nt) ? tab[(size_t)cc * a.n + t0 + ia] : cplx<T>{T(0), T(0)};
+    }
+    for (int ia = threadIdx.x; ia < TA; ia += blockDim.x) {
+      T c0 = T(0), c1 = T(0), c2 = T(0);
+      if (ia < nt) {
+        const int i = t0 + ia;
+        if (a.mode <= 1) c0 = (T)a.q[i];
+        if (a.mode >= 1) {
+          const double u0 = a.mu[3 * i], u1 = a.mu[3 * i + 1], u2 = a.mu[3 * i + 2];
+          if (a.mode == 1) {
+            c1 = (T)(u0 * a.b[0] + u1 * a.b[1] + u2 * a.b[2]);
+            c2 = (T)(u0 * a.b[3] + u1 * a.b[4] + u2 * a.b[5]);
+          } else {
+            c0 = (T)(u0 * a.b[6] + u1 * a.b[7] + u2 * a.b[8]);
+          }
+        }
+      }
+      sq[3 * ia] = c0;
+      sq[3 * ia + 1] = c1;
+      sq[3 * ia + 2] = c2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int j = 0; j < CTM; ++j) acc[rt][j] = acc_t{T(0), T(0), T(0), T(0)};
+    const int nsteps = (nt + 1) >> 1;
+    for (int s = 0; s < nsteps; ++s) {
+      const int ia = 2 * s + ja;
+      const bool aok = ia < nt;
+      const cplx<T>* row = st + (size_t)(aok ? ia : 0) * ncol;
+      T av[RT];
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
+        const cplx<T> z = row[zc[rt]];
+        const T v = take_im ? z.im : z.re;
+        av[rt] = (zok[rt] && aok) ? zs[rt] * v : T(0);
+      }
+      const T c0 = sq[3 * (aok ? ia : 0)], c1 = sq[3 * (aok ? ia : 0) + 1], c2 = sq[3 * (aok ? ia : 0) + 2];
+      T bv[CTM];
+#pragma unroll
+      for (int j = 0; j < CTM; ++j) {
+        const cplx<T> X = row[bnx[j]];
+        cplx<T> Y = row[bnyc[j]];
+        if (bneg[j]) Y.im = -Y.im;
+        const cplx<T> XY = cmul(X, Y);
+        // (c0 + i bb) XY, bb = 2pi (nx mx + ny my) in mode 1
+        const T bb = (a.mode == 1) ? tp * (bnxf[j] * c1 + bnyf[j] * c2) : T(0);
+        const T pre = c0 * XY.re - bb * XY.im;
+        const T pim = c0 * XY.im + bb * XY.re;
+        bv[j] = (bok[j] && aok) ? (kk == 0 ? pre : pim) : T(0);
+      }
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int j = 0; j < CTM; ++j) acc[rt][j] = Mma<T>::op(av[rt], bv[j], acc[rt][j]);
+    }
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int j = 0; j < CTM; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) dacc[rt][j][r] += (double)acc[rt][j][r];
+  }
+  // D: lane holds rows Rd(r) of column m; row Rd -> nz = rt*8 + Rd/2, part Rd%2
+  double* out = a.part + (size_t)blockIdx.y * 2 * a.nk_box;
+#pragma unroll
+  for (int j = 0; j < CTM; ++j) {
+    if (!bok[j]) continue;
+    const int m = (mt0 + j) * 16 + col;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int Rd = (sizeof(T) == 4) ? (4 * g + r) : (4 * r + g);
+        const int nzi = rt * 8 + (Rd >> 1);
+        if (nzi >= a.NZ) continue;
+        out[2 * ((size_t)m * a.NZ + nzi) + (Rd & 1)] = dacc[rt][j][r];
+      }
+  }
+}
+
+// S = S1 + i 2pi nz S2 from two partial sets (dipole amplitude)
+__global__ void k_sk_combine(int nk_box, int NZ, int Kz, int nchunk, double* __restrict__ p1,
+                             const double* __restrict__ p2) {
+  const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= (size_t)nchunk * nk_box) return;
+  const int k = (int)(x % nk_box);
+  const double fz = 2.0 * MDFF_PI * (double)(k % NZ - Kz);
+  const double s2r = p2[2 * x], s2i = p2[2 * x + 1];
+  p1[2 * x] -= fz * s2i;
+  p1[2 * x + 1] += fz * s2r;
+}
+
 }  // namespace
 
 // K-step counts compiled for the matrix-core gather (Kz + 1 rounded up)
@@ -751,7 +929,7 @@ int recip_setup(mdff_ctx* c) {
   chunks = std::min(chunks, std::max(1, c->n / 64));
   chunks = std::min(chunks, 64);
   c->sk_chunks = chunks;
-  HIP_TRY(c, hipMalloc(&c->sk_part, sizeof(double) * 2 * (size_t)chunks * nb));
+  HIP_TRY(c, hipMalloc(&c->sk_part, sizeof(double) * 4 * (size_t)chunks * nb));
   HIP_TRY(c, hipMalloc((void**)&c->sk, sizeof(double) * 2 * nb));
   HIP_TRY(c, hipMalloc((void**)&c->wk, sizeof(double) * 2 * nb));
   HIP_TRY(c, hipMalloc(&c->wk_t, tsz * (size_t)nb));
@@ -843,38 +1021,44 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
   }
   int chunks = div_up(c->n, s.a0_step);
   bool sk_fp64 = false;
-  if (c->use_mma && c->NZk <= 64) {
+  if (c->use_mma && c->NZk <= 56) {
     sk_fp64 = true;
-    SmArgs sm{};
+    Sm2Args sm{};
     sm.n = c->n;
     sm.Kx = c->K[0]; sm.Ky = c->K[1]; sm.Kz = c->K[2];
     sm.NY = c->NYk; sm.NZ = c->NZk; sm.M = s.M;
-    sm.RPM = use_mu ? 4 : 2;
-    const int MPT = 16 / sm.RPM;
-    sm.mtiles = div_up(s.M, MPT);
-    const int rowblocks = div_up(sm.mtiles, 4);
-    // chunks over atoms so that about 2048 waves run
-    int ch = std::max(1, std::min(c->sk_chunks, div_up(512, rowblocks)));
+    sm.mtiles = div_up(s.M, 16);
+    constexpr int CTM = 2;
+    const int mblocks = div_up(sm.mtiles, 4 * CTM);
+    int ch = std::max(1, std::min(c->sk_chunks, div_up(c->n, 512)));
     sm.chunk = div_up(c->n, ch);
     ch = div_up(c->n, sm.chunk);
     chunks = ch;
-    sm.use_q = use_q; sm.use_mu = use_mu;
     sm.nk_box = c->nk_box;
     sm.q = q; sm.mu = mu;
     for (int k = 0; k < 9; ++k) sm.b[k] = c->B[k];
     sm.tab = c->tab;
-    sm.part = reinterpret_cast<double*>(c->sk_part);
     int TA = 64;
-    auto lds = [&](int ta) {
-      return std::max(sizeof(cplx<T>) * (size_t)ncol * ta + sizeof(T) * 4 * ta, sizeof(double) * 4 * 16 * 64);
-    };
+    auto lds = [&](int ta) { return sizeof(cplx<T>) * (size_t)ncol * ta + sizeof(T) * 3 * ta; };
     while (TA > 2 && lds(TA) > 65536) TA /= 2;
     sm.TA = TA;
-    dim3 gs(rowblocks, ch);
-    if (use_mu)
-      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_mma<T, 4>), gs, dim3(256), lds(TA), c->stream, sm));
-    else
-      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_mma<T, 2>), gs, dim3(256), lds(TA), c->stream, sm));
+    dim3 gs(mblocks, ch);
+    double* p1 = reinterpret_cast<double*>(c->sk_part);
+    sm.part = p1;
+    // first pass: mode 0 (charges) or mode 1 (charges, zero when switched
+    // off, plus the x/y dipole phase); second pass for dipoles: mode 2
+    sm.mode = use_mu ? 1 : 0;
+    if (!use_q) sm.q = c->zeros;
+    PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_mma2<T, CTM>), gs, dim3(256), lds(TA), c->stream, sm));
+    if (use_mu) {
+      Sm2Args s2 = sm;
+      s2.mode = 2;
+      s2.part = p1 + (size_t)2 * ch * c->nk_box;
+      const size_t tot = (size_t)ch * c->nk_box;
+      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_mma2<T, CTM>), gs, dim3(256), lds(TA), c->stream, s2);
+                  hipLaunchKernelGGL(k_sk_combine, dim3(div_up(tot, 256)), dim3(256), 0, c->stream, c->nk_box,
+                                     c->NZk, c->K[2], ch, p1, s2.part));
+    }
   } else {
     dim3 g1(div_up((int64_t)s.M * s.NG, 256), chunks);
     PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk<T, GZ>), g1, dim3(256), shm, c->stream, s));
