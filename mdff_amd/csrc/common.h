@@ -88,6 +88,15 @@ struct ProfRec {
   hipEvent_t a, b;
 };
 
+// device sums of one force evaluation (one slot of mdff_ctx::raw)
+constexpr int RAW_PA = 0;        // 15 per-atom column sums of the pair kernel
+constexpr int RAW_EREC = 15;     // 7 reciprocal sums: energy, stress (xx yy zz xy xz yz)
+constexpr int RAW_MOM = 22;      // musq, mutot(3), qtot(3)
+constexpr int RAW_EKIN = 29;     // sum m v^2
+constexpr int RAW_SCRATCH = 32;  // 7 moments of the SCF field (slot 0 only)
+constexpr int RAW_N = 40;
+constexpr int RAW_SLOTS = 1024;  // evaluations held before the sums go to the host
+
 struct mdff_ctx {
   mdff_desc d;
   int n = 0, ntype = 0;
@@ -156,6 +165,9 @@ struct mdff_ctx {
   double* d_disp = nullptr;               // per-block max/second-max displacement partials
   int32_t* h_flag = nullptr;              // pinned host mirror
   double* h_red = nullptr;                // pinned host reduction mirror
+  double* raw = nullptr;                  // device sums of evaluations: RAW_SLOTS x RAW_N
+  double* h_raw = nullptr;                // pinned host mirror of raw
+  int timings_pending = 0;
 
   // reciprocal space
   int K[3] = {0, 0, 0};
@@ -222,6 +234,7 @@ static inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); 
 // launches implemented in the .hip files
 int nl_build(mdff_ctx* c);
 int nl_check(mdff_ctx* c, int* rebuilt);
+int reduce_cols_dev(mdff_ctx* c, const double* in, int ncol, double* dst);
 int nl_get_pairs(mdff_ctx* c, int which, int32_t* point, int32_t* list, int64_t cap, int64_t* n);
 void launch_kardir(mdff_ctx* c);
 int realspace_field(mdff_ctx* c, int mode, const double* mu_in, double* ef_out);

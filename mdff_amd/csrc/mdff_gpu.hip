@@ -63,18 +63,34 @@ __global__ void k_combine_forces(int n, const double* __restrict__ fsr, const do
 
 // ef = ef_dir + ef_rec + ef_surf + ef_self (src/ewald.f90:224-235);
 // ef_surf = -(sum q r + sum mu) 4pi/3V when lsurf; ef_self = 4 alpha^3/(3 sqrt(pi)) mu
+// total field of every atom: ef_dir + ef_rec (+ self term of the dipoles)
+// (+ surface field -(sum q r | sum mu) 4pi/3V, from the device moments
+// mom = [musq, mutot(3), qtot(3)]; which: 0 both, 1 charges, 2 dipoles)
 __global__ void k_combine_field(int n, const double* __restrict__ edir, const double* __restrict__ erec,
-                                const double* __restrict__ mu, double self2, int lsurf, double sx,
-                                double sy, double sz, double* __restrict__ ef) {
+                                const double* __restrict__ mu, double self2, int lsurf,
+                                const double* __restrict__ mom, int which, double fpi_3V,
+                                double* __restrict__ ef) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double s[3] = {sx, sy, sz};
   for (int a = 0; a < 3; ++a) {
     double e = edir[3 * i + a] + erec[3 * i + a];
-    if (lsurf) e += s[a];
+    if (lsurf) {
+      const double m = which == 0 ? mom[4 + a] + mom[1 + a] : (which == 1 ? mom[4 + a] : mom[1 + a]);
+      e += -m * fpi_3V;
+    }
     if (mu) e += self2 * mu[3 * i + a];
     ef[3 * i + a] = e;
   }
+}
+
+// column sums of the block partials in block order (the same additions, in
+// the same order, as a sequential host loop): out[k] = sum_b part[b][k]
+__global__ void k_colsum_final(int nb, int ncol, const double* __restrict__ part, double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= ncol) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[(size_t)b * ncol + k];
+  out[k] = s;
 }
 
 // per-atom columns for the self and surface terms: mu^2, mu, q r
@@ -275,6 +291,15 @@ int reduce_cols(mdff_ctx* c, const double* in, int ncol, double* host_out) {
   return MDFF_OK;
 }
 
+// the same sums left in device memory (dst[0..ncol)), no host round trip
+int reduce_cols_dev(mdff_ctx* c, const double* in, int ncol, double* dst) {
+  const int nb = std::min(1024, std::max(1, div_up(c->n, 256)));
+  hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, c->stream, c->n, ncol, in, c->red_part);
+  hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(64), 0, c->stream, nb, ncol, c->red_part, dst);
+  HIP_TRY(c, hipGetLastError());
+  return MDFF_OK;
+}
+
 static int fail(mdff_ctx* c, int code, const char* msg) {
   c->err = msg;
   return code;
@@ -443,6 +468,8 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   HIP_TRY(c, hipHostMalloc((void**)&c->h_red, sizeof(double) * nred));
   DMALLOC(c->red_part, nred);
   DMALLOC(c->red, 64);
+  DMALLOC(c->raw, (size_t)RAW_N * RAW_SLOTS);
+  HIP_TRY(c, hipHostMalloc((void**)&c->h_raw, sizeof(double) * RAW_N * RAW_SLOTS));
   HIP_TRY(c, hipMemcpy(c->itype, it0.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->q, q.data(), sizeof(double) * n, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->mass, m.data(), sizeof(double) * n, hipMemcpyHostToDevice));
@@ -477,21 +504,11 @@ static double* extra(mdff_ctx* c) { return col(c, NPA + 12); }    // 8 columns
 // ------------------------------------------------------------------ Ewald pieces
 
 // self / surface sums (src/ewald.f90:112-143)
-struct Moments {
-  double musq, mutot[3], qtot[3];
-};
-
-static int moments(mdff_ctx* c, const double* q, const double* mu, Moments* m) {
+// musq, mutot(3), qtot(3) of (q, mu) left in device memory at dst[0..7)
+static int moments_dev(mdff_ctx* c, const double* q, const double* mu, double* dst) {
   hipLaunchKernelGGL(k_moments, dim3(nblk(c->n)), dim3(256), 0, c->stream, c->n, q, mu, c->rx, c->ry, c->rz,
                      extra(c));
-  double h[7];
-  MDFF_TRY(reduce_cols(c, extra(c), 7, h));
-  m->musq = h[0];
-  for (int a = 0; a < 3; ++a) {
-    m->mutot[a] = h[1 + a];
-    m->qtot[a] = h[4 + a];
-  }
-  return MDFF_OK;
+  return reduce_cols_dev(c, extra(c), 7, dst);
 }
 
 // full Ewald field of a charge set (damped, static SCF field) or a dipole set
@@ -502,15 +519,11 @@ static int ewald_field(mdff_ctx* c, bool charges, const double* mu, double* out)
   const double fpi_3V = 2.0 * (2.0 * MDFF_PI / c->omega / 3.0);
   MDFF_TRY(realspace_field(c, charges ? 0 : 1, mu, ef_dir(c)));
   MDFF_TRY(recip_eval(c, c->q, mu, charges, !charges, false, true, ef_rec(c), nullptr, nullptr));
-  double s[3] = {0, 0, 0};
-  if (c->d.lsurf) {
-    Moments m;
-    MDFF_TRY(moments(c, c->q, charges ? nullptr : mu, &m));
-    // the SCF zeroes the charges for the induced field (src/moment_from_pola.f90:347-351)
-    for (int a = 0; a < 3; ++a) s[a] = -(charges ? m.qtot[a] : m.mutot[a]) * fpi_3V;
-  }
+  // the SCF zeroes the charges for the induced field (src/moment_from_pola.f90:347-351)
+  double* mom = c->raw + RAW_SCRATCH;
+  if (c->d.lsurf) MDFF_TRY(moments_dev(c, c->q, charges ? nullptr : mu, mom));
   hipLaunchKernelGGL(k_combine_field, dim3(nblk(c->n)), dim3(256), 0, c->stream, c->n, ef_dir(c), ef_rec(c),
-                     charges ? nullptr : mu, 2.0 * selfa2, c->d.lsurf, s[0], s[1], s[2], out);
+                     charges ? nullptr : mu, 2.0 * selfa2, c->d.lsurf, mom, charges ? 1 : 2, fpi_3V, out);
   HIP_TRY(c, hipGetLastError());
   return MDFF_OK;
 }
@@ -571,8 +584,12 @@ static int scf(mdff_ctx* c, mdff_terms* t) {
 
 static void ev_rec(mdff_ctx* c, int k) { (void)hipEventRecord(c->ev[k], c->stream); }
 
-// SUBROUTINE engforce (src/engforce_driver.f90:102-190)
-static int engforce(mdff_ctx* c, mdff_terms* t) {
+// SUBROUTINE engforce (src/engforce_driver.f90:102-190), launch half: every
+// kernel of the force evaluation is queued; the sums the energies, virials
+// and stress are made of stay in device memory at raw[0..RAW_N) and the terms
+// known on the host (list rebuild, SCF) go to t.  engforce_finish turns the
+// sums into the terms once they are on the host.
+static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t) {
   const mdff_desc& d = c->d;
   const int n = c->n;
   memset(t, 0, sizeof *t);
@@ -605,16 +622,38 @@ static int engforce(mdff_ctx* c, mdff_terms* t) {
   const bool want_field = coul && d.doefield;
   MDFF_TRY(realspace_force_impl(c, f_sr(c), f_dir(c), ef_dir(c), want_field));
   ev_rec(c, 3);
-  double erec[7] = {0, 0, 0, 0, 0, 0, 0};
   if (coul) {
     const bool dip = c->task_qd || c->task_dd;
-    MDFF_TRY(recip_eval(c, c->q, c->mu, true, dip, true, want_field, ef_rec(c), c->frec, erec));
+    MDFF_TRY(recip_eval(c, c->q, c->mu, true, dip, true, want_field, ef_rec(c), c->frec, raw + RAW_EREC));
   }
   ev_rec(c, 4);
   hipLaunchKernelGGL(k_combine_forces, dim3(nblk(n)), dim3(256), 0, c->stream, n, f_sr(c), f_dir(c), c->frec,
                      d.coul_unit, c->sr_model != 0 ? 1 : 0, coul ? 1 : 0, c->fx, c->fy, c->fz);
-  double pa[NPA];
-  MDFF_TRY(reduce_cols(c, c->peratom, NPA, pa));
+  MDFF_TRY(reduce_cols_dev(c, c->peratom, NPA, raw + RAW_PA));
+  if (coul) {
+    MDFF_TRY(moments_dev(c, c->q, c->mu, raw + RAW_MOM));
+    // ef_t = (ef_dir + ef_rec if doefield) + ef_self (+ ef_surf)
+    const double alpha2 = c->alpha * c->alpha;
+    const double selfa = c->alpha / sqrt(MDFF_PI), selfa2 = 2.0 * selfa * alpha2 / 3.0;
+    const double fpi_3V = 2.0 * (2.0 * MDFF_PI / c->omega / 3.0);
+    if (!want_field) {
+      hipLaunchKernelGGL(k_fill, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, 0.0, ef_dir(c));
+      hipLaunchKernelGGL(k_fill, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, 0.0, ef_rec(c));
+    }
+    hipLaunchKernelGGL(k_combine_field, dim3(nblk(n)), dim3(256), 0, c->stream, n, ef_dir(c), ef_rec(c), c->mu,
+                       2.0 * selfa2, d.lsurf, raw + RAW_MOM, 0, fpi_3V, c->efield);
+  }
+  ev_rec(c, 5);
+  HIP_TRY(c, hipGetLastError());
+  c->timings_pending = 1;
+  return MDFF_OK;
+}
+
+// engforce, finishing half: energies, virials and stress from the device sums
+static void engforce_finish(const mdff_ctx* c, const double* raw, mdff_terms* t) {
+  const mdff_desc& d = c->d;
+  const double* pa = raw + RAW_PA;
+  const double* erec = raw + RAW_EREC;
   // short range (full rows count every pair twice)
   const double inv_vp = 1.0 / c->omega / d.press_unit;
   double tnb[9];
@@ -628,21 +667,22 @@ static int engforce(mdff_ctx* c, mdff_terms* t) {
   if (c->sr_model == 2) { t->u_bmhft = usr; t->vir_bmhft = vir; }
   if (c->sr_model == 3) { t->u_morse = usr; t->vir_morse = vir; }
   if (c->sr_model) for (int k = 0; k < 9; ++k) { t->tau_nonb[k] = tnb[k]; t->tau[k] += tnb[k]; }
-  if (coul) {
-    Moments m;
-    MDFF_TRY(moments(c, c->q, c->mu, &m));
+  if (d.lcoulomb) {
+    const double musq = raw[RAW_MOM];
+    const double* mutot = raw + RAW_MOM + 1;
+    const double* qtot = raw + RAW_MOM + 4;
     const double qsq = c->qsq;
-    const double tpi_V = 2.0 * MDFF_PI / c->omega, tpi_3V = tpi_V / 3.0, fpi_3V = tpi_3V * 2.0;
+    const double tpi_V = 2.0 * MDFF_PI / c->omega, tpi_3V = tpi_V / 3.0;
     const double alpha2 = c->alpha * c->alpha;
     const double selfa = c->alpha / sqrt(MDFF_PI), selfa2 = 2.0 * selfa * alpha2 / 3.0;
     t->u_dir = 0.5 * pa[8];
     t->u_rec = 2.0 * erec[0] * tpi_V;
-    t->u_self = -selfa * qsq + (-selfa2 * m.musq);
+    t->u_self = -selfa * qsq + (-selfa2 * musq);
     double uq = 0, ud = 0, uqd = 0;
     for (int a = 0; a < 3; ++a) {
-      uq += m.qtot[a] * m.qtot[a];
-      ud += m.mutot[a] * m.mutot[a];
-      uqd += m.qtot[a] * m.mutot[a];
+      uq += qtot[a] * qtot[a];
+      ud += mutot[a] * mutot[a];
+      uqd += qtot[a] * mutot[a];
     }
     t->u_surf = d.lsurf ? (uq + 2.0 * uqd + ud) * tpi_3V : 0.0;
     t->u_coul = (t->u_dir + t->u_rec + t->u_surf + t->u_self + t->u_pol) * d.coul_unit;
@@ -660,19 +700,28 @@ static int engforce(mdff_ctx* c, mdff_terms* t) {
       t->tau[k] += 2.0 * t->tau_coul[k];
     }
     t->pvirial_coul = 1.0 / 3.0 * (t->tau_coul[0] + t->tau_coul[4] + t->tau_coul[8]) * d.press_unit;
-    // ef_t = (ef_dir + ef_rec if doefield) + ef_self (+ ef_surf)
-    double s[3] = {0, 0, 0};
-    if (d.lsurf)
-      for (int a = 0; a < 3; ++a) s[a] = -(m.qtot[a] + m.mutot[a]) * fpi_3V;
-    if (!want_field) {
-      hipLaunchKernelGGL(k_fill, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, 0.0, ef_dir(c));
-      hipLaunchKernelGGL(k_fill, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, 0.0, ef_rec(c));
-    }
-    hipLaunchKernelGGL(k_combine_field, dim3(nblk(n)), dim3(256), 0, c->stream, n, ef_dir(c), ef_rec(c), c->mu,
-                       2.0 * selfa2, d.lsurf, s[0], s[1], s[2], c->efield);
   }
-  ev_rec(c, 5);
+}
+
+// sums of one or more evaluations to the host: raw slots [0, nslots)
+static int fetch_raw(mdff_ctx* c, int nslots) {
+  HIP_TRY(c, hipMemcpyAsync(c->h_raw, c->raw, sizeof(double) * RAW_N * nslots, hipMemcpyDeviceToHost,
+                            c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return MDFF_OK;
+}
+
+static int engforce(mdff_ctx* c, mdff_terms* t) {
+  MDFF_TRY(engforce_launch(c, c->raw, t));
+  MDFF_TRY(fetch_raw(c, 1));
+  engforce_finish(c, c->h_raw, t);
+  return MDFF_OK;
+}
+
+// stage timings of the last evaluation (needs its events complete)
+static void read_timings(mdff_ctx* c) {
+  if (!c->timings_pending) return;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return;
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[5]); c->last_ms[0] = ms;
   (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]); c->last_ms[1] = ms;
@@ -680,8 +729,7 @@ static int engforce(mdff_ctx* c, mdff_terms* t) {
   (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]); c->last_ms[3] = ms;
   (void)hipEventElapsedTime(&ms, c->ev[1], c->ev[2]); c->last_ms[4] = ms;
   c->n_last = 5;
-  HIP_TRY(c, hipGetLastError());
-  return MDFF_OK;
+  c->timings_pending = 0;
 }
 
 hipEvent_t prof_event(mdff_ctx* c) {
@@ -758,6 +806,7 @@ int mdff_ctx_destroy(mdff_ctx* c) {
   recip_free(c);
   if (c->h_flag) (void)hipHostFree(c->h_flag);
   if (c->h_red) (void)hipHostFree(c->h_red);
+  if (c->h_raw) (void)hipHostFree(c->h_raw);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->evpool)
@@ -863,22 +912,30 @@ static double upot_of(const mdff_terms& t) { return t.u_lj + t.u_bmhft + t.u_mor
 int mdff_md_vv(mdff_ctx* c, int32_t nsteps, double dt, double* ekin, double* upot) {
   if (!c || nsteps < 0) return MDFF_ERR_BAD_ARG;
   const int n = c->n;
+  std::vector<mdff_terms> held(RAW_SLOTS);
+  int st0 = 0;   // first step of the slots in flight
   for (int st = 0; st < nsteps; ++st) {
+    const int slot = st - st0;
+    double* raw = c->raw + (size_t)RAW_N * slot;
     hipLaunchKernelGGL(k_vv_pos, dim3(nblk(n)), dim3(256), 0, c->stream, n, dt, dt * dt * 0.5, c->mass, c->rx,
                        c->ry, c->rz, c->vx, c->vy, c->vz, c->fx, c->fy, c->fz, c->fsx, c->fsy, c->fsz);
-    mdff_terms t;
-    MDFF_TRY(engforce(c, &t));
+    MDFF_TRY(engforce_launch(c, raw, &held[slot]));
     hipLaunchKernelGGL(k_vv_vel, dim3(nblk(n)), dim3(256), 0, c->stream, n, dt * 0.5, c->mass, c->vx, c->vy,
                        c->vz, c->fsx, c->fsy, c->fsz, c->fx, c->fy, c->fz, extra(c));
-    if (ekin) {
-      double h[1];
-      MDFF_TRY(reduce_cols(c, extra(c), 1, h));
-      ekin[st] = 0.5 * h[0];
-    }
-    if (upot) upot[st] = upot_of(t);
+    if (ekin) MDFF_TRY(reduce_cols_dev(c, extra(c), 1, raw + RAW_EKIN));
     if (c->d.lvnlist) {
       int rb = 0;
       MDFF_TRY(nl_check(c, &rb));
+    }
+    if (slot == RAW_SLOTS - 1 || st == nsteps - 1) {
+      MDFF_TRY(fetch_raw(c, slot + 1));
+      for (int k = 0; k <= slot; ++k) {
+        const double* h = c->h_raw + (size_t)RAW_N * k;
+        engforce_finish(c, h, &held[k]);
+        if (ekin) ekin[st0 + k] = 0.5 * h[RAW_EKIN];
+        if (upot) upot[st0 + k] = upot_of(held[k]);
+      }
+      st0 = st + 1;
     }
   }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -1026,6 +1083,7 @@ const char* mdff_profile_name(int32_t kind) { return (kind >= 0 && kind < PK_COU
 
 int mdff_last_timings(mdff_ctx* c, double* ms, int32_t* n) {
   if (!c || !ms) return MDFF_ERR_BAD_ARG;
+  read_timings(c);
   for (int k = 0; k < c->n_last; ++k) ms[k] = c->last_ms[k];
   if (n) *n = c->n_last;
   return MDFF_OK;

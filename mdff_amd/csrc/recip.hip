@@ -871,6 +871,15 @@ __global__ void k_sk_combine(int nk_box, int NZ, int Kz, int nchunk, double* __r
   p1[2 * x + 1] += fz * s2r;
 }
 
+// e[k] = sum_b part[b][k], k < 7, in block order
+__global__ void k_colsum7(int nb, const double* __restrict__ part, double* __restrict__ e) {
+  const int k = threadIdx.x;
+  if (k >= 7) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[(size_t)7 * b + k];
+  e[k] = s;
+}
+
 }  // namespace
 
 // K-step counts compiled for the matrix-core gather (Kz + 1 rounded up)
@@ -1075,13 +1084,8 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
                                    reinterpret_cast<T*>(c->wk_t), c->red_part);
               });
   if (e_out) {
-    // 7 sums of the k-blocks, reduced on the host in block order
-    std::vector<double> blk(7 * (size_t)nb);
-    HIP_TRY(c, hipMemcpyAsync(blk.data(), c->red_part, sizeof(double) * 7 * nb, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    for (int k = 0; k < 7; ++k) e_out[k] = 0.0;
-    for (int b = 0; b < nb; ++b)
-      for (int k = 0; k < 7; ++k) e_out[k] += blk[7 * (size_t)b + k];
+    // 7 sums of the k-blocks in block order, left in device memory
+    hipLaunchKernelGGL(k_colsum7, dim3(1), dim3(64), 0, c->stream, nb, c->red_part, e_out);
   }
   if (!forces && !field) return MDFF_OK;
   const bool dipf = forces && use_mu;
