@@ -528,173 +528,6 @@ __global__ void __launch_bounds__(256) k_gather_mma(GmArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Matrix-core form of phase 1: S(m, nz) = sum_i P_i(m) Z_i(nz),
-// P_i(m) = a_i(m) X_i[nx] Y_i[ny].  Real GEMM on MFMA 16x16x4 with
-// rows R = (m_local, part): part = (Re, Im) of S1 [charges: a = q + i 2pi
-// (nx m_x + ny m_y)] and, with dipoles, (Re, Im) of S2 = sum mz XY Z, so that
-// S = S1 + i 2pi nz S2; columns = nz (four tiles of 16); K = (atom, Re/Im of
-// Z).  The A value of a lane is generated from the LDS-staged tables of the
-// atom and reused by the four column tiles; the B value is a table entry.
-// Each block (4 waves = 4 row tiles) sums over one chunk of atoms; chunk
-// partials are reduced in FP64 by k_sk_reduce.
-// ---------------------------------------------------------------------------
-struct SmArgs {
-  int n, Kx, Ky, Kz, NY, NZ, M, RPM, mtiles, chunk, TA, use_q, use_mu, nk_box;
-  const double* q;
-  const double* mu;
-  double b[9];
-  const void* tab;
-  double* part;                  // [chunk][2*nk_box] FP64
-};
-
-template <typename T, int RPM>
-__global__ void __launch_bounds__(256) k_sk_mma(SmArgs a) {
-  typedef typename Mma<T>::acc_t acc_t;
-  extern __shared__ unsigned char smem_raw[];
-  const int TA = a.TA;
-  const int ncol = a.Kx + a.Ky + a.Kz + 3;
-  cplx<T>* st = reinterpret_cast<cplx<T>*>(smem_raw);          // [TA][ncol], atom-major
-  T* sq = reinterpret_cast<T*>(st + (size_t)ncol * TA);         // q, mx, my, mz
-  const cplx<T>* tab = reinterpret_cast<const cplx<T>*>(a.tab);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int g = lane >> 4, col = lane & 15;
-  constexpr int MPT = 16 / RPM;                                 // m values per row tile
-  const int mt = blockIdx.x * 4 + wave;                         // row tile of this wave
-  const bool wave_active = mt < a.mtiles;
-  const int m = mt * MPT + col / RPM;                           // A row = lane % 16
-  const int part = col % RPM;                                   // 0 S1.re 1 S1.im 2 S2.re 3 S2.im
-  const bool mok = wave_active && (m < a.M);
-  const int nx = mok ? m / a.NY : 0;
-  const int ny = mok ? m % a.NY - a.Ky : 0;
-  const int ycol = a.Kx + 1 + (ny >= 0 ? ny : -ny);
-  const int zcol = a.Kx + 1 + a.Ky + 1;
-  const int kk = g & 1;
-  const T tp = T(2.0 * MDFF_PI);
-  acc_t acc[4];
-  double dacc[4][4];
-#pragma unroll
-  for (int ct = 0; ct < 4; ++ct)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) dacc[ct][r] = 0.0;
-  // B columns of this lane: nz index per column tile
-  int bz[4];
-  bool bok[4];
-#pragma unroll
-  for (int ct = 0; ct < 4; ++ct) {
-    const int nzi = ct * 16 + col;
-    bok[ct] = nzi < a.NZ;
-    const int nz = nzi - a.Kz;
-    bz[ct] = zcol + (nz < 0 ? -nz : nz);
-  }
-  const int abeg = blockIdx.y * a.chunk;
-  const int aend = min(a.n, abeg + a.chunk);
-  for (int t0 = abeg; t0 < aend; t0 += TA) {
-    const int nt = min(TA, aend - t0);
-    __syncthreads();
-    for (int x = threadIdx.x; x < ncol * TA; x += blockDim.x) {
-      const int cc = x / TA, ia = x % TA;
-      st[ia * ncol + cc] = (ia < nt) ? tab[(size_t)cc * a.n + t0 + ia] : cplx<T>{T(0), T(0)};
-    }
-    for (int ia = threadIdx.x; ia < TA; ia += blockDim.x) {
-      T qq = T(0), mx = T(0), my = T(0), mz = T(0);
-      if (ia < nt) {
-        const int i = t0 + ia;
-        if (a.use_q) qq = (T)a.q[i];
-        if (a.use_mu) {
-          const double u0 = a.mu[3 * i], u1 = a.mu[3 * i + 1], u2 = a.mu[3 * i + 2];
-          mx = (T)(u0 * a.b[0] + u1 * a.b[1] + u2 * a.b[2]);
-          my = (T)(u0 * a.b[3] + u1 * a.b[4] + u2 * a.b[5]);
-          mz = (T)(u0 * a.b[6] + u1 * a.b[7] + u2 * a.b[8]);
-        }
-      }
-      sq[4 * ia] = qq;
-      sq[4 * ia + 1] = mx;
-      sq[4 * ia + 2] = my;
-      sq[4 * ia + 3] = mz;
-    }
-    __syncthreads();
-    if (!wave_active) continue;          // whole wave: no MFMA issued
-#pragma unroll
-    for (int ct = 0; ct < 4; ++ct) acc[ct] = acc_t{T(0), T(0), T(0), T(0)};
-    // K steps of 4: two atoms (Re Z, Im Z each) per step
-    for (int s = 0; s < (nt + 1) / 2; ++s) {
-      const int ia = 2 * s + (g >> 1);
-      const cplx<T>* row = st + (size_t)ia * ncol;
-      T av = T(0);
-      if (mok && ia < nt) {
-        const cplx<T> X = row[nx];
-        cplx<T> Y = row[ycol];
-        if (ny < 0) Y.im = -Y.im;
-        const cplx<T> XY = cmul(X, Y);
-        cplx<T> P;
-        if (part < 2) {
-          const T qq = sq[4 * ia];
-          if (RPM == 4) {
-            const T bb = tp * ((T)nx * sq[4 * ia + 1] + (T)ny * sq[4 * ia + 2]);
-            P = {qq * XY.re - bb * XY.im, qq * XY.im + bb * XY.re};
-          } else {
-            P = {qq * XY.re, qq * XY.im};
-          }
-        } else {
-          const T mzz = sq[4 * ia + 3];
-          P = {mzz * XY.re, mzz * XY.im};
-        }
-        // Re rows: (Pr, -Pi) . (Zr, Zi); Im rows: (Pi, Pr) . (Zr, Zi)
-        const bool re_row = (part & 1) == 0;
-        av = re_row ? (kk == 0 ? P.re : -P.im) : (kk == 0 ? P.im : P.re);
-      }
-      const bool aok = ia < nt;
-#pragma unroll
-      for (int ct = 0; ct < 4; ++ct) {
-        T bv = T(0);
-        if (bok[ct] && aok) {
-          const cplx<T> z = row[bz[ct]];
-          const bool neg = (ct * 16 + col) < a.Kz;
-          bv = kk == 0 ? z.re : (neg ? -z.im : z.im);
-        }
-        acc[ct] = Mma<T>::op(av, bv, acc[ct]);
-      }
-    }
-    // FP64 accumulation across atom tiles
-#pragma unroll
-    for (int ct = 0; ct < 4; ++ct)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) dacc[ct][r] += (double)acc[ct][r];
-  }
-  // write-out through LDS: every lane stores its 4 rows, then each (m, nz)
-  // combines its parts (all waves reach both barriers)
-  __syncthreads();
-  double* sd = reinterpret_cast<double*>(smem_raw) + (size_t)wave * 16 * 64;   // [R][nz col 0..63]
-#pragma unroll
-  for (int ct = 0; ct < 4; ++ct) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int Rd = (sizeof(T) == 4) ? (4 * g + r) : (4 * r + g);
-      sd[Rd * 64 + ct * 16 + col] = dacc[ct][r];
-    }
-  }
-  __syncthreads();
-  if (!wave_active) return;
-  double* out = a.part + (size_t)blockIdx.y * 2 * a.nk_box;
-  for (int x = lane; x < MPT * 64; x += 64) {
-    const int ml = x / 64, nzi = x % 64;
-    const int mm = mt * MPT + ml;
-    if (mm >= a.M || nzi >= a.NZ) continue;
-    double re = sd[(ml * RPM + 0) * 64 + nzi], im = sd[(ml * RPM + 1) * 64 + nzi];
-    if (RPM == 4) {
-      const double fz = 2.0 * MDFF_PI * (double)(nzi - a.Kz);
-      const double s2r = sd[(ml * RPM + 2) * 64 + nzi], s2i = sd[(ml * RPM + 3) * 64 + nzi];
-      re -= fz * s2i;          // S = S1 + i 2pi nz S2
-      im += fz * s2r;
-    }
-    const size_t idx = (size_t)mm * a.NZ + nzi;
-    out[2 * idx] = re;
-    out[2 * idx + 1] = im;
-  }
-}
-
-
-// ---------------------------------------------------------------------------
 // Matrix-core structure factor, transposed form: S^T(nz, m) = sum_i
 // Z_i(nz) P_i(m).  Rows R = (nz, Re/Im) in 7 tiles of 16 (8 nz each), taken
 // straight from the Z table; columns = 16 values of m = (nx, ny); K = (atom,
