@@ -34,7 +34,7 @@ sys.path.insert(0, str(ROOT))
 
 from mdff_amd import capi, configs  # noqa: E402
 
-PEAK = {"fp32": 157.3, "fp64": 78.6}      # MI355X dense vector TFLOP/s
+PEAK = {"fp32": 157.3, "fp64": 78.6}      # MI355X dense TFLOP/s (FP32 / FP64 matrix and packed vector)
 HBM_PEAK = 8000.0                          # GB/s
 # canonical algorithmic flops per unit (counting FMA = 2, sqrt/exp/sin/cos/div = 1)
 FLOPS_PER_UNIT = {
@@ -42,8 +42,13 @@ FLOPS_PER_UNIT = {
     "k_gather": 10.0,      # per (k, i): recarg 3 + force 7 (src/ewald_rec.f90:196-224)
     "k_pair_force": 106.0, # per in-cutoff Buckingham + Ewald q-q pair
 }
-KIND_NAMES = ["k_pair_force", "k_pair_field", "k_tables", "k_sk", "k_sk_reduce", "k_gather",
-              "k_gather_final", "k_nl_sweep"]
+# the kernel behind each profiler class on the default (matrix-core) path
+KERNEL_OF = {"k_sk": "k_sk_mma2<float,2>", "k_gather": "k_gather_mma<float,CT,NKS>",
+             "k_pair_force": "k_pair_force<float,false,16>"}
+# HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (KiB per dispatch,
+# scaled by the copy-kernel calibration factor); None until measured
+TRAFFIC = {}
+NKINDS = 9   # kernel classes of the library's event profiler (mdff_profile_name)
 
 
 def build_system(natm: int, relax_iters: int):
@@ -81,14 +86,27 @@ def run_form(s, precision: int, steps: int, warmup: int):
     t_start = time.perf_counter()
     ek, up = eng.md_vv(steps, s.dt)      # returns after the stream is synchronized
     t_end = time.perf_counter()
-    ms = (C.c_double * 8)()
-    cnt = (C.c_int32 * 8)()
-    lib.mdff_profile_read(eng.h, ms, cnt, 8)
+    ms = (C.c_double * NKINDS)()
+    cnt = (C.c_int32 * NKINDS)()
+    lib.mdff_profile_read(eng.h, ms, cnt, NKINDS)
     lib.mdff_profile(eng.h, 0)
-    prof = {KIND_NAMES[k]: {"ms_total": ms[k], "launches": cnt[k],
-                            "ms_avg": (ms[k] / cnt[k]) if cnt[k] else 0.0} for k in range(8)}
+    lib.mdff_profile_name.restype = C.c_char_p
+    names = [lib.mdff_profile_name(k).decode() for k in range(NKINDS)]
+    prof = {names[k]: {"ms_total": ms[k], "launches": cnt[k],
+                       "ms_avg": (ms[k] / cnt[k]) if cnt[k] else 0.0} for k in range(NKINDS)}
     eng.close()
     return {"f0": f0, "t0": t0, "sec": t_end - t_start, "ek": ek, "up": up, "prof": prof}
+
+
+def force_errors(f, fref) -> dict:
+    """max_i |dF_i| / RMS_i |F_i^ref|, the median per-atom relative error, and
+    max |dF| / max |F^ref| over components."""
+    dF = np.linalg.norm(f - fref, axis=1)
+    nref = np.linalg.norm(fref, axis=1)
+    rms = float(np.sqrt(np.mean(nref ** 2)))
+    return {"max_over_rms": float(dF.max() / rms),
+            "median_per_atom": float(np.median(dF / np.maximum(nref, 1e-30))),
+            "max_component_over_max": float(np.abs(f - fref).max() / np.abs(fref).max())}
 
 
 def drift_per_atom_per_ns(ek, up, natm, dt_fs):
@@ -125,6 +143,74 @@ def cpu_baseline(s, threads: int):
                       f"then a steady VV step ({step:.2f} s); value from the steady step"}
 
 
+def fit_leg(nsnap: int, reps: int) -> dict:
+    """fit-gradients/s: 256 PIM oxide snapshots (96-1536 ions), targets from
+    the plain FP64 engforce at the hidden parameters, one FP32 forward +
+    adjoint gradient over all snapshots per evaluation (one launch)."""
+    from mdff_amd import fit as F
+    params = [("A", 1, 2), ("RHO", 1, 2), ("A", 2, 2), ("RHO", 2, 2), ("C", 2, 2), ("ALPHA", 2, 0),
+              ("TTB", 2, 1), ("TTC", 2, 1), ("TTB", 2, 2), ("TTC", 2, 2)]
+    snaps = configs.fit_snapshots(nsnap)
+    theta_star = F.theta_of(params, configs.FIT_THETA_STAR)
+    theta0 = theta_star * (1.0 + 0.03 * np.array([1, -1, 0.5, 0.7, -0.8, 1, -0.6, 0.9, 0.4, -0.5]))
+    t0 = time.perf_counter()
+    Fref, Tref = F.plain_targets(snaps)
+    t_targets = time.perf_counter() - t0
+    P = F.FitProblem(snaps, params)
+    P.set_targets(Fref, Tref, 0.05)
+    tune = P.autotune(theta0, capi.PREC_MIXED, reps=1)
+    r32 = P.eval(theta0, capi.PREC_MIXED, grad=True, outputs=True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        P.eval(theta0, capi.PREC_MIXED, grad=True)
+    sec = (time.perf_counter() - t0) / reps
+    r64 = P.eval(theta0, capi.PREC_FP64, grad=True)
+    t0 = time.perf_counter()
+    P.eval(theta0, capi.PREC_FP64, grad=True)
+    sec64 = time.perf_counter() - t0
+    P.close()
+    natm = sum(s.natm for s in snaps)
+    return {"metric": "fit-gradients/s", "value": round(1.0 / sec, 3), "seconds_per_gradient": sec,
+            "fp64_seconds_per_gradient": sec64, "snapshots": nsnap, "ions": natm,
+            "sizes": [int(min(s.natm for s in snaps)), int(max(s.natm for s in snaps))],
+            "params": len(params), "launch": {"block": tune["block"], "grid": tune["grid"]},
+            "grad_rel_diff_fp32_vs_fp64": float(np.linalg.norm(r32.grad - r64.grad) / np.linalg.norm(r64.grad)),
+            "loss_at_start": r64.loss, "scf_iters_max": int(r32.nscf[:, 0].max()),
+            "adjoint_iters_max": int(r32.nscf[:, 1].max()), "plain_targets_seconds": t_targets}
+
+
+def extra_leg(name: str, steps: int) -> dict:
+    """cfg3 (LiF 200k, NHC-3) and cfg4 (MgO PIM 12k, NHC): a few steps of
+    each form; force error of the mixed form against FP64 at the start."""
+    if name == "lif":
+        s = configs.lif()
+        eng = capi.gpu_engine(copy.deepcopy(s.desc), s.itype)
+        s.pos, fmax = configs.relax_sd(eng, s.pos, iters=200)
+        eng.close()
+    else:
+        s = configs.mgo_pim()
+    out = {"natm": s.natm, "kmax": list(s.desc.kES)}
+    forces = {}
+    for prec, tag in ((capi.PREC_FP64, "fp64"), (capi.PREC_MIXED, "mixed")):
+        d = copy.deepcopy(s.desc)
+        d.precision = prec
+        eng = capi.gpu_engine(d, s.itype)
+        eng.set_positions(s.pos)
+        eng.set_velocities(s.vel)
+        f0, t0 = eng.engforce()
+        forces[tag] = f0
+        xi, vxi = np.zeros(3), np.zeros(3)
+        _, _, _, xi, vxi = eng.md_nhcn(2, s.dt, s.temp, 0.1 * capi.TIME_UNIT, xi, vxi)
+        ta = time.perf_counter()
+        eng.md_nhcn(steps, s.dt, s.temp, 0.1 * capi.TIME_UNIT, xi, vxi)
+        sec = (time.perf_counter() - ta) / steps
+        out[tag] = {"ms_per_step": round(sec * 1e3, 3), "ns_per_day": round(86400.0 / sec * 1e-6, 3),
+                    "n_scf": int(t0["n_scf"])}
+        eng.close()
+    out["force_error_mixed_vs_fp64"] = force_errors(forces["mixed"], forces["fp64"])
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +220,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--relax", type=int, default=150, help="steepest-descent iterations before MD")
+    ap.add_argument("--no-fit", action="store_true", help="skip the batched-fit leg")
+    ap.add_argument("--fit-snapshots", type=int, default=256)
+    ap.add_argument("--extra", action="store_true", help="also time the LiF 200k and MgO PIM 12k workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -150,7 +239,8 @@ def main() -> int:
     r64 = run_form(s, capi.PREC_FP64, args.steps, args.warmup)
     r32 = run_form(s, capi.PREC_MIXED, args.steps, args.warmup)
 
-    ferr = float(np.abs(r32["f0"] - r64["f0"]).max() / np.abs(r64["f0"]).max())
+    ferr_all = force_errors(r32["f0"], r64["f0"])
+    ferr = ferr_all["max_component_over_max"]
     ms64 = r64["sec"] / args.steps * 1e3
     ms32 = r32["sec"] / args.steps * 1e3
     nsday = lambda ms: 86400.0 / (ms * 1e-3) * dt_fs * 1e-6  # noqa: E731
@@ -165,9 +255,10 @@ def main() -> int:
     if dom in FLOPS_PER_UNIT and prof[dom]["ms_avg"] > 0:
         flops = FLOPS_PER_UNIT[dom] * units[dom]
         ach = flops / (prof[dom]["ms_avg"] * 1e-3) / 1e12
-        roof = {"kernel": dom, "bound": "valu-fp32", "achieved": round(ach, 3), "peak": PEAK["fp32"],
-                "unit": "TFLOP/s", "frac": round(ach / PEAK["fp32"], 4), "traffic": None,
-                "algorithmic_flops_per_launch": flops, "avg_launch_ms": prof[dom]["ms_avg"]}
+        roof = {"kernel": KERNEL_OF[dom], "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK["fp32"],
+                "unit": "TFLOP/s", "frac": round(ach / PEAK["fp32"], 4), "traffic": TRAFFIC.get(dom),
+                "algorithmic_flops_per_launch": flops, "avg_launch_ms": prof[dom]["ms_avg"],
+                "units_per_launch": units[dom], "flops_per_unit": FLOPS_PER_UNIT[dom]}
 
     out = {
         "metric": "ns/day mixed vs FP64 on GPU at <=1e-5 rel force err; fit-gradients/s",
@@ -191,7 +282,7 @@ def main() -> int:
                  "nve_drift_eV_per_atom_per_ns": d64},
         "mixed": {"ns_per_day": round(nsday(ms32), 3), "ms_per_step": round(ms32, 4),
                   "nve_drift_eV_per_atom_per_ns": d32,
-                  "max_rel_force_err_vs_fp64": ferr},
+                  "max_rel_force_err_vs_fp64": ferr, "force_error_vs_fp64": ferr_all},
         "speedup_mixed_vs_fp64": round(ms64 / ms32, 3),
         "drift_ratio_mixed_over_fp64": (abs(d32) / abs(d64)) if d64 else None,
         "pair_interactions_per_s": npairs / (ms32 * 1e-3),
@@ -201,6 +292,10 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": None,
     }
+    if not args.no_fit:
+        out["fit"] = fit_leg(args.fit_snapshots, reps=3)
+    if args.extra:
+        out["extra"] = {"lif_200k_nhc3": extra_leg("lif", 5), "mgo_pim_12k_nhc": extra_leg("mgo", 10)}
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(s, args.cpu_threads)

@@ -230,6 +230,51 @@ int mdff_profile(mdff_ctx *ctx, int32_t on);
 int mdff_profile_read(mdff_ctx *ctx, double *ms_total, int32_t *count, int32_t nkinds);
 const char *mdff_profile_name(int32_t kind);
 
+/* ------------------------------------------------------------------------
+ * Batched force-field fit.  No counterpart in the reference (it has no
+ * gradient code); it serves fitting of the engforce parameters on many
+ * fixed snapshots.  The forward of each snapshot is the engforce of this
+ * library restricted to Buckingham (lbmhft) + Ewald + PIM with TT damping:
+ * forces, stress (tau_nonb + tau_coul, GPa) and the SCF dipoles.  The loss is
+ *   L = sum_s [ sum_i |F_i - F*_i|^2 + w sum_ab (tau_ab - tau*_ab)^2 ]
+ * and its gradient with respect to the parameter vector theta is computed
+ * through the converged SCF by one adjoint solve per snapshot.  All
+ * snapshots run in one persistent launch (one workgroup per snapshot, largest
+ * first).
+ * ---------------------------------------------------------------------- */
+typedef struct mdff_fit mdff_fit;
+
+/* theta entry kinds: spec[3p] = kind, spec[3p+1] = t1, spec[3p+2] = t2 (1-based types) */
+#define MDFF_FIT_A      0   /* Buckingham A of pair (t1,t2)                      */
+#define MDFF_FIT_RHO    1   /* Buckingham rho = 1/B of pair (t1,t2)              */
+#define MDFF_FIT_C      2   /* Buckingham C of pair (t1,t2)                      */
+#define MDFF_FIT_ALPHA  3   /* isotropic polarisability of type t1               */
+#define MDFF_FIT_TTB    4   /* TT b of the field at a dipole of type t1 from charges of type t2 */
+#define MDFF_FIT_TTC    5   /* TT c, same entry                                   */
+#define MDFF_FIT_MAXPARAM 32
+
+/* d: the force field (types <= 4, charges, masses, TT orders, unit constants,
+ * conv_tol_ind / min / max SCF iterations for both the SCF and the adjoint
+ * solve, values of the parameters that are not fitted).  Per snapshot s:
+ * natm[s], cells[9s..9s+8] (column-major), cut_alpha[3s..3s+2] =
+ * (cutshortrange, cutlongrange, alphaES), kmax[3s..3s+2].  itype and pos
+ * (AoS 3 x total atoms, Cartesian) are the snapshots concatenated. */
+int mdff_fit_create(const mdff_desc *d, int32_t nsnap, const int32_t *natm,
+                    const double *cells, const double *cut_alpha, const int32_t *kmax,
+                    const int32_t *itype, const double *pos, int32_t nparam,
+                    const int32_t *spec, mdff_fit **out);
+/* targets: forces AoS 3 x total atoms (eV/A), stress 9 x nsnap (GPa), weight w */
+int mdff_fit_set_targets(mdff_fit *f, const double *fref, const double *tauref, double w_stress);
+/* workgroup size (64, 128 or 256) and number of workgroups */
+int mdff_fit_set_launch(mdff_fit *f, int32_t block, int32_t grid);
+/* Evaluate at theta: loss; grad[nparam] if want_grad; optional forces
+ * (3 x total atoms), stress (9 x nsnap) and iteration counts nscf[2s] (SCF),
+ * nscf[2s+1] (adjoint).  precision: MDFF_PREC_FP64 or MDFF_PREC_MIXED. */
+int mdff_fit_eval(mdff_fit *f, const double *theta, int32_t precision, int32_t want_grad,
+                  double *loss, double *grad, double *forces, double *stress, int32_t *nscf);
+int mdff_fit_destroy(mdff_fit *f);
+const char *mdff_fit_last_error(const mdff_fit *f);
+
 #ifdef __cplusplus
 }
 #endif

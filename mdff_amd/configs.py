@@ -364,3 +364,62 @@ def relax_sd(engine, pos: np.ndarray, iters: int = 150, max_step: float = 0.05,
         p += min(max_step, gain * fmax) * f / fmax
     engine.set_positions(p)
     return p, fmax
+
+
+def auto_ewald(cell: np.ndarray, rc: float, epsw: float = 1e-6) -> tuple[float, float, list[int]]:
+    """Automatic Ewald parameters of the reference (ewald_param with lautoES,
+    src/coulomb.f90:363-410, dl_poly form): the real-space cutoff clamped to
+    half the smallest cell width, alpha and kmax from the accuracy epsw.
+    cell rows are the lattice vectors.  Returns (rc, alpha, kmax)."""
+    A = np.asarray(cell, float)
+    omega = abs(np.linalg.det(A))
+    widths = [omega / np.linalg.norm(np.cross(A[(k + 1) % 3], A[(k + 2) % 3])) for k in range(3)]
+    rc = min(rc, 0.5 * min(widths))
+    eps = min(abs(epsw), 0.5)
+    tol = np.sqrt(abs(np.log(eps * rc)))
+    alpha = np.sqrt(abs(np.log(eps * rc * tol))) / rc
+    tol1 = np.sqrt(-np.log(eps * rc * (2.0 * tol * alpha) ** 2))
+    kmax = [int(np.rint(0.25 + np.linalg.norm(A[k]) * alpha * tol1 / np.pi)) for k in range(3)]
+    return float(rc), float(alpha), kmax
+
+
+# hidden "true" parameters of the fit workload: MgO-like PIM oxide
+FIT_THETA_STAR = {
+    ("A", 1, 2): 821.6, ("RHO", 1, 2): 0.3242,
+    ("A", 2, 2): 22764.0, ("RHO", 2, 2): 0.149, ("C", 2, 2): 27.88,
+    ("ALPHA", 2, 0): 1.2,
+    ("TTB", 2, 1): 3.0, ("TTC", 2, 1): 1.0, ("TTB", 2, 2): 3.0, ("TTC", 2, 2): 1.0,
+}
+
+
+def fit_snapshots(nsnap: int = 256, nmin: int = 96, nmax: int = 1536, seed: int = 5, a: float = 4.212,
+                  sigma: float = 0.08, rc: float = 8.0, epsw: float = 1e-6, tol: float = 1e-7) -> list[System]:
+    """Snapshots of the batched fit: rock-salt MgO supercells (n1 x n2 x n3
+    conventional cells, 8 n1 n2 n3 ions, sizes log-uniform in [nmin, nmax],
+    aspect ratio <= 2) with seeded Gaussian displacements sigma, each with its
+    own Ewald parameters from auto_ewald (cutoff min(rc, half width)).  The
+    force field is mgo_pim's at FIT_THETA_STAR (Buckingham + PIM on O with
+    TT damping of order 4)."""
+    rng = np.random.default_rng(seed)
+    shapes = sorted({tuple(sorted((i, j, k))) for i in range(2, 13) for j in range(2, 13) for k in range(2, 13)
+                     if nmin <= 8 * i * j * k <= nmax and max(i, j, k) <= 2 * min(i, j, k)})
+    sizes = np.array([8 * s[0] * s[1] * s[2] for s in shapes])
+    out = []
+    for s in range(nsnap):
+        target = np.exp(rng.uniform(np.log(nmin), np.log(nmax)))
+        cand = np.flatnonzero(np.abs(np.log(sizes / target)) == np.abs(np.log(sizes / target)).min())
+        cells = list(shapes[int(cand[0])])
+        rng.shuffle(cells)
+        base = mgo_pim(cells=tuple(cells), a=a, seed=int(rng.integers(1 << 30)), sigma=sigma, tol=tol)
+        d = base.desc
+        box = np.asarray(d.cell[:], float).reshape(3, 3)
+        rcs, alpha, kmax = auto_ewald(box, rc, epsw)
+        d.cutshortrange = rcs
+        d.cutlongrange = rcs
+        d.alphaES = alpha
+        d.kES[:] = kmax
+        d.skindiff = 0.1
+        base.name = f"fit{s}"
+        base.extra.update({"cells": tuple(cells), "rc": rcs, "alpha": alpha, "kmax": kmax})
+        out.append(base)
+    return out

@@ -529,8 +529,8 @@ __global__ void __launch_bounds__(256) k_gather_mma(GmArgs a) {
 
 // ---------------------------------------------------------------------------
 // Matrix-core structure factor, transposed form: S^T(nz, m) = sum_i
-// Z_i(nz) P_i(m).  Rows R = (nz, Re/Im) in 7 tiles of 16 (8 nz each), taken
-// straight from the Z table; columns = 16 values of m = (nx, ny); K = (atom,
+// Z_i(nz) P_i(m).  Rows R = (nz, Re/Im) in 7 tiles of 16 (8 nz each; 56 nz
+// per block, blockIdx.z picks the group of 56), taken straight from the Z table; columns = 16 values of m = (nx, ny); K = (atom,
 // Re/Im of P).  The generated operand P_i(m) = a_i X_i[nx] Y_i[ny] is made
 // once per lane and K step and feeds the 7 row tiles.
 //   mode 0: a = q            (charges)
@@ -566,9 +566,10 @@ __global__ void __launch_bounds__(256) k_sk_mma2(Sm2Args a) {
   int zc[RT];
   T zs[RT];
   bool zok[RT];
+  const int zoff = blockIdx.z * (RT * 8);                       // nz rows of this block
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt) {
-    const int nzi = rt * 8 + (col >> 1);
+    const int nzi = zoff + rt * 8 + (col >> 1);
     const int part = col & 1;
     const int nz = nzi - a.Kz;
     zok[rt] = nzi < a.NZ;
@@ -685,7 +686,7 @@ __global__ void __launch_bounds__(256) k_sk_mma2(Sm2Args a) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int Rd = (sizeof(T) == 4) ? (4 * g + r) : (4 * r + g);
-        const int nzi = rt * 8 + (Rd >> 1);
+        const int nzi = zoff + rt * 8 + (Rd >> 1);
         if (nzi >= a.NZ) continue;
         out[2 * ((size_t)m * a.NZ + nzi) + (Rd & 1)] = dacc[rt][j][r];
       }
@@ -863,7 +864,7 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
   }
   int chunks = div_up(c->n, s.a0_step);
   bool sk_fp64 = false;
-  if (c->use_mma && c->NZk <= 56) {
+  if (c->use_mma) {
     sk_fp64 = true;
     Sm2Args sm{};
     sm.n = c->n;
@@ -884,7 +885,7 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     auto lds = [&](int ta) { return sizeof(cplx<T>) * (size_t)ncol * ta + sizeof(T) * 3 * ta; };
     while (TA > 2 && lds(TA) > 65536) TA /= 2;
     sm.TA = TA;
-    dim3 gs(mblocks, ch);
+    dim3 gs(mblocks, ch, div_up(c->NZk, 56));
     double* p1 = reinterpret_cast<double*>(c->sk_part);
     sm.part = p1;
     // first pass: mode 0 (charges) or mode 1 (charges, zero when switched

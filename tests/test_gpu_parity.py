@@ -82,7 +82,7 @@ def _check_engforce(s, tol=TOL64, precision=capi.PREC_FP64, **over):
 
 
 @pytest.mark.parametrize("case", ["nacl_bmh", "sio2_buck", "fcc_lj", "morse", "bmhftd", "triclinic",
-                                  "lsurf", "nolist"])
+                                  "lsurf", "nolist", "large_kmax"])
 def test_engforce_fp64_parity(case):
     """engforce (src/engforce_driver.f90:102-190), FP64 form vs the oracle."""
     over = {}
@@ -116,6 +116,9 @@ def test_engforce_fp64_parity(case):
     elif case == "lsurf":
         s = configs.nacl(ncell=2)
         over["lsurf"] = 1
+    elif case == "large_kmax":
+        # 2 kmax + 1 = 67 nz values: the structure factor runs two nz groups
+        s = configs.nacl(ncell=2, alpha=0.9, kmax=33)
     else:
         s = configs.nacl(ncell=2)
         over["lvnlist"] = 0
