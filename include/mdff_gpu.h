@@ -44,7 +44,7 @@ extern "C" {
 #endif
 
 #define MDFF_NTYPEMAX 16
-#define MDFF_ABI_VERSION 1
+#define MDFF_ABI_VERSION 2
 
 typedef enum mdff_status {
   MDFF_OK = 0,
@@ -116,7 +116,8 @@ typedef struct mdff_desc {
   int32_t min_scf_pol_iter;        /* default 3                                       */
   int32_t max_scf_pol_iter;        /* default 100                                     */
   int32_t extrapolate_order;       /* ASPC order 0..4, default 0                      */
-  int32_t pad1;
+  int32_t algo_moment_from_pola;   /* 0: 'scf' (default), 1: 'scf_kO_v2'              */
+  double  omegakO;                 /* mixing of the scf_kO_v2 inner loop, default 0.7 */
 } mdff_desc;
 
 /* Energies, virials and stress of one engforce call (src/thermo.f90:41-46,

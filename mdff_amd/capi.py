@@ -16,7 +16,7 @@ from pathlib import Path
 import numpy as np
 
 NTYPEMAX = 16
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 STATUS = {
     0: "MDFF_OK",
@@ -60,7 +60,8 @@ class MdffDesc(C.Structure):
         ("ldip_damping", C.c_int32 * 4096), ("pol_damp_b", C.c_double * 4096),
         ("pol_damp_c", C.c_double * 4096), ("pol_damp_k", C.c_int32 * 4096),
         ("conv_tol_ind", C.c_double), ("min_scf_pol_iter", C.c_int32),
-        ("max_scf_pol_iter", C.c_int32), ("extrapolate_order", C.c_int32), ("pad1", C.c_int32),
+        ("max_scf_pol_iter", C.c_int32), ("extrapolate_order", C.c_int32),
+        ("algo_moment_from_pola", C.c_int32), ("omegakO", C.c_double),
     ]
 
 
@@ -118,6 +119,8 @@ def default_desc(natm: int, ntype: int, cell: np.ndarray) -> MdffDesc:
     d.min_scf_pol_iter = 3
     d.max_scf_pol_iter = 100
     d.extrapolate_order = 0
+    d.algo_moment_from_pola = 0
+    d.omegakO = 0.7
     return d
 
 

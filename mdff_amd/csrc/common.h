@@ -185,6 +185,8 @@ struct mdff_ctx {
   void* amat = nullptr;                   // matrix-core gather A operand
   int mma_nks = 0, mma_mtiles = 0, mma_ct = 0, mma_splits = 1;
   int use_mma = 1;                        // matrix-core reciprocal kernels
+  int sk_ctm = 2;                         // m-tiles per wave of the structure factor
+  double* kO_ws = nullptr;                // scf_kO_v2 work vectors (11 x 3n)
 
   double last_ms[8] = {0};
   int n_last = 0;
@@ -237,7 +239,7 @@ int nl_check(mdff_ctx* c, int* rebuilt);
 int reduce_cols_dev(mdff_ctx* c, const double* in, int ncol, double* dst);
 int nl_get_pairs(mdff_ctx* c, int which, int32_t* point, int32_t* list, int64_t cap, int64_t* n);
 void launch_kardir(mdff_ctx* c);
-int realspace_field(mdff_ctx* c, int mode, const double* mu_in, double* ef_out);
+int realspace_field(mdff_ctx* c, int mode, const double* mu_in, double* ef_out, double alpha = 0.0);
 int recip_setup(mdff_ctx* c);
 int recip_tables(mdff_ctx* c);
 int recip_eval(mdff_ctx* c, const double* q, const double* mu, bool use_q, bool use_mu,

@@ -187,6 +187,47 @@ __global__ void k_rmsd_terms(int n, const int32_t* __restrict__ polar, const dou
   out[i] = r;
 }
 
+// out = a x + b y
+__global__ void k_axpby(size_t n3, double a, const double* __restrict__ x, double b, const double* __restrict__ y,
+                        double* __restrict__ out) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n3) out[k] = a * x[k] + b * y[k];
+}
+
+// scf_kO_v2 residual f = E - mu / alpha on polarisable atoms (others keep f)
+__global__ void k_kO_resid(int n, const int32_t* __restrict__ polar, const double* __restrict__ pol,
+                           const double* __restrict__ E, const double* __restrict__ mu, double* __restrict__ f) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !polar[i]) return;
+  for (int a = 0; a < 3; ++a) f[3 * i + a] = E[3 * i + a] - mu[3 * i + a] / pol[9 * (size_t)i + a + 3 * a];
+}
+
+// find_min (src/moment_from_pola.f90:2415-2455) per-atom dot products over
+// polarisable atoms: g0.g0 and g0.(g1 - g0)
+__global__ void k_kO_dots(int n, const int32_t* __restrict__ polar, const double* __restrict__ g0,
+                          const double* __restrict__ g1, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s0 = 0.0, s1 = 0.0;
+  if (polar[i])
+    for (int k = 0; k < 3; ++k) {
+      const double a = g0[3 * i + k];
+      s0 = s0 + a * a;
+      s1 = s1 + a * (g1[3 * i + k] - a);
+    }
+  out[i] = s0;
+  out[(size_t)n + i] = s1;
+}
+
+// gmin = g0 + al (g1 - g0), dmin = al d1
+__global__ void k_kO_step(size_t n3, double al, const double* __restrict__ g0, const double* __restrict__ g1,
+                          const double* __restrict__ d1, double* __restrict__ gmin, double* __restrict__ dmin) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n3) return;
+  gmin[k] = g0[k] + al * (g1[k] - g0[k]);
+  dmin[k] = al * d1[k];
+}
+
 __global__ void k_add3(size_t n3, const double* __restrict__ a, const double* __restrict__ b,
                        double* __restrict__ o) {
   size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -489,6 +530,9 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   // matrix-core kernels do not cover)
   const char* ev = getenv("MDFF_RECIP_VALU");
   c->use_mma = (ev && ev[0] == '1') ? 0 : 1;
+  // m-tiles per wave of the matrix-core structure factor (tuning knob, 1 or 2)
+  const char* ec = getenv("MDFF_SK_CTM");
+  if (ec && ec[0] == '1') c->sk_ctm = 1;
   if (d->lcoulomb) MDFF_TRY(recip_setup(c));
   return MDFF_OK;
 }
@@ -582,6 +626,128 @@ static int scf(mdff_ctx* c, mdff_terms* t) {
   return MDFF_OK;
 }
 
+// rmsd of get_rmsd_mu (src/moment_from_pola.f90:998-1027) of (E, mu); mu may be
+// the zero vector
+static int rmsd_of(mdff_ctx* c, const double* E, const double* mu, double* out) {
+  hipLaunchKernelGGL(k_rmsd_terms, dim3(nblk(c->n)), dim3(256), 0, c->stream, c->n, c->polar, c->pol, E, mu,
+                     extra(c));
+  double h[1];
+  MDFF_TRY(reduce_cols(c, extra(c), 1, h));
+  *out = sqrt(h[0] / (double)c->npol);
+  return MDFF_OK;
+}
+
+// induced_moment_inner (src/moment_from_pola.f90:822-942): mu <- (1-w) mu +
+// w alpha (f_ext + E(mu)) with E the real-space dipole field at alphaES =
+// 0.001 plus its self (and surface) term, at most 10 steps, until the
+// residual is below a tenth of |f_ext|.  Scratch: kw[0..3) 3n vectors.
+static int kO_inner(mdff_ctx* c, const double* fext, double* mu, double* const kw[4]) {
+  const int n = c->n;
+  const size_t n3 = 3 * (size_t)n;
+  const double w = c->d.omegakO, al = 0.001;
+  double* ft = kw[0];
+  double* mprev = kw[1];
+  double* f = kw[2];
+  double* zero = kw[3];
+  double rmsd_ref;
+  MDFF_TRY(rmsd_of(c, fext, zero, &rmsd_ref));
+  hipLaunchKernelGGL(k_induced, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->pol, c->invpol, fext, mu, nullptr);
+  hipLaunchKernelGGL(k_axpby, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, w, mu, 0.0, mu, mu);
+  double rmsd_inner = 1.79769313486231571e308;
+  const double self2 = 2.0 * (2.0 * (al / sqrt(MDFF_PI)) * al * al / 3.0);
+  const double fpi_3V = 2.0 * (2.0 * MDFF_PI / c->omega / 3.0);
+  double* mom = c->raw + RAW_SCRATCH;
+  for (int it = 1; it <= 10 && rmsd_inner > 0.1 * rmsd_ref; ++it) {
+    HIP_TRY(c, hipMemcpyAsync(mprev, mu, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+    MDFF_TRY(realspace_field(c, 1, mu, ef_dir(c), al));
+    if (c->d.lsurf) MDFF_TRY(moments_dev(c, c->q, mu, mom));
+    hipLaunchKernelGGL(k_combine_field, dim3(nblk(n)), dim3(256), 0, c->stream, n, ef_dir(c), zero, mu, self2,
+                       c->d.lsurf, mom, 2, fpi_3V, f);
+    hipLaunchKernelGGL(k_add3, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, fext, f, ft);
+    MDFF_TRY(rmsd_of(c, ft, mu, &rmsd_inner));
+    hipLaunchKernelGGL(k_induced, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->pol, c->invpol, ft, mu, nullptr);
+    hipLaunchKernelGGL(k_axpby, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, 1.0 - w, mprev, w, mu, mu);
+  }
+  return MDFF_OK;
+}
+
+// moment_from_pola_scf_kO_v2 (src/moment_from_pola.f90:1335-1639): inner
+// fixed-point loop, full Ewald induced field, line minimum between the last
+// two residuals (find_min); rmsd = |g_min|.
+static int scf_kO_v2(mdff_ctx* c, mdff_terms* t) {
+  const int n = c->n;
+  const size_t n3 = 3 * (size_t)n;
+  const mdff_desc& d = c->d;
+  if (!c->kO_ws) HIP_TRY(c, hipMalloc((void**)&c->kO_ws, sizeof(double) * 11 * n3));
+  double* W = c->kO_ws;
+  double* find = W;
+  double* dmu = W + n3;
+  double* fprev = W + 2 * n3;
+  double* gmin = W + 3 * n3;
+  double* mprev = W + 4 * n3;
+  double* dmin = W + 5 * n3;
+  double* dd = W + 6 * n3;
+  double* const kw[4] = {W + 7 * n3, W + 8 * n3, W + 9 * n3, W + 10 * n3};   // kw[3]: zeros
+  hipLaunchKernelGGL(k_fill, dim3(div_up(11 * n3, 256)), dim3(256), 0, c->stream, 11 * n3, 0.0, W);
+  MDFF_TRY(ewald_field(c, true, nullptr, c->e_stat));
+  HIP_TRY(c, hipMemcpyAsync(c->e_tot, c->e_stat, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(find, c->e_stat, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_fill, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, 0.0, c->mu_ind);
+  int ext;
+  if (c->itime >= d.extrapolate_order + 1) ext = d.extrapolate_order;
+  else if (c->itime == 1 || c->itime == 0) ext = 0;
+  else ext = c->itime - 1;
+  double Bc[5] = {0, 0, 0, 0, 0}, w = 0.0;
+  switch (ext) {
+    case 0: Bc[0] = 1.0; w = 0.0; break;
+    case 1: Bc[0] = 2.0; Bc[1] = -1.0; w = 2.0 / 3.0; break;
+    case 2: Bc[0] = 2.5; Bc[1] = -2.0; Bc[2] = 0.5; w = 0.6; break;
+    case 3: Bc[0] = 2.8; Bc[1] = -2.8; Bc[2] = 1.2; Bc[3] = -0.2; w = 4.0 / 7.0; break;
+    default: Bc[0] = 3.0; Bc[1] = -24.0 / 7.0; Bc[2] = 27.0 / 14.0; Bc[3] = -4.0 / 7.0; Bc[4] = 1.0 / 14.0; w = 5.0 / 9.0; break;
+  }
+  int iscf = 0;
+  double rmsd = 1.79769313486231571e308, upol = 0.0;
+  while ((iscf < d.max_scf_pol_iter && rmsd > d.conv_tol_ind) || iscf < d.min_scf_pol_iter) {
+    iscf++;
+    if (iscf != 1 || !d.calc_md) {
+      MDFF_TRY(kO_inner(c, iscf <= 2 ? find : gmin, dmu, kw));
+      hipLaunchKernelGGL(k_add3, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, c->mu_ind, dmu, c->mu_ind);
+    } else {
+      hipLaunchKernelGGL(k_aspc_predict, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, ext, Bc[0], Bc[1],
+                         Bc[2], Bc[3], Bc[4], c->hist, c->mu_ind);
+    }
+    hipLaunchKernelGGL(k_upol, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->invpol, c->mu_ind, extra(c));
+    double h[2];
+    MDFF_TRY(reduce_cols(c, extra(c), 1, h));
+    upol = 0.5 * h[0];
+    MDFF_TRY(ewald_field(c, false, c->mu_ind, c->e_ind));
+    hipLaunchKernelGGL(k_add3, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, c->e_stat, c->e_ind, c->e_tot);
+    if (iscf == 1 && d.calc_md)
+      hipLaunchKernelGGL(k_aspc_correct, dim3(nblk(n)), dim3(256), 0, c->stream, n, w, c->pol, c->e_tot, c->mu_ind);
+    hipLaunchKernelGGL(k_kO_resid, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->polar, c->pol, c->e_tot,
+                       c->mu_ind, find);
+    if (iscf != 1) {
+      hipLaunchKernelGGL(k_axpby, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, 1.0, c->mu_ind, -1.0, mprev,
+                         dd);
+      hipLaunchKernelGGL(k_kO_dots, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->polar, fprev, find, extra(c));
+      MDFF_TRY(reduce_cols(c, extra(c), 2, h));
+      double al = -h[0] / h[1];
+      if (fabs(h[1]) < 1e-10) al = 1.0;
+      hipLaunchKernelGGL(k_kO_step, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, al, fprev, find, dd, gmin,
+                         dmin);
+      hipLaunchKernelGGL(k_add3, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, mprev, dmin, c->mu_ind);
+      MDFF_TRY(rmsd_of(c, gmin, kw[3], &rmsd));
+    }
+    HIP_TRY(c, hipMemcpyAsync(mprev, c->mu_ind, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(fprev, gmin, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIP_TRY(c, hipMemcpyAsync(c->hist, c->mu_ind, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  t->u_pol = upol;
+  t->n_scf = iscf;
+  t->rmsd = rmsd;
+  return MDFF_OK;
+}
+
 static void ev_rec(mdff_ctx* c, int k) { (void)hipEventRecord(c->ev[k], c->stream); }
 
 // SUBROUTINE engforce (src/engforce_driver.f90:102-190), launch half: every
@@ -606,7 +772,8 @@ static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t) {
   // induced dipoles (get_dipole_moments, src/moment_from_pola.f90:84-224)
   const size_t n3 = 3 * (size_t)n;
   if (coul && c->linduced) {
-    MDFF_TRY(scf(c, t));
+    if (d.algo_moment_from_pola == 1) MDFF_TRY(scf_kO_v2(c, t));
+    else MDFF_TRY(scf(c, t));
     hipLaunchKernelGGL(k_add3, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, c->dipia, c->mu_ind, c->mu);
   } else {
     HIP_TRY(c, hipMemcpyAsync(c->mu, c->dipia, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
@@ -763,6 +930,8 @@ void mdff_desc_defaults(mdff_desc* d) {
   d->conv_tol_ind = 1e-6;
   d->min_scf_pol_iter = 3;
   d->max_scf_pol_iter = 100;
+  d->algo_moment_from_pola = 0;
+  d->omegakO = 0.7;
 }
 
 const char* mdff_version(void) { return "mdff_gpu 1 (gfx950)"; }
@@ -807,6 +976,7 @@ int mdff_ctx_destroy(mdff_ctx* c) {
   if (c->h_flag) (void)hipHostFree(c->h_flag);
   if (c->h_red) (void)hipHostFree(c->h_red);
   if (c->h_raw) (void)hipHostFree(c->h_raw);
+  if (c->kO_ws) (void)hipFree(c->kO_ws);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->evpool)

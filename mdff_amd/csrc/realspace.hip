@@ -415,8 +415,9 @@ int realspace_force_impl(mdff_ctx* c, double* fsr, double* fcl, double* ef, bool
   return MDFF_OK;
 }
 
-int realspace_field(mdff_ctx* c, int mode, const double* mu_in, double* ef_out) {
+int realspace_field(mdff_ctx* c, int mode, const double* mu_in, double* ef_out, double alpha) {
   PairArgs a = make_args(c);
+  if (alpha > 0.0) a.alpha = alpha;   // alphaES of this call only (scf_kO_v2 inner loop)
   a.mu = mu_in;
   a.ef = ef_out;
   a.do_field = 1;

@@ -579,6 +579,12 @@ __global__ void __launch_bounds__(256) k_sk_mma2(Sm2Args a) {
     zs[rt] = part == 0 ? (kk == 0 ? T(1) : -sgn_im) : (kk == 0 ? sgn_im : T(1));
   }
   const bool take_im = (col & 1) ^ kk;                           // which Z component feeds A
+  int zofs[RT];                                                  // T offset of the A value in a staged row
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt) {
+    zofs[rt] = 2 * (zok[rt] ? zc[rt] : zcol) + (take_im ? 1 : 0);
+    if (!zok[rt]) zs[rt] = T(0);
+  }
   // B columns of this lane: m per m-tile
   const int mt0 = (blockIdx.x * (blockDim.x >> 6) + wave) * CTM;
   int bnx[CTM], bnyc[CTM];
@@ -637,31 +643,57 @@ __global__ void __launch_bounds__(256) k_sk_mma2(Sm2Args a) {
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
       for (int j = 0; j < CTM; ++j) acc[rt][j] = acc_t{T(0), T(0), T(0), T(0)};
+    // K steps of 4: atoms 2s and 2s+1 (Re/Im of P).  Rows beyond nt are the
+    // zero padding of the staged tile, so no step needs a mask.  The LDS
+    // operands of step s+1 are loaded while the MFMAs of step s run.
     const int nsteps = (nt + 1) >> 1;
-    for (int s = 0; s < nsteps; ++s) {
+    const T* stT = reinterpret_cast<const T*>(st);
+    const int rstride = 2 * ncol;
+    T zn[RT], xrn[CTM], xin[CTM], yrn[CTM], yin[CTM], c0n, c1n, c2n;
+    auto load_step = [&](int s) {
       const int ia = 2 * s + ja;
-      const bool aok = ia < nt;
-      const cplx<T>* row = st + (size_t)(aok ? ia : 0) * ncol;
+      const T* rowT = stT + (size_t)ia * rstride;
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) zn[rt] = rowT[zofs[rt]];
+#pragma unroll
+      for (int j = 0; j < CTM; ++j) {
+        xrn[j] = rowT[2 * bnx[j]];
+        xin[j] = rowT[2 * bnx[j] + 1];
+        yrn[j] = rowT[2 * bnyc[j]];
+        yin[j] = rowT[2 * bnyc[j] + 1];
+      }
+      c0n = sq[3 * ia];
+      c1n = sq[3 * ia + 1];
+      c2n = sq[3 * ia + 2];
+    };
+    load_step(0);
+    for (int s = 0; s < nsteps; ++s) {
+      T zv[RT], xr[CTM], xi[CTM], yr[CTM], yi[CTM];
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) zv[rt] = zn[rt];
+#pragma unroll
+      for (int j = 0; j < CTM; ++j) {
+        xr[j] = xrn[j];
+        xi[j] = xin[j];
+        yr[j] = yrn[j];
+        yi[j] = yin[j];
+      }
+      const T c0 = c0n, c1 = c1n, c2 = c2n;
+      if (s + 1 < nsteps) load_step(s + 1);
       T av[RT];
 #pragma unroll
-      for (int rt = 0; rt < RT; ++rt) {
-        const cplx<T> z = row[zc[rt]];
-        const T v = take_im ? z.im : z.re;
-        av[rt] = (zok[rt] && aok) ? zs[rt] * v : T(0);
-      }
-      const T c0 = sq[3 * (aok ? ia : 0)], c1 = sq[3 * (aok ? ia : 0) + 1], c2 = sq[3 * (aok ? ia : 0) + 2];
+      for (int rt = 0; rt < RT; ++rt) av[rt] = zs[rt] * zv[rt];
       T bv[CTM];
 #pragma unroll
       for (int j = 0; j < CTM; ++j) {
-        const cplx<T> X = row[bnx[j]];
-        cplx<T> Y = row[bnyc[j]];
-        if (bneg[j]) Y.im = -Y.im;
-        const cplx<T> XY = cmul(X, Y);
+        const T yim = bneg[j] ? -yi[j] : yi[j];
+        const T xyr = xr[j] * yr[j] - xi[j] * yim;
+        const T xyi = xr[j] * yim + xi[j] * yr[j];
         // (c0 + i bb) XY, bb = 2pi (nx mx + ny my) in mode 1
         const T bb = (a.mode == 1) ? tp * (bnxf[j] * c1 + bnyf[j] * c2) : T(0);
-        const T pre = c0 * XY.re - bb * XY.im;
-        const T pim = c0 * XY.im + bb * XY.re;
-        bv[j] = (bok[j] && aok) ? (kk == 0 ? pre : pim) : T(0);
+        const T pre = c0 * xyr - bb * xyi;
+        const T pim = c0 * xyi + bb * xyr;
+        bv[j] = bok[j] ? (kk == 0 ? pre : pim) : T(0);
       }
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
@@ -783,8 +815,29 @@ int recip_setup(mdff_ctx* c) {
   c->mma_nks = mma_bucket(Kz + 1);
   c->mma_mtiles = div_up(M, 4);
   c->mma_ct = mma_ct(c->d.precision == MDFF_PREC_MIXED, c->mma_nks);
+  // splits over m-tiles: the gather runs one wave per SIMD, so pick the split
+  // count whose wave total fills whole rounds of 4 x CUs waves best (at least
+  // 4 m-tiles per wave; fewer splits win ties)
   const int atom_waves = div_up(c->n, 16 * c->mma_ct);
-  c->mma_splits = std::max(1, std::min(c->mma_mtiles, div_up(2048, atom_waves)));
+  int cus = 256;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0)
+    cus = 256;
+  const int64_t conc = 4 * (int64_t)cus;
+  int best_s = 1;
+  double best_e = -1.0;
+  for (int sp = 1; sp <= std::min(c->mma_mtiles, 64); ++sp) {
+    if (sp > 1 && c->mma_mtiles / sp < 4) break;
+    const int mps = div_up(c->mma_mtiles, sp);
+    const int nsp = div_up(c->mma_mtiles, mps);
+    const int64_t waves = (int64_t)atom_waves * nsp;
+    const int64_t rounds = (waves + conc - 1) / conc;
+    const double e = (double)waves / (double)(rounds * conc);
+    if (e > best_e + 0.02) {
+      best_e = e;
+      best_s = nsp;
+    }
+  }
+  c->mma_splits = best_s;
   HIP_TRY(c, hipMalloc(&c->amat, tsz / 2 * (size_t)c->mma_mtiles * c->mma_nks * 64));
   splits = std::max(splits, c->mma_splits);
   c->rec_splits = splits;
@@ -834,6 +887,13 @@ static int launch_gather_mma(mdff_ctx* c, const GmArgs& gm, int ct, dim3 grid) {
   return MDFF_OK;
 }
 
+// m-tiles per wave of the structure-factor kernel (1 or 2)
+template <typename T>
+static void launch_sk_mma2(int ctm, dim3 grid, size_t lds, hipStream_t st, const Sm2Args& sm) {
+  if (ctm == 1) hipLaunchKernelGGL((k_sk_mma2<T, 1>), grid, dim3(256), lds, st, sm);
+  else hipLaunchKernelGGL((k_sk_mma2<T, 2>), grid, dim3(256), lds, st, sm);
+}
+
 template <typename T>
 static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bool use_mu, bool forces,
                   bool field, double* ef_out, double* f_out, double* e_out) {
@@ -871,7 +931,7 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     sm.Kx = c->K[0]; sm.Ky = c->K[1]; sm.Kz = c->K[2];
     sm.NY = c->NYk; sm.NZ = c->NZk; sm.M = s.M;
     sm.mtiles = div_up(s.M, 16);
-    constexpr int CTM = 2;
+    const int CTM = c->sk_ctm;
     const int mblocks = div_up(sm.mtiles, 4 * CTM);
     int ch = std::max(1, std::min(c->sk_chunks, div_up(c->n, 512)));
     sm.chunk = div_up(c->n, ch);
@@ -892,13 +952,13 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     // off, plus the x/y dipole phase); second pass for dipoles: mode 2
     sm.mode = use_mu ? 1 : 0;
     if (!use_q) sm.q = c->zeros;
-    PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_mma2<T, CTM>), gs, dim3(256), lds(TA), c->stream, sm));
+    PROF_LAUNCH(c, PK_SK, launch_sk_mma2<T>(CTM, gs, lds(TA), c->stream, sm));
     if (use_mu) {
       Sm2Args s2 = sm;
       s2.mode = 2;
       s2.part = p1 + (size_t)2 * ch * c->nk_box;
       const size_t tot = (size_t)ch * c->nk_box;
-      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_mma2<T, CTM>), gs, dim3(256), lds(TA), c->stream, s2);
+      PROF_LAUNCH(c, PK_SK, launch_sk_mma2<T>(CTM, gs, lds(TA), c->stream, s2);
                   hipLaunchKernelGGL(k_sk_combine, dim3(div_up(tot, 256)), dim3(256), 0, c->stream, c->nk_box,
                                      c->NZk, c->K[2], ch, p1, s2.part));
     }

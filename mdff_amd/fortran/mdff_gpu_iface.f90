@@ -17,7 +17,7 @@ module mdff_gpu
 
   integer, parameter, public :: dp = c_double
   integer(c_int32_t), parameter, public :: MDFF_NTYPEMAX = 16
-  integer(c_int32_t), parameter, public :: MDFF_ABI_VERSION = 1
+  integer(c_int32_t), parameter, public :: MDFF_ABI_VERSION = 2
 
   integer(c_int), parameter, public :: MDFF_OK = 0, MDFF_ERR_BAD_ARG = 1, MDFF_ERR_LIST_OVERFLOW = 2, &
        MDFF_ERR_SCF_NOT_CONVERGED = 3, MDFF_ERR_POLAR_CATASTROPHE = 4, MDFF_ERR_HIP = 5, &
@@ -48,7 +48,9 @@ module mdff_gpu
     real(c_double)     :: pol_damp_b(4096), pol_damp_c(4096)
     integer(c_int32_t) :: pol_damp_k(4096)
     real(c_double)     :: conv_tol_ind
-    integer(c_int32_t) :: min_scf_pol_iter, max_scf_pol_iter, extrapolate_order, pad1
+    integer(c_int32_t) :: min_scf_pol_iter, max_scf_pol_iter, extrapolate_order
+    integer(c_int32_t) :: algo_moment_from_pola
+    real(c_double)     :: omegakO
   end type mdff_desc
 
   ! struct mdff_terms

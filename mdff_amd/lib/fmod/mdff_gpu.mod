@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:7b86cb381fa88bd9
+﻿!mod$ v1 sum:37af09ba10df4b1c
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins
 module mdff_gpu
@@ -156,7 +156,7 @@ private::c_f_procpointer
 private::iso_c_binding$__fortran_builtins$c_associated_c_ptr
 integer(4),parameter::dp=8_4
 integer(4),parameter::mdff_ntypemax=16_4
-integer(4),parameter::mdff_abi_version=1_4
+integer(4),parameter::mdff_abi_version=2_4
 integer(4),parameter::mdff_ok=0_4
 integer(4),parameter::mdff_err_bad_arg=1_4
 integer(4),parameter::mdff_err_list_overflow=2_4
@@ -219,7 +219,8 @@ real(8)::conv_tol_ind
 integer(4)::min_scf_pol_iter
 integer(4)::max_scf_pol_iter
 integer(4)::extrapolate_order
-integer(4)::pad1
+integer(4)::algo_moment_from_pola
+real(8)::omegako
 end type
 type,bind(c)::mdff_terms
 real(8)::u_lj

@@ -1,0 +1,52 @@
+"""Short run of the a-SiO2 24,000-ion workload (the bench configuration) for
+profiling with rocprofv3: steepest-descent relaxation, then `--steps`
+velocity-Verlet steps of one precision form, then a device-to-device copy of
+`--copy-mb` MiB whose counter readings calibrate FETCH_SIZE / WRITE_SIZE
+(the copy reads and writes exactly that many bytes).
+
+    python -m mdff_amd.profile_workload --steps 20 --precision mixed
+"""
+from __future__ import annotations
+
+import argparse
+import copy
+import sys
+
+from . import capi, configs
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--natm", type=int, default=24000)
+    ap.add_argument("--relax", type=int, default=150)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--copy-mb", type=int, default=256)
+    args = ap.parse_args(argv)
+    s = configs.sio2(natm=args.natm)
+    if args.relax > 0:
+        eng = capi.gpu_engine(copy.deepcopy(s.desc), s.itype)
+        s.pos, _ = configs.relax_sd(eng, s.pos, iters=args.relax)
+        eng.close()
+    d = copy.deepcopy(s.desc)
+    d.precision = capi.PREC_MIXED if args.precision == "mixed" else capi.PREC_FP64
+    eng = capi.gpu_engine(d, s.itype)
+    eng.set_positions(s.pos)
+    eng.set_velocities(s.vel)
+    eng.engforce()
+    eng.md_vv(args.steps, s.dt)
+    eng.close()
+    print(f"profile_workload: {args.steps} {args.precision} VV steps on {s.natm} ions done", flush=True)
+    if args.copy_mb > 0:
+        import torch
+        n = args.copy_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"profile_workload: calibration copy of {args.copy_mb} MiB done", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

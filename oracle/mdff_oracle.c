@@ -627,6 +627,8 @@ typedef struct {
   int qq, qd, dd;      /* task(1:3) */
   int damp;            /* damp_ind */
   int efield, forces, stress;
+  int norec;           /* do_rec = .false. */
+  double alpha;        /* alphaES for this call when > 0 (the scf_kO_v2 inner loop) */
 } es_task;
 
 /* multipole_ES_dir (src/ewald_dir.f90:40-798) restricted to charges and
@@ -637,7 +639,7 @@ static void ewald_dir(orc_ctx *c, const double *s, const double *q, const double
   const mdff_desc *d = &c->d;
   int n = c->n;
   double cutsq = c->rc_coul * c->rc_coul;
-  double al = c->alpha, a2 = al * al, a3 = a2 * al;
+  double al = tk.alpha > 0.0 ? tk.alpha : c->alpha, a2 = al * al, a3 = a2 * al;
   double piroot = sqrt(PI);
   double u = 0.0, tt[9] = {0};
   for (int i = 0; i < n; ++i) {
@@ -891,11 +893,13 @@ static void multipole_es(orc_ctx *c, const double *s, const double *q, const dou
     qsq = qsq + q[i] * q[i];
   }
   double tpi_V = 2.0 * PI / c->omega, tpi_3V = tpi_V / 3.0, fpi_3V = tpi_3V * 2.0;
-  double alpha2 = c->alpha * c->alpha;
-  double selfa = c->alpha / sqrt(PI);
+  const double al = tk.alpha > 0.0 ? tk.alpha : c->alpha;
+  double alpha2 = al * al;
+  double selfa = al / sqrt(PI);
   double selfa2 = 2.0 * selfa * alpha2 / 3.0;
   ewald_dir(c, s, q, mu, tk, &o->u_dir, ef_dir, f_dir, tau_dir);
-  ewald_rec(c, q, mu, tk, &o->u_rec, ef_rec, f_rec, tau_rec);
+  o->u_rec = 0.0;
+  if (!tk.norec) ewald_rec(c, q, mu, tk, &o->u_rec, ef_rec, f_rec, tau_rec);
   double uq = qtot[0] * qtot[0] + qtot[1] * qtot[1] + qtot[2] * qtot[2];
   double ud = mutot[0] * mutot[0] + mutot[1] * mutot[1] + mutot[2] * mutot[2];
   double uqd = 2.0 * (qtot[0] * mutot[0] + qtot[1] * mutot[1] + qtot[2] * mutot[2]);
@@ -1060,6 +1064,110 @@ static void scf(orc_ctx *c, const double *s, double *mu_ind, double *u_pol, int 
   free(Es); free(Ei); free(E); free(zq);
 }
 
+/* induced_moment_inner (src/moment_from_pola.f90:822-942): damped fixed-point
+ * iteration mu <- (1-w) mu + w alpha (f_ext + E_dir(mu)) with a real-space
+ * dipole field at alphaES = 0.001 (no reciprocal part), at most 10 steps,
+ * until the residual falls below a tenth of |f_ext|. */
+static void kO_inner(orc_ctx *c, const double *s, const double *fext, double *mu) {
+  int n = c->n;
+  size_t N3 = 3 * (size_t)n;
+  double *zero = dalloc(N3), *f = dalloc(N3), *ft = dalloc(N3), *mprev = dalloc(N3), *zq = dalloc(n);
+  double w = c->d.omegakO;
+  double rmsd_ref = rmsd_mu(c, fext, zero);
+  induced_moment(c, fext, mu);
+  for (size_t x = 0; x < N3; ++x) mu[x] = w * mu[x];
+  double rmsd_inner = 1.79769313486231571e308;
+  es_task ti = {0, 0, 1, 0, 1, 0, 0, 1, 0.001};
+  es_out o;
+  for (int it = 1; it <= 10 && rmsd_inner > 0.1 * rmsd_ref; ++it) {
+    memcpy(mprev, mu, sizeof(double) * N3);
+    multipole_es(c, s, zq, mu, ti, f, &o);
+    for (size_t x = 0; x < N3; ++x) ft[x] = fext[x] + f[x];
+    rmsd_inner = rmsd_mu(c, ft, mu);
+    induced_moment(c, ft, mu);
+    for (size_t x = 0; x < N3; ++x) mu[x] = (1.0 - w) * mprev[x] + w * mu[x];
+  }
+  free(zero); free(f); free(ft); free(mprev); free(zq);
+}
+
+/* find_min (src/moment_from_pola.f90:2415-2455): line minimum between two
+ * residuals g0, g1 along the step d1. */
+static void kO_find_min(orc_ctx *c, const double *g0, const double *g1, const double *d1, double *gmin,
+                        double *dmin) {
+  int n = c->n;
+  double g0sq = 0.0, g0g1mg0 = 0.0;
+  for (int i = 0; i < n; ++i) {
+    if (!c->d.ldip_polar[c->itype[i] - 1]) continue;
+    for (int k = 0; k < 3; ++k) {
+      double a = g0[3 * i + k];
+      g0sq = g0sq + a * a;
+      g0g1mg0 = g0g1mg0 + a * (g1[3 * i + k] - a);
+    }
+  }
+  double al = -g0sq / g0g1mg0;
+  if (fabs(g0g1mg0) < 1e-10) al = 1.0;
+  for (size_t x = 0; x < 3 * (size_t)n; ++x) {
+    gmin[x] = g0[x] + al * (g1[x] - g0[x]);
+    dmin[x] = al * d1[x];
+  }
+}
+
+/* moment_from_pola_scf_kO_v2 (src/moment_from_pola.f90:1335-1639) */
+static void scf_kO_v2(orc_ctx *c, const double *s, double *mu_ind, double *u_pol, int *n_scf, double *rmsd_out) {
+  int n = c->n;
+  size_t N3 = 3 * (size_t)n;
+  const mdff_desc *d = &c->d;
+  double *Es = dalloc(N3), *Ei = dalloc(N3), *E = dalloc(N3), *zq = dalloc(n);
+  double *find = dalloc(N3), *dmu = dalloc(N3), *fprev = dalloc(N3), *gmin = dalloc(N3), *mprev = dalloc(N3),
+         *dmin = dalloc(N3), *zero = dalloc(N3), *dd = dalloc(N3);
+  int ldip = 0;
+  for (int i = 0; i < 48; ++i) if (d->dip[i] != 0.0) ldip = 1;
+  es_task ts = {1, ldip, ldip, 1, 1, 0, 0, 0, 0.0};
+  es_out o;
+  multipole_es(c, s, c->q, c->dipia, ts, Es, &o);
+  memcpy(E, Es, sizeof(double) * N3);
+  memcpy(find, E, sizeof(double) * N3);
+  memset(mu_ind, 0, sizeof(double) * N3);
+  es_task ti = {0, 0, 1, 0, 1, 0, 0, 0, 0.0};
+  int iscf = 0;
+  double rmsd = 1.79769313486231571e308, upol = 0.0;
+  while ((iscf < d->max_scf_pol_iter && rmsd > d->conv_tol_ind) || iscf < d->min_scf_pol_iter) {
+    iscf++;
+    if (iscf != 1 || !d->calc_md) {
+      kO_inner(c, s, iscf <= 2 ? find : gmin, dmu);
+      for (size_t x = 0; x < N3; ++x) mu_ind[x] = mu_ind[x] + dmu[x];
+    } else {
+      aspc(c, mu_ind, E, 1);
+    }
+    upol = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) upol = upol + mu_ind[3 * i + a] * c->invpol[9 * i + a + 3 * b] * mu_ind[3 * i + b];
+    upol = upol * 0.5;
+    multipole_es(c, s, zq, mu_ind, ti, Ei, &o);
+    for (size_t x = 0; x < N3; ++x) E[x] = Es[x] + Ei[x];
+    if (iscf == 1 && d->calc_md) aspc(c, mu_ind, E, 2);
+    for (int i = 0; i < n; ++i) {
+      if (!d->ldip_polar[c->itype[i] - 1]) continue;
+      for (int a = 0; a < 3; ++a) find[3 * i + a] = E[3 * i + a] - mu_ind[3 * i + a] / c->pol[9 * i + a + 3 * a];
+    }
+    if (iscf != 1) {
+      for (size_t x = 0; x < N3; ++x) dd[x] = mu_ind[x] - mprev[x];
+      kO_find_min(c, fprev, find, dd, gmin, dmin);
+      for (size_t x = 0; x < N3; ++x) mu_ind[x] = mprev[x] + dmin[x];
+      rmsd = rmsd_mu(c, gmin, zero);
+    }
+    memcpy(mprev, mu_ind, sizeof(double) * N3);
+    memcpy(fprev, gmin, sizeof(double) * N3);
+  }
+  memcpy(&c->hist[0], mu_ind, sizeof(double) * N3);
+  *u_pol = upol;
+  *n_scf = iscf;
+  *rmsd_out = rmsd;
+  free(Es); free(Ei); free(E); free(zq); free(find); free(dmu); free(fprev); free(gmin); free(mprev);
+  free(dmin); free(zero); free(dd);
+}
+
 /* ------------------------------------------------------------------------ */
 /* engforce                                                                  */
 /* ------------------------------------------------------------------------ */
@@ -1101,7 +1209,8 @@ int orc_engforce(orc_ctx *c, double *fx, double *fy, double *fz, mdff_terms *t) 
     if (c->linduced) {
       double *fsx = dalloc(n), *fsy = dalloc(n), *fsz = dalloc(n);
       memcpy(fsx, c->fx, sizeof(double) * n); memcpy(fsy, c->fy, sizeof(double) * n); memcpy(fsz, c->fz, sizeof(double) * n);
-      scf(c, s, mu_ind, &upol, &tm.n_scf, &tm.rmsd);
+      if (d->algo_moment_from_pola == 1) scf_kO_v2(c, s, mu_ind, &upol, &tm.n_scf, &tm.rmsd);
+      else scf(c, s, mu_ind, &upol, &tm.n_scf, &tm.rmsd);
       memcpy(c->fx, fsx, sizeof(double) * n); memcpy(c->fy, fsy, sizeof(double) * n); memcpy(c->fz, fsz, sizeof(double) * n);
       free(fsx); free(fsy); free(fsz);
     }

@@ -168,6 +168,28 @@ def _pim(cells=(2, 2, 3), **kw):
 
 
 @pytest.mark.parametrize("calc_md", [0, 1])
+def test_pim_scf_kO_v2_fp64_parity(calc_md):
+    """moment_from_pola_scf_kO_v2 (src/moment_from_pola.f90:1335-1639): inner
+    loop at alphaES 0.001, full-Ewald induced field, line minimum; same
+    iteration count, dipoles, forces and energies as the oracle."""
+    s = _pim(tol=1e-8)
+    note(f"pim scf_kO_v2 calc_md={calc_md} n={s.natm}")
+    g, o = pair(s, calc_md=calc_md, doefield=1, algo_moment_from_pola=1, extrapolate_order=2)
+    for it in range(3):
+        g.set_itime(it)
+        o.set_itime(it)
+        fg, tg = g.engforce()
+        fo, to = o.engforce()
+        assert tg["n_scf"] == to["n_scf"]
+        assert rel_force_err(fg, fo) < TOL64
+        bad = compare_terms(tg, to, TOL64)
+        assert not bad, bad
+        mg, _ = g.get_dipoles()
+        mo, _ = o.get_dipoles()
+        assert rel(mg, mo) < TOL64
+
+
+@pytest.mark.parametrize("calc_md", [0, 1])
 def test_pim_scf_fp64_parity(calc_md):
     """moment_from_pola_scf (src/moment_from_pola.f90:240-485): same
     iteration count, dipoles, fields, forces and energies."""

@@ -356,6 +356,27 @@ def test_scf_dipoles_solve_the_linear_system():
     assert t["n_scf"] >= 3
 
 
+def test_scf_kO_v2_reaches_the_same_fixed_point():
+    """moment_from_pola_scf_kO_v2 (src/moment_from_pola.f90:1335-1639, inner
+    loop :822-942, line minimum :2415-2455) solves the same equation as the
+    plain SCF: at a tight tolerance both give the same dipoles, energy and
+    forces; the inner iteration and the line search change only the path
+    (it takes fewer outer iterations than the plain fixed point here)."""
+    s = _pim_small()
+    s.desc.calc_md = 0
+    e1, f1, t1 = _energy(s)
+    mu1, _ = e1.get_dipoles()
+    s2 = _pim_small()
+    s2.desc.calc_md = 0
+    s2.desc.algo_moment_from_pola = 1
+    e2, f2, t2 = _energy(s2)
+    mu2, _ = e2.get_dipoles()
+    assert np.allclose(mu2, mu1, atol=1e-9)
+    assert np.allclose(f2, f1, atol=1e-8 * np.abs(f1).max())
+    assert abs(t2["u_coul"] - t1["u_coul"]) < 1e-9 * abs(t1["u_coul"])
+    assert 2 <= t2["n_scf"] < t1["n_scf"]
+
+
 # ---------------------------------------------------------------- integrators
 
 def test_velocity_verlet_conserves_energy():
