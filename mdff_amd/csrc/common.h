@@ -97,6 +97,12 @@ constexpr int RAW_SCRATCH = 32;  // 7 moments of the SCF field (slot 0 only)
 constexpr int RAW_N = 40;
 constexpr int RAW_SLOTS = 1024;  // evaluations held before the sums go to the host
 
+// one atom of the pair kernels, in cell-sorted order: fractional coordinates
+// and charge (two 16-byte loads), plus its type and original index
+struct alignas(16) AtomRec {
+  double x, y, z, q;
+};
+
 struct mdff_ctx {
   mdff_desc d;
   int n = 0, ntype = 0;
@@ -186,7 +192,9 @@ struct mdff_ctx {
   int mma_nks = 0, mma_mtiles = 0, mma_ct = 0, mma_splits = 1;
   int use_mma = 1;                        // matrix-core reciprocal kernels
   int sk_ctm = 2;                         // m-tiles per wave of the structure factor
-  double* kO_ws = nullptr;                // scf_kO_v2 work vectors (11 x 3n)
+  double* kO_ws = nullptr;
+  AtomRec* arec = nullptr;                // atoms in cell-sorted order (refreshed every evaluation)
+  int2* ameta = nullptr;                  // (type, original index) in the same order                // scf_kO_v2 work vectors (11 x 3n)
 
   double last_ms[8] = {0};
   int n_last = 0;
@@ -239,6 +247,7 @@ int nl_check(mdff_ctx* c, int* rebuilt);
 int reduce_cols_dev(mdff_ctx* c, const double* in, int ncol, double* dst);
 int nl_get_pairs(mdff_ctx* c, int which, int32_t* point, int32_t* list, int64_t cap, int64_t* n);
 void launch_kardir(mdff_ctx* c);
+void pack_atoms(mdff_ctx* c);
 int realspace_field(mdff_ctx* c, int mode, const double* mu_in, double* ef_out, double alpha = 0.0);
 int recip_setup(mdff_ctx* c);
 int recip_tables(mdff_ctx* c);

@@ -492,6 +492,8 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   DMALLOC(c->hist, 15 * (size_t)n);
   DMALLOC(c->frec, 3 * (size_t)n);
   DMALLOC(c->zeros, n);
+  DMALLOC(c->arec, n);
+  DMALLOC(c->ameta, n);
   // per-atom scratch: NPA columns + f_sr, f_dir, ef_dir, ef_rec (3n each) + 8 extra columns
   DMALLOC(c->peratom, (size_t)(NPA + 12 + 8) * n);
   DMALLOC(c->d_pp, c->pp.size());
@@ -766,6 +768,7 @@ static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t) {
   } else {
     launch_kardir(c);
   }
+  pack_atoms(c);
   ev_rec(c, 1);
   const bool coul = d.lcoulomb != 0;
   if (coul) MDFF_TRY(recip_tables(c));
@@ -966,7 +969,7 @@ int mdff_ctx_destroy(mdff_ctx* c) {
   void* ps[] = {c->rx, c->ry, c->rz, c->vx, c->vy, c->vz, c->fx, c->fy, c->fz, c->fsx, c->fsy, c->fsz,
                 c->sx, c->sy, c->sz, c->xs, c->ys, c->zs, c->bx, c->by, c->bz, c->itype, c->q, c->mass,
                 c->pol, c->invpol, c->dipia, c->mu, c->mu_ind, c->efield, c->e_stat, c->e_tot, c->e_ind,
-                c->hist, c->frec, c->zeros, c->peratom, c->red, c->red_part, c->d_pp, c->cell_of, c->cell_key,
+                c->hist, c->frec, c->zeros, c->arec, c->ameta, c->peratom, c->red, c->red_part, c->d_pp, c->cell_of, c->cell_key,
                 c->cell_atoms, c->cell_atoms_in, c->cell_start, c->nl_count, c->nl_start, c->nl_j,
                 c->nl_flag, c->sort_tmp, c->d_flag, c->d_disp};
   for (void* p : ps)

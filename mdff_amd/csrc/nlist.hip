@@ -42,6 +42,25 @@ void launch_kardir(mdff_ctx* c) {
                      c->rz, B[0], B[1], B[2], B[3], B[4], B[5], B[6], B[7], B[8], c->sx, c->sy, c->sz);
 }
 
+// atoms in the cell order of the last list build: rec[p], meta[p] of atom
+// order[p] (the rows store sorted positions p, so neighbours that are close
+// in space are close in memory)
+__global__ void k_pack_atoms(int n, const int32_t* __restrict__ order, const double* __restrict__ sx,
+                             const double* __restrict__ sy, const double* __restrict__ sz,
+                             const double* __restrict__ q, const int32_t* __restrict__ itype,
+                             AtomRec* __restrict__ rec, int2* __restrict__ meta) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i = order[p];
+  rec[p] = AtomRec{sx[i], sy[i], sz[i], q[i]};
+  meta[p] = make_int2(itype[i], i);
+}
+
+void pack_atoms(mdff_ctx* c) {
+  hipLaunchKernelGGL(k_pack_atoms, dim3(div_up(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cell_atoms, c->sx,
+                     c->sy, c->sz, c->q, c->itype, c->arec, c->ameta);
+}
+
 __device__ __forceinline__ int wrap_cell(double s, int nc) {
   double w = s - floor(s);
   int k = (int)(w * (double)nc);
@@ -147,7 +166,7 @@ __global__ void __launch_bounds__(256) k_nl_sweep(NLArgs a) {
             int off = found + __popcll(m & ((1ull << lane) - 1ull));
             int64_t e = base + off;
             if (e < a.cap) {
-              a.nl_j[e] = j;
+              a.nl_j[e] = p;                 // sorted position of j
               a.nl_flag[e] = flag;
             }
           }
@@ -343,10 +362,11 @@ int nl_check(mdff_ctx* c, int* rebuilt) {
 int nl_get_pairs(mdff_ctx* c, int which, int32_t* point, int32_t* list, int64_t cap, int64_t* nout) {
   if (!c->lists_valid) { c->err = "lists not built"; return MDFF_ERR_BAD_ARG; }
   const int n = c->n;
-  std::vector<int32_t> start(n + 1), rows(c->nl_total > 0 ? c->nl_total : 1);
+  std::vector<int32_t> start(n + 1), rows(c->nl_total > 0 ? c->nl_total : 1), order(n);
   std::vector<uint8_t> flags(c->nl_total > 0 ? c->nl_total : 1);
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   HIP_TRY(c, hipMemcpy(start.data(), c->nl_start, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(order.data(), c->cell_atoms, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
   if (c->nl_total > 0) {
     HIP_TRY(c, hipMemcpy(rows.data(), c->nl_j, sizeof(int32_t) * c->nl_total, hipMemcpyDeviceToHost));
     HIP_TRY(c, hipMemcpy(flags.data(), c->nl_flag, c->nl_total, hipMemcpyDeviceToHost));
@@ -360,7 +380,7 @@ int nl_get_pairs(mdff_ctx* c, int which, int32_t* point, int32_t* list, int64_t 
     if (point) point[i] = (int32_t)(cnt + 1);
     tmp.clear();
     for (int e = start[i]; e < start[i + 1]; ++e) {
-      int j = rows[e];
+      int j = order[rows[e]];             // rows hold cell-sorted positions
       bool own = (i > j && ((i + j) % 2 == 0)) || (i < j && ((i + j) % 2 != 0));
       if (own && (flags[e] & bit)) tmp.push_back(j + 1);
     }

@@ -27,13 +27,12 @@ struct PairArgs {
   double a11, a21, a31, a12, a22, a32, a13, a23, a33;
   double cutsq_c;                    // Coulomb cutoff^2 (d2 > cutsq cycles, src/ewald_dir.f90:202)
   double alpha;
-  const double *sx, *sy, *sz;
-  const int32_t* itype;
-  const double* q;
+  const AtomRec* rec;                // cell-sorted atoms: fractional coordinates, charge
+  const int2* meta;                  // (type, original index) in the same order
   const double* mu;                  // 3n AoS (may be null when no dipoles)
   const PairParam* pp;
   const int32_t* row;                // n + 1
-  const int32_t* nl_j;
+  const int32_t* nl_j;               // sorted positions of the neighbours
   double* fsr;                       // 3n AoS out
   double* fcl;                       // 3n AoS out
   double* ef;                        // 3n AoS out (field at i)
@@ -58,12 +57,15 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
   for (int k = threadIdx.x; k < a.ntype * a.ntype; k += blockDim.x) spp[k] = a.pp[k];
   __syncthreads();
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = gtid / LPA;
+  const int pi = gtid / LPA;             // atoms in cell order: neighbouring groups share neighbours
   const int sub = gtid % LPA;
-  if (i >= a.n) return;  // whole groups exit together
-  const double si = a.sx[i], ti = a.sy[i], ui = a.sz[i];
-  const int ita = a.itype[i];
-  const T qi = (T)a.q[i];
+  if (pi >= a.n) return;  // whole groups exit together
+  const AtomRec ri = a.rec[pi];
+  const int2 mi = a.meta[pi];
+  const int i = mi.y;
+  const double si = ri.x, ti = ri.y, ui = ri.z;
+  const int ita = mi.x;
+  const T qi = (T)ri.q;
   const T alpha = (T)a.alpha, a2 = alpha * alpha;
   const T isqpi = T(0.56418958354775628695);  // 1/sqrt(pi)
   V3<T> mui{T(0), T(0), T(0)};
@@ -77,8 +79,11 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
   double usr = 0, vsr = 0, ts[6] = {0, 0, 0, 0, 0, 0}, ucl = 0, tc[6] = {0, 0, 0, 0, 0, 0};
   const int r0 = a.row[i], r1 = a.row[i + 1];
   for (int e = r0 + sub; e < r1; e += LPA) {
-    const int j = a.nl_j[e];
-    double dxs = si - a.sx[j], dys = ti - a.sy[j], dzs = ui - a.sz[j];
+    const int pj = a.nl_j[e];
+    const AtomRec rj = a.rec[pj];
+    const int2 mj = a.meta[pj];
+    const int j = mj.y;
+    double dxs = si - rj.x, dys = ti - rj.y, dzs = ui - rj.z;
     dxs -= nint_d(dxs);
     dys -= nint_d(dys);
     dzs -= nint_d(dzs);
@@ -86,7 +91,7 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
     const double ryd = dxs * a.a21 + dys * a.a22 + dzs * a.a23;
     const double rzd = dxs * a.a31 + dys * a.a32 + dzs * a.a33;
     const double r2d = rxd * rxd + ryd * ryd + rzd * rzd;
-    const int jta = a.itype[j];
+    const int jta = mj.x;
     const PairParam& P = spp[ita + a.ntype * jta];
     const bool in_sr = a.sr_model != 0 && r2d < P.rcutsq && !(a.sr_model == 2 && P.A == 0.0);
     const bool in_cl = (a.qq | a.qd | a.dd) && !(r2d > a.cutsq_c);
@@ -141,7 +146,7 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
       ts[5] += (double)(ry * fz);
     }
     if (in_cl) {
-      const T qj = (T)a.q[j];
+      const T qj = (T)rj.q;
       const T expon = exp(-a2 * r2) * isqpi;
       const T F0 = errfc_pos<T>(alpha * d);
       const T F1 = F0 + T(2) * alpha * d * expon;
@@ -281,11 +286,14 @@ __global__ void __launch_bounds__(256) k_pair_field(PairArgs a) {
   for (int k = threadIdx.x; k < a.ntype * a.ntype; k += blockDim.x) spp[k] = a.pp[k];
   __syncthreads();
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = gtid / LPA;
+  const int pi = gtid / LPA;
   const int sub = gtid % LPA;
-  if (i >= a.n) return;
-  const double si = a.sx[i], ti = a.sy[i], ui = a.sz[i];
-  const int ita = a.itype[i];
+  if (pi >= a.n) return;
+  const AtomRec ri = a.rec[pi];
+  const int2 mi = a.meta[pi];
+  const int i = mi.y;
+  const double si = ri.x, ti = ri.y, ui = ri.z;
+  const int ita = mi.x;
   const T alpha = (T)a.alpha, a2 = alpha * alpha;
   const T isqpi = T(0.56418958354775628695);
   bool dip_i = true;
@@ -294,14 +302,17 @@ __global__ void __launch_bounds__(256) k_pair_field(PairArgs a) {
   const int r0 = a.row[i], r1 = a.row[i + 1];
   if (dip_i) {
     for (int e = r0 + sub; e < r1; e += LPA) {
-      const int j = a.nl_j[e];
+      const int pj = a.nl_j[e];
+      const AtomRec rj = a.rec[pj];
+      const int2 mj = a.meta[pj];
+      const int j = mj.y;
       T m0, m1, m2;
       if (MODE == 1) {
         double d0 = a.mu[3 * j], d1 = a.mu[3 * j + 1], d2 = a.mu[3 * j + 2];
         if (d0 == 0.0 && d1 == 0.0 && d2 == 0.0) continue;
         m0 = (T)d0; m1 = (T)d1; m2 = (T)d2;
       }
-      double dxs = si - a.sx[j], dys = ti - a.sy[j], dzs = ui - a.sz[j];
+      double dxs = si - rj.x, dys = ti - rj.y, dzs = ui - rj.z;
       dxs -= nint_d(dxs);
       dys -= nint_d(dys);
       dzs -= nint_d(dzs);
@@ -317,9 +328,9 @@ __global__ void __launch_bounds__(256) k_pair_field(PairArgs a) {
       const T F0 = errfc_pos<T>(alpha * d);
       const T F1 = F0 + T(2) * alpha * d * expon;
       if (MODE == 0) {
-        const T qj = (T)a.q[j];
+        const T qj = (T)rj.q;
         T g = qj * F1 * dm3;
-        const PairParam& P = spp[ita + a.ntype * a.itype[j]];
+        const PairParam& P = spp[ita + a.ntype * mj.x];
         if (P.ldamp) {
           T fdamp, fdd;
           tt_damp<T>((T)P.db_i, (T)P.dc_i, d, P.dk_i, fdamp, fdd);
@@ -362,9 +373,8 @@ PairArgs make_args(mdff_ctx* c) {
   a.a13 = A[6]; a.a23 = A[7]; a.a33 = A[8];
   a.cutsq_c = c->rc_coul * c->rc_coul;
   a.alpha = c->alpha;
-  a.sx = c->sx; a.sy = c->sy; a.sz = c->sz;
-  a.itype = c->itype;
-  a.q = c->q;
+  a.rec = c->arec;
+  a.meta = c->ameta;
   a.pp = c->d_pp;
   a.row = c->nl_start;
   a.nl_j = c->nl_j;
