@@ -24,7 +24,11 @@ _ip = _P(C.c_int32)
 
 
 def _lib() -> C.CDLL:
-    lib = capi.gpu_lib()
+    return bind_fit(capi.gpu_lib())
+
+
+def bind_fit(lib: C.CDLL) -> C.CDLL:
+    """Declare the mdff_fit_* signatures on a loaded library."""
     if not getattr(lib, "_fit_bound", False):
         lib.mdff_fit_create.argtypes = [_P(capi.MdffDesc), C.c_int32, _ip, _dp, _dp, _ip, _ip, _dp, C.c_int32, _ip,
                                         _P(C.c_void_p)]
@@ -59,9 +63,10 @@ class FitProblem:
     controls and the parameters that are not fitted); params: list of
     (kind, t1, t2) with kind in KINDS."""
 
-    def __init__(self, snapshots, params, scf_tol: float | None = None, max_iter: int | None = None):
+    def __init__(self, snapshots, params, scf_tol: float | None = None, max_iter: int | None = None,
+                 lib: C.CDLL | None = None):
         import copy
-        self.lib = _lib()
+        self.lib = bind_fit(lib) if lib is not None else _lib()
         d = copy.deepcopy(snapshots[0].desc)
         if scf_tol is not None:
             d.conv_tol_ind = scf_tol
