@@ -77,7 +77,7 @@ struct FitPair {
 };
 
 struct FitArgs {
-  int nsnap, ntype, want_grad, min_iter, max_iter;
+  int nsnap, ntype, want_grad, min_iter, max_iter, ncol_max;
   double tol, tol_adj, coul_unit, press_unit, w_stress;
   const FitSnap* snaps;
   const int* order;
@@ -194,40 +194,78 @@ __device__ __forceinline__ c2<T> phase(const c2<T>* tab, const FitSnap& S, int j
 // Reciprocal building blocks (VALU, thread per k or per atom)
 // ---------------------------------------------------------------------------
 
-// S(k) = sum_j amp_j(k) exp(i k.r_j), amp = (a0 + k.av)(b0 + i k.bv) with
-// mode 0: q; 1: i k.v (v = 3-column block); 2: q + i k.mu; 3: (k.g)(q + i k.mu)
+// S(k) = sum_j amp_j(k) exp(i k.r_j) with
+// mode 0: q; 1: i k.v (v = 3-column block); 2: q + i k.mu; 3: (k.g)(q + i k.mu).
+// One thread per k; the atoms' phase tables and amplitude data are staged in
+// LDS in chunks of FIT_CA atoms (coalesced loads, then every thread reads its
+// k's three factors from LDS).  Called by the whole workgroup (barriers).
+constexpr int FIT_CA = 32;
 template <typename T>
 __device__ void struct_factor(const FitArgs& a, const FitSnap& S, const c2<T>* tab, int mode, const double* v,
-                              const double* g, double* out /* [k] 2 doubles, stride FIT_KWS */) {
+                              const double* g, double* out /* [k] 2 doubles, stride FIT_KWS */, c2<T>* lt,
+                              double* la) {
   const double* q = a.q + S.a0;
-  for (int k = threadIdx.x; k < S.nk; k += blockDim.x) {
-    const int kg = S.k0 + k;
+  const int ncol = S.ncol;
+  for (int k0 = 0; k0 < S.nk; k0 += blockDim.x) {
+    const int k = k0 + threadIdx.x;
+    const bool kok = k < S.nk;
+    const int kg = S.k0 + (kok ? k : 0);
     const double kx = a.kv[5 * kg], ky = a.kv[5 * kg + 1], kz = a.kv[5 * kg + 2];
     const int nx = a.kn[3 * kg], ny = a.kn[3 * kg + 1], nz = a.kn[3 * kg + 2];
+    const int cy = S.Kx + 1 + (ny < 0 ? -ny : ny), cz = S.Kx + S.Ky + 2 + (nz < 0 ? -nz : nz);
+    const T sy = ny < 0 ? T(-1) : T(1), sz = nz < 0 ? T(-1) : T(1);
     double sr = 0.0, si = 0.0;
-    for (int j = 0; j < S.n; ++j) {
-      const c2<T> e = phase(tab, S, j, nx, ny, nz);
-      double ar = 0.0, ai = 0.0;
-      if (mode == 0) {
-        ar = q[j];
-      } else if (mode == 1) {
-        ai = kx * v[3 * j] + ky * v[3 * j + 1] + kz * v[3 * j + 2];
-      } else {
-        ar = q[j];
-        ai = kx * v[3 * j] + ky * v[3 * j + 1] + kz * v[3 * j + 2];
-        if (mode == 3) {
-          const double kg3 = kx * g[3 * j] + ky * g[3 * j + 1] + kz * g[3 * j + 2];
-          ar *= kg3;
-          ai *= kg3;
+    for (int j0 = 0; j0 < S.n; j0 += FIT_CA) {
+      const int nc = min(FIT_CA, S.n - j0);
+      __syncthreads();
+      for (int x = threadIdx.x; x < ncol * nc; x += blockDim.x) {
+        const int col = x / nc, jj = x % nc;
+        lt[jj * ncol + col] = tab[(size_t)col * S.n + j0 + jj];
+      }
+      for (int jj = threadIdx.x; jj < nc; jj += blockDim.x) {
+        const int j = j0 + jj;
+        la[7 * jj] = q[j];
+        for (int c = 0; c < 3; ++c) {
+          la[7 * jj + 1 + c] = v ? v[3 * j + c] : 0.0;
+          la[7 * jj + 4 + c] = g ? g[3 * j + c] : 0.0;
         }
       }
-      const T er = e.re, ei = e.im;
-      sr += (double)((T)ar * er - (T)ai * ei);
-      si += (double)((T)ar * ei + (T)ai * er);
+      __syncthreads();
+      if (kok) {
+        for (int jj = 0; jj < nc; ++jj) {
+          const c2<T>* row = lt + jj * ncol;
+          const c2<T> X = row[nx];
+          c2<T> Y = row[cy];
+          c2<T> Z = row[cz];
+          Y.im *= sy;
+          Z.im *= sz;
+          const c2<T> e = cm(cm(X, Y), Z);
+          const double* A = la + 7 * jj;
+          double ar = 0.0, ai = 0.0;
+          if (mode == 0) {
+            ar = A[0];
+          } else if (mode == 1) {
+            ai = kx * A[1] + ky * A[2] + kz * A[3];
+          } else {
+            ar = A[0];
+            ai = kx * A[1] + ky * A[2] + kz * A[3];
+            if (mode == 3) {
+              const double kg3 = kx * A[4] + ky * A[5] + kz * A[6];
+              ar *= kg3;
+              ai *= kg3;
+            }
+          }
+          sr += (double)((T)ar * e.re - (T)ai * e.im);
+          si += (double)((T)ar * e.im + (T)ai * e.re);
+        }
+      }
     }
-    out[(size_t)FIT_KWS * kg] = sr;
-    out[(size_t)FIT_KWS * kg + 1] = si;
+    if (kok) {
+      out[(size_t)FIT_KWS * (S.k0 + k)] = sr;
+      out[(size_t)FIT_KWS * (S.k0 + k) + 1] = si;
+    }
   }
+  __syncthreads();
 }
 
 // per atom j: im3 = sum_k k Im(C1_k e^{-ik r_j}), re3 = sum_k k Re(C2_k e^{-ik r_j}),
@@ -354,9 +392,10 @@ __device__ void field_dip_real(const FitArgs& a, const FitSnap& S, const double*
 // full dipole-field operator: out = T v (real + reciprocal + self), at
 // polarisable atoms; uses the S slot of the k scratch
 template <typename T>
-__device__ void dip_operator(const FitArgs& a, const FitSnap& S, const c2<T>* tab, const double* v, double* out) {
+__device__ void dip_operator(const FitArgs& a, const FitSnap& S, const c2<T>* tab, const double* v, double* out,
+                             c2<T>* lt, double* la) {
   field_dip_real<T>(a, S, v, out);
-  struct_factor<T>(a, S, tab, 1, v, nullptr, a.kws + K_S);
+  struct_factor<T>(a, S, tab, 1, v, nullptr, a.kws + K_S, lt, la);
   __syncthreads();
   for (int k = threadIdx.x; k < S.nk; k += blockDim.x) {
     double* K = a.kws + (size_t)FIT_KWS * (S.k0 + k);
@@ -573,7 +612,9 @@ __device__ void pair_pass(const FitArgs& a, const FitSnap& S, const double* mu, 
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_fit(FitArgs a) {
-  extern __shared__ double red[];
+  extern __shared__ double red[];                 // blockDim doubles, then staging
+  c2<T>* lt = reinterpret_cast<c2<T>*>(red + blockDim.x);            // FIT_CA x ncol_max
+  double* la = reinterpret_cast<double*>(lt + (size_t)FIT_CA * a.ncol_max);   // FIT_CA x 7
   const c2<T>* tab_all = reinterpret_cast<const c2<T>*>(a.tab);
   for (int w = blockIdx.x; w < a.nsnap; w += gridDim.x) {
     const int s = a.order[w];
@@ -595,7 +636,7 @@ __global__ void __launch_bounds__(256) k_fit(FitArgs a) {
 
     // ---- phase 1: static field E_s = real (damped) + reciprocal of the charges
     field_static<T>(a, S, vES, nullptr, nullptr);
-    struct_factor<T>(a, S, tab, 0, nullptr, nullptr, a.kws + K_SQ);
+    struct_factor<T>(a, S, tab, 0, nullptr, nullptr, a.kws + K_SQ, lt, la);
     __syncthreads();
     for (int k = threadIdx.x; k < S.nk; k += blockDim.x) {
       double* K = a.kws + (size_t)FIT_KWS * (S.k0 + k);
@@ -628,7 +669,7 @@ __global__ void __launch_bounds__(256) k_fit(FitArgs a) {
         for (int c = 0; c < 3; ++c) vMU[3 * j + c] = al * vET[3 * j + c];
       }
       __syncthreads();
-      dip_operator<T>(a, S, tab, vMU, vTMP);
+      dip_operator<T>(a, S, tab, vMU, vTMP, lt, la);
       double r2 = 0.0;
       for (int j = threadIdx.x; j < S.n; j += blockDim.x) {
         const double al = a.alpha_t[it[j]];
@@ -650,7 +691,7 @@ __global__ void __launch_bounds__(256) k_fit(FitArgs a) {
     double* Fs = a.F + 3 * (size_t)S.a0;
     const double Gdummy[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     pair_pass<T, false>(a, S, vMU, nullptr, Gdummy, Fs, nullptr, nullptr, stn, std_);
-    struct_factor<T>(a, S, tab, 2, vMU, nullptr, a.kws + K_S);
+    struct_factor<T>(a, S, tab, 2, vMU, nullptr, a.kws + K_S, lt, la);
     __syncthreads();
     double strec[6] = {0, 0, 0, 0, 0, 0};
     for (int k = threadIdx.x; k < S.nk; k += blockDim.x) {
@@ -747,7 +788,7 @@ __global__ void __launch_bounds__(256) k_fit(FitArgs a) {
     // ---- phase 5: reciprocal h
     //   h_j += -cu (8pi/V) [re3(A P) - re6(A S) g_j] + (cu/pu)(8pi/V^2) im3(w S),
     //   P = sum (k.g)(q + i k.mu) e^{ikr}, w = (tr Gs - kcoe k.Gs.k) A
-    struct_factor<T>(a, S, tab, 3, vMU, vG, a.kws + K_C2);
+    struct_factor<T>(a, S, tab, 3, vMU, vG, a.kws + K_C2, lt, la);
     __syncthreads();
     for (int k = threadIdx.x; k < S.nk; k += blockDim.x) {
       const int kg = S.k0 + k;
@@ -789,16 +830,20 @@ __global__ void __launch_bounds__(256) k_fit(FitArgs a) {
     for (int j = threadIdx.x; j < S.n; j += blockDim.x)
       for (int c = 0; c < 3; ++c) vET[3 * j + c] = vH[3 * j + c];
     __syncthreads();
+    double hn2 = 0.0;
+    for (int j = threadIdx.x; j < S.n; j += blockDim.x)
+      for (int c = 0; c < 3; ++c) hn2 += vH[3 * j + c] * vH[3 * j + c];
+    const double hn = sqrt(block_sum(hn2, red) / (double)max(S.npol, 1));
     int iadj = 0;
     double radj = 1e300;
-    while ((iadj < a.max_iter && radj > a.tol_adj) || iadj < a.min_iter) {
+    while ((iadj < a.max_iter && radj > a.tol_adj * hn) || iadj < a.min_iter) {
       ++iadj;
       for (int j = threadIdx.x; j < S.n; j += blockDim.x) {
         const double al = a.alpha_t[it[j]];
         for (int c = 0; c < 3; ++c) vLAM[3 * j + c] = al * vET[3 * j + c];
       }
       __syncthreads();
-      dip_operator<T>(a, S, tab, vLAM, vTMP);
+      dip_operator<T>(a, S, tab, vLAM, vTMP, lt, la);
       double r2 = 0.0;
       for (int j = threadIdx.x; j < S.n; j += blockDim.x) {
         const double al = a.alpha_t[it[j]];
@@ -877,7 +922,7 @@ struct mdff_fit {
   std::vector<FitPair> pairs0;    // base pair table (desc values)
   double alpha0[FIT_NT] = {0, 0, 0, 0};
   mdff_desc base;
-  int block = 256, grid = 0;
+  int block = 256, grid = 0, ncol_max = 0;
   hipStream_t stream = nullptr;
   // device
   FitSnap* d_snaps = nullptr;
@@ -1019,6 +1064,7 @@ int mdff_fit_create(const mdff_desc* d, int32_t nsnap, const int32_t* natm, cons
     S.Ky = kmax[3 * s + 1];
     S.Kz = kmax[3 * s + 2];
     S.ncol = S.Kx + S.Ky + S.Kz + 3;
+    f->ncol_max = std::max(f->ncol_max, S.ncol);
     S.tab0 = tab0;
     tab0 += (long long)S.ncol * n;
     S.k0 = k0;
@@ -1159,7 +1205,7 @@ int mdff_fit_set_targets(mdff_fit* f, const double* fref, const double* tauref, 
 }
 
 int mdff_fit_set_launch(mdff_fit* f, int32_t block, int32_t grid) {
-  if (!f || (block != 64 && block != 128 && block != 256) || grid < 1) return MDFF_ERR_BAD_ARG;
+  if (!f || (block != 32 && block != 64 && block != 128 && block != 256) || grid < 1) return MDFF_ERR_BAD_ARG;
   f->block = block;
   f->grid = std::min<int>(grid, f->nsnap);
   return MDFF_OK;
@@ -1207,7 +1253,6 @@ int mdff_fit_eval(mdff_fit* f, const double* theta, int32_t precision, int32_t w
   a.min_iter = std::max(1, f->base.min_scf_pol_iter);
   a.max_iter = std::max(a.min_iter, f->base.max_scf_pol_iter);
   a.tol = f->base.conv_tol_ind;
-  a.tol_adj = f->base.conv_tol_ind;
   a.coul_unit = f->base.coul_unit;
   a.press_unit = f->base.press_unit;
   a.w_stress = f->w_stress;
@@ -1233,7 +1278,12 @@ int mdff_fit_eval(mdff_fit* f, const double* theta, int32_t precision, int32_t w
   a.loss = f->d_loss;
   a.grad = f->d_grad;
   a.nscf = f->d_nscf;
-  const size_t shm = sizeof(double) * f->block;
+  a.ncol_max = f->ncol_max;
+  // relative tolerance of the adjoint solve: |residual| <= tol |h| (FP32: not
+  // below 2e-6, the resolution of its field sums)
+  a.tol_adj = precision == MDFF_PREC_MIXED ? std::max(a.tol, 2e-6) : a.tol;
+  const size_t tsz = precision == MDFF_PREC_MIXED ? sizeof(c2<float>) : sizeof(c2<double>);
+  const size_t shm = sizeof(double) * f->block + tsz * FIT_CA * f->ncol_max + sizeof(double) * 7 * FIT_CA;
   if (precision == MDFF_PREC_MIXED) {
     a.tab = f->d_tab32;
     hipLaunchKernelGGL((k_fit<float>), dim3(f->grid), dim3(f->block), shm, f->stream, a);

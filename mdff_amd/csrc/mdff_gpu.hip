@@ -532,9 +532,9 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   // matrix-core kernels do not cover)
   const char* ev = getenv("MDFF_RECIP_VALU");
   c->use_mma = (ev && ev[0] == '1') ? 0 : 1;
-  // m-tiles per wave of the matrix-core structure factor (tuning knob, 1 or 2)
-  const char* ec = getenv("MDFF_SK_CTM");
-  if (ec && ec[0] == '1') c->sk_ctm = 1;
+  // m-tiles per wave of the matrix-core structure factor: one in FP32 (three
+  // waves per SIMD), two in FP64 (measured on the 24k-ion a-SiO2 workload)
+  c->sk_ctm = d->precision == MDFF_PREC_MIXED ? 1 : 2;
   if (d->lcoulomb) MDFF_TRY(recip_setup(c));
   return MDFF_OK;
 }

@@ -218,7 +218,7 @@ def plain_targets(snapshots, precision: int = capi.PREC_FP64, scf_tol: float | N
         finally:
             eng.close()
         F.append(f)
-        T.append(np.asarray(t["tau_nonb"]) + np.asarray(t["tau_coul"]))
+        T.append(np.ravel(t["tau_nonb"]) + np.ravel(t["tau_coul"]))
     return np.concatenate(F), np.array(T)
 
 

@@ -4,7 +4,7 @@
 // builds and loads it, to check the fit's forward and adjoint gradient on the
 // CPU; the library itself always runs on the GPU.
 #pragma once
-#include <barrier>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -22,9 +22,25 @@
 struct dim3 { unsigned x, y, z; dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {} };
 inline thread_local dim3 threadIdx, blockIdx;
 inline dim3 blockDim, gridDim;
-inline std::barrier<>* g_bar = nullptr;
-inline void __syncthreads() { g_bar->arrive_and_wait(); }
-inline double g_shared[4096];
+// sense-reversing barrier that yields while waiting (workgroup barriers are
+// frequent; a futex-based barrier makes the emulation system-call bound)
+struct YieldBarrier {
+  std::atomic<int> count{0};
+  std::atomic<int> sense{0};
+  int n = 1;
+  void wait() {
+    const int s = sense.load(std::memory_order_acquire);
+    if (count.fetch_add(1, std::memory_order_acq_rel) + 1 == n) {
+      count.store(0, std::memory_order_relaxed);
+      sense.store(s ^ 1, std::memory_order_release);
+    } else {
+      while (sense.load(std::memory_order_acquire) == s) std::this_thread::yield();
+    }
+  }
+};
+inline YieldBarrier* g_bar = nullptr;
+inline void __syncthreads() { g_bar->wait(); }
+inline double g_shared[16384];
 typedef int hipError_t;
 constexpr int hipSuccess = 0;
 typedef void* hipStream_t;
@@ -50,7 +66,8 @@ template <typename F, typename A>
 void emu_launch(F kern, dim3 g, dim3 b, A args) {
   gridDim = g; blockDim = b;
   for (unsigned bx = 0; bx < g.x; ++bx) {
-    std::barrier<> bar((ptrdiff_t)b.x);
+    YieldBarrier bar;
+    bar.n = (int)b.x;
     g_bar = &bar;
     std::vector<std::thread> th;
     for (unsigned t = 0; t < b.x; ++t)
