@@ -25,21 +25,29 @@ __global__ void k_fill(size_t n, double v, double* p) {
 }
 
 // column sums over atoms: in [col][n] -> per-block partials [block][ncol]
+// fixed-order sum of a 256-thread block: wave shuffles, then the 4 wave sums
+// in wave order (deterministic); result valid in thread 0
+__device__ __forceinline__ double block_sum256(double v, double* sh4) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh4[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = ((sh4[0] + sh4[1]) + sh4[2]) + sh4[3];
+  __syncthreads();
+  return r;
+}
+
 __global__ void __launch_bounds__(256) k_colsum(int n, int ncol, const double* __restrict__ in,
                                                 double* __restrict__ part) {
-  __shared__ double sh[256];
+  __shared__ double sh4[4];
   for (int c = 0; c < ncol; ++c) {
     double v = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
       v += in[(size_t)c * n + i];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) part[(size_t)blockIdx.x * ncol + c] = sh[0];
-    __syncthreads();
+    v = block_sum256(v, sh4);
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x * ncol + c] = v;
   }
 }
 
@@ -83,14 +91,16 @@ __global__ void k_combine_field(int n, const double* __restrict__ edir, const do
   }
 }
 
-// column sums of the block partials in block order (the same additions, in
-// the same order, as a sequential host loop): out[k] = sum_b part[b][k]
-__global__ void k_colsum_final(int nb, int ncol, const double* __restrict__ part, double* __restrict__ out) {
-  const int k = threadIdx.x;
-  if (k >= ncol) return;
+// column sums of the block partials: one 256-thread block per column, fixed
+// order (deterministic): out[k] = sum_b part[b][k]
+__global__ void __launch_bounds__(256) k_colsum_final(int nb, int ncol, const double* __restrict__ part,
+                                                      double* __restrict__ out) {
+  __shared__ double sh4[4];
+  const int k = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[(size_t)b * ncol + k];
-  out[k] = s;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(size_t)b * ncol + k];
+  s = block_sum256(s, sh4);
+  if (threadIdx.x == 0) out[k] = s;
 }
 
 // per-atom columns for the self and surface terms: mu^2, mu, q r
@@ -336,7 +346,7 @@ int reduce_cols(mdff_ctx* c, const double* in, int ncol, double* host_out) {
 int reduce_cols_dev(mdff_ctx* c, const double* in, int ncol, double* dst) {
   const int nb = std::min(1024, std::max(1, div_up(c->n, 256)));
   hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, c->stream, c->n, ncol, in, c->red_part);
-  hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(64), 0, c->stream, nb, ncol, c->red_part, dst);
+  hipLaunchKernelGGL(k_colsum_final, dim3(ncol), dim3(256), 0, c->stream, nb, ncol, c->red_part, dst);
   HIP_TRY(c, hipGetLastError());
   return MDFF_OK;
 }

@@ -737,13 +737,17 @@ __global__ void k_sk_combine(int nk_box, int NZ, int Kz, int nchunk, double* __r
   p1[2 * x + 1] += fz * s2r;
 }
 
-// e[k] = sum_b part[b][k], k < 7, in block order
-__global__ void k_colsum7(int nb, const double* __restrict__ part, double* __restrict__ e) {
-  const int k = threadIdx.x;
-  if (k >= 7) return;
+// e[k] = sum_b part[b][k], k < 7: one 256-thread block per column, fixed order
+__global__ void __launch_bounds__(256) k_colsum7(int nb, const double* __restrict__ part, double* __restrict__ e) {
+  __shared__ double sh4[4];
+  const int k = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[(size_t)7 * b + k];
-  e[k] = s;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(size_t)7 * b + k];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh4[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) e[k] = ((sh4[0] + sh4[1]) + sh4[2]) + sh4[3];
 }
 
 }  // namespace
@@ -979,7 +983,7 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
               });
   if (e_out) {
     // 7 sums of the k-blocks in block order, left in device memory
-    hipLaunchKernelGGL(k_colsum7, dim3(1), dim3(64), 0, c->stream, nb, c->red_part, e_out);
+    hipLaunchKernelGGL(k_colsum7, dim3(7), dim3(256), 0, c->stream, nb, c->red_part, e_out);
   }
   if (!forces && !field) return MDFF_OK;
   const bool dipf = forces && use_mu;

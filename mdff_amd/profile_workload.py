@@ -1,8 +1,9 @@
 """Short run of the a-SiO2 24,000-ion workload (the bench configuration) for
 profiling with rocprofv3: steepest-descent relaxation, then `--steps`
 velocity-Verlet steps of one precision form, then a device-to-device copy of
-`--copy-mb` MiB whose counter readings calibrate FETCH_SIZE / WRITE_SIZE
-(the copy reads and writes exactly that many bytes).
+`--copy-mb` MiB (hipMemcpy device to device) whose counter readings
+calibrate FETCH_SIZE / WRITE_SIZE (the copy reads and writes exactly that
+many bytes).
 
     python -m mdff_amd.profile_workload --steps 20 --precision mixed
 """
@@ -38,12 +39,21 @@ def main(argv=None) -> int:
     eng.close()
     print(f"profile_workload: {args.steps} {args.precision} VV steps on {s.natm} ions done", flush=True)
     if args.copy_mb > 0:
-        import torch
-        n = args.copy_mb * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
+        # device-to-device copy through the HIP runtime (no framework in the
+        # profiled process): reads and writes copy_mb MiB once each
+        import ctypes as C
+        hip = C.CDLL("libamdhip64.so")
+        nbytes = args.copy_mb * (1 << 20)
+        src, dst = C.c_void_p(), C.c_void_p()
+        if hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)):
+            raise RuntimeError("hipMalloc failed")
+        hip.hipMemset(src, 1, C.c_size_t(nbytes))
+        hip.hipDeviceSynchronize()
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3):       # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+        hip.hipFree(src)
+        hip.hipFree(dst)
         print(f"profile_workload: calibration copy of {args.copy_mb} MiB done", flush=True)
     return 0
 
