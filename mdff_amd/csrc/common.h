@@ -191,10 +191,12 @@ struct mdff_ctx {
   void* amat = nullptr;                   // matrix-core gather A operand
   int mma_nks = 0, mma_mtiles = 0, mma_ct = 0, mma_splits = 1;
   int use_mma = 1;                        // matrix-core reciprocal kernels
-  int sk_ctm = 2;                         // m-tiles per wave of the structure factor
-  double* kO_ws = nullptr;
+  int sk_ctm = 1;                         // m-tiles per wave of the structure factor
+  int sk_ta = 0;                          // atoms per staged tile of the structure factor
+  int sk_mma_ch = 1;                      // atom chunks of the matrix-core structure factor
+  double* kO_ws = nullptr;                // scf_kO_v2 work vectors (11 x 3n)
   AtomRec* arec = nullptr;                // atoms in cell-sorted order (refreshed every evaluation)
-  int2* ameta = nullptr;                  // (type, original index) in the same order                // scf_kO_v2 work vectors (11 x 3n)
+  int2* ameta = nullptr;                  // (type, original index) in the same order
 
   double last_ms[8] = {0};
   int n_last = 0;

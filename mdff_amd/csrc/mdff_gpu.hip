@@ -542,9 +542,9 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   // matrix-core kernels do not cover)
   const char* ev = getenv("MDFF_RECIP_VALU");
   c->use_mma = (ev && ev[0] == '1') ? 0 : 1;
-  // m-tiles per wave of the matrix-core structure factor: one in FP32 (three
-  // waves per SIMD), two in FP64 (measured on the 24k-ion a-SiO2 workload)
-  c->sk_ctm = d->precision == MDFF_PREC_MIXED ? 1 : 2;
+  // m-tiles per wave of the matrix-core structure factor: one (two would
+  // exceed 256 VGPRs with the unrolled atom tile)
+  c->sk_ctm = 1;
   if (d->lcoulomb) MDFF_TRY(recip_setup(c));
   return MDFF_OK;
 }

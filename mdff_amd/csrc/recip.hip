@@ -419,8 +419,18 @@ struct GmArgs {
   double* out;                   // [split][3][n]
 };
 
+// Two waves per SIMD where the B operand, the double-buffered A operand and
+// the accumulators fit in 256 registers without spills (the instances
+// listed), one otherwise.
 template <typename T, int CT, int NKS>
-__global__ void __launch_bounds__(256) k_gather_mma(GmArgs a) {
+constexpr int gather_waves() {
+  return sizeof(T) == 4 ? (((CT >= 2 && NKS <= 25) || (CT == 1 && NKS <= 57)) ? 2 : 1)
+                        : (((CT == 4 && NKS <= 13) || (CT == 2 && NKS <= 25) || (CT == 1 && NKS <= 33)) ? 2 : 1);
+}
+
+template <typename T, int CT, int NKS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gather_waves<T, CT, NKS>())))
+k_gather_mma(GmArgs a) {
   typedef typename Mma<T>::acc_t acc_t;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -429,24 +439,25 @@ __global__ void __launch_bounds__(256) k_gather_mma(GmArgs a) {
   const cplx<T>* tab = reinterpret_cast<const cplx<T>*>(a.tab);
   const T* amat = reinterpret_cast<const T*>(a.amat);
   const int zcol = a.Kx + 1 + a.Ky + 1;
-  // B operand: the Z row of this lane's atoms, zero beyond NZ
+  // B operand: the Z row of this lane's atoms, zero beyond NZ.  Every load
+  // reads a clamped, valid address and the zero is selected afterwards, so
+  // all CT * NKS loads are in flight together.
   T breg[CT][NKS];
 #pragma unroll
   for (int ct = 0; ct < CT; ++ct) {
     const int ia = atom0 + ct * 16 + col;
     const bool ok = ia < a.n;
+    const int iac = min(ia, a.n - 1);
 #pragma unroll
     for (int s = 0; s < NKS; ++s) {
       const int k = 4 * s + g;
       const int t = k >> 1;
       const int nz = t - a.Kz;
-      const int az = nz < 0 ? -nz : nz;
-      T v = T(0);
-      if (ok && t < a.NZ) {
-        const cplx<T> z = tab[(size_t)(zcol + az) * a.n + ia];
-        v = (k & 1) ? (nz < 0 ? -z.im : z.im) : z.re;
-      }
-      breg[ct][s] = v;
+      const int az = min(nz < 0 ? -nz : nz, a.Kz);
+      // Re (k even) or Im (k odd) of Z, conjugated for nz < 0
+      const T v = reinterpret_cast<const T*>(tab + (size_t)(zcol + az) * a.n + iac)[k & 1];
+      const T w = (ok && t < a.NZ) ? (((k & 1) && nz < 0) ? T(-1) : T(1)) : T(0);
+      breg[ct][s] = w * v;
     }
   }
   double ax[CT], ay[CT], az3[CT];
@@ -548,60 +559,62 @@ struct Sm2Args {
   double* part;                  // [chunk][2*nk_box] FP64
 };
 
-template <typename T, int CTM>
-__global__ void __launch_bounds__(256) k_sk_mma2(Sm2Args a) {
+template <typename T, int CTM, int TA>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_sk_mma2(Sm2Args a) {
   typedef typename Mma<T>::acc_t acc_t;
   constexpr int RT = 7;                                          // row tiles: 56 nz values max
+  constexpr int TAP = TA + 2;                                    // padded atom stride of a staged column
+  static_assert(256 % TA == 0 && TA % 2 == 0, "atom tile must divide the block");
   extern __shared__ unsigned char smem_raw[];
-  const int TA = a.TA;
   const int ncol = a.Kx + a.Ky + a.Kz + 3;
-  cplx<T>* st = reinterpret_cast<cplx<T>*>(smem_raw);           // [TA][ncol]
-  T* sq = reinterpret_cast<T*>(st + (size_t)ncol * TA);          // a-coefficients per atom
+  // staged tile, column-major: st[col * TAP + atom]; then the a-coefficients
+  // sq[c * TAP + atom], c = 0..2.  A K step advances the atom by 2, a fixed
+  // byte offset, so the unrolled step loop addresses LDS with immediates.
+  cplx<T>* st = reinterpret_cast<cplx<T>*>(smem_raw);
+  T* sq = reinterpret_cast<T*>(st + (size_t)ncol * TAP);
   const cplx<T>* tab = reinterpret_cast<const cplx<T>*>(a.tab);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, col = lane & 15;
   const int kk = g & 1, ja = g >> 1;                             // K: atom 2s+ja, Re/Im kk
   const int zcol = a.Kx + 1 + a.Ky + 1;
-  // A rows of this lane (R = col): nz index and part per row tile
-  int zc[RT];
-  T zs[RT];
-  bool zok[RT];
+  // A rows of this lane (R = col): nz index and part per row tile.  A =
+  // part 0: (Zr, -Zi); part 1: (Zi, Zr), with Z(-n) = conj Z(n).  Rows
+  // beyond NZ read a valid column; their D rows are never stored.
   const int zoff = blockIdx.z * (RT * 8);                       // nz rows of this block
+  const bool take_im = (col & 1) ^ kk;                           // which Z component feeds A
+  const T* stT = reinterpret_cast<const T*>(st);
+  const T* zp[RT];
+  T zs[RT];
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt) {
     const int nzi = zoff + rt * 8 + (col >> 1);
     const int part = col & 1;
     const int nz = nzi - a.Kz;
-    zok[rt] = nzi < a.NZ;
-    zc[rt] = zcol + (nz < 0 ? -nz : nz);
-    // A = part 0: (Zr, -Zi); part 1: (Zi, Zr), with Z(-n) = conj Z(n)
+    const int zc = nzi < a.NZ ? zcol + (nz < 0 ? -nz : nz) : zcol;
     const T sgn_im = nz < 0 ? T(-1) : T(1);
     zs[rt] = part == 0 ? (kk == 0 ? T(1) : -sgn_im) : (kk == 0 ? sgn_im : T(1));
-  }
-  const bool take_im = (col & 1) ^ kk;                           // which Z component feeds A
-  int zofs[RT];                                                  // T offset of the A value in a staged row
-#pragma unroll
-  for (int rt = 0; rt < RT; ++rt) {
-    zofs[rt] = 2 * (zok[rt] ? zc[rt] : zcol) + (take_im ? 1 : 0);
-    if (!zok[rt]) zs[rt] = T(0);
+    zp[rt] = stT + 2 * ((size_t)zc * TAP + ja) + (take_im ? 1 : 0);
   }
   // B columns of this lane: m per m-tile
   const int mt0 = (blockIdx.x * (blockDim.x >> 6) + wave) * CTM;
-  int bnx[CTM], bnyc[CTM];
-  T bnyf[CTM], bnxf[CTM];
+  const cplx<T>* xp[CTM];
+  const cplx<T>* yp[CTM];
+  T bnyf[CTM], bnxf[CTM], bw[CTM];
   bool bneg[CTM], bok[CTM];
 #pragma unroll
   for (int j = 0; j < CTM; ++j) {
     const int m = (mt0 + j) * 16 + col;
     bok[j] = m < a.M;
     const int nx = bok[j] ? m / a.NY : 0, ny = bok[j] ? m % a.NY - a.Ky : 0;
-    bnx[j] = nx;
-    bnyc[j] = a.Kx + 1 + (ny >= 0 ? ny : -ny);
+    xp[j] = st + (size_t)nx * TAP + ja;
+    yp[j] = st + (size_t)(a.Kx + 1 + (ny >= 0 ? ny : -ny)) * TAP + ja;
     bneg[j] = ny < 0;
     bnxf[j] = (T)nx;
     bnyf[j] = (T)ny;
+    bw[j] = bok[j] ? T(1) : T(0);      // columns beyond M: zero B (a multiply, no branch)
   }
-  const T tp = T(2.0 * MDFF_PI);
+  const T* qp = sq + ja;
   acc_t acc[RT][CTM];
   double dacc[RT][CTM][4];
 #pragma unroll
@@ -612,14 +625,25 @@ __global__ void __launch_bounds__(256) k_sk_mma2(Sm2Args a) {
       for (int r = 0; r < 4; ++r) dacc[rt][j][r] = 0.0;
   const int abeg = blockIdx.y * a.chunk;
   const int aend = min(a.n, abeg + a.chunk);
+  // staging: thread -> (atom sia, columns scg, scg + CG, ...); loads from
+  // clamped addresses in batches of 8, so they are in flight together
+  constexpr int CG = 256 / TA;
+  const int sia = threadIdx.x % TA, scg = threadIdx.x / TA;
   for (int t0 = abeg; t0 < aend; t0 += TA) {
     const int nt = min(TA, aend - t0);
+    const bool sok = sia < nt;
+    const size_t src = (size_t)t0 + min(sia, nt - 1);
     __syncthreads();
-    for (int x = threadIdx.x; x < ncol * TA; x += blockDim.x) {
-      const int cc = x / TA, ia = x % TA;
-      st[ia * ncol + cc] = (ia < nt) ? tab[(size_t)cc * a.n + t0 + ia] : cplx<T>{T(0), T(0)};
+    for (int c0 = scg; c0 < ncol; c0 += 8 * CG) {
+      cplx<T> v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = tab[(size_t)min(c0 + u * CG, ncol - 1) * a.n + src];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (c0 + u * CG < ncol) st[(size_t)(c0 + u * CG) * TAP + sia] = sok ? v[u] : cplx<T>{T(0), T(0)};
     }
-    for (int ia = threadIdx.x; ia < TA; ia += blockDim.x) {
+    if (threadIdx.x < TA) {
+      const int ia = threadIdx.x;
       T c0 = T(0), c1 = T(0), c2 = T(0);
       if (ia < nt) {
         const int i = t0 + ia;
@@ -627,16 +651,16 @@ __global__ void __launch_bounds__(256) k_sk_mma2(Sm2Args a) {
         if (a.mode >= 1) {
           const double u0 = a.mu[3 * i], u1 = a.mu[3 * i + 1], u2 = a.mu[3 * i + 2];
           if (a.mode == 1) {
-            c1 = (T)(u0 * a.b[0] + u1 * a.b[1] + u2 * a.b[2]);
-            c2 = (T)(u0 * a.b[3] + u1 * a.b[4] + u2 * a.b[5]);
+            c1 = (T)(2.0 * MDFF_PI * (u0 * a.b[0] + u1 * a.b[1] + u2 * a.b[2]));
+            c2 = (T)(2.0 * MDFF_PI * (u0 * a.b[3] + u1 * a.b[4] + u2 * a.b[5]));
           } else {
             c0 = (T)(u0 * a.b[6] + u1 * a.b[7] + u2 * a.b[8]);
           }
         }
       }
-      sq[3 * ia] = c0;
-      sq[3 * ia + 1] = c1;
-      sq[3 * ia + 2] = c2;
+      sq[ia] = c0;
+      sq[TAP + ia] = c1;
+      sq[2 * TAP + ia] = c2;
     }
     __syncthreads();
 #pragma unroll
@@ -644,56 +668,27 @@ __global__ void __launch_bounds__(256) k_sk_mma2(Sm2Args a) {
 #pragma unroll
       for (int j = 0; j < CTM; ++j) acc[rt][j] = acc_t{T(0), T(0), T(0), T(0)};
     // K steps of 4: atoms 2s and 2s+1 (Re/Im of P).  Rows beyond nt are the
-    // zero padding of the staged tile, so no step needs a mask.  The LDS
-    // operands of step s+1 are loaded while the MFMAs of step s run.
-    const int nsteps = (nt + 1) >> 1;
-    const T* stT = reinterpret_cast<const T*>(st);
-    const int rstride = 2 * ncol;
-    T zn[RT], xrn[CTM], xin[CTM], yrn[CTM], yin[CTM], c0n, c1n, c2n;
-    auto load_step = [&](int s) {
-      const int ia = 2 * s + ja;
-      const T* rowT = stT + (size_t)ia * rstride;
+    // zero padding of the staged tile, so every tile runs all TA/2 steps and
+    // no step needs a mask.
 #pragma unroll
-      for (int rt = 0; rt < RT; ++rt) zn[rt] = rowT[zofs[rt]];
-#pragma unroll
-      for (int j = 0; j < CTM; ++j) {
-        xrn[j] = rowT[2 * bnx[j]];
-        xin[j] = rowT[2 * bnx[j] + 1];
-        yrn[j] = rowT[2 * bnyc[j]];
-        yin[j] = rowT[2 * bnyc[j] + 1];
-      }
-      c0n = sq[3 * ia];
-      c1n = sq[3 * ia + 1];
-      c2n = sq[3 * ia + 2];
-    };
-    load_step(0);
-    for (int s = 0; s < nsteps; ++s) {
-      T zv[RT], xr[CTM], xi[CTM], yr[CTM], yi[CTM];
-#pragma unroll
-      for (int rt = 0; rt < RT; ++rt) zv[rt] = zn[rt];
-#pragma unroll
-      for (int j = 0; j < CTM; ++j) {
-        xr[j] = xrn[j];
-        xi[j] = xin[j];
-        yr[j] = yrn[j];
-        yi[j] = yin[j];
-      }
-      const T c0 = c0n, c1 = c1n, c2 = c2n;
-      if (s + 1 < nsteps) load_step(s + 1);
+    for (int s = 0; s < TA / 2; ++s) {
       T av[RT];
 #pragma unroll
-      for (int rt = 0; rt < RT; ++rt) av[rt] = zs[rt] * zv[rt];
+      for (int rt = 0; rt < RT; ++rt) av[rt] = zs[rt] * zp[rt][4 * s];
+      const T c0 = qp[2 * s], c1 = qp[TAP + 2 * s], c2 = qp[2 * TAP + 2 * s];
       T bv[CTM];
 #pragma unroll
       for (int j = 0; j < CTM; ++j) {
-        const T yim = bneg[j] ? -yi[j] : yi[j];
-        const T xyr = xr[j] * yr[j] - xi[j] * yim;
-        const T xyi = xr[j] * yim + xi[j] * yr[j];
-        // (c0 + i bb) XY, bb = 2pi (nx mx + ny my) in mode 1
-        const T bb = (a.mode == 1) ? tp * (bnxf[j] * c1 + bnyf[j] * c2) : T(0);
+        const cplx<T> X = xp[j][2 * s];
+        const cplx<T> Y = yp[j][2 * s];
+        const T yim = bneg[j] ? -Y.im : Y.im;
+        const T xyr = X.re * Y.re - X.im * yim;
+        const T xyi = X.re * yim + X.im * Y.re;
+        // (c0 + i bb) XY, bb = 2pi (nx mx + ny my) in mode 1 (c1, c2 carry 2pi)
+        const T bb = bnxf[j] * c1 + bnyf[j] * c2;
         const T pre = c0 * xyr - bb * xyi;
         const T pim = c0 * xyi + bb * xyr;
-        bv[j] = bok[j] ? (kk == 0 ? pre : pim) : T(0);
+        bv[j] = bw[j] * (kk == 0 ? pre : pim);
       }
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
@@ -765,6 +760,45 @@ static int mma_ct(bool fp32, int nks) {
   return nks <= 25 ? 2 : 1;
 }
 
+// the compiled matrix-core gather for (CT, NKS), or null
+template <typename T>
+static const void* gather_kernel(int ct, int nks) {
+#define GM_CASE(CTV, NKSV) \
+  case (CTV) * 1000 + (NKSV): return reinterpret_cast<const void*>(&k_gather_mma<T, CTV, NKSV>);
+  switch (ct * 1000 + nks) {
+    GM_CASE(4, 9) GM_CASE(4, 13) GM_CASE(4, 17) GM_CASE(4, 19) GM_CASE(4, 25) GM_CASE(4, 33)
+    GM_CASE(2, 41) GM_CASE(2, 57)
+    GM_CASE(2, 9) GM_CASE(2, 13) GM_CASE(2, 17) GM_CASE(2, 19) GM_CASE(2, 25)
+    GM_CASE(1, 33) GM_CASE(1, 41) GM_CASE(1, 57)
+    default: return nullptr;
+  }
+#undef GM_CASE
+}
+
+// the compiled matrix-core structure factor for (CTM, TA), or null
+template <typename T>
+static const void* sk_kernel(int ctm, int ta) {
+#define SK_CASE(C, A) \
+  case (C) * 100 + (A): return reinterpret_cast<const void*>(&k_sk_mma2<T, C, A>);
+  switch (ctm * 100 + ta) {
+    SK_CASE(1, 64) SK_CASE(1, 32) SK_CASE(1, 16) SK_CASE(1, 8)
+    default: return nullptr;
+  }
+#undef SK_CASE
+}
+
+// resident 256-thread blocks per CU of a kernel with `lds` bytes of LDS
+static int blocks_per_cu(const void* fn, size_t lds) {
+  int nb = 0;
+  if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, lds) != hipSuccess || nb <= 0) nb = 1;
+  return nb;
+}
+
+// LDS bytes of the matrix-core structure factor for tile TA
+static size_t sk_lds(size_t tsz, int ncol, int ta) {
+  return (tsz * (size_t)ncol + tsz / 2 * 3) * (size_t)(ta + 2);
+}
+
 // k-mesh on the box with the reference's k vectors and A(n), kcoe
 // (src/kspace.f90:162-191); allocation of the reciprocal buffers.
 int recip_setup(mdff_ctx* c) {
@@ -808,7 +842,6 @@ int recip_setup(mdff_ctx* c) {
   chunks = std::min(chunks, std::max(1, c->n / 64));
   chunks = std::min(chunks, 64);
   c->sk_chunks = chunks;
-  HIP_TRY(c, hipMalloc(&c->sk_part, sizeof(double) * 4 * (size_t)chunks * nb));
   HIP_TRY(c, hipMalloc((void**)&c->sk, sizeof(double) * 2 * nb));
   HIP_TRY(c, hipMalloc((void**)&c->wk, sizeof(double) * 2 * nb));
   HIP_TRY(c, hipMalloc(&c->wk_t, tsz * (size_t)nb));
@@ -816,17 +849,19 @@ int recip_setup(mdff_ctx* c) {
   const int M = c->NXk * c->NYk;
   int splits = std::max(1, std::min(M, div_up((int64_t)262144, c->n)));
   // matrix-core gather: A operand and its own split count
+  const bool fp32 = c->d.precision == MDFF_PREC_MIXED;
   c->mma_nks = mma_bucket(Kz + 1);
   c->mma_mtiles = div_up(M, 4);
-  c->mma_ct = mma_ct(c->d.precision == MDFF_PREC_MIXED, c->mma_nks);
-  // splits over m-tiles: the gather runs one wave per SIMD, so pick the split
-  // count whose wave total fills whole rounds of 4 x CUs waves best (at least
-  // 4 m-tiles per wave; fewer splits win ties)
-  const int atom_waves = div_up(c->n, 16 * c->mma_ct);
+  c->mma_ct = mma_ct(fp32, c->mma_nks);
   int cus = 256;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0)
     cus = 256;
-  const int64_t conc = 4 * (int64_t)cus;
+  // splits over m-tiles: pick the split count whose wave total fills whole
+  // rounds of resident waves best (at least 4 m-tiles per wave; fewer splits
+  // win ties)
+  const int atom_waves = div_up(c->n, 16 * c->mma_ct);
+  const void* gfn = fp32 ? gather_kernel<float>(c->mma_ct, c->mma_nks) : gather_kernel<double>(c->mma_ct, c->mma_nks);
+  const int64_t conc = 4 * (int64_t)cus * blocks_per_cu(gfn, 0);
   int best_s = 1;
   double best_e = -1.0;
   for (int sp = 1; sp <= std::min(c->mma_mtiles, 64); ++sp) {
@@ -842,7 +877,33 @@ int recip_setup(mdff_ctx* c) {
     }
   }
   c->mma_splits = best_s;
+  // matrix-core structure factor: atoms per staged tile within 64 KiB of
+  // LDS; atom chunks chosen from the kernel's occupancy so that the blocks
+  // fill whole rounds of resident blocks with the fewest tiles per block
+  {
+    int TA = 64;
+    while (TA > 8 && sk_lds(tsz, ncol, TA) > 65536) TA /= 2;
+    c->sk_ta = sk_lds(tsz, ncol, TA) <= 65536 ? TA : 0;
+    c->sk_mma_ch = 1;
+    if (c->sk_ta) {
+      const void* sfn = fp32 ? sk_kernel<float>(c->sk_ctm, TA) : sk_kernel<double>(c->sk_ctm, TA);
+      const int64_t cap = (int64_t)cus * blocks_per_cu(sfn, sk_lds(tsz, ncol, TA));
+      const int64_t mblocks = div_up(div_up(M, 16), 4 * c->sk_ctm) * (int64_t)div_up(c->NZk, 56);
+      int64_t best = -1;
+      for (int ch = 1; ch <= 128 && (ch == 1 || div_up(c->n, ch) >= TA); ++ch) {
+        const int per = div_up(c->n, ch);
+        const int nch = div_up(c->n, per);
+        const int64_t rounds = (mblocks * nch + cap - 1) / cap;
+        const int64_t cost = rounds * div_up(per, TA);
+        if (best < 0 || cost < best) {
+          best = cost;
+          c->sk_mma_ch = nch;
+        }
+      }
+    }
+  }
   HIP_TRY(c, hipMalloc(&c->amat, tsz / 2 * (size_t)c->mma_mtiles * c->mma_nks * 64));
+  HIP_TRY(c, hipMalloc(&c->sk_part, sizeof(double) * 4 * (size_t)std::max(c->sk_chunks, c->sk_mma_ch) * nb));
   splits = std::max(splits, c->mma_splits);
   c->rec_splits = splits;
   HIP_TRY(c, hipMalloc((void**)&c->rec_part, sizeof(double) * 9 * (size_t)splits * c->n));
@@ -871,31 +932,22 @@ int recip_tables(mdff_ctx* c) {
   return MDFF_OK;
 }
 
-#define GM_CASE(CTV, NKSV)                                                                      \
-  case (CTV) * 1000 + (NKSV):                                                                   \
-    PROF_LAUNCH(c, PK_GATHER,                                                                   \
-                hipLaunchKernelGGL((k_gather_mma<T, CTV, NKSV>), grid, dim3(256), 0, c->stream, gm)); \
-    break;
-
 template <typename T>
 static int launch_gather_mma(mdff_ctx* c, const GmArgs& gm, int ct, dim3 grid) {
-  switch (ct * 1000 + gm.nks) {
-    GM_CASE(4, 9) GM_CASE(4, 13) GM_CASE(4, 17) GM_CASE(4, 19) GM_CASE(4, 25) GM_CASE(4, 33)
-    GM_CASE(2, 41) GM_CASE(2, 57)
-    GM_CASE(2, 9) GM_CASE(2, 13) GM_CASE(2, 17) GM_CASE(2, 19) GM_CASE(2, 25)
-    GM_CASE(1, 33) GM_CASE(1, 41) GM_CASE(1, 57)
-    default:
-      c->err = "no matrix-core gather kernel for this kmax";
-      return MDFF_ERR_UNSUPPORTED;
+  const void* fn = gather_kernel<T>(ct, gm.nks);
+  if (!fn) {
+    c->err = "no matrix-core gather kernel for this kmax";
+    return MDFF_ERR_UNSUPPORTED;
   }
+  void* args[] = {const_cast<GmArgs*>(&gm)};
+  PROF_LAUNCH(c, PK_GATHER, (void)hipLaunchKernel(fn, grid, dim3(256), args, 0, c->stream));
   return MDFF_OK;
 }
 
-// m-tiles per wave of the structure-factor kernel (1 or 2)
 template <typename T>
 static void launch_sk_mma2(int ctm, dim3 grid, size_t lds, hipStream_t st, const Sm2Args& sm) {
-  if (ctm == 1) hipLaunchKernelGGL((k_sk_mma2<T, 1>), grid, dim3(256), lds, st, sm);
-  else hipLaunchKernelGGL((k_sk_mma2<T, 2>), grid, dim3(256), lds, st, sm);
+  void* args[] = {const_cast<Sm2Args*>(&sm)};
+  (void)hipLaunchKernel(sk_kernel<T>(ctm, sm.TA), grid, dim3(256), args, lds, st);
 }
 
 template <typename T>
@@ -937,7 +989,7 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     sm.mtiles = div_up(s.M, 16);
     const int CTM = c->sk_ctm;
     const int mblocks = div_up(sm.mtiles, 4 * CTM);
-    int ch = std::max(1, std::min(c->sk_chunks, div_up(c->n, 512)));
+    int ch = c->sk_mma_ch;
     sm.chunk = div_up(c->n, ch);
     ch = div_up(c->n, sm.chunk);
     chunks = ch;
@@ -945,9 +997,12 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     sm.q = q; sm.mu = mu;
     for (int k = 0; k < 9; ++k) sm.b[k] = c->B[k];
     sm.tab = c->tab;
-    int TA = 64;
-    auto lds = [&](int ta) { return sizeof(cplx<T>) * (size_t)ncol * ta + sizeof(T) * 3 * ta; };
-    while (TA > 2 && lds(TA) > 65536) TA /= 2;
+    const int TA = c->sk_ta;
+    auto lds = [&](int ta) { return sk_lds(sizeof(cplx<T>), ncol, ta); };
+    if (TA == 0) {
+      c->err = "kES too large for the structure-factor kernel";
+      return MDFF_ERR_UNSUPPORTED;
+    }
     sm.TA = TA;
     dim3 gs(mblocks, ch, div_up(c->NZk, 56));
     double* p1 = reinterpret_cast<double*>(c->sk_part);
