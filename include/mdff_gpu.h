@@ -218,7 +218,8 @@ int mdff_calc_temp(mdff_ctx *ctx, double boltz_unit, double *temp, double *ekin)
 
 /* Timing of the last engforce on the context stream (HIP events), ms:
  * [0] total, [1] list build/check, [2] short-range+real-space, [3] reciprocal,
- * [4] SCF loop.  n = number of entries written (<= 8). */
+ * [4] SCF loop.  n = number of entries written (<= 8); the events are
+ * recorded only while mdff_profile is on, n = 0 for an evaluation without. */
 int mdff_last_timings(mdff_ctx *ctx, double *ms, int32_t *n);
 
 /* Kernel timing with HIP events on the context stream; replaces the

@@ -143,6 +143,11 @@ struct mdff_ctx {
   double* frec = nullptr;                 // reciprocal force scratch (3n)
   double* peratom = nullptr;              // per-atom energy/virial/stress partials (NPA x n)
   double* red = nullptr;                  // reduction outputs
+  int mu_static = 0;                      // mu holds the static dipoles (no induced dipoles)
+  double *cx = nullptr, *cy = nullptr, *cz = nullptr;   // fractional coordinates of a pending list check
+  double* h_chk = nullptr;                // pinned: block maxima of the pending check
+  hipEvent_t chk_ev = nullptr;            // the pending check's copy to h_chk is complete
+  int chk_pending = 0;
   double* red_part = nullptr;             // reduction block partials
   PairParam* d_pp = nullptr;
   int itime = 0;
@@ -246,6 +251,8 @@ static inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); 
 // launches implemented in the .hip files
 int nl_build(mdff_ctx* c);
 int nl_check(mdff_ctx* c, int* rebuilt);
+int nl_check_launch(mdff_ctx* c);
+int nl_check_finish(mdff_ctx* c, int* rebuilt);
 int reduce_cols_dev(mdff_ctx* c, const double* in, int ncol, double* dst);
 int nl_get_pairs(mdff_ctx* c, int which, int32_t* point, int32_t* list, int64_t cap, int64_t* n);
 void launch_kardir(mdff_ctx* c);

@@ -10,6 +10,7 @@
 #include <cstring>
 
 #include "common.h"
+#include "reduce.h"
 
 int realspace_force_impl(mdff_ctx* c, double* fsr, double* fcl, double* ef, bool with_field);
 
@@ -24,7 +25,9 @@ __global__ void k_fill(size_t n, double v, double* p) {
   if (i < n) p[i] = v;
 }
 
-// column sums over atoms: in [col][n] -> per-block partials [block][ncol]
+// column sums over atoms: in [col][n] (ncol <= 16) -> block partials
+// [block][ncol], then one block adds them in block order -> out[ncol]
+
 // fixed-order sum of a 256-thread block: wave shuffles, then the 4 wave sums
 // in wave order (deterministic); result valid in thread 0
 __device__ __forceinline__ double block_sum256(double v, double* sh4) {
@@ -41,14 +44,15 @@ __device__ __forceinline__ double block_sum256(double v, double* sh4) {
 
 __global__ void __launch_bounds__(256) k_colsum(int n, int ncol, const double* __restrict__ in,
                                                 double* __restrict__ part) {
-  __shared__ double sh4[4];
-  for (int c = 0; c < ncol; ++c) {
-    double v = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-      v += in[(size_t)c * n + i];
-    v = block_sum256(v, sh4);
-    if (threadIdx.x == 0) part[(size_t)blockIdx.x * ncol + c] = v;
-  }
+  __shared__ double sh[4 * kRedMaxCols];
+  colsum_partials(n, ncol, in, part, sh);
+}
+
+// out[0..ncol) = sums of the block partials in block order (one block)
+__global__ void __launch_bounds__(256) k_colsum_final(int nb, int ncol, const double* __restrict__ part,
+                                                      double* __restrict__ out) {
+  __shared__ double sh[4 * kRedMaxCols];
+  sum_block_partials(nb, ncol, part, out, sh);
 }
 
 // total force f = f_sr + (f_rec + f_dir) * coul_unit (src/ewald.f90:237-246);
@@ -73,7 +77,8 @@ __global__ void k_combine_forces(int n, const double* __restrict__ fsr, const do
 // ef_surf = -(sum q r + sum mu) 4pi/3V when lsurf; ef_self = 4 alpha^3/(3 sqrt(pi)) mu
 // total field of every atom: ef_dir + ef_rec (+ self term of the dipoles)
 // (+ surface field -(sum q r | sum mu) 4pi/3V, from the device moments
-// mom = [musq, mutot(3), qtot(3)]; which: 0 both, 1 charges, 2 dipoles)
+// mom = [musq, mutot(3), qtot(3)]; which: 0 both, 1 charges, 2 dipoles);
+// null edir / erec count as zero fields
 __global__ void k_combine_field(int n, const double* __restrict__ edir, const double* __restrict__ erec,
                                 const double* __restrict__ mu, double self2, int lsurf,
                                 const double* __restrict__ mom, int which, double fpi_3V,
@@ -81,7 +86,7 @@ __global__ void k_combine_field(int n, const double* __restrict__ edir, const do
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int a = 0; a < 3; ++a) {
-    double e = edir[3 * i + a] + erec[3 * i + a];
+    double e = (edir ? edir[3 * i + a] : 0.0) + (erec ? erec[3 * i + a] : 0.0);
     if (lsurf) {
       const double m = which == 0 ? mom[4 + a] + mom[1 + a] : (which == 1 ? mom[4 + a] : mom[1 + a]);
       e += -m * fpi_3V;
@@ -89,18 +94,6 @@ __global__ void k_combine_field(int n, const double* __restrict__ edir, const do
     if (mu) e += self2 * mu[3 * i + a];
     ef[3 * i + a] = e;
   }
-}
-
-// column sums of the block partials: one 256-thread block per column, fixed
-// order (deterministic): out[k] = sum_b part[b][k]
-__global__ void __launch_bounds__(256) k_colsum_final(int nb, int ncol, const double* __restrict__ part,
-                                                      double* __restrict__ out) {
-  __shared__ double sh4[4];
-  const int k = blockIdx.x;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(size_t)b * ncol + k];
-  s = block_sum256(s, sh4);
-  if (threadIdx.x == 0) out[k] = s;
 }
 
 // per-atom columns for the self and surface terms: mu^2, mu, q r
@@ -330,31 +323,30 @@ inline int nblk(int n) { return div_up(n, 256); }
 
 // ------------------------------------------------------------------ helpers
 
+static int fail(mdff_ctx* c, int code, const char* msg) {
+  c->err = msg;
+  return code;
+}
+
 // deterministic column sums of a [ncol][n] device array; result on the host
 int reduce_cols(mdff_ctx* c, const double* in, int ncol, double* host_out) {
-  const int nb = std::min(1024, std::max(1, div_up(c->n, 256)));
-  hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, c->stream, c->n, ncol, in, c->red_part);
-  HIP_TRY(c, hipMemcpyAsync(c->h_red, c->red_part, sizeof(double) * nb * ncol, hipMemcpyDeviceToHost, c->stream));
+  MDFF_TRY(reduce_cols_dev(c, in, ncol, c->red));
+  HIP_TRY(c, hipMemcpyAsync(c->h_red, c->red, sizeof(double) * ncol, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  for (int k = 0; k < ncol; ++k) host_out[k] = 0.0;
-  for (int b = 0; b < nb; ++b)
-    for (int k = 0; k < ncol; ++k) host_out[k] += c->h_red[(size_t)b * ncol + k];
+  for (int k = 0; k < ncol; ++k) host_out[k] = c->h_red[k];
   return MDFF_OK;
 }
 
 // the same sums left in device memory (dst[0..ncol)), no host round trip
 int reduce_cols_dev(mdff_ctx* c, const double* in, int ncol, double* dst) {
+  if (ncol < 1 || ncol > kRedMaxCols) return fail(c, MDFF_ERR_BAD_ARG, "column sum over too many columns");
   const int nb = std::min(1024, std::max(1, div_up(c->n, 256)));
   hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, c->stream, c->n, ncol, in, c->red_part);
-  hipLaunchKernelGGL(k_colsum_final, dim3(ncol), dim3(256), 0, c->stream, nb, ncol, c->red_part, dst);
+  hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(256), 0, c->stream, nb, ncol, c->red_part, dst);
   HIP_TRY(c, hipGetLastError());
   return MDFF_OK;
 }
 
-static int fail(mdff_ctx* c, int code, const char* msg) {
-  c->err = msg;
-  return code;
-}
 
 // 3x3 inverse (replaces LAPACK DGETRF/DGETRI of src/read_pos.f90:203-212)
 static void inv3(const double* m, double* o) {
@@ -516,6 +508,9 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   DMALLOC(c->nl_start, n + 1);
   DMALLOC(c->d_flag, 4);
   DMALLOC(c->d_disp, 2 * (size_t)nblk(n) + 2);
+  DMALLOC(c->cx, n); DMALLOC(c->cy, n); DMALLOC(c->cz, n);
+  HIP_TRY(c, hipHostMalloc((void**)&c->h_chk, sizeof(double) * (2 * (size_t)nblk(n) + 2)));
+  HIP_TRY(c, hipEventCreateWithFlags(&c->chk_ev, hipEventDisableTiming));
   HIP_TRY(c, hipHostMalloc((void**)&c->h_flag, sizeof(int32_t) * 4));
   const size_t nred = std::max<size_t>(64 * 1024, 2 * (size_t)nblk(n) + 2);
   HIP_TRY(c, hipHostMalloc((void**)&c->h_red, sizeof(double) * nred));
@@ -760,38 +755,78 @@ static int scf_kO_v2(mdff_ctx* c, mdff_terms* t) {
   return MDFF_OK;
 }
 
-static void ev_rec(mdff_ctx* c, int k) { (void)hipEventRecord(c->ev[k], c->stream); }
+// stage events of one evaluation, recorded only while the event profiler is
+// on (every event record costs a few microseconds of stream time)
+static void ev_rec(mdff_ctx* c, int k) {
+  if (c->prof_on) (void)hipEventRecord(c->ev[k], c->stream);
+}
+
+// whether an evaluation needs the moment sums (musq, sum mu, sum q r):
+// only with dipoles (self energy) or the surface term
+static bool has_moments(const mdff_ctx* c) { return c->task_dd || c->d.lsurf; }
 
 // SUBROUTINE engforce (src/engforce_driver.f90:102-190), launch half: every
 // kernel of the force evaluation is queued; the sums the energies, virials
 // and stress are made of stay in device memory at raw[0..RAW_N) and the terms
 // known on the host (list rebuild, SCF) go to t.  engforce_finish turns the
 // sums into the terms once they are on the host.
+// Stage events of the profiler: [0] total, [1] lists, [2] short range +
+// real space, [3] reciprocal, [4] SCF; stage k between events 2k and 2k+1.
+enum { EVS_TOTAL = 0, EVS_LIST = 1, EVS_REAL = 2, EVS_RECIP = 3, EVS_SCF = 4 };
+
 static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t) {
   const mdff_desc& d = c->d;
   const int n = c->n;
   memset(t, 0, sizeof *t);
-  ev_rec(c, 0);
+  ev_rec(c, 2 * EVS_TOTAL);
+  const bool coul = d.lcoulomb != 0;
+  const size_t n3 = 3 * (size_t)n;
+  const bool dip = c->task_qd || c->task_dd;
+  const bool want_field = coul && d.doefield;
+  // Without induced dipoles the reciprocal sums depend on the positions
+  // only: they are queued first, so that a list check still pending from the
+  // previous step (its flag on the way to the host) is finished while the
+  // device works on them.
+  const bool recip_first = coul && !c->linduced;
+  auto recip_stage = [&]() -> int {
+    ev_rec(c, 2 * EVS_RECIP);
+    if (coul) MDFF_TRY(recip_eval(c, c->q, c->mu, true, dip, true, want_field, ef_rec(c), c->frec, raw + RAW_EREC));
+    ev_rec(c, 2 * EVS_RECIP + 1);
+    return MDFF_OK;
+  };
+  if (recip_first) {
+    launch_kardir(c);
+    MDFF_TRY(recip_tables(c));
+    if (!c->mu_static) {
+      // no induced dipoles: mu is the static set for the life of the context
+      HIP_TRY(c, hipMemcpyAsync(c->mu, c->dipia, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+      c->mu_static = 1;
+    }
+    MDFF_TRY(recip_stage());
+  }
+  ev_rec(c, 2 * EVS_LIST);
+  int rb = 0;
+  MDFF_TRY(nl_check_finish(c, &rb));
   if (!d.lvnlist || !c->lists_valid) {
     MDFF_TRY(nl_build(c));
     t->list_rebuilt = 1;
-  } else {
+  } else if (!recip_first) {
     launch_kardir(c);
   }
   pack_atoms(c);
-  ev_rec(c, 1);
-  const bool coul = d.lcoulomb != 0;
-  if (coul) MDFF_TRY(recip_tables(c));
+  ev_rec(c, 2 * EVS_LIST + 1);
+  if (!recip_first && coul) MDFF_TRY(recip_tables(c));
   // induced dipoles (get_dipole_moments, src/moment_from_pola.f90:84-224)
-  const size_t n3 = 3 * (size_t)n;
+  ev_rec(c, 2 * EVS_SCF);
   if (coul && c->linduced) {
     if (d.algo_moment_from_pola == 1) MDFF_TRY(scf_kO_v2(c, t));
     else MDFF_TRY(scf(c, t));
     hipLaunchKernelGGL(k_add3, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, c->dipia, c->mu_ind, c->mu);
-  } else {
+  } else if (!c->mu_static) {
     HIP_TRY(c, hipMemcpyAsync(c->mu, c->dipia, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+    c->mu_static = 1;
   }
-  ev_rec(c, 2);
+  ev_rec(c, 2 * EVS_SCF + 1);
   if (coul && c->linduced) {
     hipLaunchKernelGGL(k_maxcomp, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->mu, extra(c));
     double h[1];
@@ -799,33 +834,29 @@ static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t) {
     if (h[0] > 0.0) return fail(c, MDFF_ERR_POLAR_CATASTROPHE, "ERROR : POLARIZATION CATASTROPH");
   }
   // short range + real-space Ewald
-  const bool want_field = coul && d.doefield;
+  ev_rec(c, 2 * EVS_REAL);
   MDFF_TRY(realspace_force_impl(c, f_sr(c), f_dir(c), ef_dir(c), want_field));
-  ev_rec(c, 3);
-  if (coul) {
-    const bool dip = c->task_qd || c->task_dd;
-    MDFF_TRY(recip_eval(c, c->q, c->mu, true, dip, true, want_field, ef_rec(c), c->frec, raw + RAW_EREC));
-  }
-  ev_rec(c, 4);
+  ev_rec(c, 2 * EVS_REAL + 1);
+  if (!recip_first) MDFF_TRY(recip_stage());
   hipLaunchKernelGGL(k_combine_forces, dim3(nblk(n)), dim3(256), 0, c->stream, n, f_sr(c), f_dir(c), c->frec,
                      d.coul_unit, c->sr_model != 0 ? 1 : 0, coul ? 1 : 0, c->fx, c->fy, c->fz);
   MDFF_TRY(reduce_cols_dev(c, c->peratom, NPA, raw + RAW_PA));
   if (coul) {
-    MDFF_TRY(moments_dev(c, c->q, c->mu, raw + RAW_MOM));
+    // moments: sum mu^2 for the dipole self energy, sum mu and sum q r for the
+    // surface terms; with neither dipoles nor lsurf they are all zero
+    if (has_moments(c)) MDFF_TRY(moments_dev(c, c->q, c->mu, raw + RAW_MOM));
     // ef_t = (ef_dir + ef_rec if doefield) + ef_self (+ ef_surf)
     const double alpha2 = c->alpha * c->alpha;
     const double selfa = c->alpha / sqrt(MDFF_PI), selfa2 = 2.0 * selfa * alpha2 / 3.0;
     const double fpi_3V = 2.0 * (2.0 * MDFF_PI / c->omega / 3.0);
-    if (!want_field) {
-      hipLaunchKernelGGL(k_fill, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, 0.0, ef_dir(c));
-      hipLaunchKernelGGL(k_fill, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, 0.0, ef_rec(c));
-    }
-    hipLaunchKernelGGL(k_combine_field, dim3(nblk(n)), dim3(256), 0, c->stream, n, ef_dir(c), ef_rec(c), c->mu,
-                       2.0 * selfa2, d.lsurf, raw + RAW_MOM, 0, fpi_3V, c->efield);
+    hipLaunchKernelGGL(k_combine_field, dim3(nblk(n)), dim3(256), 0, c->stream, n,
+                       want_field ? ef_dir(c) : nullptr, want_field ? ef_rec(c) : nullptr, c->mu, 2.0 * selfa2,
+                       d.lsurf, raw + RAW_MOM, 0, fpi_3V, c->efield);
   }
-  ev_rec(c, 5);
+  ev_rec(c, 2 * EVS_TOTAL + 1);
   HIP_TRY(c, hipGetLastError());
-  c->timings_pending = 1;
+  c->timings_pending = c->prof_on ? 1 : 0;
+  if (!c->prof_on) c->n_last = 0;
   return MDFF_OK;
 }
 
@@ -848,9 +879,11 @@ static void engforce_finish(const mdff_ctx* c, const double* raw, mdff_terms* t)
   if (c->sr_model == 3) { t->u_morse = usr; t->vir_morse = vir; }
   if (c->sr_model) for (int k = 0; k < 9; ++k) { t->tau_nonb[k] = tnb[k]; t->tau[k] += tnb[k]; }
   if (d.lcoulomb) {
-    const double musq = raw[RAW_MOM];
-    const double* mutot = raw + RAW_MOM + 1;
-    const double* qtot = raw + RAW_MOM + 4;
+    static const double zero7[7] = {0, 0, 0, 0, 0, 0, 0};
+    const double* mom = has_moments(c) ? raw + RAW_MOM : zero7;
+    const double musq = mom[0];
+    const double* mutot = mom + 1;
+    const double* qtot = mom + 4;
     const double qsq = c->qsq;
     const double tpi_V = 2.0 * MDFF_PI / c->omega, tpi_3V = tpi_V / 3.0;
     const double alpha2 = c->alpha * c->alpha;
@@ -902,12 +935,13 @@ static int engforce(mdff_ctx* c, mdff_terms* t) {
 static void read_timings(mdff_ctx* c) {
   if (!c->timings_pending) return;
   if (hipStreamSynchronize(c->stream) != hipSuccess) return;
-  float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[5]); c->last_ms[0] = ms;
-  (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]); c->last_ms[1] = ms;
-  (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]); c->last_ms[2] = ms;
-  (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]); c->last_ms[3] = ms;
-  (void)hipEventElapsedTime(&ms, c->ev[1], c->ev[2]); c->last_ms[4] = ms;
+  // stage k of last_ms: events (2k, 2k+1); a stage that does not run
+  // records both at the same point
+  for (int k = 0; k < 5; ++k) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]);
+    c->last_ms[k] = ms;
+  }
   c->n_last = 5;
   c->timings_pending = 0;
 }
@@ -981,7 +1015,7 @@ int mdff_ctx_destroy(mdff_ctx* c) {
                 c->pol, c->invpol, c->dipia, c->mu, c->mu_ind, c->efield, c->e_stat, c->e_tot, c->e_ind,
                 c->hist, c->frec, c->zeros, c->arec, c->ameta, c->peratom, c->red, c->red_part, c->d_pp, c->cell_of, c->cell_key,
                 c->cell_atoms, c->cell_atoms_in, c->cell_start, c->nl_count, c->nl_start, c->nl_j,
-                c->nl_flag, c->sort_tmp, c->d_flag, c->d_disp};
+                c->nl_flag, c->sort_tmp, c->d_flag, c->d_disp, c->raw, c->cx, c->cy, c->cz};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c->polar) (void)hipFree(c->polar);
@@ -989,6 +1023,8 @@ int mdff_ctx_destroy(mdff_ctx* c) {
   if (c->h_flag) (void)hipHostFree(c->h_flag);
   if (c->h_red) (void)hipHostFree(c->h_red);
   if (c->h_raw) (void)hipHostFree(c->h_raw);
+  if (c->h_chk) (void)hipHostFree(c->h_chk);
+  if (c->chk_ev) (void)hipEventDestroy(c->chk_ev);
   if (c->kO_ws) (void)hipFree(c->kO_ws);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1106,10 +1142,9 @@ int mdff_md_vv(mdff_ctx* c, int32_t nsteps, double dt, double* ekin, double* upo
     hipLaunchKernelGGL(k_vv_vel, dim3(nblk(n)), dim3(256), 0, c->stream, n, dt * 0.5, c->mass, c->vx, c->vy,
                        c->vz, c->fsx, c->fsy, c->fsz, c->fx, c->fy, c->fz, extra(c));
     if (ekin) MDFF_TRY(reduce_cols_dev(c, extra(c), 1, raw + RAW_EKIN));
-    if (c->d.lvnlist) {
-      int rb = 0;
-      MDFF_TRY(nl_check(c, &rb));
-    }
+    // vnlistcheck after the step: queued now, finished by the next step's
+    // engforce_launch once that step's reciprocal work is queued (or below)
+    if (c->d.lvnlist) MDFF_TRY(nl_check_launch(c));
     if (slot == RAW_SLOTS - 1 || st == nsteps - 1) {
       MDFF_TRY(fetch_raw(c, slot + 1));
       for (int k = 0; k <= slot; ++k) {
@@ -1121,6 +1156,8 @@ int mdff_md_vv(mdff_ctx* c, int32_t nsteps, double dt, double* ekin, double* upo
       st0 = st + 1;
     }
   }
+  int rb = 0;
+  MDFF_TRY(nl_check_finish(c, &rb));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return MDFF_OK;
 }

@@ -36,11 +36,13 @@ __global__ void k_kardir(int n, const double* __restrict__ rx, const double* __r
   sz[i] = x * b13 + y * b23 + z * b33;
 }
 
-void launch_kardir(mdff_ctx* c) {
+static void kardir_to(mdff_ctx* c, double* ox, double* oy, double* oz) {
   const double* B = c->B;
   hipLaunchKernelGGL(k_kardir, dim3(div_up(c->n, 256)), dim3(256), 0, c->stream, c->n, c->rx, c->ry,
-                     c->rz, B[0], B[1], B[2], B[3], B[4], B[5], B[6], B[7], B[8], c->sx, c->sy, c->sz);
+                     c->rz, B[0], B[1], B[2], B[3], B[4], B[5], B[6], B[7], B[8], ox, oy, oz);
 }
+
+void launch_kardir(mdff_ctx* c) { kardir_to(c, c->sx, c->sy, c->sz); }
 
 // atoms in the cell order of the last list build: rec[p], meta[p] of atom
 // order[p] (the rows store sorted positions p, so neighbours that are close
@@ -217,9 +219,9 @@ static NLArgs make_args(mdff_ctx* c) {
 }
 
 // vnlist_pbc: rebuild the rows from the current positions.
-int nl_build(mdff_ctx* c) {
+// list build from the fractional coordinates (px, py, pz) of the build
+static int nl_build_from(mdff_ctx* c, const double* px, const double* py, const double* pz) {
   const int n = c->n;
-  launch_kardir(c);
   // cells: width >= cutoff in every reciprocal direction (1e-9 margin against
   // rounding of the wrapped coordinates)
   double rmax = std::max(c->rc_vdw, c->rc_coul) + c->d.skindiff;
@@ -233,8 +235,8 @@ int nl_build(mdff_ctx* c) {
     HIP_TRY(c, hipMalloc((void**)&c->cell_start, sizeof(int32_t) * (ncell + 1)));
     c->ncell_alloc = ncell + 1;
   }
-  hipLaunchKernelGGL(k_cell_assign, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, c->sx, c->sy,
-                     c->sz, c->ncell[0], c->ncell[1], c->ncell[2], c->cell_of, c->cell_atoms_in);
+  hipLaunchKernelGGL(k_cell_assign, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, px, py, pz,
+                     c->ncell[0], c->ncell[1], c->ncell[2], c->cell_of, c->cell_atoms_in);
   int bits = 1;
   while ((1 << bits) <= ncell) ++bits;
   size_t tmp = 0;
@@ -251,8 +253,9 @@ int nl_build(mdff_ctx* c) {
   // fractional coordinates in cell order (coalesced candidate loads); they
   // are kept as the positions of this build
   hipLaunchKernelGGL(k_gather_sorted, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, c->cell_atoms,
-                     c->sx, c->sy, c->sz, c->bx, c->by, c->bz);
+                     px, py, pz, c->bx, c->by, c->bz);
   NLArgs a = make_args(c);
+  a.sx = px; a.sy = py; a.sz = pz;
   a.px = c->bx; a.py = c->by; a.pz = c->bz;
   dim3 grid(div_up((int64_t)n * 64, 256));
   HIP_TRY(c, hipMemsetAsync(c->nl_count + n, 0, sizeof(int32_t), c->stream));
@@ -286,6 +289,11 @@ int nl_build(mdff_ctx* c) {
   c->nl_total = total;
   c->lists_valid = 1;
   return MDFF_OK;
+}
+
+int nl_build(mdff_ctx* c) {
+  launch_kardir(c);
+  return nl_build_from(c, c->sx, c->sy, c->sz);
 }
 
 // per-atom displacement since the saved fractional coordinates, reduced to
@@ -329,31 +337,53 @@ __global__ void k_copy3(int n, const double* a, const double* b, const double* c
 }
 
 // vnlistcheck: rebuild when (largest + second largest displacement) > skindiff,
-// then save the fractional coordinates (src/tools.f90:513-526).
-int nl_check(mdff_ctx* c, int* rebuilt) {
+// then save the fractional coordinates (src/tools.f90:513-526).  Split in two
+// halves so that a caller can queue more work between them: the launch half
+// queues the displacement check of the current positions (their fractional
+// coordinates go to cx, cy, cz) and the copy of its block maxima to the host;
+// the finishing half waits for that copy and, when the skin is exceeded,
+// builds the lists from those same coordinates (the positions of the check,
+// whatever the positions are by then).
+int nl_check_launch(mdff_ctx* c) {
   const int n = c->n;
-  launch_kardir(c);
+  kardir_to(c, c->cx, c->cy, c->cz);
   NLArgs a = make_args(c);
-  int nb = div_up(n, 256);
+  a.sx = c->cx; a.sy = c->cy; a.sz = c->cz;
+  const int nb = div_up(n, 256);
   hipLaunchKernelGGL(k_disp_top2, dim3(nb), dim3(256), 0, c->stream, a, c->xs, c->ys, c->zs, c->d_disp);
-  HIP_TRY(c, hipMemcpyAsync(c->h_red, c->d_disp, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpyAsync(c->h_chk, c->d_disp, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipEventRecord(c->chk_ev, c->stream));
+  c->chk_pending = 1;
+  return MDFF_OK;
+}
+
+int nl_check_finish(mdff_ctx* c, int* rebuilt) {
+  *rebuilt = 0;
+  if (!c->chk_pending) return MDFF_OK;
+  c->chk_pending = 0;
+  const int n = c->n;
+  const int nb = div_up(n, 256);
+  HIP_TRY(c, hipEventSynchronize(c->chk_ev));
   double m1 = 0.0, m2 = 0.0;
   for (int b = 0; b < nb; ++b) {
-    double v[2] = {c->h_red[2 * b], c->h_red[2 * b + 1]};
+    double v[2] = {c->h_chk[2 * b], c->h_chk[2 * b + 1]};
     for (double d : v) {
       if (d > m1) { m2 = m1; m1 = d; }
       else if (d > m2) m2 = d;
     }
   }
-  *rebuilt = 0;
   if (m1 + m2 > c->d.skindiff) {
-    MDFF_TRY(nl_build(c));
-    hipLaunchKernelGGL(k_copy3, dim3(nb), dim3(256), 0, c->stream, n, c->sx, c->sy, c->sz, c->xs, c->ys,
+    MDFF_TRY(nl_build_from(c, c->cx, c->cy, c->cz));
+    hipLaunchKernelGGL(k_copy3, dim3(nb), dim3(256), 0, c->stream, n, c->cx, c->cy, c->cz, c->xs, c->ys,
                        c->zs);
     *rebuilt = 1;
   }
   return MDFF_OK;
+}
+
+int nl_check(mdff_ctx* c, int* rebuilt) {
+  MDFF_TRY(nl_check_launch(c));
+  return nl_check_finish(c, rebuilt);
 }
 
 // Reference CSR half lists from the device rows: ownership rule

@@ -17,6 +17,7 @@
 //            field = 2 (4 pi/V) 2 pi B U1; force = 2 (4 pi/V) [q_i 2 pi B U1
 //            + (2 pi)^2 B M m_i]                               (:196-233, :300-358)
 #include "common.h"
+#include "reduce.h"
 
 namespace {
 
@@ -168,9 +169,11 @@ __global__ void __launch_bounds__(256) k_sk_reduce(int nk_box, int nchunk, const
                                                    const double* __restrict__ kvec,
                                                    double* __restrict__ sk, double* __restrict__ wk,
                                                    T* __restrict__ wk_t, double* __restrict__ blk) {
-  __shared__ double sh[7][256];
+  __shared__ double sh[4 * kRedMaxCols];
   int k = blockIdx.x * blockDim.x + threadIdx.x;
-  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  double v[kRedMaxCols];
+#pragma unroll
+  for (int c = 0; c < kRedMaxCols; ++c) v[c] = 0.0;
   if (k < nk_box) {
     double re = 0.0, im = 0.0;
     for (int c = 0; c < nchunk; ++c) {
@@ -194,14 +197,14 @@ __global__ void __launch_bounds__(256) k_sk_reduce(int nk_box, int nchunk, const
     v[5] = -kc * kx * kz * str;
     v[6] = -kc * ky * kz * str;
   }
-  for (int c = 0; c < 7; ++c) sh[c][threadIdx.x] = v[c];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int c = 0; c < 7; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x < 7) blk[blockIdx.x * 7 + threadIdx.x] = sh[threadIdx.x][0];
+  const double r = block_sum_cols(v, 7, sh);
+  if (threadIdx.x < 7) blk[blockIdx.x * 7 + threadIdx.x] = r;
+}
+
+// e[0..7) = sums of the k-block partials in block order (one block)
+__global__ void __launch_bounds__(256) k_sum7(int nb, const double* __restrict__ part, double* __restrict__ e) {
+  __shared__ double sh[4 * kRedMaxCols];
+  sum_block_partials(nb, 7, part, e, sh);
 }
 
 struct GaArgs {
@@ -732,19 +735,6 @@ __global__ void k_sk_combine(int nk_box, int NZ, int Kz, int nchunk, double* __r
   p1[2 * x + 1] += fz * s2r;
 }
 
-// e[k] = sum_b part[b][k], k < 7: one 256-thread block per column, fixed order
-__global__ void __launch_bounds__(256) k_colsum7(int nb, const double* __restrict__ part, double* __restrict__ e) {
-  __shared__ double sh4[4];
-  const int k = blockIdx.x;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(size_t)7 * b + k];
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-  if ((threadIdx.x & 63) == 0) sh4[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) e[k] = ((sh4[0] + sh4[1]) + sh4[2]) + sh4[3];
-}
-
 }  // namespace
 
 // K-step counts compiled for the matrix-core gather (Kz + 1 rounded up)
@@ -1025,6 +1015,7 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     dim3 g1(div_up((int64_t)s.M * s.NG, 256), chunks);
     PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk<T, GZ>), g1, dim3(256), shm, c->stream, s));
   }
+  // the 7 energy / stress sums over the k-blocks stay in device memory (e_out)
   const int nb = div_up(c->nk_box, 256);
   PROF_LAUNCH(c, PK_SKRED,
               if (sk_fp64) {
@@ -1036,10 +1027,7 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
                                    reinterpret_cast<const T*>(c->sk_part), c->kA, c->kvec, c->sk, c->wk,
                                    reinterpret_cast<T*>(c->wk_t), c->red_part);
               });
-  if (e_out) {
-    // 7 sums of the k-blocks in block order, left in device memory
-    hipLaunchKernelGGL(k_colsum7, dim3(7), dim3(256), 0, c->stream, nb, c->red_part, e_out);
-  }
+  if (e_out) hipLaunchKernelGGL(k_sum7, dim3(1), dim3(256), 0, c->stream, nb, c->red_part, e_out);
   if (!forces && !field) return MDFF_OK;
   const bool dipf = forces && use_mu;
   int nsplit = 0, moments = dipf ? 9 : 3;

@@ -268,6 +268,31 @@ def test_vv_trajectory_and_skin_checks_match():
         _lists_equal(g, o, which)
 
 
+def test_vv_many_steps_in_one_call_match():
+    """The same 30 steps in one mdff_md_vv call, where each step's list check
+    is finished during the next step: energies per step, final positions,
+    velocities and the lists (built at the positions of the check that asked
+    for them) match the oracle stepping one call per step."""
+    s = configs.nacl(ncell=2, seed=5, temp_K=600.0)
+    s.desc.skindiff = 0.3
+    note("vv trajectory nacl, one call")
+    g, o = pair(s)
+    g.engforce()
+    o.engforce()
+    eg, ug = g.md_vv(30, s.dt)
+    eo, uo = [], []
+    for step in range(30):
+        e1, u1 = o.md_vv(1, s.dt)
+        eo.append(e1[0])
+        uo.append(u1[0])
+    assert np.max(np.abs(np.asarray(eg) - eo)) < 1e-9 * np.max(np.abs(eo))
+    assert np.max(np.abs(np.asarray(ug) - uo)) < 1e-9 * np.max(np.abs(uo))
+    assert rel(g.get_positions(), o.get_positions()) < 1e-10
+    assert rel(g.get_velocities(), o.get_velocities()) < 1e-8
+    for which in (0, 1):
+        _lists_equal(g, o, which)
+
+
 def test_nhcn_and_csvr_match():
     """nose_hoover_chain_n (src/integration.f90:538-701) and the CSVR rescale
     (src/kinetic.f90:122-198) with caller-supplied noise."""
