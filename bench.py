@@ -43,14 +43,14 @@ FLOPS_PER_UNIT = {
     "k_pair_force": 106.0, # per in-cutoff Buckingham + Ewald q-q pair
 }
 # the kernel behind each profiler class on the default (matrix-core) path
-KERNEL_OF = {"k_sk": "k_sk_mma2<float,2>", "k_gather": "k_gather_mma<float,CT,NKS>",
+KERNEL_OF = {"k_sk": "k_sk_mma2<float,1,TA>", "k_gather": "k_gather_mma<float,CT,NKS>",
              "k_pair_force": "k_pair_force<float,false,16>"}
 # HBM bytes per launch of the mixed-form kernels on this workload: rocprofv3
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (KiB per dispatch x 1024; the factors
 # calibrate the counters on a 256 MiB device-to-device copy), measured with
 # mdff_amd.profile_workload (profiles/engforce_sio2_24k_hbm_traffic.md)
 TRAFFIC = {"k_gather": 86.93e6, "k_sk": 168.80e6, "k_pair_force": 65.15e6}
-NKINDS = 9   # kernel classes of the library's event profiler (mdff_profile_name)
+NKINDS = 8   # kernel classes of the library's event profiler (mdff_profile_name)
 
 
 def build_system(natm: int, relax_iters: int):

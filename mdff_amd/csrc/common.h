@@ -82,7 +82,7 @@ struct DevBuf {
 };
 
 // kernel classes timed by the in-library event profiler
-enum ProfKind { PK_PAIR = 0, PK_FIELD, PK_TABLES, PK_SK, PK_SKRED, PK_GATHER, PK_GFINAL, PK_NLIST, PK_AMAT, PK_COUNT };
+enum ProfKind { PK_PAIR = 0, PK_FIELD, PK_TABLES, PK_SK, PK_SKRED, PK_GATHER, PK_GFINAL, PK_NLIST, PK_COUNT };
 struct ProfRec {
   int kind;
   hipEvent_t a, b;
@@ -96,6 +96,16 @@ constexpr int RAW_EKIN = 29;     // sum m v^2
 constexpr int RAW_SCRATCH = 32;  // 7 moments of the SCF field (slot 0 only)
 constexpr int RAW_N = 40;
 constexpr int RAW_SLOTS = 1024;  // evaluations held before the sums go to the host
+
+// A deferred fixed-order final sum: out[0..ncol) = sums over nb block
+// partials part[b * ncol + c].  Queued finals run together in one launch
+// (one block each) at flush_finals.
+struct FinalSum {
+  const double* part;
+  double* out;
+  int nb, ncol;
+};
+constexpr int kMaxFinals = 8;
 
 // one atom of the pair kernels, in cell-sorted order: fractional coordinates
 // and charge (two 16-byte loads), plus its type and original index
@@ -149,6 +159,11 @@ struct mdff_ctx {
   hipEvent_t chk_ev = nullptr;            // the pending check's copy to h_chk is complete
   int chk_pending = 0;
   double* red_part = nullptr;             // reduction block partials
+  double* fin_part = nullptr;             // block partials of the pair sums [0, 1024*16) and kinetic energy
+  double* rec_blk = nullptr;              // k-block partials of the reciprocal energy and stress
+  FinalSum fin[kMaxFinals] = {};          // queued final sums
+  int nfin = 0;
+  int efield_zero = 0;                    // efield holds zeros (no field, dipoles or surface term)
   PairParam* d_pp = nullptr;
   int itime = 0;
   int32_t* polar = nullptr;               // 1 for polarisable ions
@@ -254,12 +269,17 @@ int nl_check(mdff_ctx* c, int* rebuilt);
 int nl_check_launch(mdff_ctx* c);
 int nl_check_finish(mdff_ctx* c, int* rebuilt);
 int reduce_cols_dev(mdff_ctx* c, const double* in, int ncol, double* dst);
+int reduce_cols_part(mdff_ctx* c, const double* in, int ncol, double* part, int* nb);
+int queue_final(mdff_ctx* c, const double* part, int nb, int ncol, double* out);
+int flush_finals(mdff_ctx* c);
+int claim_partials(mdff_ctx* c, const double* part);
 int nl_get_pairs(mdff_ctx* c, int which, int32_t* point, int32_t* list, int64_t cap, int64_t* n);
 void launch_kardir(mdff_ctx* c);
 void pack_atoms(mdff_ctx* c);
 int realspace_field(mdff_ctx* c, int mode, const double* mu_in, double* ef_out, double alpha = 0.0);
 int recip_setup(mdff_ctx* c);
 int recip_tables(mdff_ctx* c);
+int recip_tables_from_r(mdff_ctx* c);     // kardir + tables in one launch
 int recip_eval(mdff_ctx* c, const double* q, const double* mu, bool use_q, bool use_mu,
                bool forces, bool field, double* ef_out, double* f_out, double* e_out);
 void recip_free(mdff_ctx* c);

@@ -55,6 +55,16 @@ __global__ void __launch_bounds__(256) k_colsum_final(int nb, int ncol, const do
   sum_block_partials(nb, ncol, part, out, sh);
 }
 
+// the queued final sums, one block each
+struct FinalArgs {
+  FinalSum f[kMaxFinals];
+};
+__global__ void __launch_bounds__(256) k_final_sums(FinalArgs a) {
+  __shared__ double sh[4 * kRedMaxCols];
+  const FinalSum f = a.f[blockIdx.x];
+  sum_block_partials(f.nb, f.ncol, f.part, f.out, sh);
+}
+
 // total force f = f_sr + (f_rec + f_dir) * coul_unit (src/ewald.f90:237-246);
 // inputs AoS, output SoA
 __global__ void k_combine_forces(int n, const double* __restrict__ fsr, const double* __restrict__ fdir,
@@ -261,18 +271,28 @@ __global__ void k_vv_pos(int n, double dt, double dtsq2, const double* __restric
 }
 
 // velocity update (:179-183) + per-atom kinetic energy column
-__global__ void k_vv_vel(int n, double dt2, const double* __restrict__ m, double* vx, double* vy, double* vz,
-                         const double* fsx, const double* fsy, const double* fsz, const double* fx,
-                         const double* fy, const double* fz, double* __restrict__ ke) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double a = vx[i] + (fsx[i] + fx[i]) * dt2 / m[i];
-  double b = vy[i] + (fsy[i] + fy[i]) * dt2 / m[i];
-  double cc = vz[i] + (fsz[i] + fz[i]) * dt2 / m[i];
-  vx[i] = a;
-  vy[i] = b;
-  vz[i] = cc;
-  ke[i] = (a * a + b * b + cc * cc) * m[i];
+
+// prop_velocity_verlet velocity update (src/integration.f90:178-182) with the
+// block partials of calc_temp's kinetic sum (:185): part[block] = sum over
+// the block's atoms of m v^2
+__global__ void __launch_bounds__(256) k_vv_vel_ke(int n, double dt2, const double* __restrict__ m, double* vx,
+                                                   double* vy, double* vz, const double* fsx, const double* fsy,
+                                                   const double* fsz, const double* fx, const double* fy,
+                                                   const double* fz, double* __restrict__ part) {
+  __shared__ double sh[4 * kRedMaxCols];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[kRedMaxCols] = {};
+  if (i < n) {
+    const double a = vx[i] + (fsx[i] + fx[i]) * dt2 / m[i];
+    const double b = vy[i] + (fsy[i] + fy[i]) * dt2 / m[i];
+    const double cc = vz[i] + (fsz[i] + fz[i]) * dt2 / m[i];
+    vx[i] = a;
+    vy[i] = b;
+    vz[i] = cc;
+    v[0] = (a * a + b * b + cc * cc) * m[i];
+  }
+  const double r = block_sum_cols(v, 1, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
 // prop_pos_vel_verlet (:420-427): half kick + drift
@@ -337,9 +357,46 @@ int reduce_cols(mdff_ctx* c, const double* in, int ncol, double* host_out) {
   return MDFF_OK;
 }
 
+// block partials of the column sums into part ([nb][ncol]); the final sum is
+// queued or launched by the caller
+int reduce_cols_part(mdff_ctx* c, const double* in, int ncol, double* part, int* nb_out) {
+  if (ncol < 1 || ncol > kRedMaxCols) return fail(c, MDFF_ERR_BAD_ARG, "column sum over too many columns");
+  MDFF_TRY(claim_partials(c, part));
+  const int nb = std::min(1024, std::max(1, div_up(c->n, 256)));
+  hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, c->stream, c->n, ncol, in, part);
+  *nb_out = nb;
+  return MDFF_OK;
+}
+
+int flush_finals(mdff_ctx* c) {
+  if (c->nfin == 0) return MDFF_OK;
+  FinalArgs a{};
+  for (int k = 0; k < c->nfin; ++k) a.f[k] = c->fin[k];
+  hipLaunchKernelGGL(k_final_sums, dim3(c->nfin), dim3(256), 0, c->stream, a);
+  c->nfin = 0;
+  HIP_TRY(c, hipGetLastError());
+  return MDFF_OK;
+}
+
+int queue_final(mdff_ctx* c, const double* part, int nb, int ncol, double* out) {
+  if (ncol < 1 || ncol > kRedMaxCols || nb < 1) return fail(c, MDFF_ERR_BAD_ARG, "bad final sum");
+  if (c->nfin == kMaxFinals) MDFF_TRY(flush_finals(c));
+  c->fin[c->nfin++] = FinalSum{part, out, nb, ncol};
+  return MDFF_OK;
+}
+
+// before a kernel writes block partials into part: run the queued finals
+// that still read it
+int claim_partials(mdff_ctx* c, const double* part) {
+  for (int k = 0; k < c->nfin; ++k)
+    if (c->fin[k].part == part) return flush_finals(c);
+  return MDFF_OK;
+}
+
 // the same sums left in device memory (dst[0..ncol)), no host round trip
 int reduce_cols_dev(mdff_ctx* c, const double* in, int ncol, double* dst) {
   if (ncol < 1 || ncol > kRedMaxCols) return fail(c, MDFF_ERR_BAD_ARG, "column sum over too many columns");
+  MDFF_TRY(claim_partials(c, c->red_part));
   const int nb = std::min(1024, std::max(1, div_up(c->n, 256)));
   hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, c->stream, c->n, ncol, in, c->red_part);
   hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(256), 0, c->stream, nb, ncol, c->red_part, dst);
@@ -516,6 +573,7 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   HIP_TRY(c, hipHostMalloc((void**)&c->h_red, sizeof(double) * nred));
   DMALLOC(c->red_part, nred);
   DMALLOC(c->red, 64);
+  DMALLOC(c->fin_part, 1024 * 16 + (size_t)nblk(n) + 16);
   DMALLOC(c->raw, (size_t)RAW_N * RAW_SLOTS);
   HIP_TRY(c, hipHostMalloc((void**)&c->h_raw, sizeof(double) * RAW_N * RAW_SLOTS));
   HIP_TRY(c, hipMemcpy(c->itype, it0.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
@@ -774,7 +832,7 @@ static bool has_moments(const mdff_ctx* c) { return c->task_dd || c->d.lsurf; }
 // real space, [3] reciprocal, [4] SCF; stage k between events 2k and 2k+1.
 enum { EVS_TOTAL = 0, EVS_LIST = 1, EVS_REAL = 2, EVS_RECIP = 3, EVS_SCF = 4 };
 
-static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t) {
+static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t, bool defer_finals) {
   const mdff_desc& d = c->d;
   const int n = c->n;
   memset(t, 0, sizeof *t);
@@ -795,8 +853,7 @@ static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t) {
     return MDFF_OK;
   };
   if (recip_first) {
-    launch_kardir(c);
-    MDFF_TRY(recip_tables(c));
+    MDFF_TRY(recip_tables_from_r(c));         // kardir and the phase tables in one launch
     if (!c->mu_static) {
       // no induced dipoles: mu is the static set for the life of the context
       HIP_TRY(c, hipMemcpyAsync(c->mu, c->dipia, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
@@ -840,19 +897,27 @@ static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t) {
   if (!recip_first) MDFF_TRY(recip_stage());
   hipLaunchKernelGGL(k_combine_forces, dim3(nblk(n)), dim3(256), 0, c->stream, n, f_sr(c), f_dir(c), c->frec,
                      d.coul_unit, c->sr_model != 0 ? 1 : 0, coul ? 1 : 0, c->fx, c->fy, c->fz);
-  MDFF_TRY(reduce_cols_dev(c, c->peratom, NPA, raw + RAW_PA));
+  int nbp = 0;
+  MDFF_TRY(reduce_cols_part(c, c->peratom, NPA, c->fin_part, &nbp));
+  MDFF_TRY(queue_final(c, c->fin_part, nbp, NPA, raw + RAW_PA));
   if (coul) {
     // moments: sum mu^2 for the dipole self energy, sum mu and sum q r for the
     // surface terms; with neither dipoles nor lsurf they are all zero
     if (has_moments(c)) MDFF_TRY(moments_dev(c, c->q, c->mu, raw + RAW_MOM));
-    // ef_t = (ef_dir + ef_rec if doefield) + ef_self (+ ef_surf)
-    const double alpha2 = c->alpha * c->alpha;
-    const double selfa = c->alpha / sqrt(MDFF_PI), selfa2 = 2.0 * selfa * alpha2 / 3.0;
-    const double fpi_3V = 2.0 * (2.0 * MDFF_PI / c->omega / 3.0);
-    hipLaunchKernelGGL(k_combine_field, dim3(nblk(n)), dim3(256), 0, c->stream, n,
-                       want_field ? ef_dir(c) : nullptr, want_field ? ef_rec(c) : nullptr, c->mu, 2.0 * selfa2,
-                       d.lsurf, raw + RAW_MOM, 0, fpi_3V, c->efield);
+    // ef_t = (ef_dir + ef_rec if doefield) + ef_self (+ ef_surf); without a
+    // field, dipoles or the surface term it is zero (written once)
+    const bool zero_field = !want_field && !has_moments(c);
+    if (!(zero_field && c->efield_zero)) {
+      const double alpha2 = c->alpha * c->alpha;
+      const double selfa = c->alpha / sqrt(MDFF_PI), selfa2 = 2.0 * selfa * alpha2 / 3.0;
+      const double fpi_3V = 2.0 * (2.0 * MDFF_PI / c->omega / 3.0);
+      hipLaunchKernelGGL(k_combine_field, dim3(nblk(n)), dim3(256), 0, c->stream, n,
+                         want_field ? ef_dir(c) : nullptr, want_field ? ef_rec(c) : nullptr, c->mu, 2.0 * selfa2,
+                         d.lsurf, raw + RAW_MOM, 0, fpi_3V, c->efield);
+    }
+    c->efield_zero = zero_field ? 1 : 0;
   }
+  if (!defer_finals) MDFF_TRY(flush_finals(c));
   ev_rec(c, 2 * EVS_TOTAL + 1);
   HIP_TRY(c, hipGetLastError());
   c->timings_pending = c->prof_on ? 1 : 0;
@@ -925,7 +990,7 @@ static int fetch_raw(mdff_ctx* c, int nslots) {
 }
 
 static int engforce(mdff_ctx* c, mdff_terms* t) {
-  MDFF_TRY(engforce_launch(c, c->raw, t));
+  MDFF_TRY(engforce_launch(c, c->raw, t, false));
   MDFF_TRY(fetch_raw(c, 1));
   engforce_finish(c, c->h_raw, t);
   return MDFF_OK;
@@ -955,9 +1020,8 @@ hipEvent_t prof_event(mdff_ctx* c) {
   return c->evpool[c->evused++];
 }
 
-static const char* kProfNames[PK_COUNT] = {"k_pair_force", "k_pair_field", "k_tables",     "k_sk",
-                                           "k_sk_reduce",  "k_gather",     "k_gather_final", "k_nl_sweep",
-                                           "k_build_amat"};
+static const char* kProfNames[PK_COUNT] = {"k_pair_force", "k_pair_field", "k_tables",       "k_sk",
+                                           "k_sk_reduce",  "k_gather",     "k_gather_final", "k_nl_sweep"};
 
 // ====================================================================== C-ABI
 
@@ -1015,7 +1079,7 @@ int mdff_ctx_destroy(mdff_ctx* c) {
                 c->pol, c->invpol, c->dipia, c->mu, c->mu_ind, c->efield, c->e_stat, c->e_tot, c->e_ind,
                 c->hist, c->frec, c->zeros, c->arec, c->ameta, c->peratom, c->red, c->red_part, c->d_pp, c->cell_of, c->cell_key,
                 c->cell_atoms, c->cell_atoms_in, c->cell_start, c->nl_count, c->nl_start, c->nl_j,
-                c->nl_flag, c->sort_tmp, c->d_flag, c->d_disp, c->raw, c->cx, c->cy, c->cz};
+                c->nl_flag, c->sort_tmp, c->d_flag, c->d_disp, c->raw, c->cx, c->cy, c->cz, c->fin_part};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c->polar) (void)hipFree(c->polar);
@@ -1138,10 +1202,13 @@ int mdff_md_vv(mdff_ctx* c, int32_t nsteps, double dt, double* ekin, double* upo
     double* raw = c->raw + (size_t)RAW_N * slot;
     hipLaunchKernelGGL(k_vv_pos, dim3(nblk(n)), dim3(256), 0, c->stream, n, dt, dt * dt * 0.5, c->mass, c->rx,
                        c->ry, c->rz, c->vx, c->vy, c->vz, c->fx, c->fy, c->fz, c->fsx, c->fsy, c->fsz);
-    MDFF_TRY(engforce_launch(c, raw, &held[slot]));
-    hipLaunchKernelGGL(k_vv_vel, dim3(nblk(n)), dim3(256), 0, c->stream, n, dt * 0.5, c->mass, c->vx, c->vy,
-                       c->vz, c->fsx, c->fsy, c->fsz, c->fx, c->fy, c->fz, extra(c));
-    if (ekin) MDFF_TRY(reduce_cols_dev(c, extra(c), 1, raw + RAW_EKIN));
+    MDFF_TRY(engforce_launch(c, raw, &held[slot], true));
+    double* ke_part = c->fin_part + 1024 * 16;
+    MDFF_TRY(claim_partials(c, ke_part));
+    hipLaunchKernelGGL(k_vv_vel_ke, dim3(nblk(n)), dim3(256), 0, c->stream, n, dt * 0.5, c->mass, c->vx, c->vy,
+                       c->vz, c->fsx, c->fsy, c->fsz, c->fx, c->fy, c->fz, ke_part);
+    if (ekin) MDFF_TRY(queue_final(c, ke_part, nblk(n), 1, raw + RAW_EKIN));
+    MDFF_TRY(flush_finals(c));
     // vnlistcheck after the step: queued now, finished by the next step's
     // engforce_launch once that step's reciprocal work is queued (or below)
     if (c->d.lvnlist) MDFF_TRY(nl_check_launch(c));

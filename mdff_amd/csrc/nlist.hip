@@ -298,14 +298,31 @@ int nl_build(mdff_ctx* c) {
 
 // per-atom displacement since the saved fractional coordinates, reduced to
 // the two largest values per block (src/tools.f90:470-500)
-__global__ void __launch_bounds__(256) k_disp_top2(NLArgs a, const double* __restrict__ xs,
+// The fractional coordinates of the check are formed here from r (kardir's
+// arithmetic) and stored in (cx, cy, cz) for a rebuild at these positions.
+__global__ void __launch_bounds__(256) k_disp_top2(NLArgs a, const double* __restrict__ rx,
+                                                   const double* __restrict__ ry,
+                                                   const double* __restrict__ rz, double b11, double b21,
+                                                   double b31, double b12, double b22, double b32, double b13,
+                                                   double b23, double b33, double* __restrict__ cx,
+                                                   double* __restrict__ cy, double* __restrict__ cz,
+                                                   const double* __restrict__ xs,
                                                    const double* __restrict__ ys,
                                                    const double* __restrict__ zs,
                                                    double* __restrict__ out) {
   __shared__ double s1[256], s2[256];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double m1 = 0.0, m2 = 0.0;
-  if (i < a.n) m1 = sqrt(min_image_r2(a, a.sx[i] - xs[i], a.sy[i] - ys[i], a.sz[i] - zs[i]));
+  if (i < a.n) {
+    const double x = rx[i], y = ry[i], z = rz[i];
+    const double u = x * b11 + y * b21 + z * b31;
+    const double v = x * b12 + y * b22 + z * b32;
+    const double w = x * b13 + y * b23 + z * b33;
+    cx[i] = u;
+    cy[i] = v;
+    cz[i] = w;
+    m1 = sqrt(min_image_r2(a, u - xs[i], v - ys[i], w - zs[i]));
+  }
   s1[threadIdx.x] = m1;
   s2[threadIdx.x] = m2;
   __syncthreads();
@@ -346,11 +363,11 @@ __global__ void k_copy3(int n, const double* a, const double* b, const double* c
 // whatever the positions are by then).
 int nl_check_launch(mdff_ctx* c) {
   const int n = c->n;
-  kardir_to(c, c->cx, c->cy, c->cz);
   NLArgs a = make_args(c);
-  a.sx = c->cx; a.sy = c->cy; a.sz = c->cz;
   const int nb = div_up(n, 256);
-  hipLaunchKernelGGL(k_disp_top2, dim3(nb), dim3(256), 0, c->stream, a, c->xs, c->ys, c->zs, c->d_disp);
+  const double* B = c->B;
+  hipLaunchKernelGGL(k_disp_top2, dim3(nb), dim3(256), 0, c->stream, a, c->rx, c->ry, c->rz, B[0], B[1], B[2],
+                     B[3], B[4], B[5], B[6], B[7], B[8], c->cx, c->cy, c->cz, c->xs, c->ys, c->zs, c->d_disp);
   HIP_TRY(c, hipMemcpyAsync(c->h_chk, c->d_disp, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipEventRecord(c->chk_ev, c->stream));
   c->chk_pending = 1;

@@ -51,6 +51,40 @@ __global__ void k_tables(int n, int Kx, int Ky, int Kz, const double* __restrict
   tab[(size_t)col * n + i] = {(T)cs, (T)sn};
 }
 
+// The same tables straight from the Cartesian positions: s = B^T r formed
+// as kardir does (src/lattice.f90:183-216, products and sums rounded
+// separately, no fused multiply-add), so s is bit-identical to the list
+// code's; the blocks of column 0 also store s (sx, sy, sz).
+template <typename T>
+__global__ void k_tables_r(int n, int Kx, int Ky, int Kz, const double* __restrict__ rx,
+                           const double* __restrict__ ry, const double* __restrict__ rz, double b11, double b21,
+                           double b31, double b12, double b22, double b32, double b13, double b23, double b33,
+                           double* __restrict__ sx, double* __restrict__ sy, double* __restrict__ sz,
+                           cplx<T>* __restrict__ tab) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int col = blockIdx.y;
+  if (i >= n) return;
+  const double x = rx[i], y = ry[i], z = rz[i];
+  auto dot = [&](double u, double v, double w) {
+    return __dadd_rn(__dadd_rn(__dmul_rn(x, u), __dmul_rn(y, v)), __dmul_rn(z, w));
+  };
+  if (col == 0) {
+    sx[i] = dot(b11, b21, b31);
+    sy[i] = dot(b12, b22, b32);
+    sz[i] = dot(b13, b23, b33);
+  }
+  double s;
+  int k;
+  if (col <= Kx) { s = dot(b11, b21, b31); k = col; }
+  else if (col <= Kx + Ky + 1) { s = dot(b12, b22, b32); k = col - Kx - 1; }
+  else { s = dot(b13, b23, b33); k = col - Kx - Ky - 2; }
+  double t = (double)k * s;
+  t -= rint(t);
+  double sn, cs;
+  sincospi(2.0 * t, &sn, &cs);
+  tab[(size_t)col * n + i] = {(T)cs, (T)sn};
+}
+
 struct SkArgs {
   int n, Kx, Ky, Kz, NY, NZ, M, NG, GZ;
   int use_q, use_mu;
@@ -161,14 +195,23 @@ __global__ void __launch_bounds__(256) k_sk(SkArgs a) {
   }
 }
 
+template <typename T>
+struct Mma;
+
 // S(n) = sum of chunk partials (FP64, fixed order); W(n) = A conj(S);
-// per-block sums of A|S|^2 and (delta_ab - kcoe k_a k_b) A|S|^2.
+// per-block sums of A|S|^2 and (delta_ab - kcoe k_a k_b) A|S|^2.  With amat,
+// also the A operand of the matrix-core gather: amat[(mtile * nks + s) * 64
+// + lane] holds, for m = 4 mtile + ml and K index kk = 4 s + lane / 16 =
+// (nz index t, Re/Im cc), row R = lane % 16 = (ml, component comp):
+// (Re, Im) of W Z as a real dot product with (Re Z, Im Z), times nz for
+// comp >= 2.  Entries outside the mesh are never written (zero from setup).
 template <typename T, typename TP>
 __global__ void __launch_bounds__(256) k_sk_reduce(int nk_box, int nchunk, const TP* __restrict__ part,
                                                    const double* __restrict__ kA,
                                                    const double* __restrict__ kvec,
                                                    double* __restrict__ sk, double* __restrict__ wk,
-                                                   T* __restrict__ wk_t, double* __restrict__ blk) {
+                                                   T* __restrict__ wk_t, double* __restrict__ blk,
+                                                   T* __restrict__ amat, int NZ, int Kz, int nks) {
   __shared__ double sh[4 * kRedMaxCols];
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   double v[kRedMaxCols];
@@ -187,6 +230,23 @@ __global__ void __launch_bounds__(256) k_sk_reduce(int nk_box, int nchunk, const
     wk[2 * k + 1] = -Ak * im;
     wk_t[2 * k] = (T)(Ak * re);
     wk_t[2 * k + 1] = (T)(-Ak * im);
+    if (amat) {
+      const double wr = Ak * re, wi = -Ak * im;
+      const int m = k / NZ, t = k - (k / NZ) * NZ;
+      const int mt = m >> 2, ml = m & 3;
+      const double fz = (double)(t - Kz);
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc) {
+        const int kk = 2 * t + cc;
+        T* dst = amat + ((size_t)mt * nks + (kk >> 2)) * 64 + 16 * (kk & 3);
+#pragma unroll
+        for (int comp = 0; comp < 4; ++comp) {
+          double v = ((comp & 1) == 0) ? (cc == 0 ? wr : -wi) : (cc == 0 ? wi : wr);
+          if (comp >= 2) v *= fz;
+          dst[Mma<T>::row_of(ml, comp)] = (T)v;
+        }
+      }
+    }
     const double str = (re * re + im * im) * Ak;
     const double kx = kvec[4 * k], ky = kvec[4 * k + 1], kz = kvec[4 * k + 2], kc = kvec[4 * k + 3];
     v[0] = str;
@@ -199,12 +259,6 @@ __global__ void __launch_bounds__(256) k_sk_reduce(int nk_box, int nchunk, const
   }
   const double r = block_sum_cols(v, 7, sh);
   if (threadIdx.x < 7) blk[blockIdx.x * 7 + threadIdx.x] = r;
-}
-
-// e[0..7) = sums of the k-block partials in block order (one block)
-__global__ void __launch_bounds__(256) k_sum7(int nb, const double* __restrict__ part, double* __restrict__ e) {
-  __shared__ double sh[4 * kRedMaxCols];
-  sum_block_partials(nb, 7, part, e, sh);
 }
 
 struct GaArgs {
@@ -357,8 +411,8 @@ __global__ void k_gather_final(int n, int nsplit, int moments, const double* __r
 // 16x16x4: rows R = (m_local, comp) of 4 values of m = (nx, ny) and four
 // components comp = (Re U0, Im U0, Re U1, Im U1), U_p(i, m) = sum_nz nz^p
 // W(m, nz) Z_i(nz); K = (nz, re/im of Z) padded to a multiple of 4; columns =
-// the 16 atoms.  The A operand (W, arranged per lane) is precomputed once per
-// evaluation (k_build_amat); the B operand (the atom's Z row) stays in
+// the 16 atoms.  The A operand (W, arranged per lane) is written once per
+// evaluation by k_sk_reduce; the B operand (the atom's Z row) stays in
 // registers for the whole kernel.  With the row order below, lane l holds the
 // four components of atom (l % 16) for m_local = l / 16 in its four
 // accumulator registers, so the (nx, ny) epilogue (multiply by
@@ -380,6 +434,7 @@ struct Mma<float> {
   }
   // A row R -> (m_local, comp) so that D register r = comp for m_local = l/16
   __host__ __device__ static __forceinline__ void row(int R, int& ml, int& comp) { ml = R >> 2; comp = R & 3; }
+  __host__ __device__ static __forceinline__ int row_of(int ml, int comp) { return 4 * ml + comp; }
 };
 template <>
 struct Mma<double> {
@@ -388,32 +443,8 @@ struct Mma<double> {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
   __host__ __device__ static __forceinline__ void row(int R, int& ml, int& comp) { comp = R >> 2; ml = R & 3; }
+  __host__ __device__ static __forceinline__ int row_of(int ml, int comp) { return 4 * comp + ml; }
 };
-
-// A operand of the gather GEMM: amat[(mtile * nks + s) * 64 + lane]
-template <typename T>
-__global__ void k_build_amat(int M, int NZ, int Kz, int nks, int mtiles, const double* __restrict__ wk,
-                             T* __restrict__ amat) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)mtiles * nks * 64) return;
-  const int lane = (int)(idx % 64);
-  const size_t rest = idx / 64;
-  const int s = (int)(rest % nks);
-  const int mt = (int)(rest / nks);
-  int ml, comp;
-  Mma<T>::row(lane % 16, ml, comp);
-  const int k = 4 * s + lane / 16;
-  const int t = k >> 1, cc = k & 1;
-  const int m = 4 * mt + ml;
-  double v = 0.0;
-  if (t < NZ && m < M) {
-    const double wr = wk[2 * ((size_t)m * NZ + t)], wi = wk[2 * ((size_t)m * NZ + t) + 1];
-    // (Re, Im) of W Z as a real dot product with (Re Z, Im Z)
-    v = ((comp & 1) == 0) ? (cc == 0 ? wr : -wi) : (cc == 0 ? wi : wr);
-    if (comp >= 2) v *= (double)(t - Kz);
-  }
-  amat[idx] = (T)v;
-}
 
 struct GmArgs {
   int n, Kx, Ky, Kz, NY, NZ, M, nks, mt_per_split, mtiles;
@@ -427,8 +458,8 @@ struct GmArgs {
 // listed), one otherwise.
 template <typename T, int CT, int NKS>
 constexpr int gather_waves() {
-  return sizeof(T) == 4 ? (((CT >= 2 && NKS <= 25) || (CT == 1 && NKS <= 57)) ? 2 : 1)
-                        : (((CT == 4 && NKS <= 13) || (CT == 2 && NKS <= 25) || (CT == 1 && NKS <= 33)) ? 2 : 1);
+  return sizeof(T) == 4 ? ((CT == 4 && NKS <= 25) ? 2 : 1)
+                        : (((CT == 2 && NKS <= 25) || (CT == 1 && NKS <= 33)) ? 2 : 1);
 }
 
 template <typename T, int CT, int NKS>
@@ -468,34 +499,48 @@ k_gather_mma(GmArgs a) {
   for (int ct = 0; ct < CT; ++ct) { ax[ct] = 0.0; ay[ct] = 0.0; az3[ct] = 0.0; }
   const int mt0 = blockIdx.y * a.mt_per_split;
   const int mt1 = min(a.mtiles, mt0 + a.mt_per_split);
-  // A operand, double buffered across m-tiles
+  // The A operand of tile mt+1 is loaded at the top of tile mt; the
+  // epilogue's X[nx] Y[ny] rows of tile mt at its top, used after its MFMAs
+  // (a scheduling barrier keeps their use, and so the wait for them, behind
+  // the MFMAs).  Addresses are clamped to valid rows, lanes beyond M get a
+  // zero weight, so nothing here branches.  The lane's m advances by 4 per
+  // tile; (nx, ny + Ky) are stepped, not divided.
+  int m = 4 * mt0 + g;
+  int nx = min(m, a.M - 1) / a.NY;
+  int nyo = min(m, a.M - 1) - nx * a.NY;                          // ny + Ky
+  int iac[CT];
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct) iac[ct] = min(atom0 + ct * 16 + col, a.n - 1);
   T abuf[NKS];
-  if (mt0 < mt1) {
-    const T* ap = amat + (size_t)mt0 * NKS * 64 + lane;
+  auto load_a = [&](int mtl) {
+    const T* ap = amat + (size_t)mtl * NKS * 64 + lane;
 #pragma unroll
     for (int s = 0; s < NKS; ++s) abuf[s] = ap[(size_t)s * 64];
-  }
+  };
+  if (mt0 < mt1) load_a(mt0);
   for (int mt = mt0; mt < mt1; ++mt) {
     T cur[NKS];
 #pragma unroll
     for (int s = 0; s < NKS; ++s) cur[s] = abuf[s];
-    if (mt + 1 < mt1) {
-      const T* ap = amat + (size_t)(mt + 1) * NKS * 64 + lane;
-#pragma unroll
-      for (int s = 0; s < NKS; ++s) abuf[s] = ap[(size_t)s * 64];
-    }
-    // epilogue operands of this m-tile (independent of the MFMA chain):
-    // clamped addresses, so the loads issue without branches
-    const int m = 4 * mt + g;
-    const bool mok = m < a.M;
-    const int nx = mok ? m / a.NY : 0, ny = mok ? m % a.NY - a.Ky : 0;
     cplx<T> Xv[CT], Yv[CT];
+    {
+      const int nyy = nyo - a.Ky;
+      const size_t xrow = (size_t)min(nx, a.Kx) * a.n, yrow = (size_t)(a.Kx + 1 + (nyy >= 0 ? nyy : -nyy)) * a.n;
 #pragma unroll
-    for (int ct = 0; ct < CT; ++ct) {
-      const int ia = min(atom0 + ct * 16 + col, a.n - 1);
-      Xv[ct] = tab[(size_t)nx * a.n + ia];
-      Yv[ct] = tab[(size_t)(a.Kx + 1 + (ny >= 0 ? ny : -ny)) * a.n + ia];
+      for (int ct = 0; ct < CT; ++ct) {
+        Xv[ct] = tab[xrow + iac[ct]];
+        Yv[ct] = tab[yrow + iac[ct]];
+      }
     }
+    const T mw = m < a.M ? T(1) : T(0);
+    const int ny = nyo - a.Ky;
+    const double fx = (double)nx, fy = (double)ny;
+    const T ysg = ny < 0 ? T(-1) : T(1);
+    // the next tile's m
+    m += 4;
+    nyo += 4;
+    while (nyo >= a.NY) { nyo -= a.NY; ++nx; }          // NY = 2 Ky + 1 may be below 4
+    if (mt + 1 < mt1) load_a(mt + 1);
     acc_t acc[CT];
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct) acc[ct] = acc_t{T(0), T(0), T(0), T(0)};
@@ -504,13 +549,14 @@ k_gather_mma(GmArgs a) {
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) acc[ct] = Mma<T>::op(cur[s], breg[ct][s], acc[ct]);
     }
-    const double fx = (double)nx, fy = (double)ny;
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct) {
       cplx<T> Y = Yv[ct];
-      if (ny < 0) Y.im = -Y.im;
+      Y.im *= ysg;
       cplx<T> XY = cmul(Xv[ct], Y);
-      if (!mok || atom0 + ct * 16 + col >= a.n) XY = {T(0), T(0)};
+      XY.re *= mw;
+      XY.im *= mw;
       const double im0 = (double)(XY.re * acc[ct][1] + XY.im * acc[ct][0]);
       const double im1 = (double)(XY.re * acc[ct][3] + XY.im * acc[ct][2]);
       ax[ct] += fx * im0;
@@ -755,12 +801,18 @@ template <typename T>
 static const void* gather_kernel(int ct, int nks) {
 #define GM_CASE(CTV, NKSV) \
   case (CTV) * 1000 + (NKSV): return reinterpret_cast<const void*>(&k_gather_mma<T, CTV, NKSV>);
-  switch (ct * 1000 + nks) {
-    GM_CASE(4, 9) GM_CASE(4, 13) GM_CASE(4, 17) GM_CASE(4, 19) GM_CASE(4, 25) GM_CASE(4, 33)
-    GM_CASE(2, 41) GM_CASE(2, 57)
-    GM_CASE(2, 9) GM_CASE(2, 13) GM_CASE(2, 17) GM_CASE(2, 19) GM_CASE(2, 25)
-    GM_CASE(1, 33) GM_CASE(1, 41) GM_CASE(1, 57)
-    default: return nullptr;
+  if constexpr (sizeof(T) == 4) {
+    switch (ct * 1000 + nks) {
+      GM_CASE(4, 9) GM_CASE(4, 13) GM_CASE(4, 17) GM_CASE(4, 19) GM_CASE(4, 25) GM_CASE(4, 33)
+      GM_CASE(2, 41) GM_CASE(2, 57)
+      default: return nullptr;
+    }
+  } else {
+    switch (ct * 1000 + nks) {
+      GM_CASE(2, 9) GM_CASE(2, 13) GM_CASE(2, 17) GM_CASE(2, 19) GM_CASE(2, 25)
+      GM_CASE(1, 33) GM_CASE(1, 41) GM_CASE(1, 57)
+      default: return nullptr;
+    }
   }
 #undef GM_CASE
 }
@@ -893,6 +945,8 @@ int recip_setup(mdff_ctx* c) {
     }
   }
   HIP_TRY(c, hipMalloc(&c->amat, tsz / 2 * (size_t)c->mma_mtiles * c->mma_nks * 64));
+  HIP_TRY(c, hipMemset(c->amat, 0, tsz / 2 * (size_t)c->mma_mtiles * c->mma_nks * 64));
+  HIP_TRY(c, hipMalloc((void**)&c->rec_blk, sizeof(double) * 7 * (size_t)div_up(nb, 256)));
   HIP_TRY(c, hipMalloc(&c->sk_part, sizeof(double) * 4 * (size_t)std::max(c->sk_chunks, c->sk_mma_ch) * nb));
   splits = std::max(splits, c->mma_splits);
   c->rec_splits = splits;
@@ -901,7 +955,7 @@ int recip_setup(mdff_ctx* c) {
 }
 
 void recip_free(mdff_ctx* c) {
-  void* ps[] = {c->kA, c->kvec, c->tab, c->sk_part, c->sk, c->wk, c->wk_t, c->rec_part, c->amat};
+  void* ps[] = {c->kA, c->kvec, c->tab, c->sk_part, c->sk, c->wk, c->wk_t, c->rec_part, c->amat, c->rec_blk};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -913,6 +967,24 @@ static void tables_t(mdff_ctx* c) {
               hipLaunchKernelGGL((k_tables<T>), dim3(div_up(c->n, 256), ncol), dim3(256), 0, c->stream, c->n,
                                  c->K[0], c->K[1], c->K[2], c->sx, c->sy, c->sz,
                                  reinterpret_cast<cplx<T>*>(c->tab)));
+}
+
+template <typename T>
+static void tables_r_t(mdff_ctx* c) {
+  const int ncol = c->K[0] + c->K[1] + c->K[2] + 3;
+  const double* B = c->B;
+  PROF_LAUNCH(c, PK_TABLES,
+              hipLaunchKernelGGL((k_tables_r<T>), dim3(div_up(c->n, 256), ncol), dim3(256), 0, c->stream, c->n,
+                                 c->K[0], c->K[1], c->K[2], c->rx, c->ry, c->rz, B[0], B[1], B[2], B[3], B[4],
+                                 B[5], B[6], B[7], B[8], c->sx, c->sy, c->sz,
+                                 reinterpret_cast<cplx<T>*>(c->tab)));
+}
+
+int recip_tables_from_r(mdff_ctx* c) {
+  if (c->d.precision == MDFF_PREC_MIXED) tables_r_t<float>(c);
+  else tables_r_t<double>(c);
+  HIP_TRY(c, hipGetLastError());
+  return MDFF_OK;
 }
 
 int recip_tables(mdff_ctx* c) {
@@ -1015,29 +1087,30 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     dim3 g1(div_up((int64_t)s.M * s.NG, 256), chunks);
     PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk<T, GZ>), g1, dim3(256), shm, c->stream, s));
   }
-  // the 7 energy / stress sums over the k-blocks stay in device memory (e_out)
+  // the 7 energy / stress sums over the k-blocks: block partials here, the
+  // final sum queued into e_out (device memory)
+  const bool dipf = forces && use_mu;
+  const bool mma_gather = (forces || field) && !dipf && c->mma_nks > 0 && c->use_mma;
+  T* amat = mma_gather ? reinterpret_cast<T*>(c->amat) : nullptr;
   const int nb = div_up(c->nk_box, 256);
+  MDFF_TRY(claim_partials(c, c->rec_blk));
   PROF_LAUNCH(c, PK_SKRED,
               if (sk_fp64) {
                 hipLaunchKernelGGL((k_sk_reduce<T, double>), dim3(nb), dim3(256), 0, c->stream, c->nk_box, chunks,
                                    reinterpret_cast<const double*>(c->sk_part), c->kA, c->kvec, c->sk, c->wk,
-                                   reinterpret_cast<T*>(c->wk_t), c->red_part);
+                                   reinterpret_cast<T*>(c->wk_t), c->rec_blk, amat, c->NZk, c->K[2],
+                                   c->mma_nks);
               } else {
                 hipLaunchKernelGGL((k_sk_reduce<T, T>), dim3(nb), dim3(256), 0, c->stream, c->nk_box, chunks,
                                    reinterpret_cast<const T*>(c->sk_part), c->kA, c->kvec, c->sk, c->wk,
-                                   reinterpret_cast<T*>(c->wk_t), c->red_part);
+                                   reinterpret_cast<T*>(c->wk_t), c->rec_blk, amat, c->NZk, c->K[2],
+                                   c->mma_nks);
               });
-  if (e_out) hipLaunchKernelGGL(k_sum7, dim3(1), dim3(256), 0, c->stream, nb, c->red_part, e_out);
+  if (e_out) MDFF_TRY(queue_final(c, c->rec_blk, nb, 7, e_out));
   if (!forces && !field) return MDFF_OK;
-  const bool dipf = forces && use_mu;
   int nsplit = 0, moments = dipf ? 9 : 3;
-  if (!dipf && c->mma_nks > 0 && c->use_mma) {
-    // matrix-core path
-    const int total = c->mma_mtiles * c->mma_nks * 64;
-    PROF_LAUNCH(c, PK_AMAT,
-                hipLaunchKernelGGL((k_build_amat<T>), dim3(div_up(total, 256)), dim3(256), 0, c->stream, s.M,
-                                   c->NZk, c->K[2], c->mma_nks, c->mma_mtiles, c->wk,
-                                   reinterpret_cast<T*>(c->amat)));
+  if (mma_gather) {
+    // matrix-core path (A operand written by k_sk_reduce)
     GmArgs gm{};
     gm.n = c->n;
     gm.Kx = c->K[0]; gm.Ky = c->K[1]; gm.Kz = c->K[2];

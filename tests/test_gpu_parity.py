@@ -82,7 +82,7 @@ def _check_engforce(s, tol=TOL64, precision=capi.PREC_FP64, **over):
 
 
 @pytest.mark.parametrize("case", ["nacl_bmh", "sio2_buck", "fcc_lj", "morse", "bmhftd", "triclinic",
-                                  "lsurf", "nolist", "large_kmax"])
+                                  "lsurf", "nolist", "large_kmax", "aniso_kmax"])
 def test_engforce_fp64_parity(case):
     """engforce (src/engforce_driver.f90:102-190), FP64 form vs the oracle."""
     over = {}
@@ -119,6 +119,11 @@ def test_engforce_fp64_parity(case):
     elif case == "large_kmax":
         # 2 kmax + 1 = 67 nz values: the structure factor runs two nz groups
         s = configs.nacl(ncell=2, alpha=0.9, kmax=33)
+    elif case == "aniso_kmax":
+        # kmax (5, 1, 7): 3 ny values per nx, fewer than the 4 m values of a
+        # gather tile, and uneven m-tiles
+        s = configs.nacl(ncell=2)
+        s.desc.kES[0], s.desc.kES[1], s.desc.kES[2] = 5, 1, 7
     else:
         s = configs.nacl(ncell=2)
         over["lvnlist"] = 0
