@@ -169,6 +169,7 @@ struct mdff_ctx {
   int32_t* polar = nullptr;               // 1 for polarisable ions
   int npol = 0;                           // number of polarisable ions
   double qsq = 0.0;                       // sum of q^2 (self term)
+  double qmax = 0.0;                      // max |q|
   double* zeros = nullptr;                // n zeros (charges switched off)
 
   // neighbour list (full rows, cut = max(rc_vdw, rc_coul) + skin)
@@ -214,6 +215,12 @@ struct mdff_ctx {
   int sk_ctm = 1;                         // m-tiles per wave of the structure factor
   int sk_ta = 0;                          // atoms per staged tile of the structure factor
   int sk_mma_ch = 1;                      // atom chunks of the matrix-core structure factor
+  // mixed form: split-half (FP16 hi + lo) matrix-core reciprocal kernels
+  int h3 = 0;
+  int h3_ch = 1, h3_nsh = 0, h3_ct = 1, h3_splits = 1, h3_mt8 = 0;
+  float h3_pscale = 1.f;                  // charge scale of the structure factor's B operand
+  void* amat_h3 = nullptr;                // gather A operand, split halves
+  float* ascale_h3 = nullptr;             // inverse scale per gather m-tile
   double* kO_ws = nullptr;                // scf_kO_v2 work vectors (11 x 3n)
   AtomRec* arec = nullptr;                // atoms in cell-sorted order (refreshed every evaluation)
   int2* ameta = nullptr;                  // (type, original index) in the same order

@@ -589,6 +589,7 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   for (int i = 0; i < n; ++i) {
     c->npol += polar[i] != 0;
     c->qsq = c->qsq + q[i] * q[i];
+    c->qmax = std::max(c->qmax, std::fabs(q[i]));
   }
   // reciprocal kernels: matrix-core by default; MDFF_RECIP_VALU=1 selects the
   // vector-ALU kernels (same sums, kept for comparison and for shapes the

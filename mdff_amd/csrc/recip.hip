@@ -770,15 +770,437 @@ k_sk_mma2(Sm2Args a) {
 }
 
 // S = S1 + i 2pi nz S2 from two partial sets (dipole amplitude)
-__global__ void k_sk_combine(int nk_box, int NZ, int Kz, int nchunk, double* __restrict__ p1,
-                             const double* __restrict__ p2) {
+template <typename TP>
+__global__ void k_sk_combine(int nk_box, int NZ, int Kz, int nchunk, TP* __restrict__ p1,
+                             const TP* __restrict__ p2) {
   const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= (size_t)nchunk * nk_box) return;
   const int k = (int)(x % nk_box);
   const double fz = 2.0 * MDFF_PI * (double)(k % NZ - Kz);
   const double s2r = p2[2 * x], s2i = p2[2 * x + 1];
-  p1[2 * x] -= fz * s2i;
-  p1[2 * x + 1] += fz * s2r;
+  p1[2 * x] = (TP)((double)p1[2 * x] - fz * s2i);
+  p1[2 * x + 1] = (TP)((double)p1[2 * x + 1] + fz * s2r);
+}
+
+// ---------------------------------------------------------------------------
+// Split-half matrix-core kernels of the mixed form.
+//
+// The FP32 operands of the two GEMMs (structure factor, gather) are each
+// split into two FP16 halves, v = hi + lo (both rounded to nearest: |v - hi
+// - lo| <= 2^-22 |v|), and a product is formed from three FP16 MFMAs with
+// FP32 accumulation: a b ~ ah bh + ah bl + al bh (the dropped al bl is below
+// 2^-22 |a b|).  That is FP32-class accuracy at the FP16 matrix rate of
+// v_mfma_f32_32x32x16_f16.  Lane layouts (checked on gfx950):
+// A(row = l%32, k = 8*(l/32) + j), B(k = 8*(l/32) + j, col = l%32),
+// D register r: row = 8*(r/4) + 4*(l/32) + r%4, col = l%32.
+// Operands whose range is not bounded (the gather's W) are scaled per tile
+// by a power of two before the split and unscaled after the product.
+// ---------------------------------------------------------------------------
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));   // 16 bytes: one half8
+
+__device__ __forceinline__ f32x16 mma_h(half8 a, half8 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+
+// v ~ hi + lo
+__device__ __forceinline__ void split_h(float v, _Float16& hi, _Float16& lo) {
+  hi = (_Float16)v;
+  lo = (_Float16)(v - (float)hi);
+}
+
+__device__ __forceinline__ uint32_t pack_h(_Float16 a, _Float16 b) {
+  return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+}
+
+// ---- structure factor: S^T(nz, m) = sum_i Z_i(nz) P_i(m), split halves ----
+// Rows R = (nz, Re/Im) in 4 tiles of 32 (16 nz each; 64 nz per block,
+// blockIdx.z picks the group); columns = 32 values of m per wave; K = (atom,
+// Re/Im of P), 8 atoms per MFMA.  The A operand (Z, signed per row part) is
+// split once per staged tile into LDS; the B operand P_i(m) = a_i X_i[nx]
+// Y_i[ny] is formed and split per lane.  Modes as k_sk_mma2; the charges
+// are scaled by pscale (a power of two) and the sums written times
+// 1/pscale, FP32 per atom chunk.
+struct Sh3Args {
+  int n, Kx, Ky, Kz, NY, NZ, M, chunk, mode, nk_box;
+  float pscale;
+  const double* q;
+  const double* mu;
+  double b[9];
+  const void* tab;               // FP32 phase tables
+  float* part;                   // [chunk][2*nk_box]
+};
+
+template <int TA>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_sk_h3(Sh3Args a) {
+  constexpr int RT = 4;                                   // row tiles of 32 rows
+  constexpr int RW = TA + 4;                              // 32-bit words per staged Z row
+  constexpr int TAP = TA + 4;                             // atoms per staged X / Y column
+  static_assert(256 % TA == 0 && TA % 8 == 0, "atom tile");
+  extern __shared__ unsigned char smem_raw[];
+  uint32_t* zs = reinterpret_cast<uint32_t*>(smem_raw);                 // [plane hi/lo][128 rows][RW]
+  const int nxy = a.Kx + a.Ky + 2;                                       // X and Y columns
+  cplx<float>* st = reinterpret_cast<cplx<float>*>(zs + 2 * 128 * RW);  // [nxy][TAP]
+  float* sq = reinterpret_cast<float*>(st + (size_t)nxy * TAP);         // [3][TAP]
+  const cplx<float>* tab = reinterpret_cast<const cplx<float>*>(a.tab);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int hl = lane >> 5, col = lane & 31;
+  const int zoff = blockIdx.z * (RT * 16);
+  const int zcol = a.Kx + 1 + a.Ky + 1;
+  // B column of this lane
+  const int m = (blockIdx.x * 4 + wave) * 32 + col;
+  const bool bok = m < a.M;
+  const int nx = bok ? m / a.NY : 0, ny = bok ? m % a.NY - a.Ky : 0;
+  const float4* xp = reinterpret_cast<const float4*>(st + (size_t)nx * TAP + 4 * hl);
+  const float4* yp = reinterpret_cast<const float4*>(st + (size_t)(a.Kx + 1 + (ny >= 0 ? ny : -ny)) * TAP + 4 * hl);
+  const float ysg = ny < 0 ? -1.f : 1.f, fnx = (float)nx, fny = (float)ny, bw = bok ? 1.f : 0.f;
+  // per-tile sums (MFMA accumulators, restarted every tile) and the chunk
+  // totals they are added to: the FP32 rounding of each accumulation is
+  // relative to a tile's sum, not to the growing chunk sum
+  f32x16 acc[RT], tot[RT];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) tot[rt][r] = 0.f;
+  const int abeg = blockIdx.y * a.chunk;
+  const int aend = min(a.n, abeg + a.chunk);
+  constexpr int CG = 256 / TA;
+  const int sia = threadIdx.x % TA, scg = threadIdx.x / TA;
+  for (int t0 = abeg; t0 < aend; t0 += TA) {
+    const int nt = min(TA, aend - t0);
+    const bool sok = sia < nt;
+    const size_t src = (size_t)t0 + min(sia, nt - 1);
+    __syncthreads();
+    // Z rows: part 0 (Zr, -Zi), part 1 (Zi, Zr), Z(-n) = conj Z(n); atoms
+    // beyond the chunk get zeros (their B is zero too)
+    for (int nzl = scg; nzl < RT * 16; nzl += CG) {
+      const int nzi = zoff + nzl;
+      const int nz = nzi - a.Kz;
+      const int az = min(nz < 0 ? -nz : nz, a.Kz);
+      cplx<float> z = tab[(size_t)(zcol + az) * a.n + src];
+      if (nz < 0) z.im = -z.im;
+      if (!sok || nzi >= a.NZ) z = cplx<float>{0.f, 0.f};
+      _Float16 rh, rl, ih, il;
+      split_h(z.re, rh, rl);
+      split_h(z.im, ih, il);
+      uint32_t* r0 = zs + (size_t)(2 * nzl) * RW + sia;
+      r0[0] = pack_h(rh, -ih);
+      r0[RW] = pack_h(ih, rh);
+      r0[128 * RW] = pack_h(rl, -il);
+      r0[129 * RW] = pack_h(il, rl);
+    }
+    // X and Y columns, clamped loads in batches of 8
+    for (int c0 = scg; c0 < nxy; c0 += 8 * CG) {
+      cplx<float> v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = tab[(size_t)min(c0 + u * CG, nxy - 1) * a.n + src];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (c0 + u * CG < nxy) st[(size_t)(c0 + u * CG) * TAP + sia] = sok ? v[u] : cplx<float>{0.f, 0.f};
+    }
+    if (threadIdx.x < TA) {
+      const int ia = threadIdx.x;
+      float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+      if (ia < nt) {
+        const int i = t0 + ia;
+        if (a.mode <= 1) c0 = (float)(a.q[i] * a.pscale);
+        if (a.mode >= 1) {
+          const double u0 = a.mu[3 * i], u1 = a.mu[3 * i + 1], u2 = a.mu[3 * i + 2];
+          if (a.mode == 1) {
+            c1 = (float)(2.0 * MDFF_PI * a.pscale * (u0 * a.b[0] + u1 * a.b[1] + u2 * a.b[2]));
+            c2 = (float)(2.0 * MDFF_PI * a.pscale * (u0 * a.b[3] + u1 * a.b[4] + u2 * a.b[5]));
+          } else {
+            c0 = (float)(a.pscale * (u0 * a.b[6] + u1 * a.b[7] + u2 * a.b[8]));
+          }
+        }
+      }
+      sq[ia] = c0;
+      sq[TAP + ia] = c1;
+      sq[2 * TAP + ia] = c2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[rt][r] = 0.f;
+    const float4* qp = reinterpret_cast<const float4*>(sq + 4 * hl);
+#pragma unroll
+    for (int s = 0; s < TA / 8; ++s) {
+      // B: P of atoms 8s + 4hl + u (u < 4) at this lane's m, split
+      const float4 x01 = xp[4 * s], x23 = xp[4 * s + 1];
+      const float4 y01 = yp[4 * s], y23 = yp[4 * s + 1];
+      const float4 c0v = qp[2 * s], c1v = qp[TAP / 4 + 2 * s], c2v = qp[TAP / 2 + 2 * s];
+      const float xr[4] = {x01.x, x01.z, x23.x, x23.z}, xi[4] = {x01.y, x01.w, x23.y, x23.w};
+      const float yr[4] = {y01.x, y01.z, y23.x, y23.z};
+      const float yi[4] = {ysg * y01.y, ysg * y01.w, ysg * y23.y, ysg * y23.w};
+      const float cq[4] = {c0v.x, c0v.y, c0v.z, c0v.w};
+      const float cx[4] = {c1v.x, c1v.y, c1v.z, c1v.w}, cy[4] = {c2v.x, c2v.y, c2v.z, c2v.w};
+      half8 bh, bl;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float xyr = xr[u] * yr[u] - xi[u] * yi[u];
+        const float xyi = xr[u] * yi[u] + xi[u] * yr[u];
+        const float bb = fnx * cx[u] + fny * cy[u];
+        const float pre = bw * (cq[u] * xyr - bb * xyi);
+        const float pim = bw * (cq[u] * xyi + bb * xyr);
+        _Float16 h, l;
+        split_h(pre, h, l);
+        bh[2 * u] = h;
+        bl[2 * u] = l;
+        split_h(pim, h, l);
+        bh[2 * u + 1] = h;
+        bl[2 * u + 1] = l;
+      }
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
+        const uint32_t* row = zs + (size_t)(32 * rt + col) * RW + 8 * s + 4 * hl;
+        const half8 ah = __builtin_bit_cast(half8, *reinterpret_cast<const u32x4*>(row));
+        const half8 al = __builtin_bit_cast(half8, *reinterpret_cast<const u32x4*>(row + 128 * RW));
+        acc[rt] = mma_h(ah, bh, acc[rt]);
+        acc[rt] = mma_h(ah, bl, acc[rt]);
+        acc[rt] = mma_h(al, bh, acc[rt]);
+      }
+    }
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) tot[rt] += acc[rt];
+  }
+  // D: lane holds rows 8*(r/4) + 4*hl + r%4 of its column m; row R of tile
+  // rt -> nz index zoff + (32 rt + R) / 2, part (32 rt + R) % 2
+  if (!bok) return;
+  float* out = a.part + (size_t)blockIdx.y * 2 * a.nk_box + 2 * (size_t)m * a.NZ;
+  const float inv = 1.0f / a.pscale;
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int R = 32 * rt + 8 * (r / 4) + 4 * hl + (r % 4);
+      const int nzi = zoff + (R >> 1);
+      if (nzi < a.NZ) out[2 * nzi + (R & 1)] = tot[rt][r] * inv;
+    }
+}
+
+// ---- gather A operand: W per m-tile of 8 m values, scaled and split ----
+// amat[((mt * NSH + s) * 64 + lane) * 2 + {0: hi, 1: lo}] = 8 halves: row
+// l%32 = (m_local = row / 4, comp = row % 4), K = 16 s + 8 (l/32) + j =
+// (nz index t, Re/Im cc): comp 0 (Re U0): (wr, -wi), comp 1 (Im U0):
+// (wi, wr), comp 2/3 the same times nz.  Values times 2^(14 - e), e the
+// exponent of the tile's largest value; ascale[mt] = 2^(e - 14).  One wave
+// per m-tile.
+template <int NSH>
+__global__ void __launch_bounds__(256) k_amat_h3(int M, int NZ, int Kz, int mtiles, const double* __restrict__ wk,
+                                                 u32x4* __restrict__ amat, float* __restrict__ ascale) {
+  const int lane = threadIdx.x & 63;
+  const int mt = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (mt >= mtiles) return;                                // whole waves only
+  const int hl = lane >> 5, row = lane & 31;
+  const int m = 8 * mt + row / 4, comp = row % 4;
+  float v[NSH][8];
+  float vmax = 0.f;
+#pragma unroll
+  for (int s = 0; s < NSH; ++s)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int kk = 16 * s + 8 * hl + j;
+      const int t = kk >> 1, cc = kk & 1;
+      double x = 0.0;
+      if (m < M && t < NZ) {
+        const double wr = wk[2 * ((size_t)m * NZ + t)], wi = wk[2 * ((size_t)m * NZ + t) + 1];
+        x = ((comp & 1) == 0) ? (cc == 0 ? wr : -wi) : (cc == 0 ? wi : wr);
+        if (comp >= 2) x *= (double)(t - Kz);
+      }
+      v[s][j] = (float)x;
+      vmax = fmaxf(vmax, fabsf(v[s][j]));
+    }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
+  int e = 0;
+  if (vmax > 0.f) (void)frexpf(vmax, &e);                  // vmax in [2^(e-1), 2^e)
+  const float sc = ldexpf(1.f, 14 - e);
+  if (lane == 0) ascale[mt] = ldexpf(1.f, e - 14);
+#pragma unroll
+  for (int s = 0; s < NSH; ++s) {
+    half8 h, l;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      _Float16 a, b;
+      split_h(v[s][j] * sc, a, b);
+      h[j] = a;
+      l[j] = b;
+    }
+    u32x4* dst = amat + ((size_t)(mt * NSH + s) * 64 + lane) * 2;
+    dst[0] = __builtin_bit_cast(u32x4, h);
+    dst[1] = __builtin_bit_cast(u32x4, l);
+  }
+}
+
+// ---- gather: U(m, i) per m-tile of 8 m values and atom tiles of 32 ----
+// D(rows (m_local, comp), atoms) = sum_K A(rows, K) B(K, atoms), B = the
+// atom's split Z row (registers, whole kernel), A staged per m-tile in LDS
+// (double buffered, shared by the block's 4 waves, which run the same
+// m-tiles).  Epilogue as k_gather_mma: Im(X Y U0) weighted by nx, ny and
+// Im(X Y U1), accumulated in FP64.
+struct Gh3Args {
+  int n, Kx, Ky, Kz, NY, NZ, M, mt_per_split, mtiles;
+  const void* tab;               // FP32 phase tables
+  const u32x4* amat;
+  const float* ascale;
+  double* out;                   // [split][3][n]
+};
+
+template <int CT, int NSH>
+constexpr int gather_h3_waves() {
+  return CT * NSH <= 14 ? 2 : 1;        // two waves per SIMD where the split B rows fit
+}
+
+template <int CT, int NSH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gather_h3_waves<CT, NSH>())))
+k_gather_h3(Gh3Args a) {
+  __shared__ u32x4 as[2][NSH * 128];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int hl = lane >> 5, col = lane & 31;
+  const int atom0 = (blockIdx.x * 4 + wave) * 32 * CT;
+  const cplx<float>* tab = reinterpret_cast<const cplx<float>*>(a.tab);
+  const int zcol = a.Kx + 1 + a.Ky + 1;
+  int iac[CT];
+  bool aok[CT];
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct) {
+    aok[ct] = atom0 + 32 * ct + col < a.n;
+    iac[ct] = min(atom0 + 32 * ct + col, a.n - 1);
+  }
+  // B: split Z rows (K = 16 s + 8 hl + j = (t, cc): Re / Im of Z(t - Kz))
+  half8 bh[CT][NSH], bl[CT][NSH];
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+    for (int s = 0; s < NSH; ++s)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int kk = 16 * s + 8 * hl + j;
+        const int t = kk >> 1, cc = kk & 1;
+        const int nz = t - a.Kz;
+        const int az = min(nz < 0 ? -nz : nz, a.Kz);
+        const float v = reinterpret_cast<const float*>(tab + (size_t)(zcol + az) * a.n + iac[ct])[cc];
+        const float w = (aok[ct] && t < a.NZ) ? ((cc && nz < 0) ? -1.f : 1.f) : 0.f;
+        _Float16 h, l;
+        split_h(w * v, h, l);
+        bh[ct][s][j] = h;
+        bl[ct][s][j] = l;
+      }
+  double ax[CT], ay[CT], az3[CT];
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct) { ax[ct] = 0.0; ay[ct] = 0.0; az3[ct] = 0.0; }
+  const int mt0 = blockIdx.y * a.mt_per_split;
+  const int mt1 = min(a.mtiles, mt0 + a.mt_per_split);
+  constexpr int NA = NSH * 128;                       // 16-byte units per staged m-tile
+  constexpr int PER = (NA + 255) / 256;
+  // staging of the A operand: thread -> unit x = threadIdx.x + 256 u (loads
+  // from clamped indices, stores only below NA)
+  u32x4 pre[PER];
+  if (mt0 < mt1) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) pre[u] = a.amat[(size_t)mt0 * NA + min((int)threadIdx.x + 256 * u, NA - 1)];
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+      if ((int)threadIdx.x + 256 * u < NA) as[0][threadIdx.x + 256 * u] = pre[u];
+  }
+  __syncthreads();
+  // the lane's m values of a tile: 8 mt + 2 q + hl, q < 4; (nx, ny + Ky) of q = 0
+  int mq = 8 * mt0 + hl;
+  int nxq = min(mq, a.M - 1) / a.NY;
+  int nyq = min(mq, a.M - 1) - nxq * a.NY;
+  for (int mt = mt0; mt < mt1; ++mt) {
+    const int buf = (mt - mt0) & 1;
+    if (mt + 1 < mt1) {
+#pragma unroll
+      for (int u = 0; u < PER; ++u)
+        pre[u] = a.amat[(size_t)(mt + 1) * NA + min((int)threadIdx.x + 256 * u, NA - 1)];
+    }
+    // epilogue operands of this tile (used after the MFMAs)
+    cplx<float> Xv[CT][4], Yv[CT][4];
+    float fx[4], fy[4], ysg[4], mw[4];
+    {
+      int nxx = nxq, nyy = nyq;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int ny = nyy - a.Ky;
+        mw[q] = (mq + 2 * q < a.M) ? 1.f : 0.f;
+        fx[q] = (float)nxx;
+        fy[q] = (float)ny;
+        ysg[q] = ny < 0 ? -1.f : 1.f;
+        const size_t xrow = (size_t)min(nxx, a.Kx) * a.n;
+        const size_t yrow = (size_t)(a.Kx + 1 + (ny >= 0 ? ny : -ny)) * a.n;
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          Xv[ct][q] = tab[xrow + iac[ct]];
+          Yv[ct][q] = tab[yrow + iac[ct]];
+        }
+        nyy += 2;
+        while (nyy >= a.NY) { nyy -= a.NY; ++nxx; }
+      }
+      mq += 8;
+      nyq += 8;
+      while (nyq >= a.NY) { nyq -= a.NY; ++nxq; }
+    }
+    f32x16 acc[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[ct][r] = 0.f;
+#pragma unroll
+    for (int s = 0; s < NSH; ++s) {
+      const half8 ah = __builtin_bit_cast(half8, as[buf][(s * 64 + lane) * 2]);
+      const half8 al = __builtin_bit_cast(half8, as[buf][(s * 64 + lane) * 2 + 1]);
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+        acc[ct] = mma_h(ah, bh[ct][s], acc[ct]);
+        acc[ct] = mma_h(ah, bl[ct][s], acc[ct]);
+        acc[ct] = mma_h(al, bh[ct][s], acc[ct]);
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const float inv = a.ascale[mt];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        cplx<float> Y = Yv[ct][q];
+        Y.im *= ysg[q];
+        cplx<float> XY = cmul(Xv[ct][q], Y);
+        const float w = mw[q] * inv;
+        XY.re *= w;
+        XY.im *= w;
+        const double im0 = (double)(XY.re * acc[ct][4 * q + 1] + XY.im * acc[ct][4 * q]);
+        const double im1 = (double)(XY.re * acc[ct][4 * q + 3] + XY.im * acc[ct][4 * q + 2]);
+        ax[ct] += (double)fx[q] * im0;
+        ay[ct] += (double)fy[q] * im0;
+        az3[ct] += im1;
+      }
+    if (mt + 1 < mt1) {
+#pragma unroll
+      for (int u = 0; u < PER; ++u)
+        if ((int)threadIdx.x + 256 * u < NA) as[buf ^ 1][threadIdx.x + 256 * u] = pre[u];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct) {
+    ax[ct] += __shfl_xor(ax[ct], 32);
+    ay[ct] += __shfl_xor(ay[ct], 32);
+    az3[ct] += __shfl_xor(az3[ct], 32);
+  }
+  if (hl == 0) {
+    double* o = a.out + (size_t)blockIdx.y * 3 * a.n;
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+      if (aok[ct]) {
+        const int ia = atom0 + 32 * ct + col;
+        o[ia] = ax[ct];
+        o[(size_t)a.n + ia] = ay[ct];
+        o[(size_t)2 * a.n + ia] = az3[ct];
+      }
+  }
 }
 
 }  // namespace
@@ -839,6 +1261,37 @@ static int blocks_per_cu(const void* fn, size_t lds) {
 // LDS bytes of the matrix-core structure factor for tile TA
 static size_t sk_lds(size_t tsz, int ncol, int ta) {
   return (tsz * (size_t)ncol + tsz / 2 * 3) * (size_t)(ta + 2);
+}
+
+// split-half kernels: staged atoms per tile, K steps (of 16) compiled for
+// the gather, LDS bytes of the structure factor
+constexpr int kH3TA = 32;
+static const int kH3Nsh[] = {4, 7, 10, 15};
+static int h3_bucket(int nsh) {
+  for (int b : kH3Nsh)
+    if (nsh <= b) return b;
+  return 0;
+}
+static size_t sk_h3_lds(int nxy, int ta) {
+  return (size_t)2 * 128 * (ta + 4) * 4 + (size_t)nxy * (ta + 4) * 8 + (size_t)3 * (ta + 4) * 4;
+}
+static const void* gather_h3_kernel(int ct, int nsh) {
+  switch (ct * 100 + nsh) {
+    case 204: return reinterpret_cast<const void*>(&k_gather_h3<2, 4>);
+    case 207: return reinterpret_cast<const void*>(&k_gather_h3<2, 7>);
+    case 110: return reinterpret_cast<const void*>(&k_gather_h3<1, 10>);
+    case 115: return reinterpret_cast<const void*>(&k_gather_h3<1, 15>);
+    default: return nullptr;
+  }
+}
+static const void* amat_h3_kernel(int nsh) {
+  switch (nsh) {
+    case 4: return reinterpret_cast<const void*>(&k_amat_h3<4>);
+    case 7: return reinterpret_cast<const void*>(&k_amat_h3<7>);
+    case 10: return reinterpret_cast<const void*>(&k_amat_h3<10>);
+    case 15: return reinterpret_cast<const void*>(&k_amat_h3<15>);
+    default: return nullptr;
+  }
 }
 
 // k-mesh on the box with the reference's k vectors and A(n), kcoe
@@ -947,15 +1400,64 @@ int recip_setup(mdff_ctx* c) {
   HIP_TRY(c, hipMalloc(&c->amat, tsz / 2 * (size_t)c->mma_mtiles * c->mma_nks * 64));
   HIP_TRY(c, hipMemset(c->amat, 0, tsz / 2 * (size_t)c->mma_mtiles * c->mma_nks * 64));
   HIP_TRY(c, hipMalloc((void**)&c->rec_blk, sizeof(double) * 7 * (size_t)div_up(nb, 256)));
-  HIP_TRY(c, hipMalloc(&c->sk_part, sizeof(double) * 4 * (size_t)std::max(c->sk_chunks, c->sk_mma_ch) * nb));
+  // mixed form: split-half kernels when their shapes fit (MDFF_RECIP_F32MMA=1
+  // keeps the FP32 matrix-core kernels)
+  c->h3 = 0;
+  {
+    const char* ev = getenv("MDFF_RECIP_F32MMA");
+    const bool off = ev && ev[0] == '1';
+    const int nxy = Kx + Ky + 2;
+    const int nsh = h3_bucket(div_up(2 * c->NZk, 16));
+    if (fp32 && c->use_mma && !off && nsh > 0 && sk_h3_lds(nxy, kH3TA) <= 65536) {
+      c->h3 = 1;
+      c->h3_nsh = nsh;
+      c->h3_ct = nsh <= 7 ? 2 : 1;
+      c->h3_mt8 = div_up(M, 8);
+      // B operand of the structure factor: max |q| scaled into [256, 512)
+      int e = 0;
+      if (c->qmax > 0.0) (void)frexp(c->qmax, &e);
+      c->h3_pscale = (float)ldexp(1.0, 9 - e);
+      // gather splits over m-tiles, as for the FP32 kernels
+      const int aw = div_up(c->n, 32 * c->h3_ct);
+      const int64_t gconc = 4 * (int64_t)cus * blocks_per_cu(gather_h3_kernel(c->h3_ct, nsh), 0);
+      int bs = 1;
+      double be = -1.0;
+      for (int sp = 1; sp <= std::min(c->h3_mt8, 64); ++sp) {
+        if (sp > 1 && c->h3_mt8 / sp < 4) break;
+        const int mps = div_up(c->h3_mt8, sp);
+        const int nsp = div_up(c->h3_mt8, mps);
+        const int64_t waves = (int64_t)aw * nsp;
+        const int64_t rounds = (waves + gconc - 1) / gconc;
+        const double e2 = (double)waves / (double)(rounds * gconc);
+        if (e2 > be + 0.02) { be = e2; bs = nsp; }
+      }
+      c->h3_splits = bs;
+      // structure-factor chunks from the kernel's occupancy
+      const size_t lds = sk_h3_lds(nxy, kH3TA);
+      const int64_t cap = (int64_t)cus * blocks_per_cu(reinterpret_cast<const void*>(&k_sk_h3<kH3TA>), lds);
+      const int64_t mblocks = div_up(div_up(M, 32), 4) * (int64_t)div_up(c->NZk, 64);
+      int64_t best = -1;
+      for (int ch = 1; ch <= 128 && (ch == 1 || div_up(c->n, ch) >= kH3TA); ++ch) {
+        const int per = div_up(c->n, ch);
+        const int nch = div_up(c->n, per);
+        const int64_t cost = ((mblocks * nch + cap - 1) / cap) * div_up(per, kH3TA);
+        if (best < 0 || cost < best) { best = cost; c->h3_ch = nch; }
+      }
+      HIP_TRY(c, hipMalloc(&c->amat_h3, 16 * 128 * (size_t)nsh * c->h3_mt8));
+      HIP_TRY(c, hipMalloc((void**)&c->ascale_h3, sizeof(float) * (size_t)c->h3_mt8));
+    }
+  }
+  HIP_TRY(c, hipMalloc(&c->sk_part, sizeof(double) * 4 * (size_t)std::max({c->sk_chunks, c->sk_mma_ch, c->h3_ch}) * nb));
   splits = std::max(splits, c->mma_splits);
+  if (c->h3) splits = std::max(splits, c->h3_splits);
   c->rec_splits = splits;
   HIP_TRY(c, hipMalloc((void**)&c->rec_part, sizeof(double) * 9 * (size_t)splits * c->n));
   return MDFF_OK;
 }
 
 void recip_free(mdff_ctx* c) {
-  void* ps[] = {c->kA, c->kvec, c->tab, c->sk_part, c->sk, c->wk, c->wk_t, c->rec_part, c->amat, c->rec_blk};
+  void* ps[] = {c->kA, c->kvec, c->tab, c->sk_part, c->sk, c->wk, c->wk_t, c->rec_part, c->amat, c->rec_blk,
+                c->amat_h3, c->ascale_h3};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -1042,7 +1544,40 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
   }
   int chunks = div_up(c->n, s.a0_step);
   bool sk_fp64 = false;
-  if (c->use_mma) {
+  const bool h3 = sizeof(T) == 4 && c->h3;
+  if (h3) {
+    // split-half structure factor, FP32 partials per chunk
+    Sh3Args sh{};
+    sh.n = c->n;
+    sh.Kx = c->K[0]; sh.Ky = c->K[1]; sh.Kz = c->K[2];
+    sh.NY = c->NYk; sh.NZ = c->NZk; sh.M = s.M;
+    int ch = c->h3_ch;
+    sh.chunk = div_up(c->n, ch);
+    ch = div_up(c->n, sh.chunk);
+    chunks = ch;
+    sh.nk_box = c->nk_box;
+    sh.q = use_q ? q : c->zeros;
+    sh.mu = mu;
+    for (int k = 0; k < 9; ++k) sh.b[k] = c->B[k];
+    sh.tab = c->tab;
+    float* p1 = reinterpret_cast<float*>(c->sk_part);
+    sh.part = p1;
+    sh.mode = use_mu ? 1 : 0;
+    sh.pscale = use_mu ? 1.f : c->h3_pscale;
+    const int nxy = c->K[0] + c->K[1] + 2;
+    const size_t lds = sk_h3_lds(nxy, kH3TA);
+    dim3 gs(div_up(div_up(s.M, 32), 4), ch, div_up(c->NZk, 64));
+    PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA>), gs, dim3(256), lds, c->stream, sh));
+    if (use_mu) {
+      Sh3Args s2 = sh;
+      s2.mode = 2;
+      s2.part = p1 + (size_t)2 * ch * c->nk_box;
+      const size_t tot = (size_t)ch * c->nk_box;
+      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA>), gs, dim3(256), lds, c->stream, s2);
+                  hipLaunchKernelGGL(k_sk_combine<float>, dim3(div_up(tot, 256)), dim3(256), 0, c->stream,
+                                     c->nk_box, c->NZk, c->K[2], ch, p1, s2.part));
+    }
+  } else if (c->use_mma) {
     sk_fp64 = true;
     Sm2Args sm{};
     sm.n = c->n;
@@ -1080,7 +1615,7 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
       s2.part = p1 + (size_t)2 * ch * c->nk_box;
       const size_t tot = (size_t)ch * c->nk_box;
       PROF_LAUNCH(c, PK_SK, launch_sk_mma2<T>(CTM, gs, lds(TA), c->stream, s2);
-                  hipLaunchKernelGGL(k_sk_combine, dim3(div_up(tot, 256)), dim3(256), 0, c->stream, c->nk_box,
+                  hipLaunchKernelGGL(k_sk_combine<double>, dim3(div_up(tot, 256)), dim3(256), 0, c->stream, c->nk_box,
                                      c->NZk, c->K[2], ch, p1, s2.part));
     }
   } else {
@@ -1090,7 +1625,7 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
   // the 7 energy / stress sums over the k-blocks: block partials here, the
   // final sum queued into e_out (device memory)
   const bool dipf = forces && use_mu;
-  const bool mma_gather = (forces || field) && !dipf && c->mma_nks > 0 && c->use_mma;
+  const bool mma_gather = (forces || field) && !dipf && c->mma_nks > 0 && c->use_mma && !h3;
   T* amat = mma_gather ? reinterpret_cast<T*>(c->amat) : nullptr;
   const int nb = div_up(c->nk_box, 256);
   MDFF_TRY(claim_partials(c, c->rec_blk));
@@ -1109,7 +1644,29 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
   if (e_out) MDFF_TRY(queue_final(c, c->rec_blk, nb, 7, e_out));
   if (!forces && !field) return MDFF_OK;
   int nsplit = 0, moments = dipf ? 9 : 3;
-  if (mma_gather) {
+  if (h3 && !dipf) {
+    // split-half gather: A operand per m-tile of 8 m values, then the GEMM
+    const int nsh = c->h3_nsh, mt8 = c->h3_mt8;
+    void* aargs[] = {(void*)&s.M, (void*)&c->NZk, (void*)&c->K[2], (void*)&mt8, (void*)&c->wk, (void*)&c->amat_h3,
+                     (void*)&c->ascale_h3};
+    PROF_LAUNCH(c, PK_GATHER, (void)hipLaunchKernel(amat_h3_kernel(nsh), dim3(div_up(mt8, 4)), dim3(256), aargs, 0,
+                                                    c->stream));
+    Gh3Args gh{};
+    gh.n = c->n;
+    gh.Kx = c->K[0]; gh.Ky = c->K[1]; gh.Kz = c->K[2];
+    gh.NY = c->NYk; gh.NZ = c->NZk; gh.M = s.M;
+    gh.mtiles = mt8;
+    gh.mt_per_split = div_up(mt8, c->h3_splits);
+    nsplit = div_up(mt8, gh.mt_per_split);
+    gh.tab = c->tab;
+    gh.amat = reinterpret_cast<const u32x4*>(c->amat_h3);
+    gh.ascale = c->ascale_h3;
+    gh.out = c->rec_part;
+    void* gargs[] = {(void*)&gh};
+    dim3 gg(div_up(c->n, 4 * 32 * c->h3_ct), nsplit);
+    PROF_LAUNCH(c, PK_GATHER, (void)hipLaunchKernel(gather_h3_kernel(c->h3_ct, nsh), gg, dim3(256), gargs, 0,
+                                                    c->stream));
+  } else if (mma_gather) {
     // matrix-core path (A operand written by k_sk_reduce)
     GmArgs gm{};
     gm.n = c->n;

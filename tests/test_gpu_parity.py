@@ -324,6 +324,29 @@ def test_nhcn_and_csvr_match():
     assert abs(tg[0] - to[0]) < 1e-9 * to[0]
 
 
+@pytest.mark.parametrize("case", ["sio2", "pim"])
+def test_mixed_reciprocal_split_half_vs_fp32_matrix_core(case, monkeypatch):
+    """The mixed form's reciprocal sums on the split-half (FP16 hi + lo)
+    matrix-core kernels and on the FP32 ones (MDFF_RECIP_F32MMA=1): both
+    within the mixed bar of the FP64 oracle, reciprocal energy and stress
+    within 1e-6 of it."""
+    if case == "sio2":
+        s = _sio2(natm=900, seed=27, kmax=9)
+    else:
+        s = _pim()
+    note(f"mixed reciprocal kernels {case} n={s.natm}")
+    _, o = pair(s)
+    fo, to = o.engforce()
+    for f32mma in ("0", "1"):
+        monkeypatch.setenv("MDFF_RECIP_F32MMA", f32mma)
+        g, _ = pair(s, precision=capi.PREC_MIXED)
+        fg, tg = g.engforce()
+        assert rel_force_err(fg, fo) < TOL_MIXED, (f32mma, rel_force_err(fg, fo))
+        assert abs(tg["u_rec"] - to["u_rec"]) < 1e-6 * abs(to["u_rec"]), (f32mma, tg["u_rec"], to["u_rec"])
+        tr = np.asarray(tg["tau_coul"]) - np.asarray(to["tau_coul"])
+        assert np.abs(tr).max() < 1e-5 * np.abs(np.asarray(to["tau_coul"])).max(), f32mma
+
+
 # ------------------------------------------------------------------ invariants at size
 
 def test_sio2_full_size_invariants_mixed_vs_fp64():
