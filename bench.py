@@ -34,7 +34,7 @@ sys.path.insert(0, str(ROOT))
 
 from mdff_amd import capi, configs  # noqa: E402
 
-PEAK = {"fp32": 157.3, "fp64": 78.6}      # MI355X dense TFLOP/s (FP32 / FP64 matrix and packed vector)
+PEAK = {"fp32": 157.3, "fp64": 78.6, "fp16": 2500.0}   # MI355X dense TFLOP/s (matrix; FP32 also packed vector)
 HBM_PEAK = 8000.0                          # GB/s
 # canonical algorithmic flops per unit (counting FMA = 2, sqrt/exp/sin/cos/div = 1)
 FLOPS_PER_UNIT = {
@@ -42,15 +42,36 @@ FLOPS_PER_UNIT = {
     "k_gather": 10.0,      # per (k, i): recarg 3 + force 7 (src/ewald_rec.f90:196-224)
     "k_pair_force": 106.0, # per in-cutoff Buckingham + Ewald q-q pair
 }
-# the kernel behind each profiler class on the default (matrix-core) path
-KERNEL_OF = {"k_sk": "k_sk_mma2<float,1,TA>", "k_gather": "k_gather_mma<float,CT,NKS>",
-             "k_pair_force": "k_pair_force<float,false,16>"}
+# the kernel behind each profiler class of the mixed form, and the unit it
+# is bound by: the reciprocal sums run on split-half FP16 matrix cores
+# (three FP16 products per FP32-class product), the pair kernel on FP32 VALU
+KERNEL_OF = {"k_sk": "k_sk_h3<32>", "k_gather": "k_gather_h3<CT,NSH>", "k_pair_force": "k_pair_force<float,false,16>"}
+BOUND_OF = {"k_sk": ("mfma", "fp16"), "k_gather": ("mfma", "fp16"), "k_pair_force": ("valu", "fp32")}
 # HBM bytes per launch of the mixed-form kernels on this workload: rocprofv3
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (KiB per dispatch x 1024; the factors
 # calibrate the counters on a 256 MiB device-to-device copy), measured with
 # mdff_amd.profile_workload (profiles/engforce_sio2_24k_hbm_traffic.md)
-TRAFFIC = {"k_gather": 86.93e6, "k_sk": 168.80e6, "k_pair_force": 65.15e6}
-NKINDS = 8   # kernel classes of the library's event profiler (mdff_profile_name)
+TRAFFIC = {"k_gather": 147.80e6, "k_sk": 140.98e6, "k_pair_force": 65.04e6}
+
+
+def mfma_issued(kind: str, natm: int, kmax) -> float:
+    """FP16 MFMA flops the split-half kernels issue per launch (32x32x16:
+    32768 flops each): gather 3 NSH per (32-atom tile, 8-m tile), NSH =
+    K steps of 16 over 2 (2 kz + 1) padded to 4, 7, 10 or 15; structure
+    factor 12 per (8 atoms, 32-m wave tile)."""
+    kx, ky, kz = kmax
+    M = (kx + 1) * (2 * ky + 1)
+    nz = 2 * kz + 1
+    nsh = next(b for b in (4, 7, 10, 15) if -(-2 * nz // 16) <= b)
+    if kind == "k_gather":
+        return -(-natm // 32) * -(-M // 8) * 3 * nsh * 32768.0
+    if kind == "k_sk":
+        waves_m = -(-(-(-M // 32)) // 4) * 4
+        return waves_m * -(-natm // 32) * 4 * 12 * -(-nz // 64) * 32768.0
+    return 0.0
+
+
+NKINDS = 9   # kernel classes of the library's event profiler (mdff_profile_name)
 
 
 def build_system(natm: int, relax_iters: int):
@@ -262,11 +283,17 @@ def main() -> int:
     roof = None
     if dom in FLOPS_PER_UNIT and prof[dom]["ms_avg"] > 0:
         flops = FLOPS_PER_UNIT[dom] * units[dom]
-        ach = flops / (prof[dom]["ms_avg"] * 1e-3) / 1e12
-        roof = {"kernel": KERNEL_OF[dom], "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK["fp32"],
-                "unit": "TFLOP/s", "frac": round(ach / PEAK["fp32"], 4), "traffic": TRAFFIC.get(dom),
+        sec = prof[dom]["ms_avg"] * 1e-3
+        ach = flops / sec / 1e12
+        bound, kind = BOUND_OF[dom]
+        roof = {"kernel": KERNEL_OF[dom], "bound": bound, "achieved": round(ach, 3), "peak": PEAK[kind],
+                "unit": "TFLOP/s", "frac": round(ach / PEAK[kind], 4), "traffic": TRAFFIC.get(dom),
                 "algorithmic_flops_per_launch": flops, "avg_launch_ms": prof[dom]["ms_avg"],
                 "units_per_launch": units[dom], "flops_per_unit": FLOPS_PER_UNIT[dom]}
+        issued = mfma_issued(dom, s.natm, kmax)
+        if issued > 0:
+            roof["mfma_fp16_issued_tflops"] = round(issued / sec / 1e12, 3)
+            roof["mfma_fp16_issued_frac"] = round(issued / sec / 1e12 / PEAK["fp16"], 4)
 
     out = {
         "metric": "ns/day mixed vs FP64 on GPU at <=1e-5 rel force err; fit-gradients/s",
@@ -279,7 +306,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair/field math, f64 accumulation",
+        "dtype": "f32 pair/field math, split-half f16 (f32-class) reciprocal products, f64 accumulation",
         "data": "synthetic (seeded random packing)",
         "config": {"workload": f"a-SiO2 {s.natm} ions, Buckingham+Ewald rc {rc} alpha {s.desc.alphaES} "
                                f"kmax {kmax[0]}, skin {s.desc.skindiff}, NVE velocity Verlet dt 1 fs",

@@ -227,7 +227,7 @@ int mdff_last_timings(mdff_ctx *ctx, double *ms, int32_t *n);
  * mdff_profile(ctx, 1) clears and starts recording one event pair around
  * each timed launch; mdff_profile_read returns, per kernel class k <
  * nkinds, the summed duration (ms) and launch count; mdff_profile_name(k)
- * names the class (8 classes). */
+ * names the class (9 classes). */
 int mdff_profile(mdff_ctx *ctx, int32_t on);
 int mdff_profile_read(mdff_ctx *ctx, double *ms_total, int32_t *count, int32_t nkinds);
 const char *mdff_profile_name(int32_t kind);

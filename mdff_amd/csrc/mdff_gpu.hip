@@ -1022,7 +1022,8 @@ hipEvent_t prof_event(mdff_ctx* c) {
 }
 
 static const char* kProfNames[PK_COUNT] = {"k_pair_force", "k_pair_field", "k_tables",       "k_sk",
-                                           "k_sk_reduce",  "k_gather",     "k_gather_final", "k_nl_sweep"};
+                                           "k_sk_reduce",  "k_gather",     "k_gather_final", "k_nl_sweep",
+                                           "k_amat"};
 
 // ====================================================================== C-ABI
 

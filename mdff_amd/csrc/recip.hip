@@ -1649,8 +1649,8 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     const int nsh = c->h3_nsh, mt8 = c->h3_mt8;
     void* aargs[] = {(void*)&s.M, (void*)&c->NZk, (void*)&c->K[2], (void*)&mt8, (void*)&c->wk, (void*)&c->amat_h3,
                      (void*)&c->ascale_h3};
-    PROF_LAUNCH(c, PK_GATHER, (void)hipLaunchKernel(amat_h3_kernel(nsh), dim3(div_up(mt8, 4)), dim3(256), aargs, 0,
-                                                    c->stream));
+    PROF_LAUNCH(c, PK_AMAT, (void)hipLaunchKernel(amat_h3_kernel(nsh), dim3(div_up(mt8, 4)), dim3(256), aargs, 0,
+                                                  c->stream));
     Gh3Args gh{};
     gh.n = c->n;
     gh.Kx = c->K[0]; gh.Ky = c->K[1]; gh.Kz = c->K[2];
