@@ -18,11 +18,10 @@
 // |x| < 2^52, so the 0.5 comparisons are exact and the result equals the
 // reference's nint for every double.
 __host__ __device__ __forceinline__ double nint_d(double x) {
-  double t = trunc(x);
-  double r = x - t;
-  if (r >= 0.5) t += 1.0;
-  else if (r <= -0.5) t -= 1.0;
-  return t;
+  const double t = trunc(x);
+  const double r = x - t;                          // exact
+  // +1 / -1 by selects (no branch): round half away from zero
+  return t + ((r >= 0.5) ? 1.0 : 0.0) - ((r <= -0.5) ? 1.0 : 0.0);
 }
 
 // Numerical Recipes erfcc, as src/functions.f90:28-40 (x >= 0 only here:

@@ -78,10 +78,24 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
   double fs0 = 0, fs1 = 0, fs2 = 0, fc0 = 0, fc1 = 0, fc2 = 0, e0 = 0, e1 = 0, e2 = 0;
   double usr = 0, vsr = 0, ts[6] = {0, 0, 0, 0, 0, 0}, ucl = 0, tc[6] = {0, 0, 0, 0, 0, 0};
   const int r0 = a.row[i], r1 = a.row[i + 1];
+  // two-stage prefetch of the neighbour chain: the row entry two steps ahead
+  // and the record of the next entry are in flight while a pair is computed
+  int pj_next = (r0 + sub + LPA < r1) ? a.nl_j[r0 + sub + LPA] : 0;
+  AtomRec rj_next{};
+  int2 mj_next{};
+  if (r0 + sub < r1) {
+    const int pj0 = a.nl_j[r0 + sub];
+    rj_next = a.rec[pj0];
+    mj_next = a.meta[pj0];
+  }
   for (int e = r0 + sub; e < r1; e += LPA) {
-    const int pj = a.nl_j[e];
-    const AtomRec rj = a.rec[pj];
-    const int2 mj = a.meta[pj];
+    const AtomRec rj = rj_next;
+    const int2 mj = mj_next;
+    if (e + LPA < r1) {
+      rj_next = a.rec[pj_next];
+      mj_next = a.meta[pj_next];
+      if (e + 2 * LPA < r1) pj_next = a.nl_j[e + 2 * LPA];
+    }
     const int j = mj.y;
     double dxs = si - rj.x, dys = ti - rj.y, dzs = ui - rj.z;
     dxs -= nint_d(dxs);
@@ -301,10 +315,23 @@ __global__ void __launch_bounds__(256) k_pair_field(PairArgs a) {
   double e0 = 0, e1 = 0, e2 = 0;
   const int r0 = a.row[i], r1 = a.row[i + 1];
   if (dip_i) {
+    // neighbour chain prefetched two stages ahead, as in k_pair_force
+    int pj_next = (r0 + sub + LPA < r1) ? a.nl_j[r0 + sub + LPA] : 0;
+    AtomRec rj_next{};
+    int2 mj_next{};
+    if (r0 + sub < r1) {
+      const int pj0 = a.nl_j[r0 + sub];
+      rj_next = a.rec[pj0];
+      mj_next = a.meta[pj0];
+    }
     for (int e = r0 + sub; e < r1; e += LPA) {
-      const int pj = a.nl_j[e];
-      const AtomRec rj = a.rec[pj];
-      const int2 mj = a.meta[pj];
+      const AtomRec rj = rj_next;
+      const int2 mj = mj_next;
+      if (e + LPA < r1) {
+        rj_next = a.rec[pj_next];
+        mj_next = a.meta[pj_next];
+        if (e + 2 * LPA < r1) pj_next = a.nl_j[e + 2 * LPA];
+      }
       const int j = mj.y;
       T m0, m1, m2;
       if (MODE == 1) {

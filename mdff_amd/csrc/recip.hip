@@ -218,8 +218,23 @@ __global__ void __launch_bounds__(256) k_sk_reduce(int nk_box, int nchunk, const
 #pragma unroll
   for (int c = 0; c < kRedMaxCols; ++c) v[c] = 0.0;
   if (k < nk_box) {
+    // chunks in order; the loads of 8 chunks are issued before their adds
     double re = 0.0, im = 0.0;
-    for (int c = 0; c < nchunk; ++c) {
+    int c = 0;
+    for (; c + 8 <= nchunk; c += 8) {
+      TP vr[8], vi[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        vr[u] = part[(size_t)(c + u) * 2 * nk_box + 2 * k];
+        vi[u] = part[(size_t)(c + u) * 2 * nk_box + 2 * k + 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        re += (double)vr[u];
+        im += (double)vi[u];
+      }
+    }
+    for (; c < nchunk; ++c) {
       re += (double)part[(size_t)c * 2 * nk_box + 2 * k];
       im += (double)part[(size_t)c * 2 * nk_box + 2 * k + 1];
     }
@@ -804,6 +819,15 @@ __device__ __forceinline__ f32x16 mma_h(half8 a, half8 b, f32x16 c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
 }
 
+// (m / d, m % d) for 0 <= m < 2^20 from a float reciprocal inv = 1/d and
+// one correction: branch-free, so loads that use the result issue together
+__device__ __forceinline__ void divmod_f(int m, int d, float inv, int& q, int& r) {
+  q = (int)((float)m * inv);
+  r = m - q * d;
+  q = r < 0 ? q - 1 : (r >= d ? q + 1 : q);
+  r = m - q * d;
+}
+
 // v ~ hi + lo
 __device__ __forceinline__ void split_h(float v, _Float16& hi, _Float16& lo) {
   hi = (_Float16)v;
@@ -865,21 +889,48 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     for (int r = 0; r < 16; ++r) tot[rt][r] = 0.f;
   const int abeg = blockIdx.y * a.chunk;
   const int aend = min(a.n, abeg + a.chunk);
+  // staging: thread -> atom sia of the tile; Z rows scg + CG u (u < ZU), X/Y
+  // columns scg + CG u (u < XU, nxy <= CG XU); the loads of tile t+1 are
+  // issued before the MFMAs of tile t and written to LDS after them
   constexpr int CG = 256 / TA;
+  constexpr int ZU = RT * 16 / CG;
+  constexpr int XU = 8;
   const int sia = threadIdx.x % TA, scg = threadIdx.x / TA;
+  cplx<float> zn[ZU], xn[XU];
+  double qd = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;        // raw a-coefficient inputs (used at store time)
+#define SKH3_FETCH(T0)                                                                        \
+  do {                                                                                        \
+    const int nt_ = min(TA, aend - (T0));                                                     \
+    const size_t src_ = (size_t)(T0) + min(sia, nt_ - 1);                                      \
+    _Pragma("unroll") for (int u = 0; u < ZU; ++u) {                                          \
+      const int nz_ = zoff + scg + CG * u - a.Kz;                                             \
+      zn[u] = tab[(size_t)(zcol + min(nz_ < 0 ? -nz_ : nz_, a.Kz)) * a.n + src_];             \
+    }                                                                                         \
+    _Pragma("unroll") for (int u = 0; u < XU; ++u)                                            \
+      xn[u] = tab[(size_t)min(scg + CG * u, nxy - 1) * a.n + src_];                           \
+    if (threadIdx.x < TA) {                                                                   \
+      const int i_ = (T0) + min((int)threadIdx.x, nt_ - 1);                                   \
+      if (a.mode <= 1) qd = a.q[i_];                                                          \
+      if (a.mode >= 1) {                                                                      \
+        m0 = a.mu[3 * i_];                                                                    \
+        m1 = a.mu[3 * i_ + 1];                                                                \
+        m2 = a.mu[3 * i_ + 2];                                                                \
+      }                                                                                       \
+    }                                                                                         \
+  } while (0)
+  if (abeg < aend) SKH3_FETCH(abeg);
   for (int t0 = abeg; t0 < aend; t0 += TA) {
     const int nt = min(TA, aend - t0);
     const bool sok = sia < nt;
-    const size_t src = (size_t)t0 + min(sia, nt - 1);
     __syncthreads();
     // Z rows: part 0 (Zr, -Zi), part 1 (Zi, Zr), Z(-n) = conj Z(n); atoms
     // beyond the chunk get zeros (their B is zero too)
-    for (int nzl = scg; nzl < RT * 16; nzl += CG) {
+#pragma unroll
+    for (int u = 0; u < ZU; ++u) {
+      const int nzl = scg + CG * u;
       const int nzi = zoff + nzl;
-      const int nz = nzi - a.Kz;
-      const int az = min(nz < 0 ? -nz : nz, a.Kz);
-      cplx<float> z = tab[(size_t)(zcol + az) * a.n + src];
-      if (nz < 0) z.im = -z.im;
+      cplx<float> z = zn[u];
+      if (nzi - a.Kz < 0) z.im = -z.im;
       if (!sok || nzi >= a.NZ) z = cplx<float>{0.f, 0.f};
       _Float16 rh, rl, ih, il;
       split_h(z.re, rh, rl);
@@ -890,36 +941,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       r0[128 * RW] = pack_h(rl, -il);
       r0[129 * RW] = pack_h(il, rl);
     }
-    // X and Y columns, clamped loads in batches of 8
-    for (int c0 = scg; c0 < nxy; c0 += 8 * CG) {
-      cplx<float> v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = tab[(size_t)min(c0 + u * CG, nxy - 1) * a.n + src];
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (c0 + u * CG < nxy) st[(size_t)(c0 + u * CG) * TAP + sia] = sok ? v[u] : cplx<float>{0.f, 0.f};
-    }
+    for (int u = 0; u < XU; ++u)
+      if (scg + CG * u < nxy) st[(size_t)(scg + CG * u) * TAP + sia] = sok ? xn[u] : cplx<float>{0.f, 0.f};
     if (threadIdx.x < TA) {
-      const int ia = threadIdx.x;
       float c0 = 0.f, c1 = 0.f, c2 = 0.f;
-      if (ia < nt) {
-        const int i = t0 + ia;
-        if (a.mode <= 1) c0 = (float)(a.q[i] * a.pscale);
-        if (a.mode >= 1) {
-          const double u0 = a.mu[3 * i], u1 = a.mu[3 * i + 1], u2 = a.mu[3 * i + 2];
-          if (a.mode == 1) {
-            c1 = (float)(2.0 * MDFF_PI * a.pscale * (u0 * a.b[0] + u1 * a.b[1] + u2 * a.b[2]));
-            c2 = (float)(2.0 * MDFF_PI * a.pscale * (u0 * a.b[3] + u1 * a.b[4] + u2 * a.b[5]));
-          } else {
-            c0 = (float)(a.pscale * (u0 * a.b[6] + u1 * a.b[7] + u2 * a.b[8]));
-          }
+      if ((int)threadIdx.x < nt) {
+        if (a.mode <= 1) c0 = (float)(qd * a.pscale);
+        if (a.mode == 1) {
+          c1 = (float)(2.0 * MDFF_PI * a.pscale * (m0 * a.b[0] + m1 * a.b[1] + m2 * a.b[2]));
+          c2 = (float)(2.0 * MDFF_PI * a.pscale * (m0 * a.b[3] + m1 * a.b[4] + m2 * a.b[5]));
+        } else if (a.mode == 2) {
+          c0 = (float)(a.pscale * (m0 * a.b[6] + m1 * a.b[7] + m2 * a.b[8]));
         }
       }
-      sq[ia] = c0;
-      sq[TAP + ia] = c1;
-      sq[2 * TAP + ia] = c2;
+      sq[threadIdx.x] = c0;
+      sq[TAP + threadIdx.x] = c1;
+      sq[2 * TAP + threadIdx.x] = c2;
     }
     __syncthreads();
+    if (t0 + TA < aend) SKH3_FETCH(t0 + TA);
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -979,52 +1020,62 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       if (nzi < a.NZ) out[2 * nzi + (R & 1)] = tot[rt][r] * inv;
     }
 }
+#undef SKH3_FETCH
 
 // ---- gather A operand: W per m-tile of 8 m values, scaled and split ----
 // amat[((mt * NSH + s) * 64 + lane) * 2 + {0: hi, 1: lo}] = 8 halves: row
 // l%32 = (m_local = row / 4, comp = row % 4), K = 16 s + 8 (l/32) + j =
 // (nz index t, Re/Im cc): comp 0 (Re U0): (wr, -wi), comp 1 (Im U0):
 // (wi, wr), comp 2/3 the same times nz.  Values times 2^(14 - e), e the
-// exponent of the tile's largest value; ascale[mt] = 2^(e - 14).  One wave
+// exponent of the tile's largest value; ascale[mt] = 2^(e - 14).  One block
 // per m-tile.
 template <int NSH>
 __global__ void __launch_bounds__(256) k_amat_h3(int M, int NZ, int Kz, int mtiles, const double* __restrict__ wk,
                                                  u32x4* __restrict__ amat, float* __restrict__ ascale) {
-  const int lane = threadIdx.x & 63;
-  const int mt = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (mt >= mtiles) return;                                // whole waves only
+  // one block per m-tile; wave w takes the K steps w, w + 4, ...
+  constexpr int SW = (NSH + 3) / 4;                        // steps per wave
+  __shared__ float wmax[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int mt = blockIdx.x;
   const int hl = lane >> 5, row = lane & 31;
   const int m = 8 * mt + row / 4, comp = row % 4;
-  float v[NSH][8];
+  float v[SW][8];
   float vmax = 0.f;
 #pragma unroll
-  for (int s = 0; s < NSH; ++s)
+  for (int u = 0; u < SW; ++u) {
+    const int s = w + 4 * u;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int kk = 16 * s + 8 * hl + j;
       const int t = kk >> 1, cc = kk & 1;
       double x = 0.0;
-      if (m < M && t < NZ) {
+      if (s < NSH && m < M && t < NZ) {
         const double wr = wk[2 * ((size_t)m * NZ + t)], wi = wk[2 * ((size_t)m * NZ + t) + 1];
         x = ((comp & 1) == 0) ? (cc == 0 ? wr : -wi) : (cc == 0 ? wi : wr);
         if (comp >= 2) x *= (double)(t - Kz);
       }
-      v[s][j] = (float)x;
-      vmax = fmaxf(vmax, fabsf(v[s][j]));
+      v[u][j] = (float)x;
+      vmax = fmaxf(vmax, fabsf(v[u][j]));
     }
+  }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
+  if (lane == 0) wmax[w] = vmax;
+  __syncthreads();
+  vmax = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
   int e = 0;
   if (vmax > 0.f) (void)frexpf(vmax, &e);                  // vmax in [2^(e-1), 2^e)
   const float sc = ldexpf(1.f, 14 - e);
-  if (lane == 0) ascale[mt] = ldexpf(1.f, e - 14);
+  if (threadIdx.x == 0) ascale[mt] = ldexpf(1.f, e - 14);
 #pragma unroll
-  for (int s = 0; s < NSH; ++s) {
+  for (int u = 0; u < SW; ++u) {
+    const int s = w + 4 * u;
+    if (s >= NSH) break;
     half8 h, l;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       _Float16 a, b;
-      split_h(v[s][j] * sc, a, b);
+      split_h(v[u][j] * sc, a, b);
       h[j] = a;
       l[j] = b;
     }
@@ -1106,10 +1157,7 @@ k_gather_h3(Gh3Args a) {
       if ((int)threadIdx.x + 256 * u < NA) as[0][threadIdx.x + 256 * u] = pre[u];
   }
   __syncthreads();
-  // the lane's m values of a tile: 8 mt + 2 q + hl, q < 4; (nx, ny + Ky) of q = 0
-  int mq = 8 * mt0 + hl;
-  int nxq = min(mq, a.M - 1) / a.NY;
-  int nyq = min(mq, a.M - 1) - nxq * a.NY;
+  const float inv_ny = 1.0f / (float)a.NY;
   for (int mt = mt0; mt < mt1; ++mt) {
     const int buf = (mt - mt0) & 1;
     if (mt + 1 < mt1) {
@@ -1117,31 +1165,27 @@ k_gather_h3(Gh3Args a) {
       for (int u = 0; u < PER; ++u)
         pre[u] = a.amat[(size_t)(mt + 1) * NA + min((int)threadIdx.x + 256 * u, NA - 1)];
     }
-    // epilogue operands of this tile (used after the MFMAs)
+    // epilogue operands of this tile (used after the MFMAs): the lane's m
+    // values 8 mt + 2 q + hl, q < 4, clamped below M (zero weight beyond)
     cplx<float> Xv[CT][4], Yv[CT][4];
     float fx[4], fy[4], ysg[4], mw[4];
-    {
-      int nxx = nxq, nyy = nyq;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int ny = nyy - a.Ky;
-        mw[q] = (mq + 2 * q < a.M) ? 1.f : 0.f;
-        fx[q] = (float)nxx;
-        fy[q] = (float)ny;
-        ysg[q] = ny < 0 ? -1.f : 1.f;
-        const size_t xrow = (size_t)min(nxx, a.Kx) * a.n;
-        const size_t yrow = (size_t)(a.Kx + 1 + (ny >= 0 ? ny : -ny)) * a.n;
+    for (int q = 0; q < 4; ++q) {
+      const int mq = 8 * mt + 2 * q + hl;
+      int nxx, nyo;
+      divmod_f(min(mq, a.M - 1), a.NY, inv_ny, nxx, nyo);
+      const int ny = nyo - a.Ky;
+      mw[q] = mq < a.M ? 1.f : 0.f;
+      fx[q] = (float)nxx;
+      fy[q] = (float)ny;
+      ysg[q] = ny < 0 ? -1.f : 1.f;
+      const size_t xrow = (size_t)nxx * a.n;
+      const size_t yrow = (size_t)(a.Kx + 1 + (ny >= 0 ? ny : -ny)) * a.n;
 #pragma unroll
-        for (int ct = 0; ct < CT; ++ct) {
-          Xv[ct][q] = tab[xrow + iac[ct]];
-          Yv[ct][q] = tab[yrow + iac[ct]];
-        }
-        nyy += 2;
-        while (nyy >= a.NY) { nyy -= a.NY; ++nxx; }
+      for (int ct = 0; ct < CT; ++ct) {
+        Xv[ct][q] = tab[xrow + iac[ct]];
+        Yv[ct][q] = tab[yrow + iac[ct]];
       }
-      mq += 8;
-      nyq += 8;
-      while (nyq >= a.NY) { nyq -= a.NY; ++nxq; }
     }
     f32x16 acc[CT];
 #pragma unroll
@@ -1251,6 +1295,29 @@ static const void* sk_kernel(int ctm, int ta) {
 #undef SK_CASE
 }
 
+// Split count of a gather over mtiles m-tiles for atom_waves waves of atoms:
+// every wave first loads its atoms' B operand (a prologue worth about
+// kGatherPrologue m-tiles), then runs its share of the m-tiles; the time is
+// taken as rounds of resident waves x (prologue + m-tiles per wave).  At
+// least 4 m-tiles per wave; fewer splits win ties.
+constexpr double kGatherPrologue = 4.0;
+static int gather_splits(int mtiles, int64_t atom_waves, int64_t conc) {
+  int best_s = 1;
+  double best = -1.0;
+  for (int sp = 1; sp <= std::min(mtiles, 64); ++sp) {
+    if (sp > 1 && mtiles / sp < 4) break;
+    const int mps = div_up(mtiles, sp);
+    const int nsp = div_up(mtiles, mps);
+    const int64_t rounds = (atom_waves * nsp + conc - 1) / conc;
+    const double cost = (double)rounds * (kGatherPrologue + mps);
+    if (best < 0.0 || cost < best - 1e-9) {
+      best = cost;
+      best_s = nsp;
+    }
+  }
+  return best_s;
+}
+
 // resident 256-thread blocks per CU of a kernel with `lds` bytes of LDS
 static int blocks_per_cu(const void* fn, size_t lds) {
   int nb = 0;
@@ -1351,26 +1418,11 @@ int recip_setup(mdff_ctx* c) {
   int cus = 256;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0)
     cus = 256;
-  // splits over m-tiles: pick the split count whose wave total fills whole
-  // rounds of resident waves best (at least 4 m-tiles per wave; fewer splits
-  // win ties)
+  // splits over m-tiles (gather_splits)
   const int atom_waves = div_up(c->n, 16 * c->mma_ct);
   const void* gfn = fp32 ? gather_kernel<float>(c->mma_ct, c->mma_nks) : gather_kernel<double>(c->mma_ct, c->mma_nks);
   const int64_t conc = 4 * (int64_t)cus * blocks_per_cu(gfn, 0);
-  int best_s = 1;
-  double best_e = -1.0;
-  for (int sp = 1; sp <= std::min(c->mma_mtiles, 64); ++sp) {
-    if (sp > 1 && c->mma_mtiles / sp < 4) break;
-    const int mps = div_up(c->mma_mtiles, sp);
-    const int nsp = div_up(c->mma_mtiles, mps);
-    const int64_t waves = (int64_t)atom_waves * nsp;
-    const int64_t rounds = (waves + conc - 1) / conc;
-    const double e = (double)waves / (double)(rounds * conc);
-    if (e > best_e + 0.02) {
-      best_e = e;
-      best_s = nsp;
-    }
-  }
+  const int best_s = gather_splits(c->mma_mtiles, atom_waves, conc);
   c->mma_splits = best_s;
   // matrix-core structure factor: atoms per staged tile within 64 KiB of
   // LDS; atom chunks chosen from the kernel's occupancy so that the blocks
@@ -1408,7 +1460,7 @@ int recip_setup(mdff_ctx* c) {
     const bool off = ev && ev[0] == '1';
     const int nxy = Kx + Ky + 2;
     const int nsh = h3_bucket(div_up(2 * c->NZk, 16));
-    if (fp32 && c->use_mma && !off && nsh > 0 && sk_h3_lds(nxy, kH3TA) <= 65536) {
+    if (fp32 && c->use_mma && !off && nsh > 0 && nxy <= 8 * (256 / kH3TA) && sk_h3_lds(nxy, kH3TA) <= 65536) {
       c->h3 = 1;
       c->h3_nsh = nsh;
       c->h3_ct = nsh <= 7 ? 2 : 1;
@@ -1417,20 +1469,10 @@ int recip_setup(mdff_ctx* c) {
       int e = 0;
       if (c->qmax > 0.0) (void)frexp(c->qmax, &e);
       c->h3_pscale = (float)ldexp(1.0, 9 - e);
-      // gather splits over m-tiles, as for the FP32 kernels
+      // gather splits over m-tiles (gather_splits)
       const int aw = div_up(c->n, 32 * c->h3_ct);
       const int64_t gconc = 4 * (int64_t)cus * blocks_per_cu(gather_h3_kernel(c->h3_ct, nsh), 0);
-      int bs = 1;
-      double be = -1.0;
-      for (int sp = 1; sp <= std::min(c->h3_mt8, 64); ++sp) {
-        if (sp > 1 && c->h3_mt8 / sp < 4) break;
-        const int mps = div_up(c->h3_mt8, sp);
-        const int nsp = div_up(c->h3_mt8, mps);
-        const int64_t waves = (int64_t)aw * nsp;
-        const int64_t rounds = (waves + gconc - 1) / gconc;
-        const double e2 = (double)waves / (double)(rounds * gconc);
-        if (e2 > be + 0.02) { be = e2; bs = nsp; }
-      }
+      const int bs = gather_splits(c->h3_mt8, aw, gconc);
       c->h3_splits = bs;
       // structure-factor chunks from the kernel's occupancy
       const size_t lds = sk_h3_lds(nxy, kH3TA);
@@ -1649,8 +1691,7 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     const int nsh = c->h3_nsh, mt8 = c->h3_mt8;
     void* aargs[] = {(void*)&s.M, (void*)&c->NZk, (void*)&c->K[2], (void*)&mt8, (void*)&c->wk, (void*)&c->amat_h3,
                      (void*)&c->ascale_h3};
-    PROF_LAUNCH(c, PK_AMAT, (void)hipLaunchKernel(amat_h3_kernel(nsh), dim3(div_up(mt8, 4)), dim3(256), aargs, 0,
-                                                  c->stream));
+    PROF_LAUNCH(c, PK_AMAT, (void)hipLaunchKernel(amat_h3_kernel(nsh), dim3(mt8), dim3(256), aargs, 0, c->stream));
     Gh3Args gh{};
     gh.n = c->n;
     gh.Kx = c->K[0]; gh.Ky = c->K[1]; gh.Kz = c->K[2];
