@@ -131,51 +131,93 @@ __device__ __forceinline__ double min_image_r2(const NLArgs& a, double dx, doubl
 
 template <bool FILL>
 __global__ void __launch_bounds__(256) k_nl_sweep(NLArgs a) {
+  // The neighbour cells' candidate ranges go to LDS once; the candidates are
+  // then taken in chunks of 64 (cell by cell, in cell order, as before) and
+  // the loads of the next chunk are issued before the current chunk is
+  // tested, so the wave does not wait on memory at every chunk.
+  __shared__ int crange[4][2][27];
   const int lane = threadIdx.x & 63;
+  const int wl = threadIdx.x >> 6;
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (wave >= a.n) return;
+  if (wave >= a.n) return;                 // whole waves: no block barrier below
   const int i = a.cell_atoms[wave];        // atoms visited in cell order (L2 locality)
   const double si = a.sx[i], ti = a.sy[i], ui = a.sz[i];
   const int ci = a.cell_of[i];
   const int cz = ci % a.nz, cy = (ci / a.nz) % a.ny, cx = ci / (a.nz * a.ny);
   int64_t base = FILL ? (int64_t)a.row_start[i] : 0;
-  int found = 0;
   const int ox0 = a.nx >= 3 ? -1 : 0, ox1 = a.nx >= 2 ? 1 : 0;
   const int oy0 = a.ny >= 3 ? -1 : 0, oy1 = a.ny >= 2 ? 1 : 0;
   const int oz0 = a.nz >= 3 ? -1 : 0, oz1 = a.nz >= 2 ? 1 : 0;
-  for (int dx = ox0; dx <= ox1; ++dx) {
-    int xx = (cx + dx + a.nx) % a.nx;
-    for (int dy = oy0; dy <= oy1; ++dy) {
-      int yy = (cy + dy + a.ny) % a.ny;
-      for (int dz = oz0; dz <= oz1; ++dz) {
-        int zz = (cz + dz + a.nz) % a.nz;
-        int c = (xx * a.ny + yy) * a.nz + zz;
-        int p0 = a.cell_start[c], p1 = a.cell_start[c + 1];
-        for (int p = p0 + lane; p - lane < p1; p += 64) {
-          bool ok = false;
-          int j = -1;
-          uint8_t flag = 0;
-          if (p < p1) {
-            j = a.cell_atoms[p];
-            if (j != i) {
-              double r2 = min_image_r2(a, si - a.px[p], ti - a.py[p], ui - a.pz[p]);
-              ok = (r2 <= a.cut2_max);
-              flag = (uint8_t)((r2 <= a.cut2_v ? 1 : 0) | (r2 <= a.cut2_c ? 2 : 0));
-            }
-          }
-          uint64_t m = __ballot(ok);
-          if (FILL && ok) {
-            int off = found + __popcll(m & ((1ull << lane) - 1ull));
-            int64_t e = base + off;
-            if (e < a.cap) {
-              a.nl_j[e] = p;                 // sorted position of j
-              a.nl_flag[e] = flag;
-            }
-          }
-          found += __popcll(m);
-        }
+  const int ncx = ox1 - ox0 + 1, ncy = oy1 - oy0 + 1, ncz = oz1 - oz0 + 1;
+  const int ncell = ncx * ncy * ncz;
+  if (lane < ncell) {
+    // cell k in the order of the nested dx, dy, dz loops
+    const int dz = oz0 + lane % ncz, dy = oy0 + (lane / ncz) % ncy, dx = ox0 + lane / (ncz * ncy);
+    const int xx = (cx + dx + a.nx) % a.nx, yy = (cy + dy + a.ny) % a.ny, zz = (cz + dz + a.nz) % a.nz;
+    const int c = (xx * a.ny + yy) * a.nz + zz;
+    crange[wl][0][lane] = a.cell_start[c];
+    crange[wl][1][lane] = a.cell_start[c + 1];
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // first non-empty chunk
+  int k = 0, p = 0;
+  while (k < ncell && crange[wl][0][k] >= crange[wl][1][k]) ++k;
+  if (k < ncell) p = crange[wl][0][k];
+  int found = 0;
+  // chunk loads: candidate j and its cell-ordered coordinates (clamped)
+  int jn = -1;
+  double xn = 0.0, yn = 0.0, zn = 0.0;
+  int pend = k < ncell ? crange[wl][1][k] : 0;
+  if (k < ncell && p + lane < pend) {
+    const int q = p + lane;
+    jn = a.cell_atoms[q];
+    xn = a.px[q];
+    yn = a.py[q];
+    zn = a.pz[q];
+  }
+  while (k < ncell) {
+    const int pc = p, pe = pend;
+    const int j = jn;
+    const double xj = xn, yj = yn, zj = zn;
+    // next chunk: the rest of this cell, else the next non-empty cell
+    p += 64;
+    if (p >= pend) {
+      ++k;
+      while (k < ncell && crange[wl][0][k] >= crange[wl][1][k]) ++k;
+      if (k < ncell) {
+        p = crange[wl][0][k];
+        pend = crange[wl][1][k];
       }
     }
+    jn = -1;
+    if (k < ncell && p + lane < pend) {
+      const int q = p + lane;
+      jn = a.cell_atoms[q];
+      xn = a.px[q];
+      yn = a.py[q];
+      zn = a.pz[q];
+    }
+    // test the current chunk
+    bool ok = false;
+    uint8_t flag = 0;
+    const int pp = pc + lane;
+    if (pp < pe && j != i) {
+      const double r2 = min_image_r2(a, si - xj, ti - yj, ui - zj);
+      ok = (r2 <= a.cut2_max);
+      flag = (uint8_t)((r2 <= a.cut2_v ? 1 : 0) | (r2 <= a.cut2_c ? 2 : 0));
+    }
+    const uint64_t m = __ballot(ok);
+    if (FILL && ok) {
+      const int off = found + __popcll(m & ((1ull << lane) - 1ull));
+      const int64_t e = base + off;
+      if (e < a.cap) {
+        a.nl_j[e] = pp;                  // sorted position of j
+        a.nl_flag[e] = flag;
+      }
+    }
+    found += __popcll(m);
   }
   if (!FILL && lane == 0) a.count[i] = found;
   if (FILL && lane == 0 && base + found > a.cap) atomicExch(a.overflow, 1);
