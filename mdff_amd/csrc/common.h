@@ -106,6 +106,62 @@ struct FinalSum {
 };
 constexpr int kMaxFinals = 8;
 
+// Last step of the reciprocal gather for one atom: the per-split partial
+// moments part[(s * moments + c) * n + i] summed over the splits, then the
+// Cartesian field and force (src/ewald_rec.f90:196-233, scaled as :300-358:
+// x2 for the half mesh, x 4pi/V).  B is column-major: b[0] = B(1,1),
+// b[1] = B(2,1), ...
+struct RecFinal {
+  int n, nsplit, moments, use_q;
+  const double* part;
+  const double* q;
+  const double* mu;               // 3n AoS (moments == 9)
+  double b[9];
+  double fpi_V;
+};
+
+// v[0..moments) = sum over splits; g = 2pi B v[0..3) (the gradient of the
+// charge phase sums)
+__device__ __forceinline__ void rec_final_moments(const RecFinal& r, int i, double v[9], double g[3]) {
+#pragma unroll
+  for (int c = 0; c < 9; ++c) v[c] = 0.0;
+  for (int s = 0; s < r.nsplit; ++s)
+    for (int c = 0; c < r.moments; ++c) v[c] += r.part[((size_t)s * r.moments + c) * r.n + i];
+  g[0] = r.b[0] * v[0] + r.b[3] * v[1] + r.b[6] * v[2];
+  g[1] = r.b[1] * v[0] + r.b[4] * v[1] + r.b[7] * v[2];
+  g[2] = r.b[2] * v[0] + r.b[5] * v[1] + r.b[8] * v[2];
+}
+
+__device__ __forceinline__ void rec_final_field(const RecFinal& r, const double g[3], double ef[3]) {
+  const double tp = 2.0 * MDFF_PI, sc = 2.0 * r.fpi_V;
+  for (int a = 0; a < 3; ++a) ef[a] = sc * tp * g[a];
+}
+
+__device__ __forceinline__ void rec_final_force(const RecFinal& r, int i, const double v[9], const double g[3],
+                                                double f[3]) {
+  const double tp = 2.0 * MDFF_PI, sc = 2.0 * r.fpi_V;
+  const double* b = r.b;
+  const double qi = r.use_q ? r.q[i] : 0.0;
+  double fx = qi * tp * g[0], fy = qi * tp * g[1], fz = qi * tp * g[2];
+  if (r.moments == 9) {
+    const double u0 = r.mu[3 * i], u1 = r.mu[3 * i + 1], u2 = r.mu[3 * i + 2];
+    // m = B^T mu; p = (symmetric second moments) m
+    const double m0 = u0 * b[0] + u1 * b[1] + u2 * b[2];
+    const double m1 = u0 * b[3] + u1 * b[4] + u2 * b[5];
+    const double m2 = u0 * b[6] + u1 * b[7] + u2 * b[8];
+    const double p0 = v[3] * m0 + v[6] * m1 + v[7] * m2;
+    const double p1 = v[6] * m0 + v[4] * m1 + v[8] * m2;
+    const double p2 = v[7] * m0 + v[8] * m1 + v[5] * m2;
+    const double t2 = tp * tp;
+    fx += t2 * (b[0] * p0 + b[3] * p1 + b[6] * p2);
+    fy += t2 * (b[1] * p0 + b[4] * p1 + b[7] * p2);
+    fz += t2 * (b[2] * p0 + b[5] * p1 + b[8] * p2);
+  }
+  f[0] = sc * fx;
+  f[1] = sc * fy;
+  f[2] = sc * fz;
+}
+
 // one atom of the pair kernels, in cell-sorted order: fractional coordinates
 // and charge (two 16-byte loads), plus its type and original index
 struct alignas(16) AtomRec {
@@ -154,13 +210,16 @@ struct mdff_ctx {
   double* red = nullptr;                  // reduction outputs
   int mu_static = 0;                      // mu holds the static dipoles (no induced dipoles)
   double *cx = nullptr, *cy = nullptr, *cz = nullptr;   // fractional coordinates of a pending list check
-  double* h_chk = nullptr;                // pinned: block maxima of the pending check
-  hipEvent_t chk_ev = nullptr;            // the pending check's copy to h_chk is complete
+  double* h_chk = nullptr;                // mapped pinned host memory: block maxima of the pending check
+  double* d_chk = nullptr;                // the device address of h_chk
+  hipEvent_t chk_ev = nullptr;            // the pending check's maxima are in h_chk
   int chk_pending = 0;
   double* red_part = nullptr;             // reduction block partials
   double* fin_part = nullptr;             // block partials of the pair sums [0, 1024*16) and kinetic energy
   double* rec_blk = nullptr;              // k-block partials of the reciprocal energy and stress
   FinalSum fin[kMaxFinals] = {};          // queued final sums
+  RecFinal rfin = {};                     // reciprocal forces left to the force kernel
+  int rfin_pending = 0;
   int nfin = 0;
   int efield_zero = 0;                    // efield holds zeros (no field, dipoles or surface term)
   PairParam* d_pp = nullptr;
@@ -188,7 +247,6 @@ struct mdff_ctx {
   size_t sort_tmp_bytes = 0;
   int lists_valid = 0;
   int32_t* d_flag = nullptr;              // device flags: [0] rebuild, [1] overflow
-  double* d_disp = nullptr;               // per-block max/second-max displacement partials
   int32_t* h_flag = nullptr;              // pinned host mirror
   double* h_red = nullptr;                // pinned host reduction mirror
   double* raw = nullptr;                  // device sums of evaluations: RAW_SLOTS x RAW_N
@@ -274,6 +332,10 @@ int nl_build(mdff_ctx* c);
 int nl_check(mdff_ctx* c, int* rebuilt);
 int nl_check_launch(mdff_ctx* c);
 int nl_check_finish(mdff_ctx* c, int* rebuilt);
+// velocity Verlet: position update; step end + list check (+ the next
+// step's position update)
+int vv_pos_launch(mdff_ctx* c, double dt);
+int vv_end_launch(mdff_ctx* c, double dt, bool next, double* ke_part);
 int reduce_cols_dev(mdff_ctx* c, const double* in, int ncol, double* dst);
 int reduce_cols_part(mdff_ctx* c, const double* in, int ncol, double* part, int* nb);
 int queue_final(mdff_ctx* c, const double* part, int nb, int ncol, double* out);
@@ -285,7 +347,11 @@ void pack_atoms(mdff_ctx* c);
 int realspace_field(mdff_ctx* c, int mode, const double* mu_in, double* ef_out, double alpha = 0.0);
 int recip_setup(mdff_ctx* c);
 int recip_tables(mdff_ctx* c);
-int recip_tables_from_r(mdff_ctx* c);     // kardir + tables in one launch
+// kardir + tables in one launch; pack: also pack_atoms (in the current cell
+// order)
+int recip_tables_from_r(mdff_ctx* c, bool pack);
+// forces with a null f_out: the split partials stay in rec_part and c->rfin
+// describes them, for a later kernel that finishes the forces itself
 int recip_eval(mdff_ctx* c, const double* q, const double* mu, bool use_q, bool use_mu,
                bool forces, bool field, double* ef_out, double* f_out, double* e_out);
 void recip_free(mdff_ctx* c);

@@ -65,22 +65,49 @@ __global__ void __launch_bounds__(256) k_final_sums(FinalArgs a) {
   sum_block_partials(f.nb, f.ncol, f.part, f.out, sh);
 }
 
-// total force f = f_sr + (f_rec + f_dir) * coul_unit (src/ewald.f90:237-246);
-// inputs AoS, output SoA
-__global__ void k_combine_forces(int n, const double* __restrict__ fsr, const double* __restrict__ fdir,
-                                 const double* __restrict__ frec, double coul, int has_sr, int has_cl,
-                                 double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double f[3];
-  for (int a = 0; a < 3; ++a) {
-    double v = has_sr ? fsr[3 * i + a] : 0.0;
-    if (has_cl) v = v + (frec[3 * i + a] + fdir[3 * i + a]) * coul;
-    f[a] = v;
+// Total force f = f_sr + (f_rec + f_dir) * coul_unit (src/ewald.f90:237-246),
+// AoS inputs, SoA output; f_rec from the array frec, or finished here from
+// the reciprocal split partials (rf_on).  The same blocks then write the
+// block partials of the pair kernel's per-atom columns (column sums over
+// atoms, as k_colsum).
+struct ForceSumArgs {
+  int n, ncol;
+  const double* fsr;
+  const double* fdir;
+  const double* frec;
+  RecFinal rf;
+  int rf_on, has_sr, has_cl;
+  double coul;
+  double *fx, *fy, *fz;
+  const double* in;              // [ncol][n]
+  double* part;                  // [block][ncol]
+};
+__global__ void __launch_bounds__(256) k_force_sums(ForceSumArgs a) {
+  __shared__ double sh[4 * kRedMaxCols];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+    double fr[3] = {0.0, 0.0, 0.0};
+    if (a.has_cl) {
+      if (a.rf_on) {
+        double v[9], g[3];
+        rec_final_moments(a.rf, i, v, g);
+        rec_final_force(a.rf, i, v, g, fr);
+      } else {
+        fr[0] = a.frec[3 * i];
+        fr[1] = a.frec[3 * i + 1];
+        fr[2] = a.frec[3 * i + 2];
+      }
+    }
+    double f[3];
+    for (int d = 0; d < 3; ++d) {
+      double v = a.has_sr ? a.fsr[3 * i + d] : 0.0;
+      if (a.has_cl) v = v + (fr[d] + a.fdir[3 * i + d]) * a.coul;
+      f[d] = v;
+    }
+    a.fx[i] = f[0];
+    a.fy[i] = f[1];
+    a.fz[i] = f[2];
   }
-  fx[i] = f[0];
-  fy[i] = f[1];
-  fz[i] = f[2];
+  colsum_partials(a.n, a.ncol, a.in, a.part, sh);
 }
 
 // ef = ef_dir + ef_rec + ef_surf + ef_self (src/ewald.f90:224-235);
@@ -256,45 +283,6 @@ __global__ void k_maxcomp(int n, const double* __restrict__ mu, double* __restri
   out[i] = m > 10.0 ? 1.0 : 0.0;
 }
 
-// prop_velocity_verlet position update (src/integration.f90:165-169)
-__global__ void k_vv_pos(int n, double dt, double dtsq2, const double* __restrict__ m, double* rx, double* ry,
-                         double* rz, const double* vx, const double* vy, const double* vz, const double* fx,
-                         const double* fy, const double* fz, double* fsx, double* fsy, double* fsz) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  fsx[i] = fx[i];
-  fsy[i] = fy[i];
-  fsz[i] = fz[i];
-  rx[i] = rx[i] + vx[i] * dt + (fx[i] * dtsq2) / m[i];
-  ry[i] = ry[i] + vy[i] * dt + (fy[i] * dtsq2) / m[i];
-  rz[i] = rz[i] + vz[i] * dt + (fz[i] * dtsq2) / m[i];
-}
-
-// velocity update (:179-183) + per-atom kinetic energy column
-
-// prop_velocity_verlet velocity update (src/integration.f90:178-182) with the
-// block partials of calc_temp's kinetic sum (:185): part[block] = sum over
-// the block's atoms of m v^2
-__global__ void __launch_bounds__(256) k_vv_vel_ke(int n, double dt2, const double* __restrict__ m, double* vx,
-                                                   double* vy, double* vz, const double* fsx, const double* fsy,
-                                                   const double* fsz, const double* fx, const double* fy,
-                                                   const double* fz, double* __restrict__ part) {
-  __shared__ double sh[4 * kRedMaxCols];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double v[kRedMaxCols] = {};
-  if (i < n) {
-    const double a = vx[i] + (fsx[i] + fx[i]) * dt2 / m[i];
-    const double b = vy[i] + (fsy[i] + fy[i]) * dt2 / m[i];
-    const double cc = vz[i] + (fsz[i] + fz[i]) * dt2 / m[i];
-    vx[i] = a;
-    vy[i] = b;
-    vz[i] = cc;
-    v[0] = (a * a + b * b + cc * cc) * m[i];
-  }
-  const double r = block_sum_cols(v, 1, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = r;
-}
-
 // prop_pos_vel_verlet (:420-427): half kick + drift
 __global__ void k_kick_drift(int n, double dt2, double dt, const double* __restrict__ m, double* rx, double* ry,
                              double* rz, double* vx, double* vy, double* vz, const double* fx, const double* fy,
@@ -357,12 +345,15 @@ int reduce_cols(mdff_ctx* c, const double* in, int ncol, double* host_out) {
   return MDFF_OK;
 }
 
+// blocks of a column-sum launch over the atoms
+static int colsum_blocks(int n) { return std::min(1024, std::max(1, div_up(n, 256))); }
+
 // block partials of the column sums into part ([nb][ncol]); the final sum is
 // queued or launched by the caller
 int reduce_cols_part(mdff_ctx* c, const double* in, int ncol, double* part, int* nb_out) {
   if (ncol < 1 || ncol > kRedMaxCols) return fail(c, MDFF_ERR_BAD_ARG, "column sum over too many columns");
   MDFF_TRY(claim_partials(c, part));
-  const int nb = std::min(1024, std::max(1, div_up(c->n, 256)));
+  const int nb = colsum_blocks(c->n);
   hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, c->stream, c->n, ncol, in, part);
   *nb_out = nb;
   return MDFF_OK;
@@ -397,7 +388,7 @@ int claim_partials(mdff_ctx* c, const double* part) {
 int reduce_cols_dev(mdff_ctx* c, const double* in, int ncol, double* dst) {
   if (ncol < 1 || ncol > kRedMaxCols) return fail(c, MDFF_ERR_BAD_ARG, "column sum over too many columns");
   MDFF_TRY(claim_partials(c, c->red_part));
-  const int nb = std::min(1024, std::max(1, div_up(c->n, 256)));
+  const int nb = colsum_blocks(c->n);
   hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, c->stream, c->n, ncol, in, c->red_part);
   hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(256), 0, c->stream, nb, ncol, c->red_part, dst);
   HIP_TRY(c, hipGetLastError());
@@ -564,9 +555,12 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   DMALLOC(c->nl_count, n + 1);
   DMALLOC(c->nl_start, n + 1);
   DMALLOC(c->d_flag, 4);
-  DMALLOC(c->d_disp, 2 * (size_t)nblk(n) + 2);
   DMALLOC(c->cx, n); DMALLOC(c->cy, n); DMALLOC(c->cz, n);
-  HIP_TRY(c, hipHostMalloc((void**)&c->h_chk, sizeof(double) * (2 * (size_t)nblk(n) + 2)));
+  // written by the check kernels over the bus, read by the host after the
+  // check's event: mapped, coherent (uncached on the device side)
+  HIP_TRY(c, hipHostMalloc((void**)&c->h_chk, sizeof(double) * (2 * (size_t)nblk(n) + 2),
+                           hipHostMallocMapped | hipHostMallocCoherent));
+  HIP_TRY(c, hipHostGetDevicePointer((void**)&c->d_chk, c->h_chk, 0));
   HIP_TRY(c, hipEventCreateWithFlags(&c->chk_ev, hipEventDisableTiming));
   HIP_TRY(c, hipHostMalloc((void**)&c->h_flag, sizeof(int32_t) * 4));
   const size_t nred = std::max<size_t>(64 * 1024, 2 * (size_t)nblk(n) + 2);
@@ -849,12 +843,16 @@ static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t, bool defer_f
   const bool recip_first = coul && !c->linduced;
   auto recip_stage = [&]() -> int {
     ev_rec(c, 2 * EVS_RECIP);
-    if (coul) MDFF_TRY(recip_eval(c, c->q, c->mu, true, dip, true, want_field, ef_rec(c), c->frec, raw + RAW_EREC));
+    if (coul) MDFF_TRY(recip_eval(c, c->q, c->mu, true, dip, true, want_field, ef_rec(c), nullptr, raw + RAW_EREC));
     ev_rec(c, 2 * EVS_RECIP + 1);
     return MDFF_OK;
   };
+  // atoms packed for the pair kernels by the table kernel (valid unless the
+  // lists are rebuilt below)
+  bool packed = false;
   if (recip_first) {
-    MDFF_TRY(recip_tables_from_r(c));         // kardir and the phase tables in one launch
+    packed = d.lvnlist && c->lists_valid;
+    MDFF_TRY(recip_tables_from_r(c, packed));  // kardir and the phase tables in one launch
     if (!c->mu_static) {
       // no induced dipoles: mu is the static set for the life of the context
       HIP_TRY(c, hipMemcpyAsync(c->mu, c->dipia, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
@@ -871,7 +869,7 @@ static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t, bool defer_f
   } else if (!recip_first) {
     launch_kardir(c);
   }
-  pack_atoms(c);
+  if (!packed || rb || t->list_rebuilt) pack_atoms(c);
   ev_rec(c, 2 * EVS_LIST + 1);
   if (!recip_first && coul) MDFF_TRY(recip_tables(c));
   // induced dipoles (get_dipole_moments, src/moment_from_pola.f90:84-224)
@@ -896,11 +894,29 @@ static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t, bool defer_f
   MDFF_TRY(realspace_force_impl(c, f_sr(c), f_dir(c), ef_dir(c), want_field));
   ev_rec(c, 2 * EVS_REAL + 1);
   if (!recip_first) MDFF_TRY(recip_stage());
-  hipLaunchKernelGGL(k_combine_forces, dim3(nblk(n)), dim3(256), 0, c->stream, n, f_sr(c), f_dir(c), c->frec,
-                     d.coul_unit, c->sr_model != 0 ? 1 : 0, coul ? 1 : 0, c->fx, c->fy, c->fz);
-  int nbp = 0;
-  MDFF_TRY(reduce_cols_part(c, c->peratom, NPA, c->fin_part, &nbp));
-  MDFF_TRY(queue_final(c, c->fin_part, nbp, NPA, raw + RAW_PA));
+  {
+    // total forces (the reciprocal ones finished from their split partials)
+    // and the block partials of the per-atom columns, in one launch
+    MDFF_TRY(claim_partials(c, c->fin_part));
+    ForceSumArgs fa{};
+    fa.n = n;
+    fa.ncol = NPA;
+    fa.fsr = f_sr(c);
+    fa.fdir = f_dir(c);
+    fa.frec = c->frec;
+    fa.rf = c->rfin;
+    fa.rf_on = (coul && c->rfin_pending) ? 1 : 0;
+    fa.has_sr = c->sr_model != 0 ? 1 : 0;
+    fa.has_cl = coul ? 1 : 0;
+    fa.coul = d.coul_unit;
+    fa.fx = c->fx; fa.fy = c->fy; fa.fz = c->fz;
+    fa.in = c->peratom;
+    fa.part = c->fin_part;
+    c->rfin_pending = 0;
+    const int nbp = colsum_blocks(n);
+    hipLaunchKernelGGL(k_force_sums, dim3(nbp), dim3(256), 0, c->stream, fa);
+    MDFF_TRY(queue_final(c, c->fin_part, nbp, NPA, raw + RAW_PA));
+  }
   if (coul) {
     // moments: sum mu^2 for the dipole self energy, sum mu and sum q r for the
     // surface terms; with neither dipoles nor lsurf they are all zero
@@ -1081,7 +1097,7 @@ int mdff_ctx_destroy(mdff_ctx* c) {
                 c->pol, c->invpol, c->dipia, c->mu, c->mu_ind, c->efield, c->e_stat, c->e_tot, c->e_ind,
                 c->hist, c->frec, c->zeros, c->arec, c->ameta, c->peratom, c->red, c->red_part, c->d_pp, c->cell_of, c->cell_key,
                 c->cell_atoms, c->cell_atoms_in, c->cell_start, c->nl_count, c->nl_start, c->nl_j,
-                c->nl_flag, c->sort_tmp, c->d_flag, c->d_disp, c->raw, c->cx, c->cy, c->cz, c->fin_part};
+                c->nl_flag, c->sort_tmp, c->d_flag, c->raw, c->cx, c->cy, c->cz, c->fin_part};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c->polar) (void)hipFree(c->polar);
@@ -1202,18 +1218,18 @@ int mdff_md_vv(mdff_ctx* c, int32_t nsteps, double dt, double* ekin, double* upo
   for (int st = 0; st < nsteps; ++st) {
     const int slot = st - st0;
     double* raw = c->raw + (size_t)RAW_N * slot;
-    hipLaunchKernelGGL(k_vv_pos, dim3(nblk(n)), dim3(256), 0, c->stream, n, dt, dt * dt * 0.5, c->mass, c->rx,
-                       c->ry, c->rz, c->vx, c->vy, c->vz, c->fx, c->fy, c->fz, c->fsx, c->fsy, c->fsz);
+    // the first position update; later ones are made by the previous step's
+    // vv_end_launch
+    if (st == 0) MDFF_TRY(vv_pos_launch(c, dt));
     MDFF_TRY(engforce_launch(c, raw, &held[slot], true));
     double* ke_part = c->fin_part + 1024 * 16;
     MDFF_TRY(claim_partials(c, ke_part));
-    hipLaunchKernelGGL(k_vv_vel_ke, dim3(nblk(n)), dim3(256), 0, c->stream, n, dt * 0.5, c->mass, c->vx, c->vy,
-                       c->vz, c->fsx, c->fsy, c->fsz, c->fx, c->fy, c->fz, ke_part);
+    // velocity update, kinetic sum, vnlistcheck after the step (finished by
+    // the next step's engforce_launch once that step's reciprocal work is
+    // queued, or below) and the next step's position update
+    MDFF_TRY(vv_end_launch(c, dt, st + 1 < nsteps, ke_part));
     if (ekin) MDFF_TRY(queue_final(c, ke_part, nblk(n), 1, raw + RAW_EKIN));
     MDFF_TRY(flush_finals(c));
-    // vnlistcheck after the step: queued now, finished by the next step's
-    // engforce_launch once that step's reciprocal work is queued (or below)
-    if (c->d.lvnlist) MDFF_TRY(nl_check_launch(c));
     if (slot == RAW_SLOTS - 1 || st == nsteps - 1) {
       MDFF_TRY(fetch_raw(c, slot + 1));
       for (int k = 0; k <= slot; ++k) {

@@ -22,6 +22,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include "common.h"
+#include "reduce.h"
 
 // kardir (src/lattice.f90:183-216): s = B^T r in the reference's order.
 __global__ void k_kardir(int n, const double* __restrict__ rx, const double* __restrict__ ry,
@@ -338,33 +339,9 @@ int nl_build(mdff_ctx* c) {
   return nl_build_from(c, c->sx, c->sy, c->sz);
 }
 
-// per-atom displacement since the saved fractional coordinates, reduced to
-// the two largest values per block (src/tools.f90:470-500)
-// The fractional coordinates of the check are formed here from r (kardir's
-// arithmetic) and stored in (cx, cy, cz) for a rebuild at these positions.
-__global__ void __launch_bounds__(256) k_disp_top2(NLArgs a, const double* __restrict__ rx,
-                                                   const double* __restrict__ ry,
-                                                   const double* __restrict__ rz, double b11, double b21,
-                                                   double b31, double b12, double b22, double b32, double b13,
-                                                   double b23, double b33, double* __restrict__ cx,
-                                                   double* __restrict__ cy, double* __restrict__ cz,
-                                                   const double* __restrict__ xs,
-                                                   const double* __restrict__ ys,
-                                                   const double* __restrict__ zs,
-                                                   double* __restrict__ out) {
-  __shared__ double s1[256], s2[256];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double m1 = 0.0, m2 = 0.0;
-  if (i < a.n) {
-    const double x = rx[i], y = ry[i], z = rz[i];
-    const double u = x * b11 + y * b21 + z * b31;
-    const double v = x * b12 + y * b22 + z * b32;
-    const double w = x * b13 + y * b23 + z * b33;
-    cx[i] = u;
-    cy[i] = v;
-    cz[i] = w;
-    m1 = sqrt(min_image_r2(a, u - xs[i], v - ys[i], w - zs[i]));
-  }
+// the two largest of the block's values m1 (m2 = 0 on entry) -> out[2 * block]
+// (all threads of the block take part)
+__device__ inline void block_top2(double m1, double m2, double* s1, double* s2, double* __restrict__ out) {
   s1[threadIdx.x] = m1;
   s2[threadIdx.x] = m2;
   __syncthreads();
@@ -386,6 +363,104 @@ __global__ void __launch_bounds__(256) k_disp_top2(NLArgs a, const double* __res
   }
 }
 
+// displacement of atom i since the saved fractional coordinates
+// (src/tools.f90:470-500); its fractional coordinates (kardir's arithmetic)
+// go to (cx, cy, cz) for a rebuild at these positions
+__device__ __forceinline__ double check_disp(const NLArgs& a, int i, double x, double y, double z, const double* b,
+                                             double* __restrict__ cx, double* __restrict__ cy,
+                                             double* __restrict__ cz, const double* __restrict__ xs,
+                                             const double* __restrict__ ys, const double* __restrict__ zs) {
+  const double u = x * b[0] + y * b[1] + z * b[2];
+  const double v = x * b[3] + y * b[4] + z * b[5];
+  const double w = x * b[6] + y * b[7] + z * b[8];
+  cx[i] = u;
+  cy[i] = v;
+  cz[i] = w;
+  return sqrt(min_image_r2(a, u - xs[i], v - ys[i], w - zs[i]));
+}
+
+struct CheckArgs {
+  NLArgs a;
+  double b[9];                    // B column-major
+  double *cx, *cy, *cz;
+  const double *xs, *ys, *zs;
+  double* out;                    // [block][2] (mapped host memory)
+};
+
+// per-atom displacement, reduced to the two largest values per block
+__global__ void __launch_bounds__(256) k_disp_top2(CheckArgs k, const double* __restrict__ rx,
+                                                   const double* __restrict__ ry,
+                                                   const double* __restrict__ rz) {
+  __shared__ double s1[256], s2[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double m1 = 0.0;
+  if (i < k.a.n) m1 = check_disp(k.a, i, rx[i], ry[i], rz[i], k.b, k.cx, k.cy, k.cz, k.xs, k.ys, k.zs);
+  block_top2(m1, 0.0, s1, s2, k.out);
+}
+
+// prop_velocity_verlet position update (src/integration.f90:165-169):
+// f_saved = f, r += v dt + f dt^2/2 / m
+__global__ void k_vv_pos(int n, double dt, double dtsq2, const double* __restrict__ m, double* rx, double* ry,
+                         double* rz, const double* vx, const double* vy, const double* vz, const double* fx,
+                         const double* fy, const double* fz, double* fsx, double* fsy, double* fsz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fsx[i] = fx[i];
+  fsy[i] = fy[i];
+  fsz[i] = fz[i];
+  rx[i] = rx[i] + vx[i] * dt + (fx[i] * dtsq2) / m[i];
+  ry[i] = ry[i] + vy[i] * dt + (fy[i] * dtsq2) / m[i];
+  rz[i] = rz[i] + vz[i] * dt + (fz[i] * dtsq2) / m[i];
+}
+
+// End of a velocity Verlet step (src/integration.f90:178-185), the list
+// check after it (vnlistcheck, src/tools.f90:444-537) and, when another step
+// follows, that step's position update (:165-169), one thread per atom:
+//   v += (f_saved + f) dt/2 / m; ke[block] = sum m v^2 over the block;
+//   the check's displacement of the positions of this step (block top-2);
+//   then f_saved = f and r += v dt + f dt^2/2 / m.
+struct VvEndArgs {
+  CheckArgs k;
+  int n, check, next;
+  double dt2, dt, dtsq2;
+  const double* m;
+  double *rx, *ry, *rz, *vx, *vy, *vz;
+  double *fsx, *fsy, *fsz;
+  const double *fx, *fy, *fz;
+  double* ke;                     // [block]
+};
+__global__ void __launch_bounds__(256) k_vv_end(VvEndArgs e) {
+  __shared__ double sh[4 * kRedMaxCols];
+  __shared__ double s1[256], s2[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double kv[kRedMaxCols] = {};
+  double m1 = 0.0;
+  if (i < e.n) {
+    const double m = e.m[i];
+    const double fx = e.fx[i], fy = e.fy[i], fz = e.fz[i];
+    const double a = e.vx[i] + (e.fsx[i] + fx) * e.dt2 / m;
+    const double b = e.vy[i] + (e.fsy[i] + fy) * e.dt2 / m;
+    const double cc = e.vz[i] + (e.fsz[i] + fz) * e.dt2 / m;
+    e.vx[i] = a;
+    e.vy[i] = b;
+    e.vz[i] = cc;
+    kv[0] = (a * a + b * b + cc * cc) * m;
+    const double x = e.rx[i], y = e.ry[i], z = e.rz[i];
+    if (e.check) m1 = check_disp(e.k.a, i, x, y, z, e.k.b, e.k.cx, e.k.cy, e.k.cz, e.k.xs, e.k.ys, e.k.zs);
+    if (e.next) {
+      e.fsx[i] = fx;
+      e.fsy[i] = fy;
+      e.fsz[i] = fz;
+      e.rx[i] = x + a * e.dt + (fx * e.dtsq2) / m;
+      e.ry[i] = y + b * e.dt + (fy * e.dtsq2) / m;
+      e.rz[i] = z + cc * e.dt + (fz * e.dtsq2) / m;
+    }
+  }
+  const double r = block_sum_cols(kv, 1, sh);
+  if (threadIdx.x == 0) e.ke[blockIdx.x] = r;
+  if (e.check) block_top2(m1, 0.0, s1, s2, e.k.out);
+}
+
 __global__ void k_copy3(int n, const double* a, const double* b, const double* cc, double* x, double* y,
                         double* z) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -398,22 +473,62 @@ __global__ void k_copy3(int n, const double* a, const double* b, const double* c
 // vnlistcheck: rebuild when (largest + second largest displacement) > skindiff,
 // then save the fractional coordinates (src/tools.f90:513-526).  Split in two
 // halves so that a caller can queue more work between them: the launch half
-// queues the displacement check of the current positions (their fractional
-// coordinates go to cx, cy, cz) and the copy of its block maxima to the host;
-// the finishing half waits for that copy and, when the skin is exceeded,
-// builds the lists from those same coordinates (the positions of the check,
-// whatever the positions are by then).
-int nl_check_launch(mdff_ctx* c) {
-  const int n = c->n;
-  NLArgs a = make_args(c);
-  const int nb = div_up(n, 256);
-  const double* B = c->B;
-  hipLaunchKernelGGL(k_disp_top2, dim3(nb), dim3(256), 0, c->stream, a, c->rx, c->ry, c->rz, B[0], B[1], B[2],
-                     B[3], B[4], B[5], B[6], B[7], B[8], c->cx, c->cy, c->cz, c->xs, c->ys, c->zs, c->d_disp);
-  HIP_TRY(c, hipMemcpyAsync(c->h_chk, c->d_disp, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->stream));
+// (nl_check_launch, or vv_end_launch at the end of an MD step) queues the
+// displacement check of the current positions (their fractional coordinates
+// go to cx, cy, cz; its block maxima to mapped host memory); the finishing
+// half waits for it and, when the skin is exceeded, builds the lists from
+// those same coordinates (the positions of the check, whatever the positions
+// are by then).
+static CheckArgs check_args(mdff_ctx* c) {
+  CheckArgs k{};
+  k.a = make_args(c);
+  for (int j = 0; j < 9; ++j) k.b[j] = c->B[j];
+  k.cx = c->cx; k.cy = c->cy; k.cz = c->cz;
+  k.xs = c->xs; k.ys = c->ys; k.zs = c->zs;
+  k.out = c->d_chk;
+  return k;
+}
+
+// the check's block maxima are written straight into mapped host memory
+// (h_chk); the event says when they are there
+static int check_posted(mdff_ctx* c) {
   HIP_TRY(c, hipEventRecord(c->chk_ev, c->stream));
   c->chk_pending = 1;
   return MDFF_OK;
+}
+
+int nl_check_launch(mdff_ctx* c) {
+  const int nb = div_up(c->n, 256);
+  hipLaunchKernelGGL(k_disp_top2, dim3(nb), dim3(256), 0, c->stream, check_args(c), c->rx, c->ry, c->rz);
+  HIP_TRY(c, hipGetLastError());
+  return check_posted(c);
+}
+
+int vv_pos_launch(mdff_ctx* c, double dt) {
+  hipLaunchKernelGGL(k_vv_pos, dim3(div_up(c->n, 256)), dim3(256), 0, c->stream, c->n, dt, dt * dt * 0.5, c->mass,
+                     c->rx, c->ry, c->rz, c->vx, c->vy, c->vz, c->fx, c->fy, c->fz, c->fsx, c->fsy, c->fsz);
+  HIP_TRY(c, hipGetLastError());
+  return MDFF_OK;
+}
+
+int vv_end_launch(mdff_ctx* c, double dt, bool next, double* ke_part) {
+  VvEndArgs e{};
+  e.k = check_args(c);
+  e.n = c->n;
+  e.check = c->d.lvnlist ? 1 : 0;
+  e.next = next ? 1 : 0;
+  e.dt2 = dt * 0.5;
+  e.dt = dt;
+  e.dtsq2 = dt * dt * 0.5;
+  e.m = c->mass;
+  e.rx = c->rx; e.ry = c->ry; e.rz = c->rz;
+  e.vx = c->vx; e.vy = c->vy; e.vz = c->vz;
+  e.fsx = c->fsx; e.fsy = c->fsy; e.fsz = c->fsz;
+  e.fx = c->fx; e.fy = c->fy; e.fz = c->fz;
+  e.ke = ke_part;
+  hipLaunchKernelGGL(k_vv_end, dim3(div_up(c->n, 256)), dim3(256), 0, c->stream, e);
+  HIP_TRY(c, hipGetLastError());
+  return e.check ? check_posted(c) : MDFF_OK;
 }
 
 int nl_check_finish(mdff_ctx* c, int* rebuilt) {

@@ -55,34 +55,54 @@ __global__ void k_tables(int n, int Kx, int Ky, int Kz, const double* __restrict
 // as kardir does (src/lattice.f90:183-216, products and sums rounded
 // separately, no fused multiply-add), so s is bit-identical to the list
 // code's; the blocks of column 0 also store s (sx, sy, sz).
+struct TabRArgs {
+  int n, Kx, Ky, Kz;
+  const double *rx, *ry, *rz;
+  double b[9];                   // B column-major
+  double *sx, *sy, *sz;
+  void* tab;
+  // pack != 0: also the atoms in the cell order of the last list build
+  // (rec[p], meta[p] of atom order[p], as pack_atoms)
+  int pack;
+  const int32_t* order;
+  const double* q;
+  const int32_t* itype;
+  AtomRec* rec;
+  int2* meta;
+};
+
 template <typename T>
-__global__ void k_tables_r(int n, int Kx, int Ky, int Kz, const double* __restrict__ rx,
-                           const double* __restrict__ ry, const double* __restrict__ rz, double b11, double b21,
-                           double b31, double b12, double b22, double b32, double b13, double b23, double b33,
-                           double* __restrict__ sx, double* __restrict__ sy, double* __restrict__ sz,
-                           cplx<T>* __restrict__ tab) {
+__global__ void k_tables_r(TabRArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int col = blockIdx.y;
-  if (i >= n) return;
-  const double x = rx[i], y = ry[i], z = rz[i];
-  auto dot = [&](double u, double v, double w) {
+  if (i >= a.n) return;
+  auto dot = [](double x, double y, double z, double u, double v, double w) {
     return __dadd_rn(__dadd_rn(__dmul_rn(x, u), __dmul_rn(y, v)), __dmul_rn(z, w));
   };
+  const double* b = a.b;
+  const double x = a.rx[i], y = a.ry[i], z = a.rz[i];
   if (col == 0) {
-    sx[i] = dot(b11, b21, b31);
-    sy[i] = dot(b12, b22, b32);
-    sz[i] = dot(b13, b23, b33);
+    a.sx[i] = dot(x, y, z, b[0], b[1], b[2]);
+    a.sy[i] = dot(x, y, z, b[3], b[4], b[5]);
+    a.sz[i] = dot(x, y, z, b[6], b[7], b[8]);
+    if (a.pack) {
+      const int j = a.order[i];
+      const double xj = a.rx[j], yj = a.ry[j], zj = a.rz[j];
+      a.rec[i] = AtomRec{dot(xj, yj, zj, b[0], b[1], b[2]), dot(xj, yj, zj, b[3], b[4], b[5]),
+                         dot(xj, yj, zj, b[6], b[7], b[8]), a.q[j]};
+      a.meta[i] = make_int2(a.itype[j], j);
+    }
   }
   double s;
   int k;
-  if (col <= Kx) { s = dot(b11, b21, b31); k = col; }
-  else if (col <= Kx + Ky + 1) { s = dot(b12, b22, b32); k = col - Kx - 1; }
-  else { s = dot(b13, b23, b33); k = col - Kx - Ky - 2; }
+  if (col <= a.Kx) { s = dot(x, y, z, b[0], b[1], b[2]); k = col; }
+  else if (col <= a.Kx + a.Ky + 1) { s = dot(x, y, z, b[3], b[4], b[5]); k = col - a.Kx - 1; }
+  else { s = dot(x, y, z, b[6], b[7], b[8]); k = col - a.Kx - a.Ky - 2; }
   double t = (double)k * s;
   t -= rint(t);
   double sn, cs;
   sincospi(2.0 * t, &sn, &cs);
-  tab[(size_t)col * n + i] = {(T)cs, (T)sn};
+  reinterpret_cast<cplx<T>*>(a.tab)[(size_t)col * a.n + i] = {(T)cs, (T)sn};
 }
 
 struct SkArgs {
@@ -374,47 +394,25 @@ __global__ void __launch_bounds__(256) k_gather(GaArgs a) {
 
 // sum over splits, then field / force in Cartesian (src/ewald_rec.f90:196-233,
 // scaled as :300-358: x2 for the half mesh, x 4pi/V)
-__global__ void k_gather_final(int n, int nsplit, int moments, const double* __restrict__ part,
-                               const double* __restrict__ q, const double* __restrict__ mu, double b11,
-                               double b21, double b31, double b12, double b22, double b32, double b13,
-                               double b23, double b33, double fpi_V, int use_q, int field, int forces,
-                               double* __restrict__ ef, double* __restrict__ f) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int s = 0; s < nsplit; ++s)
-    for (int c = 0; c < moments; ++c) v[c] += part[((size_t)s * moments + c) * n + i];
-  const double tp = 2.0 * MDFF_PI;
-  // B U1 (Cartesian), B column-major: (B u)_a = sum_d B(a,d) u_d
-  const double gx = b11 * v[0] + b12 * v[1] + b13 * v[2];
-  const double gy = b21 * v[0] + b22 * v[1] + b23 * v[2];
-  const double gz = b31 * v[0] + b32 * v[1] + b33 * v[2];
-  const double sc = 2.0 * fpi_V;
+__global__ void k_gather_final(RecFinal r, int field, int forces, double* __restrict__ ef,
+                               double* __restrict__ f) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= r.n) return;
+  double v[9], g[3];
+  rec_final_moments(r, i, v, g);
   if (field) {
-    ef[3 * i] = sc * tp * gx;
-    ef[3 * i + 1] = sc * tp * gy;
-    ef[3 * i + 2] = sc * tp * gz;
+    double e[3];
+    rec_final_field(r, g, e);
+    ef[3 * i] = e[0];
+    ef[3 * i + 1] = e[1];
+    ef[3 * i + 2] = e[2];
   }
   if (forces) {
-    double qi = use_q ? q[i] : 0.0;
-    double fxv = qi * tp * gx, fyv = qi * tp * gy, fzv = qi * tp * gz;
-    if (moments == 9) {
-      const double u0 = mu[3 * i], u1 = mu[3 * i + 1], u2 = mu[3 * i + 2];
-      // m = B^T mu
-      const double m0 = u0 * b11 + u1 * b21 + u2 * b31;
-      const double m1 = u0 * b12 + u1 * b22 + u2 * b32;
-      const double m2 = u0 * b13 + u1 * b23 + u2 * b33;
-      const double p0 = v[3] * m0 + v[6] * m1 + v[7] * m2;
-      const double p1 = v[6] * m0 + v[4] * m1 + v[8] * m2;
-      const double p2 = v[7] * m0 + v[8] * m1 + v[5] * m2;
-      const double t2 = tp * tp;
-      fxv += t2 * (b11 * p0 + b12 * p1 + b13 * p2);
-      fyv += t2 * (b21 * p0 + b22 * p1 + b23 * p2);
-      fzv += t2 * (b31 * p0 + b32 * p1 + b33 * p2);
-    }
-    f[3 * i] = sc * fxv;
-    f[3 * i + 1] = sc * fyv;
-    f[3 * i + 2] = sc * fzv;
+    double fv[3];
+    rec_final_force(r, i, v, g, fv);
+    f[3 * i] = fv[0];
+    f[3 * i + 1] = fv[1];
+    f[3 * i + 2] = fv[2];
   }
 }
 
@@ -1514,19 +1512,28 @@ static void tables_t(mdff_ctx* c) {
 }
 
 template <typename T>
-static void tables_r_t(mdff_ctx* c) {
+static void tables_r_t(mdff_ctx* c, bool pack) {
   const int ncol = c->K[0] + c->K[1] + c->K[2] + 3;
-  const double* B = c->B;
+  TabRArgs a{};
+  a.n = c->n;
+  a.Kx = c->K[0]; a.Ky = c->K[1]; a.Kz = c->K[2];
+  a.rx = c->rx; a.ry = c->ry; a.rz = c->rz;
+  for (int k = 0; k < 9; ++k) a.b[k] = c->B[k];
+  a.sx = c->sx; a.sy = c->sy; a.sz = c->sz;
+  a.tab = c->tab;
+  a.pack = pack ? 1 : 0;
+  a.order = c->cell_atoms;
+  a.q = c->q;
+  a.itype = c->itype;
+  a.rec = c->arec;
+  a.meta = c->ameta;
   PROF_LAUNCH(c, PK_TABLES,
-              hipLaunchKernelGGL((k_tables_r<T>), dim3(div_up(c->n, 256), ncol), dim3(256), 0, c->stream, c->n,
-                                 c->K[0], c->K[1], c->K[2], c->rx, c->ry, c->rz, B[0], B[1], B[2], B[3], B[4],
-                                 B[5], B[6], B[7], B[8], c->sx, c->sy, c->sz,
-                                 reinterpret_cast<cplx<T>*>(c->tab)));
+              hipLaunchKernelGGL((k_tables_r<T>), dim3(div_up(c->n, 256), ncol), dim3(256), 0, c->stream, a));
 }
 
-int recip_tables_from_r(mdff_ctx* c) {
-  if (c->d.precision == MDFF_PREC_MIXED) tables_r_t<float>(c);
-  else tables_r_t<double>(c);
+int recip_tables_from_r(mdff_ctx* c, bool pack) {
+  if (c->d.precision == MDFF_PREC_MIXED) tables_r_t<float>(c, pack);
+  else tables_r_t<double>(c, pack);
   HIP_TRY(c, hipGetLastError());
   return MDFF_OK;
 }
@@ -1747,12 +1754,27 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     if (dipf) PROF_LAUNCH(c, PK_GATHER, hipLaunchKernelGGL((k_gather<T, 9>), g2, dim3(nthr), shm2, c->stream, g));
     else PROF_LAUNCH(c, PK_GATHER, hipLaunchKernelGGL((k_gather<T, 3>), g2, dim3(nthr), shm2, c->stream, g));
   }
-  const double* B = c->B;
-  const double fpi_V = 2.0 * (2.0 * MDFF_PI / c->omega);
-  PROF_LAUNCH(c, PK_GFINAL,
-              hipLaunchKernelGGL(k_gather_final, dim3(div_up(c->n, 256)), dim3(256), 0, c->stream, c->n, nsplit,
-                                 moments, c->rec_part, q, mu, B[0], B[1], B[2], B[3], B[4], B[5], B[6], B[7],
-                                 B[8], fpi_V, use_q ? 1 : 0, field ? 1 : 0, forces ? 1 : 0, ef_out, f_out));
+  RecFinal rf{};
+  rf.n = c->n;
+  rf.nsplit = nsplit;
+  rf.moments = moments;
+  rf.use_q = use_q ? 1 : 0;
+  rf.part = c->rec_part;
+  rf.q = q;
+  rf.mu = mu;
+  for (int k = 0; k < 9; ++k) rf.b[k] = c->B[k];
+  rf.fpi_V = 2.0 * (2.0 * MDFF_PI / c->omega);
+  // forces wanted with no output array: the caller's force kernel finishes
+  // them from the split partials (c->rfin)
+  const bool defer = forces && !f_out;
+  if (defer) {
+    c->rfin = rf;
+    c->rfin_pending = 1;
+  }
+  if (field || !defer)
+    PROF_LAUNCH(c, PK_GFINAL,
+                hipLaunchKernelGGL(k_gather_final, dim3(div_up(c->n, 256)), dim3(256), 0, c->stream, rf,
+                                   field ? 1 : 0, (forces && !defer) ? 1 : 0, ef_out, f_out));
   HIP_TRY(c, hipGetLastError());
   return MDFF_OK;
 }
