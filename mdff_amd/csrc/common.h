@@ -242,6 +242,8 @@ struct mdff_ctx {
   uint8_t* nl_flag = nullptr;             // bit0: r2 <= (rc_vdw+skin)^2, bit1: <= (rc_coul+skin)^2
   int64_t nl_cap = 0, nl_total = 0;
   int ncell[3] = {1, 1, 1};
+  int nb_lo[3] = {0, 0, 0}, nb_hi[3] = {0, 0, 0}, nb_all[3] = {1, 1, 1};  // neighbour cell offsets
+  float4* bf = nullptr;                   // wrapped single-precision copy of (bx, by, bz)
   int ncell_alloc = 0;
   void* sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;

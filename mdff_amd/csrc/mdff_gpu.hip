@@ -527,6 +527,7 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   DMALLOC(c->sx, n); DMALLOC(c->sy, n); DMALLOC(c->sz, n);
   DMALLOC(c->xs, n); DMALLOC(c->ys, n); DMALLOC(c->zs, n);
   DMALLOC(c->bx, n); DMALLOC(c->by, n); DMALLOC(c->bz, n);
+  DMALLOC(c->bf, n);
   DMALLOC(c->itype, n);
   DMALLOC(c->q, n);
   DMALLOC(c->mass, n);
@@ -1093,7 +1094,7 @@ int mdff_ctx_destroy(mdff_ctx* c) {
   if (!c) return MDFF_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* ps[] = {c->rx, c->ry, c->rz, c->vx, c->vy, c->vz, c->fx, c->fy, c->fz, c->fsx, c->fsy, c->fsz,
-                c->sx, c->sy, c->sz, c->xs, c->ys, c->zs, c->bx, c->by, c->bz, c->itype, c->q, c->mass,
+                c->sx, c->sy, c->sz, c->xs, c->ys, c->zs, c->bx, c->by, c->bz, c->bf, c->itype, c->q, c->mass,
                 c->pol, c->invpol, c->dipia, c->mu, c->mu_ind, c->efield, c->e_stat, c->e_tot, c->e_ind,
                 c->hist, c->frec, c->zeros, c->arec, c->ameta, c->peratom, c->red, c->red_part, c->d_pp, c->cell_of, c->cell_key,
                 c->cell_atoms, c->cell_atoms_in, c->cell_start, c->nl_count, c->nl_start, c->nl_j,

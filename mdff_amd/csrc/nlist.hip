@@ -92,15 +92,21 @@ __global__ void k_cell_start(int n, int ncell, const int32_t* __restrict__ key,
     for (int c = k + 1; c <= ncell; ++c) start[c] = n;
 }
 
+// fractional coordinates in cell order: the doubles, and a wrapped
+// single-precision copy (s - floor(s), exact in double, then rounded) for
+// the sweep's prefilter
 __global__ void k_gather_sorted(int n, const int32_t* __restrict__ order, const double* __restrict__ sx,
                                 const double* __restrict__ sy, const double* __restrict__ sz,
-                                double* __restrict__ ox, double* __restrict__ oy, double* __restrict__ oz) {
+                                double* __restrict__ ox, double* __restrict__ oy, double* __restrict__ oz,
+                                float4* __restrict__ of) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   int i = order[p];
-  ox[p] = sx[i];
-  oy[p] = sy[i];
-  oz[p] = sz[i];
+  const double x = sx[i], y = sy[i], z = sz[i];
+  ox[p] = x;
+  oy[p] = y;
+  oz[p] = z;
+  of[p] = make_float4((float)(x - floor(x)), (float)(y - floor(y)), (float)(z - floor(z)), 0.f);
 }
 
 struct NLArgs {
@@ -109,6 +115,15 @@ struct NLArgs {
   double cut2_max, cut2_v, cut2_c;
   const double *sx, *sy, *sz;             // fractional coords, original order
   const double *px, *py, *pz;             // fractional coords, cell order
+  const float4* pf;                       // wrapped single-precision copy, cell order
+  // neighbour cells: offsets ox0..ox1 (x), oy0..oy1 (y) around the atom's
+  // cell; in z either oz0..oz1 around it or, zall != 0, every z cell
+  int ox0, ox1, oy0, oy1, oz0, oz1, zall;
+  // prefilter: r^2 in single precision; decided exactly (FP64, reference
+  // rounding) when within marg of a cutoff or when prefilter == 0
+  int prefilter;
+  float a11f, a21f, a31f, a12f, a22f, a32f, a13f, a23f, a33f;
+  float c2f_max, c2f_v, c2f_c, marg;
   const int32_t* cell_atoms;
   const int32_t* cell_start;
   const int32_t* cell_of;
@@ -130,91 +145,128 @@ __device__ __forceinline__ double min_image_r2(const NLArgs& a, double dx, doubl
   return rx * rx + ry * ry + rz * rz;
 }
 
+// Candidate index v of the wave's concatenated ranges -> sorted position:
+// range k holds v in [pref[k], pref[k + 1]) starting at position st[k];
+// unused entries hold pref = total, so the branch-free search over 64
+// entries ends on the range that holds v.
+__device__ __forceinline__ int range_pos(const int* pref, const int* st, int v) {
+  int k = 0;
+#pragma unroll
+  for (int step = 32; step > 0; step >>= 1) k += (pref[k + step] <= v) ? step : 0;
+  return st[k] + (v - pref[k]);
+}
+
+// One wave per atom (atoms in cell order).  The neighbour cells form, per
+// (x, y) column, one or two contiguous ranges of the cell-sorted atoms (the
+// z cells of a column are consecutive cells); the ranges are concatenated
+// and swept 64 candidates at a time, the loads of the next 64 issued before
+// the current ones are tested.  A candidate is tested in single precision
+// (wrapped coordinates, rint minimum image); one whose r^2 lies within marg
+// of a cutoff is tested again in FP64 exactly as the reference does, so the
+// accepted set and the flags equal the reference's bit for bit.
 template <bool FILL>
 __global__ void __launch_bounds__(256) k_nl_sweep(NLArgs a) {
-  // The neighbour cells' candidate ranges go to LDS once; the candidates are
-  // then taken in chunks of 64 (cell by cell, in cell order, as before) and
-  // the loads of the next chunk are issued before the current chunk is
-  // tested, so the wave does not wait on memory at every chunk.
-  __shared__ int crange[4][2][27];
+  __shared__ int rpref[4][66], rst[4][65];
   const int lane = threadIdx.x & 63;
   const int wl = threadIdx.x >> 6;
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (wave >= a.n) return;                 // whole waves: no block barrier below
-  const int i = a.cell_atoms[wave];        // atoms visited in cell order (L2 locality)
+  const int i = a.cell_atoms[wave];
   const double si = a.sx[i], ti = a.sy[i], ui = a.sz[i];
+  const float sif = (float)(si - floor(si)), tif = (float)(ti - floor(ti)), uif = (float)(ui - floor(ui));
   const int ci = a.cell_of[i];
   const int cz = ci % a.nz, cy = (ci / a.nz) % a.ny, cx = ci / (a.nz * a.ny);
-  int64_t base = FILL ? (int64_t)a.row_start[i] : 0;
-  const int ox0 = a.nx >= 3 ? -1 : 0, ox1 = a.nx >= 2 ? 1 : 0;
-  const int oy0 = a.ny >= 3 ? -1 : 0, oy1 = a.ny >= 2 ? 1 : 0;
-  const int oz0 = a.nz >= 3 ? -1 : 0, oz1 = a.nz >= 2 ? 1 : 0;
-  const int ncx = ox1 - ox0 + 1, ncy = oy1 - oy0 + 1, ncz = oz1 - oz0 + 1;
-  const int ncell = ncx * ncy * ncz;
-  if (lane < ncell) {
-    // cell k in the order of the nested dx, dy, dz loops
-    const int dz = oz0 + lane % ncz, dy = oy0 + (lane / ncz) % ncy, dx = ox0 + lane / (ncz * ncy);
-    const int xx = (cx + dx + a.nx) % a.nx, yy = (cy + dy + a.ny) % a.ny, zz = (cz + dz + a.nz) % a.nz;
-    const int c = (xx * a.ny + yy) * a.nz + zz;
-    crange[wl][0][lane] = a.cell_start[c];
-    crange[wl][1][lane] = a.cell_start[c + 1];
+  const int64_t base = FILL ? (int64_t)a.row_start[i] : 0;
+  const int ncx = a.ox1 - a.ox0 + 1, ncy = a.oy1 - a.oy0 + 1;
+  const int ncol = ncx * ncy;              // <= 25 columns, <= 2 ranges each
+  int b0 = 0, e0 = 0, b1 = 0, e1 = 0;
+  if (lane < ncol) {
+    const int dy = a.oy0 + lane % ncy, dx = a.ox0 + lane / ncy;
+    const int xx = (cx + dx + a.nx) % a.nx, yy = (cy + dy + a.ny) % a.ny;
+    const int c0 = (xx * a.ny + yy) * a.nz;  // cell (xx, yy, 0)
+    int z0 = 0, z1 = a.nz - 1, w0 = 0, w1 = -1;   // [z0, z1] and the wrapped [w0, w1]
+    if (!a.zall) {
+      z0 = cz + a.oz0;
+      z1 = cz + a.oz1;
+      if (z0 < 0) { w0 = z0 + a.nz; w1 = a.nz - 1; z0 = 0; }
+      else if (z1 >= a.nz) { w0 = 0; w1 = z1 - a.nz; z1 = a.nz - 1; }
+    }
+    b0 = a.cell_start[c0 + z0];
+    e0 = a.cell_start[c0 + z1 + 1];
+    if (w1 >= w0) {
+      b1 = a.cell_start[c0 + w0];
+      e1 = a.cell_start[c0 + w1 + 1];
+    }
   }
+  // prefix of the range lengths over the lanes (2 ranges per lane)
+  const int len = (e0 - b0) + (e1 - b1);
+  int incl = len;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  const int total = __shfl(incl, 63, 64);
+  const int excl = incl - len;
+  // range 2l: [b0, e0), range 2l+1: [b1, e1); 64 entries (2 * 25 + pad)
+  if (lane < 32) {
+    rpref[wl][2 * lane] = lane < ncol ? excl : total;
+    rpref[wl][2 * lane + 1] = lane < ncol ? excl + (e0 - b0) : total;
+    rst[wl][2 * lane] = b0;
+    rst[wl][2 * lane + 1] = b1;
+  }
+  if (lane < 2) rpref[wl][64 + lane] = total;
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // first non-empty chunk
-  int k = 0, p = 0;
-  while (k < ncell && crange[wl][0][k] >= crange[wl][1][k]) ++k;
-  if (k < ncell) p = crange[wl][0][k];
+  const int* pref = rpref[wl];
+  const int* st = rst[wl];
   int found = 0;
-  // chunk loads: candidate j and its cell-ordered coordinates (clamped)
-  int jn = -1;
-  double xn = 0.0, yn = 0.0, zn = 0.0;
-  int pend = k < ncell ? crange[wl][1][k] : 0;
-  if (k < ncell && p + lane < pend) {
-    const int q = p + lane;
-    jn = a.cell_atoms[q];
-    xn = a.px[q];
-    yn = a.py[q];
-    zn = a.pz[q];
+  // chunk loads: candidate sorted position, its atom and coordinates (clamped)
+  int qn = 0, jn = -1;
+  float4 fn = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (lane < total) {
+    qn = range_pos(pref, st, lane);
+    jn = a.cell_atoms[qn];
+    fn = a.pf[qn];
   }
-  while (k < ncell) {
-    const int pc = p, pe = pend;
-    const int j = jn;
-    const double xj = xn, yj = yn, zj = zn;
-    // next chunk: the rest of this cell, else the next non-empty cell
-    p += 64;
-    if (p >= pend) {
-      ++k;
-      while (k < ncell && crange[wl][0][k] >= crange[wl][1][k]) ++k;
-      if (k < ncell) {
-        p = crange[wl][0][k];
-        pend = crange[wl][1][k];
-      }
+  for (int v0 = 0; v0 < total; v0 += 64) {
+    const int q = qn, j = jn;
+    const float4 f = fn;
+    const bool live = v0 + lane < total;
+    if (v0 + 64 + lane < total) {
+      qn = range_pos(pref, st, v0 + 64 + lane);
+      jn = a.cell_atoms[qn];
+      fn = a.pf[qn];
     }
-    jn = -1;
-    if (k < ncell && p + lane < pend) {
-      const int q = p + lane;
-      jn = a.cell_atoms[q];
-      xn = a.px[q];
-      yn = a.py[q];
-      zn = a.pz[q];
-    }
-    // test the current chunk
     bool ok = false;
     uint8_t flag = 0;
-    const int pp = pc + lane;
-    if (pp < pe && j != i) {
-      const double r2 = min_image_r2(a, si - xj, ti - yj, ui - zj);
-      ok = (r2 <= a.cut2_max);
-      flag = (uint8_t)((r2 <= a.cut2_v ? 1 : 0) | (r2 <= a.cut2_c ? 2 : 0));
+    if (live && j != i) {
+      float dx = sif - f.x, dy = tif - f.y, dz = uif - f.z;
+      dx -= rintf(dx);
+      dy -= rintf(dy);
+      dz -= rintf(dz);
+      const float rx = fmaf(dx, a.a11f, fmaf(dy, a.a12f, dz * a.a13f));
+      const float ry = fmaf(dx, a.a21f, fmaf(dy, a.a22f, dz * a.a23f));
+      const float rz = fmaf(dx, a.a31f, fmaf(dy, a.a32f, dz * a.a33f));
+      const float r2f = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+      const bool near = fabsf(r2f - a.c2f_max) <= a.marg || fabsf(r2f - a.c2f_v) <= a.marg ||
+                        fabsf(r2f - a.c2f_c) <= a.marg;
+      if (!a.prefilter || near) {
+        const double r2 = min_image_r2(a, si - a.px[q], ti - a.py[q], ui - a.pz[q]);
+        ok = (r2 <= a.cut2_max);
+        flag = (uint8_t)((r2 <= a.cut2_v ? 1 : 0) | (r2 <= a.cut2_c ? 2 : 0));
+      } else {
+        ok = (r2f <= a.c2f_max);
+        flag = (uint8_t)((r2f <= a.c2f_v ? 1 : 0) | (r2f <= a.c2f_c ? 2 : 0));
+      }
     }
     const uint64_t m = __ballot(ok);
     if (FILL && ok) {
       const int off = found + __popcll(m & ((1ull << lane) - 1ull));
       const int64_t e = base + off;
       if (e < a.cap) {
-        a.nl_j[e] = pp;                  // sorted position of j
+        a.nl_j[e] = q;                   // sorted position of j
         a.nl_flag[e] = flag;
       }
     }
@@ -249,6 +301,35 @@ static NLArgs make_args(mdff_ctx* c) {
   a.cut2_c = rcl * rcl;
   a.cut2_max = a.cut2_v > a.cut2_c ? a.cut2_v : a.cut2_c;
   a.sx = c->sx; a.sy = c->sy; a.sz = c->sz;
+  a.pf = c->bf;
+  a.ox0 = c->nb_lo[0]; a.ox1 = c->nb_hi[0];
+  a.oy0 = c->nb_lo[1]; a.oy1 = c->nb_hi[1];
+  a.oz0 = c->nb_lo[2]; a.oz1 = c->nb_hi[2];
+  a.zall = c->nb_all[2];
+  // Single-precision prefilter.  The wrapped coordinates carry an error of
+  // at most 2^-24, the difference, the minimum image and each Cartesian
+  // component add a few units of 2^-24 relative to |A| (row sums of |A|),
+  // so |r^2(single) - r^2| <= (2 sqrt(3) rmax max_row |A| + r^2) 2^-21
+  // near the cutoffs; marg is 16 times that.  It is valid when every
+  // perpendicular width exceeds 2 rmax (with room): a pair within range is
+  // then far from a half-cell fractional offset, and the two nearest
+  // integers cannot differ between the single and the double difference.
+  {
+    double amax = 0.0;
+    for (int r = 0; r < 3; ++r)
+      amax = std::max(amax, std::fabs(A[r]) + std::fabs(A[r + 3]) + std::fabs(A[r + 6]));
+    const double rmax = std::sqrt(a.cut2_max);
+    a.marg = (float)(16.0 * (2.0 * std::sqrt(3.0) * rmax * amax + a.cut2_max) * std::ldexp(1.0, -21));
+    a.prefilter = 1;
+    for (int d = 0; d < 3; ++d)
+      if (!(c->W[d] > 2.0 * rmax * (1.0 + 1e-3))) a.prefilter = 0;
+    a.a11f = (float)A[0]; a.a21f = (float)A[1]; a.a31f = (float)A[2];
+    a.a12f = (float)A[3]; a.a22f = (float)A[4]; a.a32f = (float)A[5];
+    a.a13f = (float)A[6]; a.a23f = (float)A[7]; a.a33f = (float)A[8];
+    a.c2f_max = (float)a.cut2_max;
+    a.c2f_v = (float)a.cut2_v;
+    a.c2f_c = (float)a.cut2_c;
+  }
   a.cell_atoms = c->cell_atoms;
   a.cell_start = c->cell_start;
   a.cell_of = c->cell_of;
@@ -265,12 +346,24 @@ static NLArgs make_args(mdff_ctx* c) {
 // list build from the fractional coordinates (px, py, pz) of the build
 static int nl_build_from(mdff_ctx* c, const double* px, const double* py, const double* pz) {
   const int n = c->n;
-  // cells: width >= cutoff in every reciprocal direction (1e-9 margin against
-  // rounding of the wrapped coordinates)
+  // cells: width >= cutoff / 2 in a reciprocal direction with room for at
+  // least 5 of them (partners within +-2 cells), else width >= cutoff
+  // (partners within +-1 cell); 1e-9 margin against rounding of the wrapped
+  // coordinates.  With fewer cells than the reach covers, every cell of that
+  // direction is visited once.
   double rmax = std::max(c->rc_vdw, c->rc_coul) + c->d.skindiff;
   for (int d = 0; d < 3; ++d) {
-    int k = (int)floor(c->W[d] / (rmax * (1.0 + 1e-9)));
+    const int k2 = (int)floor(c->W[d] / (0.5 * rmax * (1.0 + 1e-9)));
+    int reach = 2, k = k2;
+    if (k2 < 5) {
+      reach = 1;
+      k = (int)floor(c->W[d] / (rmax * (1.0 + 1e-9)));
+    }
     c->ncell[d] = k < 1 ? 1 : k;
+    const bool all = c->ncell[d] < 2 * reach + 1;
+    c->nb_lo[d] = all ? 0 : -reach;
+    c->nb_hi[d] = all ? c->ncell[d] - 1 : reach;
+    c->nb_all[d] = all ? 1 : 0;
   }
   int ncell = c->ncell[0] * c->ncell[1] * c->ncell[2];
   if (ncell + 1 > c->ncell_alloc) {
@@ -296,7 +389,7 @@ static int nl_build_from(mdff_ctx* c, const double* px, const double* py, const 
   // fractional coordinates in cell order (coalesced candidate loads); they
   // are kept as the positions of this build
   hipLaunchKernelGGL(k_gather_sorted, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, c->cell_atoms,
-                     px, py, pz, c->bx, c->by, c->bz);
+                     px, py, pz, c->bx, c->by, c->bz, c->bf);
   NLArgs a = make_args(c);
   a.sx = px; a.sy = py; a.sz = pz;
   a.px = c->bx; a.py = c->by; a.pz = c->bz;
