@@ -854,8 +854,11 @@ struct Sh3Args {
   float* part;                   // [chunk][2*nk_box]
 };
 
-template <int TA>
+template <int TA, bool CQ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_sk_h3(Sh3Args a) {
+  // CQ: P = c X Y with one coefficient per atom (modes 0 and 2): c X is
+  // formed once per tile when X is staged; otherwise (mode 1) P =
+  // (c0 - i (nx c1 + ny c2)) X Y is formed per column
   constexpr int RT = 4;                                   // row tiles of 32 rows
   constexpr int RW = TA + 4;                              // 32-bit words per staged Z row
   constexpr int TAP = TA + 4;                             // atoms per staged X / Y column
@@ -863,8 +866,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   extern __shared__ unsigned char smem_raw[];
   uint32_t* zs = reinterpret_cast<uint32_t*>(smem_raw);                 // [plane hi/lo][128 rows][RW]
   const int nxy = a.Kx + a.Ky + 2;                                       // X and Y columns
-  cplx<float>* st = reinterpret_cast<cplx<float>*>(zs + 2 * 128 * RW);  // [nxy][TAP]
-  float* sq = reinterpret_cast<float*>(st + (size_t)nxy * TAP);         // [3][TAP]
+  // staged columns: X(nx) (times c when CQ), nx = 0..Kx, then Y(ny),
+  // ny = -Ky..Ky (Y(-n) = conj Y(n))
+  const int nst = a.Kx + 1 + 2 * a.Ky + 1;
+  cplx<float>* st = reinterpret_cast<cplx<float>*>(zs + 2 * 128 * RW);  // [nst][TAP]
+  float* sq = reinterpret_cast<float*>(st + (size_t)nst * TAP);         // [3][TAP]
   const cplx<float>* tab = reinterpret_cast<const cplx<float>*>(a.tab);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int hl = lane >> 5, col = lane & 31;
@@ -875,8 +881,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   const bool bok = m < a.M;
   const int nx = bok ? m / a.NY : 0, ny = bok ? m % a.NY - a.Ky : 0;
   const float4* xp = reinterpret_cast<const float4*>(st + (size_t)nx * TAP + 4 * hl);
-  const float4* yp = reinterpret_cast<const float4*>(st + (size_t)(a.Kx + 1 + (ny >= 0 ? ny : -ny)) * TAP + 4 * hl);
-  const float ysg = ny < 0 ? -1.f : 1.f, fnx = (float)nx, fny = (float)ny, bw = bok ? 1.f : 0.f;
+  const float4* yp = reinterpret_cast<const float4*>(st + (size_t)(a.Kx + 1 + a.Ky + ny) * TAP + 4 * hl);
+  // columns m >= M (bok false) read valid columns; their D columns are not
+  // stored, and every D column depends on its own B column only
+  const float fnx = (float)nx, fny = (float)ny;
   // per-tile sums (MFMA accumulators, restarted every tile) and the chunk
   // totals they are added to: the FP32 rounding of each accumulation is
   // relative to a tile's sum, not to the growing chunk sum
@@ -906,8 +914,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     }                                                                                         \
     _Pragma("unroll") for (int u = 0; u < XU; ++u)                                            \
       xn[u] = tab[(size_t)min(scg + CG * u, nxy - 1) * a.n + src_];                           \
-    if (threadIdx.x < TA) {                                                                   \
-      const int i_ = (T0) + min((int)threadIdx.x, nt_ - 1);                                   \
+    if (CQ || threadIdx.x < TA) {                                                             \
+      const int i_ = (int)src_;                                                               \
       if (a.mode <= 1) qd = a.q[i_];                                                          \
       if (a.mode >= 1) {                                                                      \
         m0 = a.mu[3 * i_];                                                                    \
@@ -939,10 +947,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       r0[128 * RW] = pack_h(rl, -il);
       r0[129 * RW] = pack_h(il, rl);
     }
+    float cs = 1.f;                                       // c of atom sia (CQ)
+    if (CQ) cs = a.mode == 0 ? (float)(qd * a.pscale) : (float)(a.pscale * (m0 * a.b[6] + m1 * a.b[7] + m2 * a.b[8]));
 #pragma unroll
-    for (int u = 0; u < XU; ++u)
-      if (scg + CG * u < nxy) st[(size_t)(scg + CG * u) * TAP + sia] = sok ? xn[u] : cplx<float>{0.f, 0.f};
-    if (threadIdx.x < TA) {
+    for (int u = 0; u < XU; ++u) {
+      const int cidx = scg + CG * u;
+      if (cidx < nxy) {
+        cplx<float> v = sok ? xn[u] : cplx<float>{0.f, 0.f};
+        if (cidx <= a.Kx) {
+          if (CQ) v = cplx<float>{cs * v.re, cs * v.im};
+          st[(size_t)cidx * TAP + sia] = v;
+        } else {
+          const int nyc = cidx - a.Kx - 1;
+          st[(size_t)(a.Kx + 1 + a.Ky + nyc) * TAP + sia] = v;
+          if (nyc > 0) st[(size_t)(a.Kx + 1 + a.Ky - nyc) * TAP + sia] = cplx<float>{v.re, -v.im};
+        }
+      }
+    }
+    if (!CQ && threadIdx.x < TA) {
       float c0 = 0.f, c1 = 0.f, c2 = 0.f;
       if ((int)threadIdx.x < nt) {
         if (a.mode <= 1) c0 = (float)(qd * a.pscale);
@@ -969,20 +991,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       // B: P of atoms 8s + 4hl + u (u < 4) at this lane's m, split
       const float4 x01 = xp[4 * s], x23 = xp[4 * s + 1];
       const float4 y01 = yp[4 * s], y23 = yp[4 * s + 1];
-      const float4 c0v = qp[2 * s], c1v = qp[TAP / 4 + 2 * s], c2v = qp[TAP / 2 + 2 * s];
       const float xr[4] = {x01.x, x01.z, x23.x, x23.z}, xi[4] = {x01.y, x01.w, x23.y, x23.w};
-      const float yr[4] = {y01.x, y01.z, y23.x, y23.z};
-      const float yi[4] = {ysg * y01.y, ysg * y01.w, ysg * y23.y, ysg * y23.w};
-      const float cq[4] = {c0v.x, c0v.y, c0v.z, c0v.w};
-      const float cx[4] = {c1v.x, c1v.y, c1v.z, c1v.w}, cy[4] = {c2v.x, c2v.y, c2v.z, c2v.w};
+      const float yr[4] = {y01.x, y01.z, y23.x, y23.z}, yi[4] = {y01.y, y01.w, y23.y, y23.w};
+      float cq[4] = {0.f, 0.f, 0.f, 0.f}, cx[4] = {0.f, 0.f, 0.f, 0.f}, cy[4] = {0.f, 0.f, 0.f, 0.f};
+      if (!CQ) {
+        const float4 c0v = qp[2 * s], c1v = qp[TAP / 4 + 2 * s], c2v = qp[TAP / 2 + 2 * s];
+        cq[0] = c0v.x; cq[1] = c0v.y; cq[2] = c0v.z; cq[3] = c0v.w;
+        cx[0] = c1v.x; cx[1] = c1v.y; cx[2] = c1v.z; cx[3] = c1v.w;
+        cy[0] = c2v.x; cy[1] = c2v.y; cy[2] = c2v.z; cy[3] = c2v.w;
+      }
       half8 bh, bl;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const float xyr = xr[u] * yr[u] - xi[u] * yi[u];
         const float xyi = xr[u] * yi[u] + xi[u] * yr[u];
-        const float bb = fnx * cx[u] + fny * cy[u];
-        const float pre = bw * (cq[u] * xyr - bb * xyi);
-        const float pim = bw * (cq[u] * xyi + bb * xyr);
+        float pre = xyr, pim = xyi;                         // CQ: c is in X
+        if (!CQ) {
+          const float bb = fnx * cx[u] + fny * cy[u];
+          pre = cq[u] * xyr - bb * xyi;
+          pim = cq[u] * xyi + bb * xyr;
+        }
         _Float16 h, l;
         split_h(pre, h, l);
         bh[2 * u] = h;
@@ -1337,8 +1365,9 @@ static int h3_bucket(int nsh) {
     if (nsh <= b) return b;
   return 0;
 }
-static size_t sk_h3_lds(int nxy, int ta) {
-  return (size_t)2 * 128 * (ta + 4) * 4 + (size_t)nxy * (ta + 4) * 8 + (size_t)3 * (ta + 4) * 4;
+static size_t sk_h3_lds(int kx, int ky, int ta) {
+  const int nst = kx + 1 + 2 * ky + 1;
+  return (size_t)2 * 128 * (ta + 4) * 4 + (size_t)nst * (ta + 4) * 8 + (size_t)3 * (ta + 4) * 4;
 }
 static const void* gather_h3_kernel(int ct, int nsh) {
   switch (ct * 100 + nsh) {
@@ -1458,7 +1487,8 @@ int recip_setup(mdff_ctx* c) {
     const bool off = ev && ev[0] == '1';
     const int nxy = Kx + Ky + 2;
     const int nsh = h3_bucket(div_up(2 * c->NZk, 16));
-    if (fp32 && c->use_mma && !off && nsh > 0 && nxy <= 8 * (256 / kH3TA) && sk_h3_lds(nxy, kH3TA) <= 65536) {
+    if (fp32 && c->use_mma && !off && nsh > 0 && nxy <= 8 * (256 / kH3TA) &&
+        sk_h3_lds(c->K[0], c->K[1], kH3TA) <= 65536) {
       c->h3 = 1;
       c->h3_nsh = nsh;
       c->h3_ct = nsh <= 7 ? 2 : 1;
@@ -1473,8 +1503,9 @@ int recip_setup(mdff_ctx* c) {
       const int bs = gather_splits(c->h3_mt8, aw, gconc);
       c->h3_splits = bs;
       // structure-factor chunks from the kernel's occupancy
-      const size_t lds = sk_h3_lds(nxy, kH3TA);
-      const int64_t cap = (int64_t)cus * blocks_per_cu(reinterpret_cast<const void*>(&k_sk_h3<kH3TA>), lds);
+      const size_t lds = sk_h3_lds(c->K[0], c->K[1], kH3TA);
+      const int64_t cap =
+          (int64_t)cus * blocks_per_cu(reinterpret_cast<const void*>(&k_sk_h3<kH3TA, true>), lds);
       const int64_t mblocks = div_up(div_up(M, 32), 4) * (int64_t)div_up(c->NZk, 64);
       int64_t best = -1;
       for (int ch = 1; ch <= 128 && (ch == 1 || div_up(c->n, ch) >= kH3TA); ++ch) {
@@ -1613,16 +1644,16 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     sh.part = p1;
     sh.mode = use_mu ? 1 : 0;
     sh.pscale = use_mu ? 1.f : c->h3_pscale;
-    const int nxy = c->K[0] + c->K[1] + 2;
-    const size_t lds = sk_h3_lds(nxy, kH3TA);
+    const size_t lds = sk_h3_lds(c->K[0], c->K[1], kH3TA);
     dim3 gs(div_up(div_up(s.M, 32), 4), ch, div_up(c->NZk, 64));
-    PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA>), gs, dim3(256), lds, c->stream, sh));
+    if (use_mu) PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, false>), gs, dim3(256), lds, c->stream, sh));
+    else PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, true>), gs, dim3(256), lds, c->stream, sh));
     if (use_mu) {
       Sh3Args s2 = sh;
       s2.mode = 2;
       s2.part = p1 + (size_t)2 * ch * c->nk_box;
       const size_t tot = (size_t)ch * c->nk_box;
-      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA>), gs, dim3(256), lds, c->stream, s2);
+      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, true>), gs, dim3(256), lds, c->stream, s2);
                   hipLaunchKernelGGL(k_sk_combine<float>, dim3(div_up(tot, 256)), dim3(256), 0, c->stream,
                                      c->nk_box, c->NZk, c->K[2], ch, p1, s2.part));
     }
