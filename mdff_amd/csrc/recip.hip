@@ -32,29 +32,67 @@ __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
 }
 
 // Phase tables, layout [col][atom], col = 0..Kx (X), Kx+1..Kx+Ky+1 (Y), then Z.
+// A thread writes a run of up to kTabRun consecutive columns of one
+// direction: exp(2 pi i k0 s) by sincospi of the reduced angle k0 s - rint(k0 s),
+// then E(k + 1) = E(k) E(1) in FP64 (at most kTabRun - 1 products, each
+// within a few units of 2^-53).
+constexpr int kTabRun = 8;
+
+// runs of the three directions: blockIdx.y -> (first column, k0, count, which s)
+struct TabRun {
+  int col, k0, cnt, dir;
+};
+__device__ __forceinline__ TabRun tab_run(int y, int Kx, int Ky, int Kz) {
+  const int rx = (Kx + kTabRun) / kTabRun, ry = (Ky + kTabRun) / kTabRun;
+  TabRun r;
+  int K, base;
+  if (y < rx) { r.dir = 0; K = Kx; base = 0; }
+  else if (y < rx + ry) { r.dir = 1; K = Ky; base = Kx + 1; y -= rx; }
+  else { r.dir = 2; K = Kz; base = Kx + Ky + 2; y -= rx + ry; }
+  r.k0 = y * kTabRun;
+  r.cnt = min(kTabRun, K + 1 - r.k0);
+  r.col = base + r.k0;
+  return r;
+}
+static int tab_runs(const int* K) {
+  return (K[0] + kTabRun) / kTabRun + (K[1] + kTabRun) / kTabRun + (K[2] + kTabRun) / kTabRun;
+}
+
+template <typename T>
+__device__ __forceinline__ void table_run(double s, const TabRun& r, int n, int i, cplx<T>* __restrict__ tab) {
+  double t = (double)r.k0 * s;
+  t -= rint(t);
+  double sn, cs;
+  sincospi(2.0 * t, &sn, &cs);
+  double t1 = s - rint(s);
+  double s1, c1;
+  sincospi(2.0 * t1, &s1, &c1);
+  cplx<T>* dst = tab + (size_t)r.col * n + i;
+  dst[0] = {(T)cs, (T)sn};
+  for (int u = 1; u < r.cnt; ++u) {
+    const double c2 = cs * c1 - sn * s1, s2 = cs * s1 + sn * c1;
+    cs = c2;
+    sn = s2;
+    dst[(size_t)u * n] = {(T)cs, (T)sn};
+  }
+}
+
 template <typename T>
 __global__ void k_tables(int n, int Kx, int Ky, int Kz, const double* __restrict__ sx,
                          const double* __restrict__ sy, const double* __restrict__ sz,
                          cplx<T>* __restrict__ tab) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int col = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double s;
-  int k;
-  if (col <= Kx) { s = sx[i]; k = col; }
-  else if (col <= Kx + Ky + 1) { s = sy[i]; k = col - Kx - 1; }
-  else { s = sz[i]; k = col - Kx - Ky - 2; }
-  double t = (double)k * s;
-  t -= rint(t);
-  double sn, cs;
-  sincospi(2.0 * t, &sn, &cs);
-  tab[(size_t)col * n + i] = {(T)cs, (T)sn};
+  const TabRun r = tab_run(blockIdx.y, Kx, Ky, Kz);
+  const double s = r.dir == 0 ? sx[i] : (r.dir == 1 ? sy[i] : sz[i]);
+  table_run<T>(s, r, n, i, tab);
 }
 
 // The same tables straight from the Cartesian positions: s = B^T r formed
 // as kardir does (src/lattice.f90:183-216, products and sums rounded
 // separately, no fused multiply-add), so s is bit-identical to the list
-// code's; the blocks of column 0 also store s (sx, sy, sz).
+// code's; the blocks of the first run (X columns from 0) also store s (sx,
+// sy, sz).
 struct TabRArgs {
   int n, Kx, Ky, Kz;
   const double *rx, *ry, *rz;
@@ -74,14 +112,14 @@ struct TabRArgs {
 template <typename T>
 __global__ void k_tables_r(TabRArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int col = blockIdx.y;
   if (i >= a.n) return;
   auto dot = [](double x, double y, double z, double u, double v, double w) {
     return __dadd_rn(__dadd_rn(__dmul_rn(x, u), __dmul_rn(y, v)), __dmul_rn(z, w));
   };
   const double* b = a.b;
   const double x = a.rx[i], y = a.ry[i], z = a.rz[i];
-  if (col == 0) {
+  const TabRun r = tab_run(blockIdx.y, a.Kx, a.Ky, a.Kz);
+  if (blockIdx.y == 0) {
     a.sx[i] = dot(x, y, z, b[0], b[1], b[2]);
     a.sy[i] = dot(x, y, z, b[3], b[4], b[5]);
     a.sz[i] = dot(x, y, z, b[6], b[7], b[8]);
@@ -93,16 +131,9 @@ __global__ void k_tables_r(TabRArgs a) {
       a.meta[i] = make_int2(a.itype[j], j);
     }
   }
-  double s;
-  int k;
-  if (col <= a.Kx) { s = dot(x, y, z, b[0], b[1], b[2]); k = col; }
-  else if (col <= a.Kx + a.Ky + 1) { s = dot(x, y, z, b[3], b[4], b[5]); k = col - a.Kx - 1; }
-  else { s = dot(x, y, z, b[6], b[7], b[8]); k = col - a.Kx - a.Ky - 2; }
-  double t = (double)k * s;
-  t -= rint(t);
-  double sn, cs;
-  sincospi(2.0 * t, &sn, &cs);
-  reinterpret_cast<cplx<T>*>(a.tab)[(size_t)col * a.n + i] = {(T)cs, (T)sn};
+  const double s = r.dir == 0 ? dot(x, y, z, b[0], b[1], b[2])
+                              : (r.dir == 1 ? dot(x, y, z, b[3], b[4], b[5]) : dot(x, y, z, b[6], b[7], b[8]));
+  table_run<T>(s, r, a.n, i, reinterpret_cast<cplx<T>*>(a.tab));
 }
 
 struct SkArgs {
@@ -1048,67 +1079,131 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 }
 #undef SKH3_FETCH
 
-// ---- gather A operand: W per m-tile of 8 m values, scaled and split ----
+// ---- chunk sums of S and the gather A operand, per m-tile of 8 m values ----
+// One block per m-tile (k = m NZ + t for its 8 values of m, every nz): S(k)
+// as k_sk_reduce (chunks in order), W = A conj(S), the energy / stress block
+// partials, and the split A operand of the gather:
 // amat[((mt * NSH + s) * 64 + lane) * 2 + {0: hi, 1: lo}] = 8 halves: row
 // l%32 = (m_local = row / 4, comp = row % 4), K = 16 s + 8 (l/32) + j =
 // (nz index t, Re/Im cc): comp 0 (Re U0): (wr, -wi), comp 1 (Im U0):
 // (wi, wr), comp 2/3 the same times nz.  Values times 2^(14 - e), e the
-// exponent of the tile's largest value; ascale[mt] = 2^(e - 14).  One block
-// per m-tile.
+// exponent of the tile's largest value; ascale[mt] = 2^(e - 14).  The tile is
+// assembled in LDS (zeros where t >= NZ or m >= M) and stored coalesced.
 template <int NSH>
-__global__ void __launch_bounds__(256) k_amat_h3(int M, int NZ, int Kz, int mtiles, const double* __restrict__ wk,
-                                                 u32x4* __restrict__ amat, float* __restrict__ ascale) {
-  // one block per m-tile; wave w takes the K steps w, w + 4, ...
-  constexpr int SW = (NSH + 3) / 4;                        // steps per wave
+__global__ void __launch_bounds__(256) k_sk_reduce_h3(int nk_box, int nchunk, const float* __restrict__ part,
+                                                      const double* __restrict__ kA,
+                                                      const double* __restrict__ kvec, double* __restrict__ sk,
+                                                      double* __restrict__ wk, float* __restrict__ wk_t,
+                                                      double* __restrict__ blk, int NZ, int Kz,
+                                                      u32x4* __restrict__ amat, float* __restrict__ ascale) {
+  constexpr int NU = (NSH * 16 * 4 + 255) / 256;           // k entries per thread (8 NZ <= 64 NSH)
+  constexpr int NT = NSH * 128 * 4;                        // 32-bit words of the tile
+  __shared__ double sh[4 * kRedMaxCols];
+  __shared__ uint32_t tile[NT];
   __shared__ float wmax[4];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int mt = blockIdx.x;
-  const int hl = lane >> 5, row = lane & 31;
-  const int m = 8 * mt + row / 4, comp = row % 4;
-  float v[SW][8];
+  const int k0 = 8 * mt * NZ;
+  const int kn = min(nk_box, k0 + 8 * NZ) - k0;
+  for (int x = threadIdx.x; x < NT; x += 256) tile[x] = 0u;
+  double v[kRedMaxCols];
+#pragma unroll
+  for (int c = 0; c < kRedMaxCols; ++c) v[c] = 0.0;
+  // chunk sums in chunk order for the NU entries of the thread; the loads of
+  // 16 chunks of every entry are issued before their adds (clamped indices)
+  constexpr int CQ = 16;
+  double sre[NU], sim[NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) { sre[u] = 0.0; sim[u] = 0.0; }
+  for (int c = 0; c < nchunk; c += CQ) {
+    float2 pv[NU][CQ];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int k = k0 + min((int)threadIdx.x + 256 * u, kn - 1);
+#pragma unroll
+      for (int q = 0; q < CQ; ++q)
+        pv[u][q] = *reinterpret_cast<const float2*>(part + (size_t)min(c + q, nchunk - 1) * 2 * nk_box + 2 * k);
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u)
+#pragma unroll
+      for (int q = 0; q < CQ; ++q)
+        if (c + q < nchunk) {
+          sre[u] += (double)pv[u][q].x;
+          sim[u] += (double)pv[u][q].y;
+        }
+  }
+  float val[NU][8];
   float vmax = 0.f;
 #pragma unroll
-  for (int u = 0; u < SW; ++u) {
-    const int s = w + 4 * u;
+  for (int u = 0; u < NU; ++u) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int kk = 16 * s + 8 * hl + j;
-      const int t = kk >> 1, cc = kk & 1;
-      double x = 0.0;
-      if (s < NSH && m < M && t < NZ) {
-        const double wr = wk[2 * ((size_t)m * NZ + t)], wi = wk[2 * ((size_t)m * NZ + t) + 1];
-        x = ((comp & 1) == 0) ? (cc == 0 ? wr : -wi) : (cc == 0 ? wi : wr);
-        if (comp >= 2) x *= (double)(t - Kz);
+    for (int j = 0; j < 8; ++j) val[u][j] = 0.f;
+    const int kl = threadIdx.x + 256 * u;
+    if (kl >= kn) continue;
+    const int k = k0 + kl;
+    const double re = sre[u], im = sim[u];
+    sk[2 * k] = re;
+    sk[2 * k + 1] = im;
+    const double Ak = kA[k];
+    const double wr = Ak * re, wi = -Ak * im;
+    wk[2 * k] = wr;
+    wk[2 * k + 1] = wi;
+    wk_t[2 * k] = (float)wr;
+    wk_t[2 * k + 1] = (float)wi;
+    const double str = (re * re + im * im) * Ak;
+    const double kx = kvec[4 * k], ky = kvec[4 * k + 1], kz = kvec[4 * k + 2], kc = kvec[4 * k + 3];
+    v[0] += str;
+    v[1] += (1.0 - kc * kx * kx) * str;
+    v[2] += (1.0 - kc * ky * ky) * str;
+    v[3] += (1.0 - kc * kz * kz) * str;
+    v[4] += -kc * kx * ky * str;
+    v[5] += -kc * kx * kz * str;
+    v[6] += -kc * ky * kz * str;
+    const double fz = (double)(kl % NZ - Kz);
+#pragma unroll
+    for (int comp = 0; comp < 4; ++comp)
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc) {
+        double x = ((comp & 1) == 0) ? (cc == 0 ? wr : -wi) : (cc == 0 ? wi : wr);
+        if (comp >= 2) x *= fz;
+        val[u][2 * comp + cc] = (float)x;
+        vmax = fmaxf(vmax, fabsf((float)x));
       }
-      v[u][j] = (float)x;
-      vmax = fmaxf(vmax, fabsf(v[u][j]));
-    }
   }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
   if (lane == 0) wmax[w] = vmax;
-  __syncthreads();
+  __syncthreads();                                       // also orders the tile zeroing
   vmax = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
   int e = 0;
   if (vmax > 0.f) (void)frexpf(vmax, &e);                  // vmax in [2^(e-1), 2^e)
   const float sc = ldexpf(1.f, 14 - e);
   if (threadIdx.x == 0) ascale[mt] = ldexpf(1.f, e - 14);
+  uint16_t* th = reinterpret_cast<uint16_t*>(tile);
 #pragma unroll
-  for (int u = 0; u < SW; ++u) {
-    const int s = w + 4 * u;
-    if (s >= NSH) break;
-    half8 h, l;
+  for (int u = 0; u < NU; ++u) {
+    const int kl = threadIdx.x + 256 * u;
+    if (kl >= kn) continue;
+    const int ml = kl / NZ, t = kl - ml * NZ;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      _Float16 a, b;
-      split_h(v[u][j] * sc, a, b);
-      h[j] = a;
-      l[j] = b;
-    }
-    u32x4* dst = amat + ((size_t)(mt * NSH + s) * 64 + lane) * 2;
-    dst[0] = __builtin_bit_cast(u32x4, h);
-    dst[1] = __builtin_bit_cast(u32x4, l);
+    for (int comp = 0; comp < 4; ++comp)
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc) {
+        _Float16 h, l;
+        split_h(val[u][2 * comp + cc] * sc, h, l);
+        const int K = 2 * t + cc, row = 4 * ml + comp;
+        const int ln = 32 * ((K >> 3) & 1) + row;
+        const int at = (((K >> 4) * 64 + ln) * 2) * 8 + (K & 7);
+        th[at] = __builtin_bit_cast(uint16_t, h);
+        th[at + 8] = __builtin_bit_cast(uint16_t, l);
+      }
   }
+  __syncthreads();
+  const u32x4* t4 = reinterpret_cast<const u32x4*>(tile);
+  for (int x = threadIdx.x; x < NSH * 128; x += 256) amat[(size_t)mt * NSH * 128 + x] = t4[x];
+  const double r = block_sum_cols(v, 7, sh);
+  if (threadIdx.x < 7) blk[blockIdx.x * 7 + threadIdx.x] = r;
 }
 
 // ---- gather: U(m, i) per m-tile of 8 m values and atom tiles of 32 ----
@@ -1378,12 +1473,12 @@ static const void* gather_h3_kernel(int ct, int nsh) {
     default: return nullptr;
   }
 }
-static const void* amat_h3_kernel(int nsh) {
+static const void* sk_reduce_h3_kernel(int nsh) {
   switch (nsh) {
-    case 4: return reinterpret_cast<const void*>(&k_amat_h3<4>);
-    case 7: return reinterpret_cast<const void*>(&k_amat_h3<7>);
-    case 10: return reinterpret_cast<const void*>(&k_amat_h3<10>);
-    case 15: return reinterpret_cast<const void*>(&k_amat_h3<15>);
+    case 4: return reinterpret_cast<const void*>(&k_sk_reduce_h3<4>);
+    case 7: return reinterpret_cast<const void*>(&k_sk_reduce_h3<7>);
+    case 10: return reinterpret_cast<const void*>(&k_sk_reduce_h3<10>);
+    case 15: return reinterpret_cast<const void*>(&k_sk_reduce_h3<15>);
     default: return nullptr;
   }
 }
@@ -1478,7 +1573,9 @@ int recip_setup(mdff_ctx* c) {
   }
   HIP_TRY(c, hipMalloc(&c->amat, tsz / 2 * (size_t)c->mma_mtiles * c->mma_nks * 64));
   HIP_TRY(c, hipMemset(c->amat, 0, tsz / 2 * (size_t)c->mma_mtiles * c->mma_nks * 64));
-  HIP_TRY(c, hipMalloc((void**)&c->rec_blk, sizeof(double) * 7 * (size_t)div_up(nb, 256)));
+  // block partials of the energy / stress: k-blocks of 256, or m-tiles of 8
+  const size_t nblk_rec = std::max<size_t>(div_up(nb, 256), div_up((int64_t)c->NXk * c->NYk, 8));
+  HIP_TRY(c, hipMalloc((void**)&c->rec_blk, sizeof(double) * 7 * nblk_rec));
   // mixed form: split-half kernels when their shapes fit (MDFF_RECIP_F32MMA=1
   // keeps the FP32 matrix-core kernels)
   c->h3 = 0;
@@ -1535,16 +1632,14 @@ void recip_free(mdff_ctx* c) {
 
 template <typename T>
 static void tables_t(mdff_ctx* c) {
-  const int ncol = c->K[0] + c->K[1] + c->K[2] + 3;
   PROF_LAUNCH(c, PK_TABLES,
-              hipLaunchKernelGGL((k_tables<T>), dim3(div_up(c->n, 256), ncol), dim3(256), 0, c->stream, c->n,
+              hipLaunchKernelGGL((k_tables<T>), dim3(div_up(c->n, 256), tab_runs(c->K)), dim3(256), 0, c->stream, c->n,
                                  c->K[0], c->K[1], c->K[2], c->sx, c->sy, c->sz,
                                  reinterpret_cast<cplx<T>*>(c->tab)));
 }
 
 template <typename T>
 static void tables_r_t(mdff_ctx* c, bool pack) {
-  const int ncol = c->K[0] + c->K[1] + c->K[2] + 3;
   TabRArgs a{};
   a.n = c->n;
   a.Kx = c->K[0]; a.Ky = c->K[1]; a.Kz = c->K[2];
@@ -1559,7 +1654,8 @@ static void tables_r_t(mdff_ctx* c, bool pack) {
   a.rec = c->arec;
   a.meta = c->ameta;
   PROF_LAUNCH(c, PK_TABLES,
-              hipLaunchKernelGGL((k_tables_r<T>), dim3(div_up(c->n, 256), ncol), dim3(256), 0, c->stream, a));
+              hipLaunchKernelGGL((k_tables_r<T>), dim3(div_up(c->n, 256), tab_runs(c->K)), dim3(256), 0, c->stream,
+                                 a));
 }
 
 int recip_tables_from_r(mdff_ctx* c, bool pack) {
@@ -1706,11 +1802,21 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
   // final sum queued into e_out (device memory)
   const bool dipf = forces && use_mu;
   const bool mma_gather = (forces || field) && !dipf && c->mma_nks > 0 && c->use_mma && !h3;
+  // split-half gather: its A operand is written by the reduction (per m-tile)
+  const bool h3_gather = h3 && !dipf && (forces || field);
   T* amat = mma_gather ? reinterpret_cast<T*>(c->amat) : nullptr;
-  const int nb = div_up(c->nk_box, 256);
+  const int nb = h3_gather ? c->h3_mt8 : div_up(c->nk_box, 256);
   MDFF_TRY(claim_partials(c, c->rec_blk));
   PROF_LAUNCH(c, PK_SKRED,
-              if (sk_fp64) {
+              if (h3_gather) {
+                const float* prt = reinterpret_cast<const float*>(c->sk_part);
+                float* wkt = reinterpret_cast<float*>(c->wk_t);
+                u32x4* am = reinterpret_cast<u32x4*>(c->amat_h3);
+                void* rargs[] = {(void*)&c->nk_box, (void*)&chunks, (void*)&prt, (void*)&c->kA, (void*)&c->kvec,
+                                 (void*)&c->sk, (void*)&c->wk, (void*)&wkt, (void*)&c->rec_blk, (void*)&c->NZk,
+                                 (void*)&c->K[2], (void*)&am, (void*)&c->ascale_h3};
+                (void)hipLaunchKernel(sk_reduce_h3_kernel(c->h3_nsh), dim3(nb), dim3(256), rargs, 0, c->stream);
+              } else if (sk_fp64) {
                 hipLaunchKernelGGL((k_sk_reduce<T, double>), dim3(nb), dim3(256), 0, c->stream, c->nk_box, chunks,
                                    reinterpret_cast<const double*>(c->sk_part), c->kA, c->kvec, c->sk, c->wk,
                                    reinterpret_cast<T*>(c->wk_t), c->rec_blk, amat, c->NZk, c->K[2],
@@ -1724,12 +1830,9 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
   if (e_out) MDFF_TRY(queue_final(c, c->rec_blk, nb, 7, e_out));
   if (!forces && !field) return MDFF_OK;
   int nsplit = 0, moments = dipf ? 9 : 3;
-  if (h3 && !dipf) {
-    // split-half gather: A operand per m-tile of 8 m values, then the GEMM
+  if (h3_gather) {
+    // split-half gather (A operand per m-tile of 8 m values from the reduction)
     const int nsh = c->h3_nsh, mt8 = c->h3_mt8;
-    void* aargs[] = {(void*)&s.M, (void*)&c->NZk, (void*)&c->K[2], (void*)&mt8, (void*)&c->wk, (void*)&c->amat_h3,
-                     (void*)&c->ascale_h3};
-    PROF_LAUNCH(c, PK_AMAT, (void)hipLaunchKernel(amat_h3_kernel(nsh), dim3(mt8), dim3(256), aargs, 0, c->stream));
     Gh3Args gh{};
     gh.n = c->n;
     gh.Kx = c->K[0]; gh.Ky = c->K[1]; gh.Kz = c->K[2];
