@@ -125,8 +125,20 @@ struct RecFinal {
 __device__ __forceinline__ void rec_final_moments(const RecFinal& r, int i, double v[9], double g[3]) {
 #pragma unroll
   for (int c = 0; c < 9; ++c) v[c] = 0.0;
-  for (int s = 0; s < r.nsplit; ++s)
-    for (int c = 0; c < r.moments; ++c) v[c] += r.part[((size_t)s * r.moments + c) * r.n + i];
+  // splits in order, the loads of 4 splits issued before their adds
+  for (int s0 = 0; s0 < r.nsplit; s0 += 4) {
+    double t[4][9];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < 9; ++c)
+        t[u][c] = (s0 + u < r.nsplit && c < r.moments) ? r.part[((size_t)(s0 + u) * r.moments + c) * r.n + i] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < 9; ++c)
+        if (s0 + u < r.nsplit && c < r.moments) v[c] += t[u][c];
+  }
   g[0] = r.b[0] * v[0] + r.b[3] * v[1] + r.b[6] * v[2];
   g[1] = r.b[1] * v[0] + r.b[4] * v[1] + r.b[7] * v[2];
   g[2] = r.b[2] * v[0] + r.b[5] * v[1] + r.b[8] * v[2];

@@ -15,6 +15,8 @@
 // Precision: the minimum image and the cutoff tests are FP64 in the
 // reference's order; the pair functions run in T (double for the FP64 form,
 // float for the mixed form); every per-atom accumulator is FP64.
+#include <algorithm>
+
 #include "common.h"
 
 namespace {
@@ -24,6 +26,11 @@ constexpr int NPA = 15;  // per-atom columns: u_sr, vir_sr, tau_sr(6), u_cl, tau
 struct PairArgs {
   int n, ntype, sr_model, lbmhftd, trunc;
   int qq, qd, dd, do_field, want_stress;
+  // exact_image: Fortran nint for the minimum image (ties away from zero);
+  // otherwise rint, which equals it for every listed pair when every
+  // perpendicular width exceeds twice the list cutoff (no listed pair can sit
+  // at a half-integer fractional offset)
+  int exact_image;
   double a11, a21, a31, a12, a22, a32, a13, a23, a33;
   double cutsq_c;                    // Coulomb cutoff^2 (d2 > cutsq cycles, src/ewald_dir.f90:202)
   double alpha;
@@ -98,9 +105,15 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
     }
     const int j = mj.y;
     double dxs = si - rj.x, dys = ti - rj.y, dzs = ui - rj.z;
-    dxs -= nint_d(dxs);
-    dys -= nint_d(dys);
-    dzs -= nint_d(dzs);
+    if (a.exact_image) {
+      dxs -= nint_d(dxs);
+      dys -= nint_d(dys);
+      dzs -= nint_d(dzs);
+    } else {
+      dxs -= rint(dxs);
+      dys -= rint(dys);
+      dzs -= rint(dzs);
+    }
     const double rxd = dxs * a.a11 + dys * a.a12 + dzs * a.a13;
     const double ryd = dxs * a.a21 + dys * a.a22 + dzs * a.a23;
     const double rzd = dxs * a.a31 + dys * a.a32 + dzs * a.a33;
@@ -340,9 +353,15 @@ __global__ void __launch_bounds__(256) k_pair_field(PairArgs a) {
         m0 = (T)d0; m1 = (T)d1; m2 = (T)d2;
       }
       double dxs = si - rj.x, dys = ti - rj.y, dzs = ui - rj.z;
-      dxs -= nint_d(dxs);
-      dys -= nint_d(dys);
-      dzs -= nint_d(dzs);
+      if (a.exact_image) {
+        dxs -= nint_d(dxs);
+        dys -= nint_d(dys);
+        dzs -= nint_d(dzs);
+      } else {
+        dxs -= rint(dxs);
+        dys -= rint(dys);
+        dzs -= rint(dzs);
+      }
       const double rxd = dxs * a.a11 + dys * a.a12 + dzs * a.a13;
       const double ryd = dxs * a.a21 + dys * a.a22 + dzs * a.a23;
       const double rzd = dxs * a.a31 + dys * a.a32 + dzs * a.a33;
@@ -399,6 +418,12 @@ PairArgs make_args(mdff_ctx* c) {
   a.a12 = A[3]; a.a22 = A[4]; a.a32 = A[5];
   a.a13 = A[6]; a.a23 = A[7]; a.a33 = A[8];
   a.cutsq_c = c->rc_coul * c->rc_coul;
+  {
+    const double rlist = std::max(c->rc_vdw, c->rc_coul) + c->d.skindiff;
+    a.exact_image = 0;
+    for (int d = 0; d < 3; ++d)
+      if (!(c->W[d] > 2.0 * rlist * (1.0 + 1e-6))) a.exact_image = 1;
+  }
   a.alpha = c->alpha;
   a.rec = c->arec;
   a.meta = c->ameta;
