@@ -248,11 +248,12 @@ struct mdff_ctx {
   int32_t* cell_atoms = nullptr;          // atoms sorted by cell
   int32_t* cell_atoms_in = nullptr;
   int32_t* cell_start = nullptr;          // ncell + 1
-  int32_t* nl_count = nullptr;            // n
-  int32_t* nl_start = nullptr;            // n + 1
+  int32_t* nl_count = nullptr;            // n (+1): row lengths
+  int32_t* nl_start = nullptr;            // n + 1: row starts
   int32_t* nl_j = nullptr;                // capacity entries
   uint8_t* nl_flag = nullptr;             // bit0: r2 <= (rc_vdw+skin)^2, bit1: <= (rc_coul+skin)^2
-  int64_t nl_cap = 0, nl_total = 0;
+  int64_t nl_cap = 0;
+  int nl_row_w = 0;                       // fixed row width of the one-pass build (0: rows from a count)
   int ncell[3] = {1, 1, 1};
   int nb_lo[3] = {0, 0, 0}, nb_hi[3] = {0, 0, 0}, nb_all[3] = {1, 1, 1};  // neighbour cell offsets
   float4* bf = nullptr;                   // wrapped single-precision copy of (bx, by, bz)

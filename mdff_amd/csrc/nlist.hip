@@ -19,6 +19,7 @@
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include "common.h"
@@ -129,6 +130,11 @@ struct NLArgs {
   const int32_t* cell_of;
   int32_t* count;
   const int32_t* row_start;
+  // row_w > 0: rows of fixed width row_w (row i at i * row_w), written in one
+  // pass that also stores the row's start (row_out) and length (count);
+  // row_w == 0: rows at the exclusive scan of a counting pass (row_start)
+  int row_w;
+  int32_t* row_out;
   int32_t* nl_j;
   uint8_t* nl_flag;
   int64_t cap;
@@ -176,7 +182,8 @@ __global__ void __launch_bounds__(256) k_nl_sweep(NLArgs a) {
   const float sif = (float)(si - floor(si)), tif = (float)(ti - floor(ti)), uif = (float)(ui - floor(ui));
   const int ci = a.cell_of[i];
   const int cz = ci % a.nz, cy = (ci / a.nz) % a.ny, cx = ci / (a.nz * a.ny);
-  const int64_t base = FILL ? (int64_t)a.row_start[i] : 0;
+  const int64_t base = FILL ? (a.row_w ? (int64_t)i * a.row_w : (int64_t)a.row_start[i]) : 0;
+  const int64_t lim = a.row_w ? base + a.row_w : a.cap;
   const int ncx = a.ox1 - a.ox0 + 1, ncy = a.oy1 - a.oy0 + 1;
   const int ncol = ncx * ncy;              // <= 25 columns, <= 2 ranges each
   int b0 = 0, e0 = 0, b1 = 0, e1 = 0;
@@ -265,15 +272,18 @@ __global__ void __launch_bounds__(256) k_nl_sweep(NLArgs a) {
     if (FILL && ok) {
       const int off = found + __popcll(m & ((1ull << lane) - 1ull));
       const int64_t e = base + off;
-      if (e < a.cap) {
+      if (e < lim) {
         a.nl_j[e] = q;                   // sorted position of j
         a.nl_flag[e] = flag;
       }
     }
     found += __popcll(m);
   }
-  if (!FILL && lane == 0) a.count[i] = found;
-  if (FILL && lane == 0 && base + found > a.cap) atomicExch(a.overflow, 1);
+  if (lane == 0) {
+    if (!FILL || a.row_w) a.count[i] = found;
+    if (FILL && a.row_w) a.row_out[i] = (int32_t)base;
+    if (FILL && base + found > lim) atomicExch(a.overflow, 1);
+  }
 }
 
 static int ensure(mdff_ctx* c, void** p, size_t& have, size_t need) {
@@ -335,6 +345,8 @@ static NLArgs make_args(mdff_ctx* c) {
   a.cell_of = c->cell_of;
   a.count = c->nl_count;
   a.row_start = c->nl_start;
+  a.row_out = c->nl_start;
+  a.row_w = 0;
   a.nl_j = c->nl_j;
   a.nl_flag = c->nl_flag;
   a.cap = c->nl_cap;
@@ -378,10 +390,12 @@ static int nl_build_from(mdff_ctx* c, const double* px, const double* py, const 
   size_t tmp = 0;
   HIP_TRY(c, rocprim::radix_sort_pairs(nullptr, tmp, c->cell_of, c->cell_key, c->cell_atoms_in,
                                        c->cell_atoms, (unsigned)n, 0u, (unsigned)bits, c->stream));
-  size_t tmp2 = 0;
+  size_t tmp2 = 0, tmp3 = 0;
   HIP_TRY(c, rocprim::exclusive_scan(nullptr, tmp2, c->nl_count, c->nl_start, 0, (size_t)n + 1,
                                      rocprim::plus<int32_t>(), c->stream));
-  MDFF_TRY(ensure(c, &c->sort_tmp, c->sort_tmp_bytes, std::max(tmp, tmp2)));
+  HIP_TRY(c, rocprim::reduce(nullptr, tmp3, c->nl_count, c->d_flag + 2, 0, (size_t)n,
+                             rocprim::maximum<int32_t>(), c->stream));
+  MDFF_TRY(ensure(c, &c->sort_tmp, c->sort_tmp_bytes, std::max(tmp, std::max(tmp2, tmp3))));
   HIP_TRY(c, rocprim::radix_sort_pairs(c->sort_tmp, tmp, c->cell_of, c->cell_key, c->cell_atoms_in,
                                        c->cell_atoms, (unsigned)n, 0u, (unsigned)bits, c->stream));
   hipLaunchKernelGGL(k_cell_start, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, ncell, c->cell_key,
@@ -394,12 +408,29 @@ static int nl_build_from(mdff_ctx* c, const double* px, const double* py, const 
   a.sx = px; a.sy = py; a.sz = pz;
   a.px = c->bx; a.py = c->by; a.pz = c->bz;
   dim3 grid(div_up((int64_t)n * 64, 256));
+  if (c->nl_row_w > 0) {
+    // one pass into rows of the width sized at an earlier build; a row that
+    // does not fit sends the build through the counting pass below
+    a.row_w = c->nl_row_w;
+    HIP_TRY(c, hipMemsetAsync(c->d_flag + 1, 0, sizeof(int32_t), c->stream));
+    PROF_LAUNCH(c, PK_NLIST, hipLaunchKernelGGL(k_nl_sweep<true>, grid, dim3(256), 0, c->stream, a));
+    HIP_TRY(c, hipMemcpyAsync(c->h_flag + 1, c->d_flag + 1, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->h_flag[1] == 0) {
+      c->lists_valid = 1;
+      return MDFF_OK;
+    }
+    a.row_w = 0;
+  }
   HIP_TRY(c, hipMemsetAsync(c->nl_count + n, 0, sizeof(int32_t), c->stream));
   PROF_LAUNCH(c, PK_NLIST, hipLaunchKernelGGL(k_nl_sweep<false>, grid, dim3(256), 0, c->stream, a));
   HIP_TRY(c, rocprim::exclusive_scan(c->sort_tmp, tmp2, c->nl_count, c->nl_start, 0, (size_t)n + 1,
                                      rocprim::plus<int32_t>(), c->stream));
+  HIP_TRY(c, rocprim::reduce(c->sort_tmp, tmp3, c->nl_count, c->d_flag + 2, 0, (size_t)n,
+                             rocprim::maximum<int32_t>(), c->stream));
   int32_t total = 0;
   HIP_TRY(c, hipMemcpyAsync(&total, c->nl_start + n, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(c->h_flag + 2, c->d_flag + 2, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (total < 0) { c->err = "neighbour list exceeds 2^31 entries"; return MDFF_ERR_LIST_OVERFLOW; }
   // reference capacity: vnlmax*natm entries per half list (src/config.f90:43,
@@ -408,21 +439,28 @@ static int nl_build_from(mdff_ctx* c, const double* px, const double* py, const 
     c->err = "ERROR: out of bound list in vnlist_pbc";
     return MDFF_ERR_LIST_OVERFLOW;
   }
-  if (total > c->nl_cap) {
-    int64_t cap = (int64_t)(total * 1.25) + 1024;
+  // fixed-width rows for this and later builds: the widest row with room for
+  // growth, while the rows stay within the reference capacity (a one-pass
+  // build cannot exceed it then) and int32 positions
+  const int64_t widest = c->h_flag[2];
+  int64_t w = ((widest + widest / 4 + 16) + 31) / 32 * 32;
+  if (w > 2 * 2000 || w * n >= ((int64_t)1 << 31)) w = 0;
+  const int64_t need = w > 0 ? w * n : (int64_t)(total * 1.25) + 1024;
+  if (need > c->nl_cap) {
     if (c->nl_j) HIP_TRY(c, hipFree(c->nl_j));
     if (c->nl_flag) HIP_TRY(c, hipFree(c->nl_flag));
-    HIP_TRY(c, hipMalloc((void**)&c->nl_j, sizeof(int32_t) * cap));
-    HIP_TRY(c, hipMalloc((void**)&c->nl_flag, cap));
-    c->nl_cap = cap;
-    a.nl_j = c->nl_j;
-    a.nl_flag = c->nl_flag;
-    a.cap = cap;
+    HIP_TRY(c, hipMalloc((void**)&c->nl_j, sizeof(int32_t) * need));
+    HIP_TRY(c, hipMalloc((void**)&c->nl_flag, need));
+    c->nl_cap = need;
   }
+  a.nl_j = c->nl_j;
+  a.nl_flag = c->nl_flag;
+  a.cap = c->nl_cap;
+  a.row_w = (int)w;
+  c->nl_row_w = (int)w;
   HIP_TRY(c, hipMemsetAsync(c->d_flag + 1, 0, sizeof(int32_t), c->stream));
   PROF_LAUNCH(c, PK_NLIST, hipLaunchKernelGGL(k_nl_sweep<true>, grid, dim3(256), 0, c->stream, a));
   HIP_TRY(c, hipGetLastError());
-  c->nl_total = total;
   c->lists_valid = 1;
   return MDFF_OK;
 }
@@ -659,24 +697,31 @@ int nl_check(mdff_ctx* c, int* rebuilt) {
 int nl_get_pairs(mdff_ctx* c, int which, int32_t* point, int32_t* list, int64_t cap, int64_t* nout) {
   if (!c->lists_valid) { c->err = "lists not built"; return MDFF_ERR_BAD_ARG; }
   const int n = c->n;
-  std::vector<int32_t> start(n + 1), rows(c->nl_total > 0 ? c->nl_total : 1), order(n);
-  std::vector<uint8_t> flags(c->nl_total > 0 ? c->nl_total : 1);
+  std::vector<int32_t> start(n), count(n), order(n);
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  HIP_TRY(c, hipMemcpy(start.data(), c->nl_start, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(start.data(), c->nl_start, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(count.data(), c->nl_count, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
   HIP_TRY(c, hipMemcpy(order.data(), c->cell_atoms, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-  if (c->nl_total > 0) {
-    HIP_TRY(c, hipMemcpy(rows.data(), c->nl_j, sizeof(int32_t) * c->nl_total, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy(flags.data(), c->nl_flag, c->nl_total, hipMemcpyDeviceToHost));
+  int64_t span = 0, total = 0;
+  for (int i = 0; i < n; ++i) {
+    span = std::max<int64_t>(span, (int64_t)start[i] + count[i]);
+    total += count[i];
+  }
+  std::vector<int32_t> rows(span > 0 ? span : 1);
+  std::vector<uint8_t> flags(span > 0 ? span : 1);
+  if (span > 0) {
+    HIP_TRY(c, hipMemcpy(rows.data(), c->nl_j, sizeof(int32_t) * span, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(flags.data(), c->nl_flag, span, hipMemcpyDeviceToHost));
   }
   const uint8_t bit = which ? 2 : 1;
   int64_t cnt = 0;
   std::vector<int32_t> tmp;
   std::vector<int32_t> out;
-  out.reserve(c->nl_total / 2 + 1);
+  out.reserve(total / 2 + 1);
   for (int i = 0; i < n; ++i) {
     if (point) point[i] = (int32_t)(cnt + 1);
     tmp.clear();
-    for (int e = start[i]; e < start[i + 1]; ++e) {
+    for (int e = start[i]; e < start[i] + count[i]; ++e) {
       int j = order[rows[e]];             // rows hold cell-sorted positions
       bool own = (i > j && ((i + j) % 2 == 0)) || (i < j && ((i + j) % 2 != 0));
       if (own && (flags[e] & bit)) tmp.push_back(j + 1);

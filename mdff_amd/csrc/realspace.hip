@@ -38,7 +38,8 @@ struct PairArgs {
   const int2* meta;                  // (type, original index) in the same order
   const double* mu;                  // 3n AoS (may be null when no dipoles)
   const PairParam* pp;
-  const int32_t* row;                // n + 1
+  const int32_t* row;                // row starts (n)
+  const int32_t* cnt;                // row lengths (n)
   const int32_t* nl_j;               // sorted positions of the neighbours
   double* fsr;                       // 3n AoS out
   double* fcl;                       // 3n AoS out
@@ -84,7 +85,7 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
   }
   double fs0 = 0, fs1 = 0, fs2 = 0, fc0 = 0, fc1 = 0, fc2 = 0, e0 = 0, e1 = 0, e2 = 0;
   double usr = 0, vsr = 0, ts[6] = {0, 0, 0, 0, 0, 0}, ucl = 0, tc[6] = {0, 0, 0, 0, 0, 0};
-  const int r0 = a.row[i], r1 = a.row[i + 1];
+  const int r0 = a.row[i], r1 = r0 + a.cnt[i];
   // two-stage prefetch of the neighbour chain: the row entry two steps ahead
   // and the record of the next entry are in flight while a pair is computed
   int pj_next = (r0 + sub + LPA < r1) ? a.nl_j[r0 + sub + LPA] : 0;
@@ -326,7 +327,7 @@ __global__ void __launch_bounds__(256) k_pair_field(PairArgs a) {
   bool dip_i = true;
   if (MODE == 1) dip_i = (a.mu[3 * i] != 0.0 || a.mu[3 * i + 1] != 0.0 || a.mu[3 * i + 2] != 0.0);
   double e0 = 0, e1 = 0, e2 = 0;
-  const int r0 = a.row[i], r1 = a.row[i + 1];
+  const int r0 = a.row[i], r1 = r0 + a.cnt[i];
   if (dip_i) {
     // neighbour chain prefetched two stages ahead, as in k_pair_force
     int pj_next = (r0 + sub + LPA < r1) ? a.nl_j[r0 + sub + LPA] : 0;
@@ -429,6 +430,7 @@ PairArgs make_args(mdff_ctx* c) {
   a.meta = c->ameta;
   a.pp = c->d_pp;
   a.row = c->nl_start;
+  a.cnt = c->nl_count;
   a.nl_j = c->nl_j;
   return a;
 }
