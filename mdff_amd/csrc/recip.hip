@@ -1287,16 +1287,16 @@ k_gather_h3(Gh3Args a) {
         pre[u] = a.amat[(size_t)(mt + 1) * NA + min((int)threadIdx.x + 256 * u, NA - 1)];
     }
     // epilogue operands of this tile (used after the MFMAs): the lane's m
-    // values 8 mt + 2 q + hl, q < 4, clamped below M (zero weight beyond)
+    // values 8 mt + 2 q + hl, q < 4, clamped below M (the A rows of m >= M
+    // are zero, so those D rows are zero)
     cplx<float> Xv[CT][4], Yv[CT][4];
-    float fx[4], fy[4], ysg[4], mw[4];
+    float fx[4], fy[4], ysg[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int mq = 8 * mt + 2 * q + hl;
       int nxx, nyo;
       divmod_f(min(mq, a.M - 1), a.NY, inv_ny, nxx, nyo);
       const int ny = nyo - a.Ky;
-      mw[q] = mq < a.M ? 1.f : 0.f;
       fx[q] = (float)nxx;
       fy[q] = (float)ny;
       ysg[q] = ny < 0 ? -1.f : 1.f;
@@ -1325,23 +1325,27 @@ k_gather_h3(Gh3Args a) {
       }
     }
     __builtin_amdgcn_sched_barrier(0);
+    // Im(X Y U0) weighted by nx, ny and Im(X Y U1) over the lane's 4 m
+    // values in FP32, times the tile's inverse scale, then added in FP64
     const float inv = a.ascale[mt];
 #pragma unroll
-    for (int ct = 0; ct < CT; ++ct)
+    for (int ct = 0; ct < CT; ++ct) {
+      float sx = 0.f, sy = 0.f, sz = 0.f;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         cplx<float> Y = Yv[ct][q];
         Y.im *= ysg[q];
-        cplx<float> XY = cmul(Xv[ct][q], Y);
-        const float w = mw[q] * inv;
-        XY.re *= w;
-        XY.im *= w;
-        const double im0 = (double)(XY.re * acc[ct][4 * q + 1] + XY.im * acc[ct][4 * q]);
-        const double im1 = (double)(XY.re * acc[ct][4 * q + 3] + XY.im * acc[ct][4 * q + 2]);
-        ax[ct] += (double)fx[q] * im0;
-        ay[ct] += (double)fy[q] * im0;
-        az3[ct] += im1;
+        const cplx<float> XY = cmul(Xv[ct][q], Y);
+        const float im0 = XY.re * acc[ct][4 * q + 1] + XY.im * acc[ct][4 * q];
+        const float im1 = XY.re * acc[ct][4 * q + 3] + XY.im * acc[ct][4 * q + 2];
+        sx += fx[q] * im0;
+        sy += fy[q] * im0;
+        sz += im1;
       }
+      ax[ct] += (double)(sx * inv);
+      ay[ct] += (double)(sy * inv);
+      az3[ct] += (double)(sz * inv);
+    }
     if (mt + 1 < mt1) {
 #pragma unroll
       for (int u = 0; u < PER; ++u)
