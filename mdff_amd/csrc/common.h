@@ -26,9 +26,12 @@ __host__ __device__ __forceinline__ double nint_d(double x) {
 
 // Numerical Recipes erfcc, as src/functions.f90:28-40 (x >= 0 only here:
 // every caller passes alpha*d >= 0).
+// The FP32 form takes t from the hardware reciprocal (within 1 ulp).
 template <typename T>
 __device__ __forceinline__ T errfc_pos(T z) {
-  T t = T(1) / (T(1) + T(0.5) * z);
+  T t;
+  if constexpr (sizeof(T) == 4) t = __builtin_amdgcn_rcpf(1.f + 0.5f * z);
+  else t = T(1) / (T(1) + T(0.5) * z);
   T p = t * (T(1.48851587) + t * (T(-.82215223) + t * T(.17087277)));
   p = t * (T(-1.13520398) + p);
   p = t * (T(.27886807) + p);

@@ -52,6 +52,20 @@ struct V3 {
   T x, y, z;
 };
 
+// d and 1/d from r^2: d correctly rounded (the short-range exponentials are
+// sensitive to it); 1/d in FP32 from the hardware reciprocal (within 1 ulp),
+// in FP64 by a division
+template <typename T>
+__device__ __forceinline__ void dist_inv(T r2, T& d, T& dm1) {
+  if constexpr (sizeof(T) == 4) {
+    d = sqrtf(r2);
+    dm1 = __builtin_amdgcn_rcpf(d);
+  } else {
+    d = sqrt(r2);
+    dm1 = T(1) / d;
+  }
+}
+
 template <int LPA>
 __device__ __forceinline__ double group_sum(double v) {
 #pragma unroll
@@ -125,8 +139,8 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
     const bool in_cl = (a.qq | a.qd | a.dd) && !(r2d > a.cutsq_c);
     if (!in_sr && !in_cl) continue;
     const T rx = (T)rxd, ry = (T)ryd, rz = (T)rzd, r2 = (T)r2d;
-    const T d = sqrt(r2);
-    const T dm1 = T(1) / d;
+    T d, dm1;
+    dist_inv(r2, d, dm1);
     const T dm2 = dm1 * dm1;
     if (in_sr) {
       T w = T(0), u = T(0);
@@ -369,8 +383,9 @@ __global__ void __launch_bounds__(256) k_pair_field(PairArgs a) {
       const double r2d = rxd * rxd + ryd * ryd + rzd * rzd;
       if (r2d > a.cutsq_c) continue;
       const T rx = (T)rxd, ry = (T)ryd, rz = (T)rzd, r2 = (T)r2d;
-      const T d = sqrt(r2);
-      const T dm1 = T(1) / d, dm2 = dm1 * dm1, dm3 = dm1 * dm2;
+      T d, dm1;
+      dist_inv(r2, d, dm1);
+      const T dm2 = dm1 * dm1, dm3 = dm1 * dm2;
       const T expon = exp(-a2 * r2) * isqpi;
       const T F0 = errfc_pos<T>(alpha * d);
       const T F1 = F0 + T(2) * alpha * d * expon;
