@@ -229,6 +229,11 @@ struct mdff_ctx {
   double* d_chk = nullptr;                // the device address of h_chk
   hipEvent_t chk_ev = nullptr;            // the pending check's maxima are in h_chk
   int chk_pending = 0;
+  // instead of the event, the next final-sum launch writes chk_seq into
+  // h_chk[2 nb] once the check kernel has completed (1: to be attached to
+  // that launch, 2: attached); the host waits for the value
+  int chk_seq_wait = 0;
+  double chk_seq = 0.0;
   double* red_part = nullptr;             // reduction block partials
   double* fin_part = nullptr;             // block partials of the pair sums [0, 1024*16) and kinetic energy
   double* rec_blk = nullptr;              // k-block partials of the reciprocal energy and stress

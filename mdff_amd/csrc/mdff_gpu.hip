@@ -56,11 +56,18 @@ __global__ void __launch_bounds__(256) k_colsum_final(int nb, int ncol, const do
 }
 
 // the queued final sums, one block each
+// (+ the skin check's completion signal: seq_val into mapped host memory;
+// this launch starts after the check kernel has completed)
 struct FinalArgs {
   FinalSum f[kMaxFinals];
+  int nfin;
+  double* seq_out;
+  double seq_val;
 };
 __global__ void __launch_bounds__(256) k_final_sums(FinalArgs a) {
   __shared__ double sh[4 * kRedMaxCols];
+  if (a.seq_out && blockIdx.x == 0 && threadIdx.x == 0) a.seq_out[0] = a.seq_val;
+  if ((int)blockIdx.x >= a.nfin) return;            // whole block
   const FinalSum f = a.f[blockIdx.x];
   sum_block_partials(f.nb, f.ncol, f.part, f.out, sh);
 }
@@ -360,10 +367,17 @@ int reduce_cols_part(mdff_ctx* c, const double* in, int ncol, double* part, int*
 }
 
 int flush_finals(mdff_ctx* c) {
-  if (c->nfin == 0) return MDFF_OK;
+  const bool sig = c->chk_seq_wait == 1;
+  if (c->nfin == 0 && !sig) return MDFF_OK;
   FinalArgs a{};
   for (int k = 0; k < c->nfin; ++k) a.f[k] = c->fin[k];
-  hipLaunchKernelGGL(k_final_sums, dim3(c->nfin), dim3(256), 0, c->stream, a);
+  a.nfin = c->nfin;
+  if (sig) {
+    a.seq_out = c->d_chk + 2 * nblk(c->n);
+    a.seq_val = c->chk_seq;
+    c->chk_seq_wait = 2;
+  }
+  hipLaunchKernelGGL(k_final_sums, dim3(std::max(c->nfin, 1)), dim3(256), 0, c->stream, a);
   c->nfin = 0;
   HIP_TRY(c, hipGetLastError());
   return MDFF_OK;
@@ -562,6 +576,7 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   HIP_TRY(c, hipHostMalloc((void**)&c->h_chk, sizeof(double) * (2 * (size_t)nblk(n) + 2),
                            hipHostMallocMapped | hipHostMallocCoherent));
   HIP_TRY(c, hipHostGetDevicePointer((void**)&c->d_chk, c->h_chk, 0));
+  memset(c->h_chk, 0, sizeof(double) * (2 * (size_t)nblk(n) + 2));
   HIP_TRY(c, hipEventCreateWithFlags(&c->chk_ev, hipEventDisableTiming));
   HIP_TRY(c, hipHostMalloc((void**)&c->h_flag, sizeof(int32_t) * 4));
   const size_t nred = std::max<size_t>(64 * 1024, 2 * (size_t)nblk(n) + 2);

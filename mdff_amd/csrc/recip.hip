@@ -885,8 +885,9 @@ struct Sh3Args {
   float* part;                   // [chunk][2*nk_box]
 };
 
-template <int TA, bool CQ>
+template <int TA, int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_sk_h3(Sh3Args a) {
+  constexpr bool CQ = MODE != 1;
   // CQ: P = c X Y with one coefficient per atom (modes 0 and 2): c X is
   // formed once per tile when X is staged; otherwise (mode 1) P =
   // (c0 - i (nx c1 + ny c2)) X Y is formed per column
@@ -947,8 +948,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       xn[u] = tab[(size_t)min(scg + CG * u, nxy - 1) * a.n + src_];                           \
     if (CQ || threadIdx.x < TA) {                                                             \
       const int i_ = (int)src_;                                                               \
-      if (a.mode <= 1) qd = a.q[i_];                                                          \
-      if (a.mode >= 1) {                                                                      \
+      if (MODE <= 1) qd = a.q[i_];                                                          \
+      if (MODE >= 1) {                                                                      \
         m0 = a.mu[3 * i_];                                                                    \
         m1 = a.mu[3 * i_ + 1];                                                                \
         m2 = a.mu[3 * i_ + 2];                                                                \
@@ -979,7 +980,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       r0[129 * RW] = pack_h(il, rl);
     }
     float cs = 1.f;                                       // c of atom sia (CQ)
-    if (CQ) cs = a.mode == 0 ? (float)(qd * a.pscale) : (float)(a.pscale * (m0 * a.b[6] + m1 * a.b[7] + m2 * a.b[8]));
+    if (CQ) cs = MODE == 0 ? (float)(qd * a.pscale) : (float)(a.pscale * (m0 * a.b[6] + m1 * a.b[7] + m2 * a.b[8]));
 #pragma unroll
     for (int u = 0; u < XU; ++u) {
       const int cidx = scg + CG * u;
@@ -998,11 +999,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     if (!CQ && threadIdx.x < TA) {
       float c0 = 0.f, c1 = 0.f, c2 = 0.f;
       if ((int)threadIdx.x < nt) {
-        if (a.mode <= 1) c0 = (float)(qd * a.pscale);
-        if (a.mode == 1) {
+        if (MODE <= 1) c0 = (float)(qd * a.pscale);
+        if (MODE == 1) {
           c1 = (float)(2.0 * MDFF_PI * a.pscale * (m0 * a.b[0] + m1 * a.b[1] + m2 * a.b[2]));
           c2 = (float)(2.0 * MDFF_PI * a.pscale * (m0 * a.b[3] + m1 * a.b[4] + m2 * a.b[5]));
-        } else if (a.mode == 2) {
+        } else if (MODE == 2) {
           c0 = (float)(a.pscale * (m0 * a.b[6] + m1 * a.b[7] + m2 * a.b[8]));
         }
       }
@@ -1606,7 +1607,7 @@ int recip_setup(mdff_ctx* c) {
       // structure-factor chunks from the kernel's occupancy
       const size_t lds = sk_h3_lds(c->K[0], c->K[1], kH3TA);
       const int64_t cap =
-          (int64_t)cus * blocks_per_cu(reinterpret_cast<const void*>(&k_sk_h3<kH3TA, true>), lds);
+          (int64_t)cus * blocks_per_cu(reinterpret_cast<const void*>(&k_sk_h3<kH3TA, 0>), lds);
       const int64_t mblocks = div_up(div_up(M, 32), 4) * (int64_t)div_up(c->NZk, 64);
       int64_t best = -1;
       for (int ch = 1; ch <= 128 && (ch == 1 || div_up(c->n, ch) >= kH3TA); ++ch) {
@@ -1746,14 +1747,14 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     sh.pscale = use_mu ? 1.f : c->h3_pscale;
     const size_t lds = sk_h3_lds(c->K[0], c->K[1], kH3TA);
     dim3 gs(div_up(div_up(s.M, 32), 4), ch, div_up(c->NZk, 64));
-    if (use_mu) PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, false>), gs, dim3(256), lds, c->stream, sh));
-    else PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, true>), gs, dim3(256), lds, c->stream, sh));
+    if (use_mu) PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, 1>), gs, dim3(256), lds, c->stream, sh));
+    else PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, 0>), gs, dim3(256), lds, c->stream, sh));
     if (use_mu) {
       Sh3Args s2 = sh;
       s2.mode = 2;
       s2.part = p1 + (size_t)2 * ch * c->nk_box;
       const size_t tot = (size_t)ch * c->nk_box;
-      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, true>), gs, dim3(256), lds, c->stream, s2);
+      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, 2>), gs, dim3(256), lds, c->stream, s2);
                   hipLaunchKernelGGL(k_sk_combine<float>, dim3(div_up(tot, 256)), dim3(256), 0, c->stream,
                                      c->nk_box, c->NZk, c->K[2], ch, p1, s2.part));
     }
