@@ -296,7 +296,8 @@ struct mdff_ctx {
   int sk_ta = 0;                          // atoms per staged tile of the structure factor
   int sk_mma_ch = 1;                      // atom chunks of the matrix-core structure factor
   // mixed form: split-half (FP16 hi + lo) matrix-core reciprocal kernels
-  int h3 = 0;
+  int h3 = 0;                             // split-half gather (and its A operand from the reduction)
+  int h3_sk = 0;                          // split-half structure factor as well
   int h3_ch = 1, h3_nsh = 0, h3_ct = 1, h3_splits = 1, h3_mt8 = 0;
   float h3_pscale = 1.f;                  // charge scale of the structure factor's B operand
   void* amat_h3 = nullptr;                // gather A operand, split halves

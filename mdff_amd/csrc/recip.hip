@@ -16,6 +16,8 @@
 //   phase 2  per atom: U1_d = sum_n n_d Im(W e_i), M_de = sum_n n_d n_e Re(W e_i)
 //            field = 2 (4 pi/V) 2 pi B U1; force = 2 (4 pi/V) [q_i 2 pi B U1
 //            + (2 pi)^2 B M m_i]                               (:196-233, :300-358)
+#include <type_traits>
+
 #include "common.h"
 #include "reduce.h"
 
@@ -1090,8 +1092,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 // (wi, wr), comp 2/3 the same times nz.  Values times 2^(14 - e), e the
 // exponent of the tile's largest value; ascale[mt] = 2^(e - 14).  The tile is
 // assembled in LDS (zeros where t >= NZ or m >= M) and stored coalesced.
-template <int NSH>
-__global__ void __launch_bounds__(256) k_sk_reduce_h3(int nk_box, int nchunk, const float* __restrict__ part,
+template <int NSH, typename TP>
+__global__ void __launch_bounds__(256) k_sk_reduce_h3(int nk_box, int nchunk, const TP* __restrict__ part,
                                                       const double* __restrict__ kA,
                                                       const double* __restrict__ kvec, double* __restrict__ sk,
                                                       double* __restrict__ wk, float* __restrict__ wk_t,
@@ -1116,13 +1118,14 @@ __global__ void __launch_bounds__(256) k_sk_reduce_h3(int nk_box, int nchunk, co
 #pragma unroll
   for (int u = 0; u < NU; ++u) { sre[u] = 0.0; sim[u] = 0.0; }
   for (int c = 0; c < nchunk; c += CQ) {
-    float2 pv[NU][CQ];
+    using TP2 = typename std::conditional<sizeof(TP) == 4, float2, double2>::type;
+    TP2 pv[NU][CQ];
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
       const int k = k0 + min((int)threadIdx.x + 256 * u, kn - 1);
 #pragma unroll
       for (int q = 0; q < CQ; ++q)
-        pv[u][q] = *reinterpret_cast<const float2*>(part + (size_t)min(c + q, nchunk - 1) * 2 * nk_box + 2 * k);
+        pv[u][q] = *reinterpret_cast<const TP2*>(part + (size_t)min(c + q, nchunk - 1) * 2 * nk_box + 2 * k);
     }
 #pragma unroll
     for (int u = 0; u < NU; ++u)
@@ -1179,6 +1182,9 @@ __global__ void __launch_bounds__(256) k_sk_reduce_h3(int nk_box, int nchunk, co
   vmax = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
   int e = 0;
   if (vmax > 0.f) (void)frexpf(vmax, &e);                  // vmax in [2^(e-1), 2^e)
+  // tiles of vanishing W (A_k far below FP32's normal range at large k):
+  // the scale stays finite, their halves flush to zero
+  e = max(e, -100);
   const float sc = ldexpf(1.f, 14 - e);
   if (threadIdx.x == 0) ascale[mt] = ldexpf(1.f, e - 14);
   uint16_t* th = reinterpret_cast<uint16_t*>(tile);
@@ -1478,12 +1484,13 @@ static const void* gather_h3_kernel(int ct, int nsh) {
     default: return nullptr;
   }
 }
+template <typename TP>
 static const void* sk_reduce_h3_kernel(int nsh) {
   switch (nsh) {
-    case 4: return reinterpret_cast<const void*>(&k_sk_reduce_h3<4>);
-    case 7: return reinterpret_cast<const void*>(&k_sk_reduce_h3<7>);
-    case 10: return reinterpret_cast<const void*>(&k_sk_reduce_h3<10>);
-    case 15: return reinterpret_cast<const void*>(&k_sk_reduce_h3<15>);
+    case 4: return reinterpret_cast<const void*>(&k_sk_reduce_h3<4, TP>);
+    case 7: return reinterpret_cast<const void*>(&k_sk_reduce_h3<7, TP>);
+    case 10: return reinterpret_cast<const void*>(&k_sk_reduce_h3<10, TP>);
+    case 15: return reinterpret_cast<const void*>(&k_sk_reduce_h3<15, TP>);
     default: return nullptr;
   }
 }
@@ -1584,14 +1591,18 @@ int recip_setup(mdff_ctx* c) {
   // mixed form: split-half kernels when their shapes fit (MDFF_RECIP_F32MMA=1
   // keeps the FP32 matrix-core kernels)
   c->h3 = 0;
+  c->h3_sk = 0;
   {
     const char* ev = getenv("MDFF_RECIP_F32MMA");
     const bool off = ev && ev[0] == '1';
+    const bool sk_off = ev && ev[0] == '2';         // FP32 matrix-core structure factor only
     const int nxy = Kx + Ky + 2;
     const int nsh = h3_bucket(div_up(2 * c->NZk, 16));
-    if (fp32 && c->use_mma && !off && nsh > 0 && nxy <= 8 * (256 / kH3TA) &&
-        sk_h3_lds(c->K[0], c->K[1], kH3TA) <= 65536) {
+    if (fp32 && c->use_mma && !off && nsh > 0) {
+      // the gather for any kz its K steps cover; the structure factor where
+      // its staged columns fit (else the FP32 matrix-core one feeds it)
       c->h3 = 1;
+      c->h3_sk = !sk_off && nxy <= 8 * (256 / kH3TA) && sk_h3_lds(c->K[0], c->K[1], kH3TA) <= 65536;
       c->h3_nsh = nsh;
       c->h3_ct = nsh <= 7 ? 2 : 1;
       c->h3_mt8 = div_up(M, 8);
@@ -1605,16 +1616,18 @@ int recip_setup(mdff_ctx* c) {
       const int bs = gather_splits(c->h3_mt8, aw, gconc);
       c->h3_splits = bs;
       // structure-factor chunks from the kernel's occupancy
-      const size_t lds = sk_h3_lds(c->K[0], c->K[1], kH3TA);
-      const int64_t cap =
-          (int64_t)cus * blocks_per_cu(reinterpret_cast<const void*>(&k_sk_h3<kH3TA, 0>), lds);
-      const int64_t mblocks = div_up(div_up(M, 32), 4) * (int64_t)div_up(c->NZk, 64);
-      int64_t best = -1;
-      for (int ch = 1; ch <= 128 && (ch == 1 || div_up(c->n, ch) >= kH3TA); ++ch) {
-        const int per = div_up(c->n, ch);
-        const int nch = div_up(c->n, per);
-        const int64_t cost = ((mblocks * nch + cap - 1) / cap) * div_up(per, kH3TA);
-        if (best < 0 || cost < best) { best = cost; c->h3_ch = nch; }
+      if (c->h3_sk) {
+        const size_t lds = sk_h3_lds(c->K[0], c->K[1], kH3TA);
+        const int64_t cap =
+            (int64_t)cus * blocks_per_cu(reinterpret_cast<const void*>(&k_sk_h3<kH3TA, 0>), lds);
+        const int64_t mblocks = div_up(div_up(M, 32), 4) * (int64_t)div_up(c->NZk, 64);
+        int64_t best = -1;
+        for (int ch = 1; ch <= 128 && (ch == 1 || div_up(c->n, ch) >= kH3TA); ++ch) {
+          const int per = div_up(c->n, ch);
+          const int nch = div_up(c->n, per);
+          const int64_t cost = ((mblocks * nch + cap - 1) / cap) * div_up(per, kH3TA);
+          if (best < 0 || cost < best) { best = cost; c->h3_ch = nch; }
+        }
       }
       HIP_TRY(c, hipMalloc(&c->amat_h3, 16 * 128 * (size_t)nsh * c->h3_mt8));
       HIP_TRY(c, hipMalloc((void**)&c->ascale_h3, sizeof(float) * (size_t)c->h3_mt8));
@@ -1725,8 +1738,8 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
   }
   int chunks = div_up(c->n, s.a0_step);
   bool sk_fp64 = false;
-  const bool h3 = sizeof(T) == 4 && c->h3;
-  if (h3) {
+  const bool h3 = sizeof(T) == 4 && c->h3;           // split-half gather
+  if (h3 && c->h3_sk) {
     // split-half structure factor, FP32 partials per chunk
     Sh3Args sh{};
     sh.n = c->n;
@@ -1814,13 +1827,14 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
   MDFF_TRY(claim_partials(c, c->rec_blk));
   PROF_LAUNCH(c, PK_SKRED,
               if (h3_gather) {
-                const float* prt = reinterpret_cast<const float*>(c->sk_part);
+                const void* prt = c->sk_part;        // FP32 (split-half) or FP64 (FP32 matrix-core) partials
                 float* wkt = reinterpret_cast<float*>(c->wk_t);
                 u32x4* am = reinterpret_cast<u32x4*>(c->amat_h3);
                 void* rargs[] = {(void*)&c->nk_box, (void*)&chunks, (void*)&prt, (void*)&c->kA, (void*)&c->kvec,
                                  (void*)&c->sk, (void*)&c->wk, (void*)&wkt, (void*)&c->rec_blk, (void*)&c->NZk,
                                  (void*)&c->K[2], (void*)&am, (void*)&c->ascale_h3};
-                (void)hipLaunchKernel(sk_reduce_h3_kernel(c->h3_nsh), dim3(nb), dim3(256), rargs, 0, c->stream);
+                const void* fn = sk_fp64 ? sk_reduce_h3_kernel<double>(c->h3_nsh) : sk_reduce_h3_kernel<float>(c->h3_nsh);
+                (void)hipLaunchKernel(fn, dim3(nb), dim3(256), rargs, 0, c->stream);
               } else if (sk_fp64) {
                 hipLaunchKernelGGL((k_sk_reduce<T, double>), dim3(nb), dim3(256), 0, c->stream, c->nk_box, chunks,
                                    reinterpret_cast<const double*>(c->sk_part), c->kA, c->kvec, c->sk, c->wk,

@@ -231,7 +231,7 @@ def test_pim_extrapolation_order_parity():
 
 # ------------------------------------------------------------------ mixed precision
 
-@pytest.mark.parametrize("case", ["sio2", "nacl", "pim"])
+@pytest.mark.parametrize("case", ["sio2", "nacl", "pim", "wide_kxy"])
 def test_mixed_precision_force_error(case):
     """FP32 pair / field arithmetic with FP64 accumulation: max |dF|/max|F|
     against the FP64 oracle below 1e-5."""
@@ -239,6 +239,11 @@ def test_mixed_precision_force_error(case):
         s = _sio2(natm=900, seed=26)
     elif case == "nacl":
         s = configs.nacl(ncell=3)
+    elif case == "wide_kxy":
+        # kmax (34, 34, 8): more X/Y columns than the split-half structure
+        # factor stages, so the FP32 matrix-core one feeds the split-half gather
+        s = configs.nacl(ncell=2, alpha=0.9, kmax=34)
+        s.desc.kES[2] = 8
     else:
         s = _pim()
     note(f"mixed {case} n={s.natm}")
