@@ -73,17 +73,17 @@ __global__ void __launch_bounds__(256) k_final_sums(FinalArgs a) {
 }
 
 // Total force f = f_sr + (f_rec + f_dir) * coul_unit (src/ewald.f90:237-246),
-// AoS inputs, SoA output; f_rec from the array frec, or finished here from
-// the reciprocal split partials (rf_on).  The same blocks then write the
+// AoS inputs, SoA output; the pair kernel delivers f_sr + coul_unit f_dir
+// (combined per pair) in fsr; f_rec from the array frec, or finished here
+// from the reciprocal split partials (rf_on).  The same blocks then write the
 // block partials of the pair kernel's per-atom columns (column sums over
 // atoms, as k_colsum).
 struct ForceSumArgs {
   int n, ncol;
   const double* fsr;
-  const double* fdir;
   const double* frec;
   RecFinal rf;
-  int rf_on, has_sr, has_cl;
+  int rf_on, has_cl;
   double coul;
   double *fx, *fy, *fz;
   const double* in;              // [ncol][n]
@@ -106,8 +106,9 @@ __global__ void __launch_bounds__(256) k_force_sums(ForceSumArgs a) {
     }
     double f[3];
     for (int d = 0; d < 3; ++d) {
-      double v = a.has_sr ? a.fsr[3 * i + d] : 0.0;
-      if (a.has_cl) v = v + (fr[d] + a.fdir[3 * i + d]) * a.coul;
+      // fsr: the pair kernel's f_sr + coul_unit f_dir, combined per pair
+      double v = a.fsr[3 * i + d];
+      if (a.has_cl) v = v + fr[d] * a.coul;
       f[d] = v;
     }
     a.fx[i] = f[0];
@@ -918,11 +919,9 @@ static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t, bool defer_f
     fa.n = n;
     fa.ncol = NPA;
     fa.fsr = f_sr(c);
-    fa.fdir = f_dir(c);
     fa.frec = c->frec;
     fa.rf = c->rfin;
     fa.rf_on = (coul && c->rfin_pending) ? 1 : 0;
-    fa.has_sr = c->sr_model != 0 ? 1 : 0;
     fa.has_cl = coul ? 1 : 0;
     fa.coul = d.coul_unit;
     fa.fx = c->fx; fa.fy = c->fy; fa.fz = c->fz;

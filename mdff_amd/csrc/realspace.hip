@@ -40,9 +40,10 @@ struct PairArgs {
   const PairParam* pp;
   const int32_t* row;                // row starts (n)
   const int32_t* cnt;                // row lengths (n)
+  double coul;                       // coul_unit
   const int32_t* nl_j;               // sorted positions of the neighbours
-  double* fsr;                       // 3n AoS out
-  double* fcl;                       // 3n AoS out
+  double* fsr;                       // 3n AoS out: f_sr + coul_unit f_dir
+  double* fcl;                       // unused by the force kernel
   double* ef;                        // 3n AoS out (field at i)
   double* peratom;                   // NPA x n (column-major: col * n + i)
 };
@@ -97,8 +98,11 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
     dip_i = (m0 != 0.0 || m1 != 0.0 || m2 != 0.0);
     mui = {(T)m0, (T)m1, (T)m2};
   }
-  double fs0 = 0, fs1 = 0, fs2 = 0, fc0 = 0, fc1 = 0, fc2 = 0, e0 = 0, e1 = 0, e2 = 0;
-  double usr = 0, vsr = 0, ts[6] = {0, 0, 0, 0, 0, 0}, ucl = 0, tc[6] = {0, 0, 0, 0, 0, 0};
+  // f: the pair forces on i, short range + coul_unit x real-space Ewald,
+  // combined per pair in T
+  double f0 = 0, f1 = 0, f2 = 0, e0 = 0, e1 = 0, e2 = 0;
+  double usr = 0, ts[6] = {0, 0, 0, 0, 0, 0}, ucl = 0, tc[6] = {0, 0, 0, 0, 0, 0};
+  const T coul = (T)a.coul;
   const int r0 = a.row[i], r1 = r0 + a.cnt[i];
   // two-stage prefetch of the neighbour chain: the row entry two steps ahead
   // and the record of the next entry are in flight while a pair is computed
@@ -142,6 +146,7 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
     T d, dm1;
     dist_inv(r2, d, dm1);
     const T dm2 = dm1 * dm1;
+    T pfx = T(0), pfy = T(0), pfz = T(0);
     if (in_sr) {
       T w = T(0), u = T(0);
       if (a.sr_model == 2) {
@@ -175,11 +180,10 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
         w = T(2) * (T)P.epsmor * (T)P.rhomor * erh * (T)P.rs * x * dm1;
       }
       const T fx = w * rx, fy = w * ry, fz = w * rz;
-      fs0 += (double)fx;
-      fs1 += (double)fy;
-      fs2 += (double)fz;
+      pfx = fx;
+      pfy = fy;
+      pfz = fz;
       usr += (double)u;
-      vsr += (double)(w * r2);
       ts[0] += (double)(rx * fx);
       ts[1] += (double)(ry * fy);
       ts[2] += (double)(rz * fz);
@@ -267,9 +271,9 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
           }
         }
       }
-      fc0 -= (double)fij0;
-      fc1 -= (double)fij1;
-      fc2 -= (double)fij2;
+      pfx -= coul * fij0;
+      pfy -= coul * fij1;
+      pfz -= coul * fij2;
       ucl += (double)u;
       tc[0] -= (double)(T(2) * rx * fij0);
       tc[1] -= (double)(T(2) * ry * fij1);
@@ -278,13 +282,13 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
       tc[4] -= (double)(rx * fij2 + rz * fij0);
       tc[5] -= (double)(ry * fij2 + rz * fij1);
     }
+    f0 += (double)pfx;
+    f1 += (double)pfy;
+    f2 += (double)pfz;
   }
-  fs0 = group_sum<LPA>(fs0);
-  fs1 = group_sum<LPA>(fs1);
-  fs2 = group_sum<LPA>(fs2);
-  fc0 = group_sum<LPA>(fc0);
-  fc1 = group_sum<LPA>(fc1);
-  fc2 = group_sum<LPA>(fc2);
+  f0 = group_sum<LPA>(f0);
+  f1 = group_sum<LPA>(f1);
+  f2 = group_sum<LPA>(f2);
   if (a.do_field) {
     e0 = group_sum<LPA>(e0);
     e1 = group_sum<LPA>(e1);
@@ -293,20 +297,18 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
   double cols[NPA];
   if (a.want_stress) {
     cols[0] = group_sum<LPA>(usr);
-    cols[1] = group_sum<LPA>(vsr);
+    cols[1] = 0.0;                        // set from the stress trace below
 #pragma unroll
     for (int k = 0; k < 6; ++k) cols[2 + k] = group_sum<LPA>(ts[k]);
+    cols[1] = (cols[2] + cols[3]) + cols[4];  // virial w r^2 = trace of r (w r)
     cols[8] = group_sum<LPA>(ucl);
 #pragma unroll
     for (int k = 0; k < 6; ++k) cols[9 + k] = group_sum<LPA>(tc[k]);
   }
   if (sub == 0) {
-    a.fsr[3 * i] = fs0;
-    a.fsr[3 * i + 1] = fs1;
-    a.fsr[3 * i + 2] = fs2;
-    a.fcl[3 * i] = fc0;
-    a.fcl[3 * i + 1] = fc1;
-    a.fcl[3 * i + 2] = fc2;
+    a.fsr[3 * i] = f0;
+    a.fsr[3 * i + 1] = f1;
+    a.fsr[3 * i + 2] = f2;
     if (a.do_field) {
       a.ef[3 * i] = e0;
       a.ef[3 * i + 1] = e1;
@@ -446,6 +448,7 @@ PairArgs make_args(mdff_ctx* c) {
   a.pp = c->d_pp;
   a.row = c->nl_start;
   a.cnt = c->nl_count;
+  a.coul = c->d.coul_unit;
   a.nl_j = c->nl_j;
   return a;
 }
