@@ -70,6 +70,33 @@ def test_pair_sets_bit_exact(case):
     assert n1 > 0 or s.desc.cutlongrange == 0.0  # LJ-only: the coul list is empty
 
 
+def test_pair_sets_after_rows_outgrow_fixed_width():
+    """After the first (counted) build the rows have a fixed width; a later
+    build whose rows outgrow it goes back through the counting pass.  Both
+    builds give the reference's pair sets (src/tools.f90:273-368)."""
+    s = _sio2(seed=28)
+    g, o = pair(s)
+    g.vnlist_build()
+    o.vnlist_build()
+    _lists_equal(g, o, 0)
+    # pull the atoms of one region together: its rows grow well past the
+    # width sized at the first build
+    A = np.array(s.desc.cell[:]).reshape(3, 3, order="F")
+    c0 = A @ np.array([0.5, 0.5, 0.5])
+    d = s.pos - c0
+    near = np.linalg.norm(d, axis=1) < 7.0
+    pos = s.pos.copy()
+    pos[near] = c0 + 0.55 * d[near]
+    note(f"pair sets after row growth: {int(near.sum())} atoms moved")
+    g.set_positions(pos)
+    o.set_positions(pos)
+    g.vnlist_build()
+    o.vnlist_build()
+    n0 = _lists_equal(g, o, 0)
+    n1 = _lists_equal(g, o, 1)
+    assert n0 > 0 and n1 > 0
+
+
 # ------------------------------------------------------------------ FP64 engforce
 
 def _check_engforce(s, tol=TOL64, precision=capi.PREC_FP64, **over):
