@@ -109,6 +109,16 @@ struct FinalSum {
 };
 constexpr int kMaxFinals = 8;
 
+// Queued final sums handed to a launch that runs them in extra blocks (one
+// block each), plus the skin check's completion signal (seq_val written to
+// seq_out by the first of them; the launch starts after the check kernel).
+struct FinalBatch {
+  FinalSum f[kMaxFinals];
+  int nfin;
+  double* seq_out;
+  double seq_val;
+};
+
 // Last step of the reciprocal gather for one atom: the per-split partial
 // moments part[(s * moments + c) * n + i] summed over the splits, then the
 // Cartesian field and force (src/ewald_rec.f90:196-233, scaled as :300-358:
@@ -365,6 +375,9 @@ int reduce_cols_part(mdff_ctx* c, const double* in, int ncol, double* part, int*
 int queue_final(mdff_ctx* c, const double* part, int nb, int ncol, double* out);
 int flush_finals(mdff_ctx* c);
 int claim_partials(mdff_ctx* c, const double* part);
+// the queued finals (and a pending skin-check signal) as a batch for the next
+// launch, which must run them; the queue is left empty
+FinalBatch take_finals(mdff_ctx* c);
 int nl_get_pairs(mdff_ctx* c, int which, int32_t* point, int32_t* list, int64_t cap, int64_t* n);
 void launch_kardir(mdff_ctx* c);
 void pack_atoms(mdff_ctx* c);

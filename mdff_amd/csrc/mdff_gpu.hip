@@ -58,18 +58,9 @@ __global__ void __launch_bounds__(256) k_colsum_final(int nb, int ncol, const do
 // the queued final sums, one block each
 // (+ the skin check's completion signal: seq_val into mapped host memory;
 // this launch starts after the check kernel has completed)
-struct FinalArgs {
-  FinalSum f[kMaxFinals];
-  int nfin;
-  double* seq_out;
-  double seq_val;
-};
-__global__ void __launch_bounds__(256) k_final_sums(FinalArgs a) {
+__global__ void __launch_bounds__(256) k_final_sums(FinalBatch a) {
   __shared__ double sh[4 * kRedMaxCols];
-  if (a.seq_out && blockIdx.x == 0 && threadIdx.x == 0) a.seq_out[0] = a.seq_val;
-  if ((int)blockIdx.x >= a.nfin) return;            // whole block
-  const FinalSum f = a.f[blockIdx.x];
-  sum_block_partials(f.nb, f.ncol, f.part, f.out, sh);
+  run_final_batch(a, blockIdx.x, sh);
 }
 
 // Total force f = f_sr + (f_rec + f_dir) * coul_unit (src/ewald.f90:237-246),
@@ -367,19 +358,23 @@ int reduce_cols_part(mdff_ctx* c, const double* in, int ncol, double* part, int*
   return MDFF_OK;
 }
 
-int flush_finals(mdff_ctx* c) {
-  const bool sig = c->chk_seq_wait == 1;
-  if (c->nfin == 0 && !sig) return MDFF_OK;
-  FinalArgs a{};
+FinalBatch take_finals(mdff_ctx* c) {
+  FinalBatch a{};
   for (int k = 0; k < c->nfin; ++k) a.f[k] = c->fin[k];
   a.nfin = c->nfin;
-  if (sig) {
+  if (c->chk_seq_wait == 1) {
     a.seq_out = c->d_chk + 2 * nblk(c->n);
     a.seq_val = c->chk_seq;
     c->chk_seq_wait = 2;
   }
-  hipLaunchKernelGGL(k_final_sums, dim3(std::max(c->nfin, 1)), dim3(256), 0, c->stream, a);
   c->nfin = 0;
+  return a;
+}
+
+int flush_finals(mdff_ctx* c) {
+  if (c->nfin == 0 && c->chk_seq_wait != 1) return MDFF_OK;
+  const FinalBatch a = take_finals(c);
+  hipLaunchKernelGGL(k_final_sums, dim3(std::max(a.nfin, 1)), dim3(256), 0, c->stream, a);
   HIP_TRY(c, hipGetLastError());
   return MDFF_OK;
 }
@@ -1244,7 +1239,9 @@ int mdff_md_vv(mdff_ctx* c, int32_t nsteps, double dt, double* ekin, double* upo
     // queued, or below) and the next step's position update
     MDFF_TRY(vv_end_launch(c, dt, st + 1 < nsteps, ke_part));
     if (ekin) MDFF_TRY(queue_final(c, ke_part, nblk(n), 1, raw + RAW_EKIN));
-    MDFF_TRY(flush_finals(c));
+    // the step's final sums run in the next step's first launch (the phase
+    // tables), or here before the sums go to the host
+    if (slot == RAW_SLOTS - 1 || st == nsteps - 1) MDFF_TRY(flush_finals(c));
     if (slot == RAW_SLOTS - 1 || st == nsteps - 1) {
       MDFF_TRY(fetch_raw(c, slot + 1));
       for (int k = 0; k <= slot; ++k) {

@@ -111,8 +111,15 @@ struct TabRArgs {
   int2* meta;
 };
 
+// fin_row >= 0: the blocks of row blockIdx.y == fin_row run the queued final
+// sums of the previous MD step instead (block x: sum x).
 template <typename T>
-__global__ void k_tables_r(TabRArgs a) {
+__global__ void __launch_bounds__(256) k_tables_r(TabRArgs a, FinalBatch fb, int fin_row) {
+  if ((int)blockIdx.y == fin_row) {
+    __shared__ double sh[4 * kRedMaxCols];
+    run_final_batch(fb, blockIdx.x, sh);
+    return;
+  }
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   auto dot = [](double x, double y, double z, double u, double v, double w) {
@@ -1671,9 +1678,15 @@ static void tables_r_t(mdff_ctx* c, bool pack) {
   a.itype = c->itype;
   a.rec = c->arec;
   a.meta = c->ameta;
+  // the previous MD step's final sums ride along as one more block row
+  const int nbx = div_up(c->n, 256);
+  if (c->nfin > nbx) (void)flush_finals(c);
+  const FinalBatch fb = take_finals(c);
+  const int runs = tab_runs(c->K);
+  const bool extra = fb.nfin > 0 || fb.seq_out;
   PROF_LAUNCH(c, PK_TABLES,
-              hipLaunchKernelGGL((k_tables_r<T>), dim3(div_up(c->n, 256), tab_runs(c->K)), dim3(256), 0, c->stream,
-                                 a));
+              hipLaunchKernelGGL((k_tables_r<T>), dim3(nbx, runs + (extra ? 1 : 0)), dim3(256), 0, c->stream, a,
+                                 fb, extra ? runs : -1));
 }
 
 int recip_tables_from_r(mdff_ctx* c, bool pack) {

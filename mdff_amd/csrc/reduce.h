@@ -70,3 +70,13 @@ __device__ inline void sum_block_partials(int nb, int ncol, const double* __rest
   const double r = block_sum_cols(v, ncol, sh);
   if ((int)threadIdx.x < ncol) out[threadIdx.x] = r;
 }
+
+// (FinalBatch and FinalSum: common.h, included first.)
+// Block b of a launch that carries a FinalBatch: final sum b (b < nfin) and,
+// in block 0, the completion signal.  Whole blocks (256 threads).
+__device__ inline void run_final_batch(const FinalBatch& fb, int b, double* sh) {
+  if (fb.seq_out && b == 0 && threadIdx.x == 0) fb.seq_out[0] = fb.seq_val;
+  if (b >= fb.nfin) return;
+  const FinalSum f = fb.f[b];
+  sum_block_partials(f.nb, f.ncol, f.part, f.out, sh);
+}
