@@ -119,6 +119,16 @@ struct FinalBatch {
   double seq_val;
 };
 
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs by
+// linear id.  xcd_order maps a linear id to a logical one so that each XCD
+// runs one contiguous range of the logical order (the first nbk % 8 XCDs get
+// one block more): neighbouring logical blocks, which read the same atoms'
+// table rows, then share one XCD's L2.  A bijection of [0, nbk).
+__device__ __forceinline__ int xcd_order(int bid, int nbk) {
+  const int xcd = bid & 7, slot = bid >> 3, per = nbk >> 3, ext = nbk & 7;
+  return (xcd < ext ? xcd * (per + 1) : ext * (per + 1) + (xcd - ext) * per) + slot;
+}
+
 // Last step of the reciprocal gather for one atom: the per-split partial
 // moments part[(s * moments + c) * n + i] summed over the splits, then the
 // Cartesian field and force (src/ewald_rec.f90:196-233, scaled as :300-358:

@@ -915,10 +915,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   const cplx<float>* tab = reinterpret_cast<const cplx<float>*>(a.tab);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int hl = lane >> 5, col = lane & 31;
-  const int zoff = blockIdx.z * (RT * 16);
+  // logical block (m block bx fastest, then chunk by, then z block bz) in
+  // XCD order: the m blocks of one atom chunk run on one XCD
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int lin = xcd_order(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * (int)gridDim.z);
+  const int bx = lin % gx, by = (lin / gx) % gy, bz = lin / (gx * gy);
+  const int zoff = bz * (RT * 16);
   const int zcol = a.Kx + 1 + a.Ky + 1;
   // B column of this lane
-  const int m = (blockIdx.x * 4 + wave) * 32 + col;
+  const int m = (bx * 4 + wave) * 32 + col;
   const bool bok = m < a.M;
   const int nx = bok ? m / a.NY : 0, ny = bok ? m % a.NY - a.Ky : 0;
   const float4* xp = reinterpret_cast<const float4*>(st + (size_t)nx * TAP + 4 * hl);
@@ -934,7 +939,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
     for (int r = 0; r < 16; ++r) tot[rt][r] = 0.f;
-  const int abeg = blockIdx.y * a.chunk;
+  const int abeg = by * a.chunk;
   const int aend = min(a.n, abeg + a.chunk);
   // staging: thread -> atom sia of the tile; Z rows scg + CG u (u < ZU), X/Y
   // columns scg + CG u (u < XU, nxy <= CG XU); the loads of tile t+1 are
@@ -1076,7 +1081,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   // D: lane holds rows 8*(r/4) + 4*hl + r%4 of its column m; row R of tile
   // rt -> nz index zoff + (32 rt + R) / 2, part (32 rt + R) % 2
   if (!bok) return;
-  float* out = a.part + (size_t)blockIdx.y * 2 * a.nk_box + 2 * (size_t)m * a.NZ;
+  float* out = a.part + (size_t)by * 2 * a.nk_box + 2 * (size_t)m * a.NZ;
   const float inv = 1.0f / a.pscale;
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt)
