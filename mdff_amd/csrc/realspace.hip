@@ -31,6 +31,9 @@ struct PairArgs {
   // perpendicular width exceeds twice the list cutoff (no listed pair can sit
   // at a half-integer fractional offset)
   int exact_image;
+  // ortho: the cell matrix is diagonal, so r = diag(A) s (the off-diagonal
+  // products of the general form are exact zeros: the same bits)
+  int ortho;
   double a11, a21, a31, a12, a22, a32, a13, a23, a33;
   double cutsq_c;                    // Coulomb cutoff^2 (d2 > cutsq cycles, src/ewald_dir.f90:202)
   double alpha;
@@ -133,9 +136,16 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
       dys -= rint(dys);
       dzs -= rint(dzs);
     }
-    const double rxd = dxs * a.a11 + dys * a.a12 + dzs * a.a13;
-    const double ryd = dxs * a.a21 + dys * a.a22 + dzs * a.a23;
-    const double rzd = dxs * a.a31 + dys * a.a32 + dzs * a.a33;
+    double rxd, ryd, rzd;
+    if (a.ortho) {
+      rxd = dxs * a.a11;
+      ryd = dys * a.a22;
+      rzd = dzs * a.a33;
+    } else {
+      rxd = dxs * a.a11 + dys * a.a12 + dzs * a.a13;
+      ryd = dxs * a.a21 + dys * a.a22 + dzs * a.a23;
+      rzd = dxs * a.a31 + dys * a.a32 + dzs * a.a33;
+    }
     const double r2d = rxd * rxd + ryd * ryd + rzd * rzd;
     const int jta = mj.x;
     const PairParam& P = spp[ita + a.ntype * jta];
@@ -379,9 +389,16 @@ __global__ void __launch_bounds__(256) k_pair_field(PairArgs a) {
         dys -= rint(dys);
         dzs -= rint(dzs);
       }
-      const double rxd = dxs * a.a11 + dys * a.a12 + dzs * a.a13;
-      const double ryd = dxs * a.a21 + dys * a.a22 + dzs * a.a23;
-      const double rzd = dxs * a.a31 + dys * a.a32 + dzs * a.a33;
+      double rxd, ryd, rzd;
+      if (a.ortho) {
+        rxd = dxs * a.a11;
+        ryd = dys * a.a22;
+        rzd = dzs * a.a33;
+      } else {
+        rxd = dxs * a.a11 + dys * a.a12 + dzs * a.a13;
+        ryd = dxs * a.a21 + dys * a.a22 + dzs * a.a23;
+        rzd = dxs * a.a31 + dys * a.a32 + dzs * a.a33;
+      }
       const double r2d = rxd * rxd + ryd * ryd + rzd * rzd;
       if (r2d > a.cutsq_c) continue;
       const T rx = (T)rxd, ry = (T)ryd, rz = (T)rzd, r2 = (T)r2d;
@@ -435,6 +452,7 @@ PairArgs make_args(mdff_ctx* c) {
   a.a11 = A[0]; a.a21 = A[1]; a.a31 = A[2];
   a.a12 = A[3]; a.a22 = A[4]; a.a32 = A[5];
   a.a13 = A[6]; a.a23 = A[7]; a.a33 = A[8];
+  a.ortho = (A[1] == 0.0 && A[2] == 0.0 && A[3] == 0.0 && A[5] == 0.0 && A[6] == 0.0 && A[7] == 0.0) ? 1 : 0;
   a.cutsq_c = c->rc_coul * c->rc_coul;
   {
     const double rlist = std::max(c->rc_vdw, c->rc_coul) + c->d.skindiff;
