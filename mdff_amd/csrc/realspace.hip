@@ -104,7 +104,12 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
   // f: the pair forces on i, short range + coul_unit x real-space Ewald,
   // combined per pair in T
   double f0 = 0, f1 = 0, f2 = 0, e0 = 0, e1 = 0, e2 = 0;
-  double usr = 0, ts[6] = {0, 0, 0, 0, 0, 0}, ucl = 0, tc[6] = {0, 0, 0, 0, 0, 0};
+  // energies and stresses: summed in T over the lane's share of the row,
+  // then in FP64 across the lanes and atoms (each lane holds a few dozen
+  // pairs, so the T rounding of these global sums stays far below the
+  // force error of the mixed form; the forces are summed in FP64 per pair)
+  using EA = T;
+  EA usr = 0, ts[6] = {0, 0, 0, 0, 0, 0}, ucl = 0, tc[6] = {0, 0, 0, 0, 0, 0};
   const T coul = (T)a.coul;
   const int r0 = a.row[i], r1 = r0 + a.cnt[i];
   // two-stage prefetch of the neighbour chain: the row entry two steps ahead
@@ -193,13 +198,13 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
       pfx = fx;
       pfy = fy;
       pfz = fz;
-      usr += (double)u;
-      ts[0] += (double)(rx * fx);
-      ts[1] += (double)(ry * fy);
-      ts[2] += (double)(rz * fz);
-      ts[3] += (double)(rx * fy);
-      ts[4] += (double)(rx * fz);
-      ts[5] += (double)(ry * fz);
+      usr += (EA)u;
+      ts[0] += (EA)(rx * fx);
+      ts[1] += (EA)(ry * fy);
+      ts[2] += (EA)(rz * fz);
+      ts[3] += (EA)(rx * fy);
+      ts[4] += (EA)(rx * fz);
+      ts[5] += (EA)(ry * fz);
     }
     if (in_cl) {
       const T qj = (T)rj.q;
@@ -284,13 +289,13 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
       pfx -= coul * fij0;
       pfy -= coul * fij1;
       pfz -= coul * fij2;
-      ucl += (double)u;
-      tc[0] -= (double)(T(2) * rx * fij0);
-      tc[1] -= (double)(T(2) * ry * fij1);
-      tc[2] -= (double)(T(2) * rz * fij2);
-      tc[3] -= (double)(rx * fij1 + ry * fij0);
-      tc[4] -= (double)(rx * fij2 + rz * fij0);
-      tc[5] -= (double)(ry * fij2 + rz * fij1);
+      ucl += (EA)u;
+      tc[0] -= (EA)(T(2) * rx * fij0);
+      tc[1] -= (EA)(T(2) * ry * fij1);
+      tc[2] -= (EA)(T(2) * rz * fij2);
+      tc[3] -= (EA)(rx * fij1 + ry * fij0);
+      tc[4] -= (EA)(rx * fij2 + rz * fij0);
+      tc[5] -= (EA)(ry * fij2 + rz * fij1);
     }
     f0 += (double)pfx;
     f1 += (double)pfy;
@@ -306,14 +311,14 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
   }
   double cols[NPA];
   if (a.want_stress) {
-    cols[0] = group_sum<LPA>(usr);
+    cols[0] = group_sum<LPA>((double)usr);
     cols[1] = 0.0;                        // set from the stress trace below
 #pragma unroll
-    for (int k = 0; k < 6; ++k) cols[2 + k] = group_sum<LPA>(ts[k]);
+    for (int k = 0; k < 6; ++k) cols[2 + k] = group_sum<LPA>((double)ts[k]);
     cols[1] = (cols[2] + cols[3]) + cols[4];  // virial w r^2 = trace of r (w r)
-    cols[8] = group_sum<LPA>(ucl);
+    cols[8] = group_sum<LPA>((double)ucl);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) cols[9 + k] = group_sum<LPA>(tc[k]);
+    for (int k = 0; k < 6; ++k) cols[9 + k] = group_sum<LPA>((double)tc[k]);
   }
   if (sub == 0) {
     a.fsr[3 * i] = f0;
