@@ -51,7 +51,7 @@ BOUND_OF = {"k_sk": ("mfma", "fp16"), "k_gather": ("mfma", "fp16"), "k_pair_forc
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (KiB per dispatch x 1024; the factors
 # calibrate the counters on a 256 MiB device-to-device copy), measured with
 # mdff_amd.profile_workload (profiles/engforce_sio2_24k_hbm_traffic.md)
-TRAFFIC = {"k_gather": 65.12e6, "k_sk": 141.82e6, "k_pair_force": 63.68e6}
+TRAFFIC = {"k_gather": 65.12e6, "k_sk": 40.82e6, "k_pair_force": 63.18e6}
 
 
 def mfma_issued(kind: str, natm: int, kmax) -> float:
