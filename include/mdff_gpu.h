@@ -59,8 +59,10 @@ typedef enum mdff_status {
 
 typedef enum mdff_precision {
   MDFF_PREC_FP64 = 0,   /* every operation in FP64 (parity form)                         */
-  MDFF_PREC_MIXED = 1   /* FP32 pair / dipole-field arithmetic, FP64 accumulation,
-                           FP64 minimum image, FP64 k.r phases, FP64 SCF residual        */
+  MDFF_PREC_MIXED = 1   /* FP32 pair / dipole-field arithmetic, FP64 force and field
+                           accumulation (pair energies / stresses: FP32 within one
+                           lane's row share, FP64 beyond), FP64 minimum image,
+                           FP64 k.r phases, FP64 SCF residual                            */
 } mdff_precision;
 
 typedef enum mdff_trunc {  /* src/non_bonded.f90:54-56 ctrunc */
