@@ -209,6 +209,12 @@ struct mdff_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[12] = {};
+  // pair kernel overlapped with the reciprocal kernels (charges, valid
+  // lists): it runs on stream2 after the phase-table launch (ov_tab) and the
+  // force sums wait for it (ov_pair)
+  int overlap = 0;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ov_tab = nullptr, ov_pair = nullptr;
   std::string err;
 
   // cell

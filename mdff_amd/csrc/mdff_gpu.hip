@@ -574,6 +574,14 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   HIP_TRY(c, hipHostGetDevicePointer((void**)&c->d_chk, c->h_chk, 0));
   memset(c->h_chk, 0, sizeof(double) * (2 * (size_t)nblk(n) + 2));
   HIP_TRY(c, hipEventCreateWithFlags(&c->chk_ev, hipEventDisableTiming));
+  HIP_TRY(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+  HIP_TRY(c, hipEventCreateWithFlags(&c->ov_tab, hipEventDisableTiming));
+  HIP_TRY(c, hipEventCreateWithFlags(&c->ov_pair, hipEventDisableTiming));
+  {
+    // MDFF_PAIR_OVERLAP=0 keeps every kernel on the one stream
+    const char* ov = getenv("MDFF_PAIR_OVERLAP");
+    c->overlap = (ov && ov[0] == '0') ? 0 : 1;
+  }
   HIP_TRY(c, hipHostMalloc((void**)&c->h_flag, sizeof(int32_t) * 4));
   const size_t nred = std::max<size_t>(64 * 1024, 2 * (size_t)nblk(n) + 2);
   HIP_TRY(c, hipHostMalloc((void**)&c->h_red, sizeof(double) * nred));
@@ -862,9 +870,12 @@ static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t, bool defer_f
   // atoms packed for the pair kernels by the table kernel (valid unless the
   // lists are rebuilt below)
   bool packed = false;
+  bool ov = false;
   if (recip_first) {
     packed = d.lvnlist && c->lists_valid;
     MDFF_TRY(recip_tables_from_r(c, packed));  // kardir and the phase tables in one launch
+    ov = c->overlap && packed && !dip && !want_field && !c->prof_on;
+    if (ov) HIP_TRY(c, hipEventRecord(c->ov_tab, c->stream));
     if (!c->mu_static) {
       // no induced dipoles: mu is the static set for the life of the context
       HIP_TRY(c, hipMemcpyAsync(c->mu, c->dipia, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
@@ -903,7 +914,20 @@ static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t, bool defer_f
   }
   // short range + real-space Ewald
   ev_rec(c, 2 * EVS_REAL);
-  MDFF_TRY(realspace_force_impl(c, f_sr(c), f_dir(c), ef_dir(c), want_field));
+  if (ov && !rb && !t->list_rebuilt) {
+    // the atoms packed by the table launch and the unchanged lists are all
+    // the pair kernel reads: it runs beside the reciprocal kernels
+    HIP_TRY(c, hipStreamWaitEvent(c->stream2, c->ov_tab, 0));
+    hipStream_t s0 = c->stream;
+    c->stream = c->stream2;
+    const int rc = realspace_force_impl(c, f_sr(c), f_dir(c), ef_dir(c), want_field);
+    c->stream = s0;
+    MDFF_TRY(rc);
+    HIP_TRY(c, hipEventRecord(c->ov_pair, c->stream2));
+    HIP_TRY(c, hipStreamWaitEvent(c->stream, c->ov_pair, 0));
+  } else {
+    MDFF_TRY(realspace_force_impl(c, f_sr(c), f_dir(c), ef_dir(c), want_field));
+  }
   ev_rec(c, 2 * EVS_REAL + 1);
   if (!recip_first) MDFF_TRY(recip_stage());
   {
@@ -1117,6 +1141,9 @@ int mdff_ctx_destroy(mdff_ctx* c) {
   if (c->h_raw) (void)hipHostFree(c->h_raw);
   if (c->h_chk) (void)hipHostFree(c->h_chk);
   if (c->chk_ev) (void)hipEventDestroy(c->chk_ev);
+  if (c->ov_tab) (void)hipEventDestroy(c->ov_tab);
+  if (c->ov_pair) (void)hipEventDestroy(c->ov_pair);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->kO_ws) (void)hipFree(c->kO_ws);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);

@@ -4,6 +4,7 @@ form within 1e-10 relative on energies, forces, stress, fields and induced
 dipoles; mixed form within 1e-5 relative force error (max |dF| / max |F|)."""
 from __future__ import annotations
 
+import copy
 import json
 from pathlib import Path
 
@@ -356,6 +357,31 @@ def test_nhcn_and_csvr_match():
     assert abs(tg[0] - to[0]) < 1e-9 * to[0]
 
 
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_MIXED])
+def test_md_pair_kernel_beside_reciprocal_same_bits(precision, monkeypatch):
+    """MD steps with the pair kernel on its own stream beside the reciprocal
+    kernels (the default) and with every kernel on one stream
+    (MDFF_PAIR_OVERLAP=0): the same kernels on the same inputs, so energies,
+    positions and velocities agree bit for bit, across steps that rebuild
+    the lists (serial) and steps that reuse them (overlapped)."""
+    s = _sio2(natm=900, seed=31, kmax=9, skin=0.3)
+    note(f"pair overlap vs serial sio2 n={s.natm} precision={precision}")
+    runs = []
+    for ov in ("0", "1"):
+        monkeypatch.setenv("MDFF_PAIR_OVERLAP", ov)
+        d = copy.deepcopy(s.desc)
+        d.precision = precision
+        e = capi.gpu_engine(d, s.itype)
+        e.set_positions(s.pos)
+        e.set_velocities(s.vel)
+        e.engforce()
+        en, u = e.md_vv(60, s.dt)
+        runs.append((np.asarray(en), np.asarray(u), e.get_positions(), e.get_velocities()))
+        e.close()
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("case", ["sio2", "pim"])
 def test_mixed_reciprocal_split_half_vs_fp32_matrix_core(case, monkeypatch):
     """The mixed form's reciprocal sums on the split-half (FP16 hi + lo)
@@ -391,7 +417,6 @@ def test_sio2_full_size_invariants_mixed_vs_fp64():
     g64 = capi.gpu_engine(d64, s.itype)
     g64.set_positions(s.pos)
     f64, t64 = g64.engforce()
-    import copy
     d32 = copy.deepcopy(s.desc)
     d32.precision = capi.PREC_MIXED
     g32 = capi.gpu_engine(d32, s.itype)
