@@ -610,8 +610,8 @@ __device__ void pair_pass(const FitArgs& a, const FitSnap& S, const double* mu, 
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_fit(FitArgs a) {
+template <typename T, int LB>
+__global__ void __launch_bounds__(LB) k_fit(FitArgs a) {
   extern __shared__ double red[];                 // blockDim doubles, then staging
   c2<T>* lt = reinterpret_cast<c2<T>*>(red + blockDim.x);            // FIT_CA x ncol_max
   double* la = reinterpret_cast<double*>(lt + (size_t)FIT_CA * a.ncol_max);   // FIT_CA x 7
@@ -1205,7 +1205,9 @@ int mdff_fit_set_targets(mdff_fit* f, const double* fref, const double* tauref, 
 }
 
 int mdff_fit_set_launch(mdff_fit* f, int32_t block, int32_t grid) {
-  if (!f || (block != 32 && block != 64 && block != 128 && block != 256) || grid < 1) return MDFF_ERR_BAD_ARG;
+  if (!f || (block != 32 && block != 64 && block != 128 && block != 256 && block != 512 && block != 1024) ||
+      grid < 1)
+    return MDFF_ERR_BAD_ARG;
   f->block = block;
   f->grid = std::min<int>(grid, f->nsnap);
   return MDFF_OK;
@@ -1286,10 +1288,12 @@ int mdff_fit_eval(mdff_fit* f, const double* theta, int32_t precision, int32_t w
   const size_t shm = sizeof(double) * f->block + tsz * FIT_CA * f->ncol_max + sizeof(double) * 7 * FIT_CA;
   if (precision == MDFF_PREC_MIXED) {
     a.tab = f->d_tab32;
-    hipLaunchKernelGGL((k_fit<float>), dim3(f->grid), dim3(f->block), shm, f->stream, a);
+    if (f->block <= 256) hipLaunchKernelGGL((k_fit<float, 256>), dim3(f->grid), dim3(f->block), shm, f->stream, a);
+    else hipLaunchKernelGGL((k_fit<float, 1024>), dim3(f->grid), dim3(f->block), shm, f->stream, a);
   } else {
     a.tab = f->d_tab64;
-    hipLaunchKernelGGL((k_fit<double>), dim3(f->grid), dim3(f->block), shm, f->stream, a);
+    if (f->block <= 256) hipLaunchKernelGGL((k_fit<double, 256>), dim3(f->grid), dim3(f->block), shm, f->stream, a);
+    else hipLaunchKernelGGL((k_fit<double, 1024>), dim3(f->grid), dim3(f->block), shm, f->stream, a);
   }
   FIT_TRY(f, hipGetLastError());
   std::vector<double> ls(f->nsnap), gr((size_t)FIT_MAXP * f->nsnap);

@@ -143,7 +143,7 @@ class FitProblem:
                          F.reshape(-1, 3) if outputs else None, tau.reshape(-1, 9) if outputs else None,
                          ns.reshape(-1, 2) if outputs else None)
 
-    def autotune(self, theta, precision: int = capi.PREC_MIXED, blocks=(64, 128, 256),
+    def autotune(self, theta, precision: int = capi.PREC_MIXED, blocks=(64, 128, 256, 512, 1024),
                  grids=None, reps: int = 2) -> dict:
         """Pick the workgroup size and count with the lowest measured time of a
         full forward + gradient evaluation; keeps the winner."""

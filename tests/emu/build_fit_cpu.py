@@ -18,10 +18,9 @@ def build() -> Path:
     subs = [
         ('#include "common.h"', '#include "hip_cpu_shim.h"'),
         ("extern __shared__ double red[];", "double* red = g_shared;"),
-        ("hipLaunchKernelGGL((k_fit<float>), dim3(f->grid), dim3(f->block), shm, f->stream, a);",
-         "emu_launch(k_fit<float>, dim3(f->grid), dim3(f->block), a);"),
-        ("hipLaunchKernelGGL((k_fit<double>), dim3(f->grid), dim3(f->block), shm, f->stream, a);",
-         "emu_launch(k_fit<double>, dim3(f->grid), dim3(f->block), a);"),
+        *[(f"hipLaunchKernelGGL((k_fit<{t}, {lb}>), dim3(f->grid), dim3(f->block), shm, f->stream, a);",
+           f"emu_launch((k_fit<{t}, {lb}>), dim3(f->grid), dim3(f->block), a);")
+          for t in ("float", "double") for lb in (256, 1024)],
     ]
     for a, b in subs:
         if a not in src:
