@@ -31,8 +31,10 @@
  *    on these conditions (src/tools.f90:337-340, src/moment_from_pola.f90:176-216,
  *    src/pim.f90:201-204); the C side returns the code and keeps a message
  *    readable through mdff_last_error().
- *  - A context owns one HIP stream on the device current at creation and is
- *    not thread safe (the reference is single-threaded per rank).
+ *  - A context owns two HIP streams (default priority) on the device current
+ *    at creation: its work stream and one on which the pair kernel runs beside
+ *    the reciprocal kernels; every call returns with both joined.  It is not
+ *    thread safe (the reference is single-threaded per rank).
  */
 #ifndef MDFF_GPU_H
 #define MDFF_GPU_H
