@@ -18,6 +18,16 @@ from mdff_amd import capi, configs
 pytestmark = pytest.mark.skipif(not FI.DRIVER.exists(), reason="Fortran driver not built (flang absent)")
 
 
+def test_driver_binary_is_current():
+    """The driver is a build output (not tracked): it must be newer than every
+    Fortran source and the library it links, or the tests below would run a
+    stale program."""
+    t = FI.DRIVER.stat().st_mtime
+    srcs = sorted((FI.ROOT / "mdff_amd" / "fortran").glob("*.f90"))
+    stale = [p.name for p in srcs if p.stat().st_mtime > t]
+    assert not stale, f"{FI.DRIVER} is older than {stale}: run __graft_entry__.build()"
+
+
 def _run(workdir, *args, timeout=120):
     env = dict(os.environ)
     return subprocess.run([str(FI.DRIVER), "control.F", *args], cwd=workdir, capture_output=True, text=True,
