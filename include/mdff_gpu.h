@@ -198,13 +198,16 @@ int mdff_set_itime(mdff_ctx *ctx, int32_t itime);
  * each step r += v dt + f dt^2/2m, engforce, v += (f_old+f) dt/2m, then the
  * skin check (src/runmd.f90:294-304).  dt in internal time units.  ekin/upot
  * (may be NULL) receive per-step kinetic and potential energy (eV), nsteps
- * entries each. */
+ * entries each.  Every step advances the step counter (md%itime,
+ * mdff_set_itime) by one before its force call, as the reference's MD loop
+ * does (src/runmd.f90:257); the ASPC order follows it. */
 int mdff_md_vv(mdff_ctx *ctx, int32_t nsteps, double dt, double *ekin, double *upot);
 
 /* Nose-Hoover chain NVT steps (src/integration.f90:538-701): chain_nhcn,
  * prop_pos_vel_verlet, chain_nhcn.  temp = kB*T (eV), Q from timesca_thermo
  * as in the reference.  xi/vxi: nhc_n thermostat positions/momenta, updated
- * in place.  e_nvt (may be NULL) receives the extended-system energy per step. */
+ * in place.  e_nvt (may be NULL) receives the extended-system energy per step.
+ * Each step advances md%itime by one, as mdff_md_vv. */
 int mdff_md_nhcn(mdff_ctx *ctx, int32_t nsteps, double dt, double temp,
                  double timesca_thermo, int32_t nhc_n, int32_t yosh_order,
                  int32_t nhc_mults, double *xi, double *vxi,

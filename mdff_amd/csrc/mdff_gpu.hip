@@ -1258,6 +1258,7 @@ int mdff_md_vv(mdff_ctx* c, int32_t nsteps, double dt, double* ekin, double* upo
     // the first position update; later ones are made by the previous step's
     // vv_end_launch
     if (st == 0) MDFF_TRY(vv_pos_launch(c, dt));
+    c->itime++;   // md%itime of this step (src/runmd.f90:257), read by the ASPC order
     MDFF_TRY(engforce_launch(c, raw, &held[slot], true));
     double* ke_part = c->fin_part + 1024 * 16;
     MDFF_TRY(claim_partials(c, ke_part));
@@ -1296,13 +1297,33 @@ static int kinetic(mdff_ctx* c, double* kin, double* vsum) {
   return MDFF_OK;
 }
 
+// get_yoshida_weigth (src/integration.f90:1059-1104); order 5 keeps the
+// reference's placement of the middle weight last
+static void yoshida_weights(int order, double w[9]) {
+  for (int k = 0; k < 9; ++k) w[k] = 0.0;
+  w[0] = 1.0;
+  if (order == 3) {
+    w[0] = 1.0 / (2.0 - pow(2.0, 1.0 / 3.0)); w[1] = 1.0 - 2.0 * w[0]; w[2] = w[0];
+  } else if (order == 5) {
+    w[0] = 1.0 / (4.0 - pow(4.0, 1.0 / 3.0)); w[1] = w[2] = w[3] = w[0]; w[4] = 1.0 - 4.0 * w[0];
+  } else if (order == 7) {
+    w[0] = .78451361047756; w[1] = .235573213359357; w[2] = -1.17767998417887;
+    w[3] = 1.0 - 2.0 * (w[0] + w[1] + w[2]);
+    w[4] = w[2]; w[5] = w[1]; w[6] = w[0];
+  } else if (order == 9) {
+    w[0] = 0.192; w[1] = 0.554910818409783619692725006662999;
+    w[2] = 0.124659619941888644216504240951585; w[3] = -0.843182063596933505315033808282941;
+    w[4] = 1.0 - 2.0 * (w[0] + w[1] + w[2] + w[3]);
+    w[5] = w[3]; w[6] = w[2]; w[7] = w[1]; w[8] = w[0];
+  }
+}
+
 // chain_nhcn (src/integration.f90:619-701): thermostat variables on the host,
 // the velocity scale s applied on the device
 static int chain(mdff_ctx* c, double* kin, double* vxi, double* xi, const double* Q, double L, double temp,
                  double dt, int nhc, int yosh, int mults) {
-  double w[9] = {1, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (yosh == 3) { w[0] = 1.0 / (2.0 - pow(2.0, 1.0 / 3.0)); w[1] = 1.0 - 2.0 * w[0]; w[2] = w[0]; }
-  else if (yosh == 5) { w[0] = 1.0 / (4.0 - pow(4.0, 1.0 / 3.0)); w[1] = w[2] = w[3] = w[0]; w[4] = 1.0 - 4.0 * w[0]; }
+  double w[9];
+  yoshida_weights(yosh, w);
   double G[16] = {0};
   double s = 1.0;
   G[0] = 2.0 * (*kin) - L * temp;
@@ -1334,7 +1355,7 @@ static int chain(mdff_ctx* c, double* kin, double* vxi, double* xi, const double
 
 int mdff_md_nhcn(mdff_ctx* c, int32_t nsteps, double dt, double temp, double timesca, int32_t nhc, int32_t yosh,
                  int32_t mults, double* xi, double* vxi, double* ekin, double* upot, double* e_nvt) {
-  if (!c || nhc < 2 || nhc > 16 || !xi || !vxi || (yosh != 1 && yosh != 3 && yosh != 5) || mults < 1)
+  if (!c || nhc < 2 || nhc > 16 || !xi || !vxi || (yosh != 1 && yosh != 3 && yosh != 5 && yosh != 7 && yosh != 9) || mults < 1)
     return MDFF_ERR_BAD_ARG;
   const int n = c->n;
   const double L = 3.0 * (double)n;
@@ -1348,6 +1369,7 @@ int mdff_md_nhcn(mdff_ctx* c, int32_t nsteps, double dt, double temp, double tim
     hipLaunchKernelGGL(k_kick_drift, dim3(nblk(n)), dim3(256), 0, c->stream, n, dt * 0.5, dt, c->mass, c->rx, c->ry,
                        c->rz, c->vx, c->vy, c->vz, c->fx, c->fy, c->fz);
     mdff_terms t;
+    c->itime++;   // md%itime of this step (src/runmd.f90:257)
     MDFF_TRY(engforce(c, &t));
     hipLaunchKernelGGL(k_kick, dim3(nblk(n)), dim3(256), 0, c->stream, n, dt * 0.5, c->mass, c->vx, c->vy, c->vz,
                        c->fx, c->fy, c->fz);

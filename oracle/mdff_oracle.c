@@ -33,14 +33,28 @@
  *                    rule; half list + reversed = full set within rc+skin.
  *  nmlj            - CCD LJ cluster energies (examples/ex02/config/REFERENCE)
  *                    and the 4000-atom FCC sample run (doc/mdff.tex:303-421).
- *  bmhftd / morse  - F = -dU/dr and stress = dU/dstrain by central finite
- *                    differences; Newton's third law.
+ *  bmhftd          - one isolated pair against A e^{-Br} - f6 C/r^6 - f8 D/r^8
+ *                    written from the TT series (energy, force, virial,
+ *                    stress), the A = 0 skip and the strict cutoff; F = -dU/dr
+ *                    and stress = dU/dstrain by central finite differences.
+ *  morse           - harmonic period of a pair near its minimum; FD forces.
  *  kmesh           - closed-form nk (src/coulomb.f90:472); k = 2 pi B n.
  *  ewald q-q, mu-mu- Nymand & Linse values (examples/ex04/run_ex4.sh:145,179);
  *                    NaCl Madelung constant; alpha independence of the total;
  *                    FD forces and stress.
- *  scf             - converged mu against a dense solve (alpha^-1 - T) mu = E.
- *  integrators     - NVE energy conservation; harmonic closed form.
+ *  scf             - converged mu against a dense solve (alpha^-1 - T) mu = E;
+ *                    F = -dU/dr of the converged PIM energy (damped terms).
+ *  damped q-mu     - an isolated charge + polarisable ion: mu -> pol q f_k(br)/r^2
+ *                    after extrapolating three box sizes (1/L^3 image trend).
+ *  aspc            - coefficients equal Kolafa's closed form (J. Comput. Chem.
+ *                    25, 335 (2004)), linear histories predicted exactly; the
+ *                    MD history rules against a numpy restatement step by step.
+ *  integrators     - NVE energy conservation; harmonic closed form; NHC of two
+ *                    conserves the extended energy; longer chains equal the
+ *                    MTK half step with the reference's zeroed middle forces;
+ *                    Yoshida weights: sum w = 1, sum w^3 = 0.
+ *  csvr            - mean and variance of the new kinetic energy over sampled
+ *                    noise (Bussi et al. 2007, eq. A7); COM removal.
  */
 #include "mdff_oracle.h"
 #include <math.h>
@@ -996,20 +1010,28 @@ static double rmsd_mu(orc_ctx *c, const double *E, const double *mu) {
   return sqrt(r / (double)npol);
 }
 
+/* Predictor coefficients B[0..ext] (B[k] multiplies the dipoles of k steps
+ * back) and corrector weight w of the ASPC order ext in use
+ * (src/moment_from_pola.f90:526-562). */
+void orc_aspc_coeffs(int ext, double B[6], double *w) {
+  for (int k = 0; k < 6; ++k) B[k] = 0.0;
+  switch (ext) {
+    case 0: B[0] = 1.0; *w = 0.0; break;
+    case 1: B[0] = 2.0; B[1] = -1.0; *w = 2.0 / 3.0; break;
+    case 2: B[0] = 2.5; B[1] = -2.0; B[2] = 0.5; *w = 0.6; break;
+    case 3: B[0] = 2.8; B[1] = -2.8; B[2] = 1.2; B[3] = -0.2; *w = 4.0 / 7.0; break;
+    default: B[0] = 3.0; B[1] = -24.0 / 7.0; B[2] = 27.0 / 14.0; B[3] = -4.0 / 7.0; B[4] = 1.0 / 14.0; *w = 5.0 / 9.0; break;
+  }
+}
+
 /* extrapolate_dipole_aspc (src/moment_from_pola.f90:493-598) */
 static void aspc(orc_ctx *c, double *mu, const double *E, int key) {
   int n = c->n, ord = c->d.extrapolate_order, ext;
   if (c->itime >= ord + 1) ext = ord;
   else if (c->itime == 1 || c->itime == 0) ext = 0;
   else ext = c->itime - 1;
-  double Bc[6] = {0}, w = 0.0;
-  switch (ext) {
-    case 0: Bc[0] = 1.0; w = 0.0; break;
-    case 1: Bc[0] = 2.0; Bc[1] = -1.0; w = 2.0 / 3.0; break;
-    case 2: Bc[0] = 2.5; Bc[1] = -2.0; Bc[2] = 0.5; w = 0.6; break;
-    case 3: Bc[0] = 2.8; Bc[1] = -2.8; Bc[2] = 1.2; Bc[3] = -0.2; w = 4.0 / 7.0; break;
-    default: Bc[0] = 3.0; Bc[1] = -24.0 / 7.0; Bc[2] = 27.0 / 14.0; Bc[3] = -4.0 / 7.0; Bc[4] = 1.0 / 14.0; w = 5.0 / 9.0; break;
-  }
+  double Bc[6], w;
+  orc_aspc_coeffs(ext, Bc, &w);
   size_t N3 = 3 * (size_t)n;
   if (key == 1) {
     for (size_t x = 0; x < N3; ++x) {
@@ -1270,6 +1292,7 @@ int orc_md_vv(orc_ctx *c, int32_t nsteps, double dt, double *ekin, double *upot)
       c->rz[i] = c->rz[i] + c->vz[i] * dt + (c->fz[i] * dtsq2) / c->mass[i];
     }
     mdff_terms t;
+    c->itime++;   /* md%itime of this step (src/runmd.f90:257) */
     int rc = orc_engforce(c, NULL, NULL, NULL, &t);
     if (rc) { free(fsx); free(fsy); free(fsz); return rc; }
     double kin = 0.0;
@@ -1287,24 +1310,37 @@ int orc_md_vv(orc_ctx *c, int32_t nsteps, double dt, double *ekin, double *upot)
   return MDFF_OK;
 }
 
-static void yoshida(int order, double *w) {
-  /* get_yoshida_weigth (src/integration.f90:1059-1104) */
+/* get_yoshida_weigth (src/integration.f90:1059-1104) */
+void orc_yoshida(int order, double *w) {
   if (order == 3) {
     w[0] = 1.0 / (2.0 - pow(2.0, 1.0 / 3.0)); w[1] = 1.0 - 2.0 * w[0]; w[2] = w[0];
   } else if (order == 5) {
+    /* the reference puts the middle weight last (not a symmetric composition) */
     w[0] = 1.0 / (4.0 - pow(4.0, 1.0 / 3.0)); w[1] = w[0]; w[2] = w[0]; w[3] = w[0]; w[4] = 1.0 - 4.0 * w[0];
+  } else if (order == 7) {
+    w[0] = .78451361047756; w[1] = .235573213359357; w[2] = -1.17767998417887;
+    w[3] = 1.0 - 2.0 * (w[0] + w[1] + w[2]);
+    w[4] = w[2]; w[5] = w[1]; w[6] = w[0];
+  } else if (order == 9) {
+    w[0] = 0.192; w[1] = 0.554910818409783619692725006662999;
+    w[2] = 0.124659619941888644216504240951585; w[3] = -0.843182063596933505315033808282941;
+    w[4] = 1.0 - 2.0 * (w[0] + w[1] + w[2] + w[3]);
+    w[5] = w[3]; w[6] = w[2]; w[7] = w[1]; w[8] = w[0];
   } else {
     w[0] = 1.0;
   }
 }
 
-/* chain_nhcn (src/integration.f90:619-701) */
-static void chain_nhcn(orc_ctx *c, double *kin, double *vxi, double *xi, const double *Q, double L,
-                       double temp, double dt, int nhc, int yosh, int mults) {
+/* chain_nhcn (src/integration.f90:619-701), thermostat part: propagates
+ * (xi, vxi) by one half step for kinetic energy kin and returns the velocity
+ * scale s.  Like the reference, G(2..nhc-1) start at zero on every call (they
+ * are set only by the forward sweep of the first Yoshida sub-step). */
+double orc_nhc_chain(double kin, double *vxi, double *xi, const double *Q, double L, double temp,
+                     double dt, int nhc, int yosh, int mults) {
   double w[9], G[16] = {0};
-  yoshida(yosh, w);
+  orc_yoshida(yosh, w);
   double s = 1.0;
-  G[0] = 2.0 * (*kin) - L * temp;
+  G[0] = 2.0 * kin - L * temp;
   for (int k = 0; k < mults; ++k)
     for (int j = 0; j < yosh; ++j) {
       double dts = (w[j] * dt) / (double)mults;
@@ -1318,7 +1354,7 @@ static void chain_nhcn(orc_ctx *c, double *kin, double *vxi, double *xi, const d
       }
       s = s * exp(-vxi[0] * dts2 / Q[0]);
       for (int h = 0; h < nhc; ++h) xi[h] = xi[h] + vxi[h] * dts2 / Q[h];
-      G[0] = 2.0 * (*kin) * s * s - L * temp;
+      G[0] = 2.0 * kin * s * s - L * temp;
       for (int h = 0; h < nhc - 1; ++h) {
         vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
         vxi[h] = vxi[h] + G[h] * dts4;
@@ -1327,6 +1363,13 @@ static void chain_nhcn(orc_ctx *c, double *kin, double *vxi, double *xi, const d
       }
       vxi[nhc - 1] = vxi[nhc - 1] + G[nhc - 1] * dts4;
     }
+  return s;
+}
+
+/* chain_nhcn: the thermostat half step, then the velocities scaled by s */
+static void chain_nhcn(orc_ctx *c, double *kin, double *vxi, double *xi, const double *Q, double L,
+                       double temp, double dt, int nhc, int yosh, int mults) {
+  double s = orc_nhc_chain(*kin, vxi, xi, Q, L, temp, dt, nhc, yosh, mults);
   double k2 = 0.0;
   for (int i = 0; i < c->n; ++i) {
     c->vx[i] = s * c->vx[i]; c->vy[i] = s * c->vy[i]; c->vz[i] = s * c->vz[i];
@@ -1365,6 +1408,7 @@ int orc_md_nhcn(orc_ctx *c, int32_t nsteps, double dt, double temp, double times
       c->rz[i] = c->rz[i] + c->vz[i] * dt;
     }
     mdff_terms t;
+    c->itime++;   /* md%itime of this step (src/runmd.f90:257) */
     int rc = orc_engforce(c, NULL, NULL, NULL, &t);
     if (rc) return rc;
     for (int i = 0; i < n; ++i) {

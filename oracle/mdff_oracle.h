@@ -56,6 +56,15 @@ int    orc_multipole_es(orc_ctx *c, const double *mu, const int32_t task[3], int
                         double *ef, double *fx, double *fy, double *fz, double tau_coul[9],
                         double *u_dir, double *u_rec, double *u_self, double *u_surf);
 
+/* ASPC predictor coefficients and corrector weight of order ext (0..4) */
+void   orc_aspc_coeffs(int ext, double B[6], double *w);
+/* Yoshida-Suzuki weights of order 1, 3, 5, 7 or 9 (w[0..order)) */
+void   orc_yoshida(int order, double *w);
+/* thermostat half step of chain_nhcn for kinetic energy kin; returns the
+ * velocity scale */
+double orc_nhc_chain(double kin, double *vxi, double *xi, const double *Q, double L, double temp,
+                     double dt, int nhc, int yosh, int mults);
+
 /* induced-dipole field operator E_ind(mu) (charges zero, undamped) */
 int    orc_dipole_field(orc_ctx *c, const double *mu, double *ef);
 

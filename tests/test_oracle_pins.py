@@ -421,3 +421,449 @@ def test_harmonic_closed_form_vv():
     xs = np.array(xs)
     assert abs(xs[-1] - amp) < 2e-2 * amp  # back after one period
     assert abs(xs[199] + amp) < 2e-2 * amp  # opposite after half a period
+
+
+# ---------------------------------------------------------------- BMH(FTD) pair value
+
+def _tt(n, x):
+    """Tang-Toennies f_n(x) = 1 - e^{-x} sum_{k<=n} x^k/k! (c = 1), written
+    from the series (Tang & Toennies, J. Chem. Phys. 80, 3726 (1984))."""
+    return 1.0 - math.exp(-x) * sum(x ** k / math.factorial(k) for k in range(n + 1))
+
+
+def _pair_potential(r, A, B, C, D, BD):
+    """A e^{-Br} - f6 C/r^6 - f8 D/r^8 (f = 1 when BD is None)."""
+    f6 = 1.0 if BD is None else _tt(6, BD * r)
+    f8 = 1.0 if BD is None else _tt(8, BD * r)
+    return A * math.exp(-B * r) - f6 * C / r ** 6 - f8 * D / r ** 8
+
+
+def _isolated_pair(r, L=30.0, damped=False, A=18003.7572, B=4.87318, C=33.4, D=0.0, BD=0.0):
+    d = configs._base(2, 2, configs.cubic(L), [28.0855, 15.9994], [0.0, 0.0])
+    d.lbmhftd = 1 if damped else 0
+    d.lbmhft = 0 if damped else 1
+    d.lcoulomb = 0
+    d.cutshortrange = 9.0
+    for name, v in (("Abmhftd", A), ("Bbmhftd", B), ("Cbmhftd", C), ("Dbmhftd", D), ("BDbmhftd", BD)):
+        set_pair(d, name, 1, 2, v)
+    pos = np.array([[3.0, 4.0, 5.0], [3.0 + r, 4.0, 5.0]])
+    e = O.oracle(d, np.array([1, 2], np.int32))
+    e.set_positions(pos)
+    f, t = e.engforce()
+    return f, t, L ** 3
+
+
+@pytest.mark.parametrize("damped,r,params", [
+    (False, 1.6, dict(A=18003.7572, B=4.87318, C=33.4, D=0.0)),          # BKS-type Si-O contact
+    (False, 2.9, dict(A=1388.7730, B=2.76, C=43.75, D=0.0)),             # O-O
+    (True, 2.2, dict(A=1500.0, B=3.1, C=20.0, D=50.0, BD=3.5)),          # BMHFTD, damping active
+    (True, 4.7, dict(A=1500.0, B=3.1, C=20.0, D=50.0, BD=1.2)),
+])
+def test_bmh_pair_energy_force_stress_closed_form(damped, r, params):
+    """engforce_bmhftd_pbc (src/non_bonded.f90:1156-1205) on one pair against
+    V(r) = A e^{-Br} - f6(BD r) C/r^6 - f8(BD r) D/r^8 written from the
+    series: energy, force -dV/dr (five-point difference of the closed form),
+    virial r dV/dr / 3 and tau_xx = -r dV/dr / (V press_unit)."""
+    BD = params.get("BD") if damped else None
+    f, t, V = _isolated_pair(r, damped=damped, **params)
+    u = _pair_potential(r, params["A"], params["B"], params["C"], params["D"], BD)
+    assert abs(t["u_bmhft"] - u) <= 1e-12 * max(1.0, abs(u))
+    h = 1e-3
+    g = lambda x: _pair_potential(x, params["A"], params["B"], params["C"], params["D"], BD)  # noqa: E731
+    dvdr = (-g(r + 2 * h) + 8 * g(r + h) - 8 * g(r - h) + g(r - 2 * h)) / (12 * h)
+    # atom 2 sits at +r along x: F_2x = -dV/dr, Newton's third law on atom 1
+    assert abs(f[1, 0] - (-dvdr)) <= 1e-8 * max(1.0, abs(dvdr))
+    assert abs(f[0, 0] + f[1, 0]) <= 1e-12 * max(1.0, abs(dvdr))
+    assert np.abs(f[:, 1:]).max() == 0.0
+    assert abs(t["vir_bmhft"] - (-dvdr * r / 3.0)) <= 1e-8 * max(1.0, abs(dvdr * r))
+    tau = t["tau_nonb"]
+    assert abs(tau[0, 0] - (-dvdr * r / V / PRESS_UNIT)) <= 1e-8 * max(1.0, abs(tau[0, 0]))
+    assert np.abs(tau - np.diag(np.diag(tau))).max() == 0.0 and tau[1, 1] == 0.0
+
+
+def test_bmh_pair_skipped_when_A_is_zero_and_beyond_cutoff():
+    """A = 0 pairs are skipped whatever C (src/non_bonded.f90:1153); pairs at
+    or beyond the cutoff contribute nothing (strict r^2 < rc^2, :1154)."""
+    f, t, _ = _isolated_pair(2.5, A=0.0, C=40.0)
+    assert t["u_bmhft"] == 0.0 and np.abs(f).max() == 0.0
+    f, t, _ = _isolated_pair(9.5, A=1000.0, C=40.0)
+    assert t["u_bmhft"] == 0.0 and np.abs(f).max() == 0.0
+    f, t, _ = _isolated_pair(8.5, A=1000.0, B=2.0, C=40.0)
+    assert abs(t["u_bmhft"] - _pair_potential(8.5, 1000.0, 2.0, 40.0, 0.0, None)) < 1e-15
+
+
+# ---------------------------------------------------------------- damped charge -> dipole field
+
+def _charge_and_polarisable(L, r, pol=1.5, b=2.0, c=0.8, k=4, q=1.0):
+    """A charge q (type 1) and a neutral polarisable ion (type 2) a distance r
+    apart along x in a cubic box L; the field at the ion from the charge is
+    Tang-Toennies damped (b, c, order k).  Ewald converged to ~1e-12."""
+    d = configs._base(2, 2, configs.cubic(L), [10.0, 10.0], [q, 0.0])
+    d.lcoulomb = 1
+    d.cutlongrange = L / 2
+    d.alphaES = 0.4
+    d.kES[:] = [int(np.ceil(5.2 * 0.4 * L / np.pi))] * 3
+    d.skindiff = 0.1
+    configs.set_pim(d, polar_type=2, pol=pol, tt_b=b, tt_c=c, tt_k=k, ntype=2)
+    d.conv_tol_ind = 1e-13
+    d.max_scf_pol_iter = 200
+    d.calc_md = 0
+    e = O.oracle(d, np.array([1, 2], np.int32))
+    e.set_positions(np.array([[1.0, 1.0, 1.0], [1.0 + r, 1.0, 1.0]]))
+    f, t = e.engforce()
+    mu, ef = e.get_dipoles()
+    return mu[1], f, t
+
+
+def test_damped_charge_dipole_field_closed_form_and_image_trend():
+    """The SCF static field is the TT-damped charge field (src/ewald_dir.f90:216-260,
+    src/moment_from_pola.f90:327-330): for an isolated charge-polarisable pair
+    mu -> pol q f_k(br)/r^2 along the bond, with f_k = 1 - c e^{-br} sum (br)^j/j!.
+    The periodic images add a 1/L^3 term: three box sizes fix it and the limit."""
+    r, pol, b, c, k, q = 2.0, 1.5, 2.0, 0.8, 4, 1.0
+    Ls = [24.0, 32.0, 40.0, 48.0]
+    mus = [_charge_and_polarisable(L, r, pol, b, c, k, q)[0] for L in Ls]
+    for m in mus:
+        assert abs(m[1]) < 1e-12 and abs(m[2]) < 1e-12
+    mx = np.array([m[0] for m in mus])
+    Lv = np.array(Ls)
+    # mu(L) = mu_inf + a / L^3 + b / L^5 from three boxes, predicted for the fourth
+    M = np.stack([np.ones(3), Lv[:3] ** -3, Lv[:3] ** -5], 1)
+    mu_inf, a3, a5 = np.linalg.solve(M, mx[:3])
+    fk = 1.0 - c * math.exp(-b * r) * sum((b * r) ** j / math.factorial(j) for j in range(k + 1))
+    want = pol * q * fk / r ** 2
+    assert abs(mu_inf - want) < 1e-7 * want, (mu_inf, want)
+    assert a3 < 0 and abs(mu_inf + a3 * Lv[3] ** -3 + a5 * Lv[3] ** -5 - mx[3]) < 1e-3 * abs(mx[3] - want)
+    # without damping the same pair gives the bare field (c = 0 -> f = 1)
+    m0 = np.array([_charge_and_polarisable(L, r, pol, b, 0.0, k, q)[0][0] for L in Ls[:3]])
+    bare = np.linalg.solve(M, m0)[0]
+    assert abs(bare - pol * q / r ** 2) < 1e-7 * pol * q / r ** 2
+
+
+def test_pim_forces_are_minus_gradient_of_converged_energy():
+    """At SCF convergence the energy (u_bmhft + u_coul, u_pol included,
+    src/engforce_driver.f90:146-179) is stationary in mu, so the forces with
+    the damped charge-dipole terms (src/ewald_dir.f90:599-644) are -dU/dr."""
+    s = _pim_small(seed=9)
+    s.desc.calc_md = 0
+    s.desc.conv_tol_ind = 1e-13
+    s.desc.max_scf_pol_iter = 400
+    _, f, t = _energy(s)
+    rng = np.random.default_rng(2)
+    h = 1e-5
+    for ia in rng.choice(s.natm, 4, replace=False):
+        for a in range(3):
+            p = s.pos.copy()
+            p[ia, a] += h
+            up = _energy(_pim_small(seed=9), p, calc_md=0, conv_tol_ind=1e-13, max_scf_pol_iter=400)[2]
+            p[ia, a] -= 2 * h
+            um = _energy(_pim_small(seed=9), p, calc_md=0, conv_tol_ind=1e-13, max_scf_pol_iter=400)[2]
+            fd = -((up["u_bmhft"] + up["u_coul"]) - (um["u_bmhft"] + um["u_coul"])) / (2 * h)
+            assert abs(fd - f[ia, a]) < 2e-6 * max(1.0, abs(f[ia, a])), (ia, a, fd, f[ia, a])
+
+
+# ---------------------------------------------------------------- ASPC
+
+def _kolafa(k):
+    """Always-stable predictor-corrector of Kolafa (J. Comput. Chem. 25, 335
+    (2004), eqs. 11-12): B_j = (-1)^{j+1} j C(2k+4, k+2-j) / C(2k+2, k+1),
+    j = 1..k+2, and omega = (k+2)/(2k+3)."""
+    B = [(-1) ** (j + 1) * j * math.comb(2 * k + 4, k + 2 - j) / math.comb(2 * k + 2, k + 1)
+         for j in range(1, k + 3)]
+    return B, (k + 2) / (2 * k + 3)
+
+
+def test_aspc_coefficients_are_kolafas():
+    """extrapolate_dipole_aspc (src/moment_from_pola.f90:526-562): order ext >= 1
+    is Kolafa's k = ext - 1; order 0 keeps the last dipoles (B = 1, no
+    corrector).  Every predictor reproduces a linear history exactly."""
+    L = O.lib()
+    B = (C.c_double * 6)()
+    w = C.c_double()
+    L.orc_aspc_coeffs.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.orc_aspc_coeffs(0, B, C.byref(w))
+    assert list(B) == [1.0, 0, 0, 0, 0, 0] and w.value == 0.0
+    for ext in range(1, 5):
+        L.orc_aspc_coeffs(ext, B, C.byref(w))
+        kb, kw = _kolafa(ext - 1)
+        assert np.allclose(list(B)[:ext + 1], kb, rtol=1e-15, atol=1e-15), ext
+        assert all(x == 0.0 for x in list(B)[ext + 1:])
+        assert abs(w.value - kw) < 1e-15
+        # linear history x(t - j) = a - b j, j = 1..ext+1: predicts x(t) = a
+        jj = np.arange(1, ext + 2)
+        assert abs(np.dot(list(B)[:ext + 1], 3.0 - 0.7 * jj) - 3.0) < 1e-13
+
+
+def test_aspc_history_in_md_follows_the_reference_rules():
+    """One SCF iteration per step (min = max = 1) makes every step's dipoles
+    the ASPC predictor + corrector itself: mu_t = w pol (E_s + T mu_p) +
+    (1 - w) mu_p, mu_p = sum_k B_k h_k.  A numpy restatement of the history
+    rules (order from itime, src/moment_from_pola.f90:514-522; shift of the
+    first ext entries, :566-575; h_0 <- mu after the solve, :465) with
+    Kolafa's coefficients and the oracle's field operators reproduces the
+    oracle's MD dipoles step by step."""
+    s = _pim_small(seed=11)
+    d = s.desc
+    d.extrapolate_order = 4
+    d.min_scf_pol_iter = 1
+    d.max_scf_pol_iter = 1
+    d.calc_md = 1
+    e = O.oracle(d, s.itype)
+    e.set_positions(s.pos)
+    e.set_velocities(s.vel * 3.0)
+    e.set_itime(0)
+    e.engforce()
+    probe = O.oracle(_pim_small(seed=11).desc, s.itype)
+    L = O.lib()
+    dp = C.POINTER(C.c_double)
+    n = s.natm
+    pol = 1.2
+    polar = (s.itype == 2)[:, None]
+    hist = np.zeros((5, n, 3))
+
+    def fields(pos, mu):
+        probe.set_positions(pos)
+        es = np.zeros((n, 3))
+        zero = np.zeros((n, 3))
+        fx, fy, fz, tau = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(9)
+        ud, ur, us, usf = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        L.orc_multipole_es(probe.h, zero.ctypes.data_as(dp), (C.c_int32 * 3)(1, 0, 0), 1,
+                           es.ctypes.data_as(dp), fx.ctypes.data_as(dp), fy.ctypes.data_as(dp),
+                           fz.ctypes.data_as(dp), tau.ctypes.data_as(dp), C.byref(ud), C.byref(ur),
+                           C.byref(us), C.byref(usf))
+        ei = np.zeros((n, 3))
+        m = np.ascontiguousarray(mu)
+        L.orc_dipole_field(probe.h, m.ctypes.data_as(dp), ei.ctypes.data_as(dp))
+        return es, ei
+
+    def step(itime, pos):
+        ext = 4 if itime >= 5 else (0 if itime in (0, 1) else itime - 1)
+        if ext == 0:
+            B, w = [1.0], 0.0
+        else:
+            B, w = _kolafa(ext - 1)
+        mu_p = sum(B[k] * hist[k] for k in range(ext + 1))
+        for k in range(ext, 0, -1):
+            hist[k] = hist[k - 1]
+        es, ei = fields(pos, mu_p)
+        mu = w * pol * (es + ei) * polar + (1.0 - w) * mu_p
+        hist[0] = mu
+        return mu
+
+    mu0 = step(0, s.pos)
+    assert np.allclose(e.get_dipoles()[0], mu0, atol=1e-12)
+    for it in range(1, 9):
+        e.md_vv(1, s.dt)
+        mu = step(it, e.get_positions())
+        got = e.get_dipoles()[0]
+        assert np.abs(got - mu).max() < 1e-11 * max(1.0, np.abs(mu).max()), it
+
+
+# ---------------------------------------------------------------- NHC-n, Yoshida
+
+def test_yoshida_weights_cancel_the_odd_error_terms():
+    """get_yoshida_weigth (src/integration.f90:1059-1104): the Suzuki-Yoshida
+    compositions have sum w = 1 and sum w^3 = 0 (third-order error term
+    cancelled; order 7 and 9 also sum w^5 = 0).  Orders 3, 7, 9 are
+    palindromes; the reference's order 5 places the middle weight 1 - 4 w1 last."""
+    L = O.lib()
+    L.orc_yoshida.argtypes = [C.c_int, C.POINTER(C.c_double)]
+    for order in (3, 5, 7, 9):
+        w = (C.c_double * 9)()
+        L.orc_yoshida(order, w)
+        ww = np.array(list(w)[:order])
+        assert abs(ww.sum() - 1.0) < 1e-15
+        assert abs((ww ** 3).sum()) < 1e-13
+        if order >= 7:
+            assert abs((ww ** 5).sum()) < 1e-13
+        if order == 5:
+            assert np.all(ww[:4] == ww[0]) and ww[4] == 1.0 - 4.0 * ww[0]
+        else:
+            assert np.allclose(ww, ww[::-1], rtol=0, atol=0)
+    w = (C.c_double * 9)()
+    L.orc_yoshida(1, w)
+    assert w[0] == 1.0
+
+
+def _morse_fluid(seed=1, temp_K=240.0, n=4):
+    """Argon-like Morse fluid (fcc start, 256 atoms): a smooth pair potential
+    whose cutoff jump is ~1e-6 eV, so energy conservation is limited by the
+    time step alone."""
+    a = 5.3
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    cells = np.array([[i, j, k] for i in range(n) for j in range(n) for k in range(n)], float)
+    pos = (cells[:, None, :] + base[None]).reshape(-1, 3) * a
+    N = len(pos)
+    d = configs._base(N, 1, configs.cubic(n * a), [39.948], [0.0])
+    d.lmorse = 1
+    d.cutshortrange = n * a / 2 - 0.01
+    d.skindiff = 0.3
+    set_pair(d, "epsmor", 1, 1, 0.0104)
+    set_pair(d, "sigmamor", 1, 1, 3.75)
+    set_pair(d, "rhomor", 1, 1, 1.3)
+    rng = np.random.default_rng(seed)
+    v = configs.maxwell_boltzmann(rng, np.full(N, 39.948), temp_K)
+    return d, np.ones(N, np.int32), pos + rng.normal(0, 0.05, pos.shape), v
+
+
+KB = 8.61734229648141e-05
+TU = 98.2269514139276
+
+
+@pytest.mark.parametrize("yosh,mults", [(1, 1), (3, 1), (5, 1), (7, 2), (9, 1)])
+def test_nhc2_conserves_the_extended_energy(yosh, mults):
+    """nose_hoover_chain_n + chain_nhcn (src/integration.f90:538-701) with a
+    chain of two: K + U + sum p_xi^2/2Q + L kT xi_1 + kT xi_2 is a constant of
+    the motion (to the time-step error of plain NVE on the same fluid, ~2e-4
+    of the energy exchanged), while K relaxes from 240 K to the 120 K bath."""
+    d, it, pos, v = _morse_fluid()
+    e = O.oracle(d, it)
+    e.set_positions(pos)
+    e.set_velocities(v)
+    e.engforce()
+    xi, vxi = np.zeros(2), np.zeros(2)
+    ek, up, env, xi, vxi = e.md_nhcn(400, 0.002 * TU, 120.0 * KB, 0.1 * TU, xi, vxi, yosh, mults)
+    H = ek + up + env
+    spread = np.ptp(ek)
+    assert ek[-20:].mean() < 0.55 * ek[:20].mean()          # the bath took the energy
+    assert np.ptp(H) < 5e-4 * spread, (np.ptp(H), spread)
+    assert abs(H[:50].mean() - H[-50:].mean()) < 5e-4 * spread
+
+
+def _mtk_half_step(kin, vxi, xi, Q, L, kT, dt, w, mults, zero_middle=True):
+    """Thermostat half step exp(iL_NHC dt/2) of Martyna, Tuckerman, Tobias,
+    Klein (Mol. Phys. 87, 1117 (1996), sec. 4) with Suzuki-Yoshida weights w
+    and mults sub-steps.  zero_middle: the chain forces G_2..G_{M-1} start at
+    zero instead of (p_{j-1}^2/Q_{j-1} - kT)."""
+    M = len(vxi)
+    vxi, xi = vxi.copy(), xi.copy()
+    G = np.zeros(M)
+    G[0] = 2 * kin - L * kT
+    if not zero_middle:
+        for j in range(1, M):
+            G[j] = vxi[j - 1] ** 2 / Q[j - 1] - kT
+    s = 1.0
+    for _ in range(mults):
+        for wj in w:
+            dl = wj * dt / mults
+            G[M - 1] = vxi[M - 2] ** 2 / Q[M - 2] - kT
+            vxi[M - 1] += G[M - 1] * dl / 4
+            for j in range(M - 2, -1, -1):
+                f = math.exp(-vxi[j + 1] * dl / 8 / Q[j + 1])
+                vxi[j] = (vxi[j] * f + G[j] * dl / 4) * f
+            s *= math.exp(-vxi[0] * dl / 2 / Q[0])
+            xi += vxi * dl / 2 / Q
+            G[0] = 2 * kin * s * s - L * kT
+            for j in range(M - 1):
+                f = math.exp(-vxi[j + 1] * dl / 8 / Q[j + 1])
+                vxi[j] = (vxi[j] * f + G[j] * dl / 4) * f
+                G[j + 1] = vxi[j] ** 2 / Q[j] - kT
+            vxi[M - 1] += G[M - 1] * dl / 4
+    return s, vxi, xi
+
+
+def test_nhc_longer_chains_keep_the_references_zeroed_middle_forces():
+    """For chains of three or more the reference's chain_nhcn allocates G = 0
+    on every call and sets G_2..G_{M-1} only in the forward sweep
+    (src/integration.f90:640-690), so the first backward sweep pushes the
+    middle thermostats with zero force.  The oracle's half step equals the
+    MTK half step with that zeroing, not without it; the extended energy of a
+    three-chain then drifts by orders of magnitude more than with two."""
+    L_ = O.lib()
+    dp = C.POINTER(C.c_double)
+    L_.orc_nhc_chain.argtypes = [C.c_double, dp, dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                 C.c_int]
+    L_.orc_nhc_chain.restype = C.c_double
+    L_.orc_yoshida.argtypes = [C.c_int, dp]
+    rng = np.random.default_rng(3)
+    kT, L = 0.02, 300.0
+    for M, yosh, mults in [(3, 3, 2), (4, 1, 1), (5, 5, 1)]:
+        Q = np.full(M, 0.7)
+        Q[0] *= L
+        vxi = rng.normal(0, 0.3, M)
+        xi = rng.normal(0, 0.1, M)
+        kin = 4.1
+        w = (C.c_double * 9)()
+        L_.orc_yoshida(yosh, w)
+        want_s, want_v, want_x = _mtk_half_step(kin, vxi, xi, Q, L, kT, 0.4, list(w)[:yosh], mults)
+        alt_s, alt_v, _ = _mtk_half_step(kin, vxi, xi, Q, L, kT, 0.4, list(w)[:yosh], mults, zero_middle=False)
+        v, x = vxi.copy(), xi.copy()
+        got = L_.orc_nhc_chain(kin, v.ctypes.data_as(dp), x.ctypes.data_as(dp), Q.ctypes.data_as(dp), L, kT,
+                               0.4, M, yosh, mults)
+        assert abs(got - want_s) < 1e-14 and np.allclose(v, want_v, rtol=1e-13, atol=1e-15)
+        assert np.allclose(x, want_x, rtol=1e-13, atol=1e-15)
+        assert np.abs(v - alt_v).max() > 1e-6           # the zeroing is visible
+    d, it, pos, vel = _morse_fluid()
+    e = O.oracle(d, it)
+    e.set_positions(pos)
+    e.set_velocities(vel)
+    e.engforce()
+    ek, up, env, _, _ = e.md_nhcn(400, 0.002 * TU, 120.0 * KB, 0.1 * TU, np.zeros(3), np.zeros(3), 1, 1)
+    assert np.ptp(ek + up + env) > 1e-2 * np.ptp(ek)
+
+
+# ---------------------------------------------------------------- CSVR
+
+def _csvr_system():
+    s = configs.nacl(ncell=2, seed=6)
+    e = O.oracle(s.desc, s.itype)
+    e.set_positions(s.pos)
+    v = s.vel - s.vel.mean(axis=0)       # zero mean velocity: the removal after the rescale is then a no-op
+    return s, e, v
+
+
+def _kin(e):
+    return e.calc_temp()[1]
+
+
+@pytest.mark.parametrize("taut", [0.0, 5.0])
+def test_csvr_samples_bussis_kinetic_energy_distribution(taut):
+    """rescale_velocities with lcsvr + resamplekin (src/kinetic.f90:122-198,
+    :586-635; Bussi, Donadio, Parrinello, J. Chem. Phys. 126, 014101 (2007),
+    eq. A7): with r1 ~ N(0,1) and sumnoise ~ chi^2(ndeg - 1), the new kinetic
+    energy has mean K + (1 - f)(sigma - K) and variance
+    2 (1-f)^2 sigma^2 / ndeg + 4 K sigma (1-f) f / ndeg, f = exp(-1/taut)
+    (f = 0 for taut <= 0.1: K' ~ Gamma(ndeg/2, kT)); conint accumulates -(K' - K)."""
+    s, e, v = _csvr_system()
+    n = s.natm
+    ndeg = 3 * n - 3
+    kT = 300.0 * 8.61734229648141e-05
+    sigma = ndeg * kT * 0.5
+    rng = np.random.default_rng(17)
+    e.set_velocities(v * 1.3)
+    K = _kin(e)
+    f = math.exp(-1.0 / taut) if taut > 0.1 else 0.0
+    ks = []
+    for _ in range(3000):
+        e.set_velocities(v * 1.3)
+        r1 = float(rng.standard_normal())
+        sn = float(rng.chisquare(ndeg - 1))
+        ci = e.csvr_rescale(kT, taut, r1, sn, 0.0)
+        k2 = _kin(e)
+        assert abs(ci + (k2 - K)) < 1e-10 * K
+        ks.append(k2)
+    ks = np.array(ks)
+    mean = K + (1 - f) * (sigma - K)
+    var = 2 * (1 - f) ** 2 * sigma ** 2 / ndeg + 4 * K * sigma * (1 - f) * f / ndeg
+    se = math.sqrt(var / len(ks))
+    assert abs(ks.mean() - mean) < 4.5 * se, (ks.mean(), mean, se)
+    assert abs(ks.var() / var - 1.0) < 0.12, (ks.var(), var)
+
+
+def test_csvr_removes_centre_of_mass_velocity():
+    """After the rescale the velocity sum is zero (src/kinetic.f90:165-180)
+    and, for a zero-momentum start, K' is the closed form of eq. A7."""
+    s, e, v = _csvr_system()
+    vv = v + np.array([0.01, -0.02, 0.005])
+    e.set_velocities(vv)
+    e.csvr_rescale(0.02, 2.0, 0.3, 180.0, 0.0)
+    assert np.abs(e.get_velocities().sum(0)).max() < 1e-15 * s.natm
+    e.set_velocities(v)
+    K = _kin(e)
+    ndeg = 3 * s.natm - 3
+    kT, taut, r1, sn = 0.02, 2.0, 0.3, 180.0
+    sigma = ndeg * kT / 2
+    f = math.exp(-1.0 / taut)
+    want = K + (1 - f) * (sigma * (sn + r1 ** 2) / ndeg - K) + 2 * r1 * math.sqrt(K * sigma / ndeg * (1 - f) * f)
+    e.csvr_rescale(kT, taut, r1, sn, 0.0)
+    assert abs(_kin(e) - want) < 1e-12 * want
