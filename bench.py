@@ -145,7 +145,61 @@ def drift_per_atom_per_ns(ek, up, natm, dt_fs):
     return float(np.polyfit(t_ns, H, 1)[0])
 
 
-def cpu_baseline(s, threads: int):
+def nve_drift_leg(s, equil: int, steps: int) -> dict:
+    """NVE energy drift of both forms, outside the timed region: from the
+    workload's start, `equil` velocity-Verlet steps, then `steps` more whose
+    total energy H(t) = K + U is fitted by a line; the slope per atom per ns
+    of each form and their ratio (mixed over FP64)."""
+    out = {"equil_steps": equil, "steps": steps, "dt_fs": 1.0}
+    for prec, tag in ((capi.PREC_FP64, "fp64"), (capi.PREC_MIXED, "mixed")):
+        d = copy.deepcopy(s.desc)
+        d.precision = prec
+        eng = capi.gpu_engine(d, s.itype)
+        eng.set_positions(s.pos)
+        eng.set_velocities(s.vel)
+        eng.engforce()
+        if equil:
+            eng.md_vv(equil, s.dt)
+        ek, up = eng.md_vv(steps, s.dt)
+        eng.close()
+        H = (ek + up) / s.natm
+        out[tag] = {"drift_eV_per_atom_per_ns": drift_per_atom_per_ns(ek, up, s.natm, 1.0),
+                    "H_rms_fluct_eV_per_atom": float(np.std(H)), "H_first_last_eV_per_atom": [float(H[0]), float(H[-1])]}
+    d64 = out["fp64"]["drift_eV_per_atom_per_ns"]
+    d32 = out["mixed"]["drift_eV_per_atom_per_ns"]
+    out["ratio_mixed_over_fp64"] = abs(d32) / abs(d64) if d64 else None
+    return out
+
+
+def term_errors(s) -> dict:
+    """Force error of the mixed form against the FP64 form at the workload's
+    start, with the short-range term alone (Coulomb off) and the Coulomb
+    terms alone (Buckingham off), each normalised by the RMS of the full
+    FP64 force."""
+    def forces(prec, **over):
+        d = copy.deepcopy(s.desc)
+        d.precision = prec
+        for k, v in over.items():
+            setattr(d, k, v)
+        eng = capi.gpu_engine(d, s.itype)
+        eng.set_positions(s.pos)
+        f, _ = eng.engforce()
+        eng.close()
+        return f
+    rms = float(np.sqrt(np.mean(np.sum(forces(capi.PREC_FP64) ** 2, axis=1))))
+    out = {}
+    for tag, over in (("short_range_only", {"lcoulomb": 0}), ("coulomb_only", {"lbmhft": 0})):
+        dF = np.linalg.norm(forces(capi.PREC_MIXED, **over) - forces(capi.PREC_FP64, **over), axis=1)
+        out[tag] = {"max_over_rms_of_full": float(dF.max() / rms)}
+    return out
+
+
+def cpu_baseline(s, threads: int, gpu_f64=None, gpu_t64=None, steady: int = 4):
+    """The oracle on the GPU box's host cores, bounded sample: one O(N^2)
+    list build, one engforce, a first VV step (it rebuilds the lists: the
+    saved coordinates start at 0), then `steady` VV steps; value from their
+    mean.  The oracle's engforce at the workload's positions is also the
+    check of the GPU FP64 form at full size (gpu_f64, gpu_t64)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib
     lib = oracle_lib.lib()
@@ -156,19 +210,30 @@ def cpu_baseline(s, threads: int):
     t0 = time.perf_counter()
     o.vnlist_build()
     t1 = time.perf_counter()
-    o.engforce()
+    fo, to = o.engforce()
     t2 = time.perf_counter()
-    o.md_vv(1, s.dt)   # the first skin check always rebuilds (saved coordinates start at 0)
-    t3 = time.perf_counter()
     o.md_vv(1, s.dt)
+    t3 = time.perf_counter()
+    o.md_vv(steady, s.dt)
     t4 = time.perf_counter()
-    step = t4 - t3
+    step = (t4 - t3) / steady
     ns_day = 86400.0 / step * 1e-6 * 1.0
-    return {"value": ns_day, "unit": "ns/day", "cores": threads, "kind": "port",
-            "sample": f"oracle (FP64 C restatement of the reference, OpenMP over k-vectors) on the "
-                      f"same {s.natm}-ion workload: 1 O(N^2) list build ({t1 - t0:.2f} s), 1 engforce "
-                      f"({t2 - t1:.2f} s), a first VV step with its list rebuild ({t3 - t2:.2f} s), "
-                      f"then a steady VV step ({step:.2f} s); value from the steady step"}
+    out = {"value": ns_day, "unit": "ns/day", "cores": threads, "kind": "port",
+           "sample": f"oracle (FP64 C restatement of the reference, OpenMP over k-vectors) on the "
+                     f"same {s.natm}-ion workload: 1 O(N^2) list build ({t1 - t0:.2f} s), 1 engforce "
+                     f"({t2 - t1:.2f} s), a first VV step with its list rebuild ({t3 - t2:.2f} s), "
+                     f"then {steady} VV steps ({step:.2f} s each); value from their mean"}
+    if gpu_f64 is not None:
+        ferr = force_errors(gpu_f64, fo)
+        terms = {}
+        for k in ("u_bmhft", "u_coul", "u_dir", "u_rec", "u_self"):
+            if abs(to[k]) > 0:
+                terms[k] = abs(gpu_t64[k] - to[k]) / abs(to[k])
+        for k in ("tau_nonb", "tau_coul"):
+            ref = np.asarray(to[k])
+            terms[k] = float(np.abs(np.asarray(gpu_t64[k]) - ref).max() / max(np.abs(ref).max(), 1e-300))
+        out["gpu_fp64_vs_oracle"] = {"force": ferr, "terms_rel": terms}
+    return out
 
 
 def fit_leg(nsnap: int, reps: int) -> dict:
@@ -251,6 +316,9 @@ def main() -> int:
     ap.add_argument("--no-fit", action="store_true", help="skip the batched-fit leg")
     ap.add_argument("--fit-snapshots", type=int, default=256)
     ap.add_argument("--extra", action="store_true", help="also time the LiF 200k and MgO PIM 12k workloads")
+    ap.add_argument("--drift-steps", type=int, default=10000,
+                    help="NVE steps per form of the untimed drift leg (0: skip)")
+    ap.add_argument("--drift-equil", type=int, default=1000, help="NVE steps before the drift fit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -269,12 +337,10 @@ def main() -> int:
     r32 = run_form(s, capi.PREC_MIXED, args.steps, args.warmup, prof_steps)
 
     ferr_all = force_errors(r32["f0"], r64["f0"])
-    ferr = ferr_all["max_component_over_max"]
+    ferr = ferr_all["max_over_rms"]
     ms64 = r64["sec"] / args.steps * 1e3
     ms32 = r32["sec"] / args.steps * 1e3
     nsday = lambda ms: 86400.0 / (ms * 1e-3) * dt_fs * 1e-6  # noqa: E731
-    d64 = drift_per_atom_per_ns(r64["ek"], r64["up"], s.natm, dt_fs)
-    d32 = drift_per_atom_per_ns(r32["ek"], r32["up"], s.natm, dt_fs)
 
     # dominant kernel of the mixed form
     prof = r32["prof"]
@@ -313,13 +379,12 @@ def main() -> int:
                    "natm": s.natm, "nk": nk, "unique_pairs_in_rc": npairs,
                    "prep": f"steepest descent {args.relax} iterations (max|F| after: "
                            f"{s.extra.get('relax_fmax', float('nan')):.3g} eV/A), 300 K velocities"},
-        "fp64": {"ns_per_day": round(nsday(ms64), 3), "ms_per_step": round(ms64, 4),
-                 "nve_drift_eV_per_atom_per_ns": d64},
+        "fp64": {"ns_per_day": round(nsday(ms64), 3), "ms_per_step": round(ms64, 4)},
         "mixed": {"ns_per_day": round(nsday(ms32), 3), "ms_per_step": round(ms32, 4),
-                  "nve_drift_eV_per_atom_per_ns": d32,
-                  "max_rel_force_err_vs_fp64": ferr, "force_error_vs_fp64": ferr_all},
+                  "max_rel_force_err_vs_fp64": ferr,
+                  "force_error_definition": "max_i |F_i(mixed) - F_i(fp64)| / RMS_i |F_i(fp64)|",
+                  "force_error_vs_fp64": ferr_all},
         "speedup_mixed_vs_fp64": round(ms64 / ms32, 3),
-        "drift_ratio_mixed_over_fp64": (abs(d32) / abs(d64)) if d64 else None,
         "pair_interactions_per_s": npairs / (ms32 * 1e-3),
         "k_terms_per_s": nk * s.natm / (ms32 * 1e-3),
         "kernel_ms_per_step_mixed": {k: round(v["ms_total"] / prof_steps, 4) for k, v in prof.items()},
@@ -328,13 +393,20 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": None,
     }
+    out["mixed"]["force_error_by_term"] = term_errors(s)
+    if args.drift_steps > 0:
+        nd = nve_drift_leg(s, args.drift_equil, args.drift_steps)
+        out["nve_drift"] = nd
+        out["fp64"]["nve_drift_eV_per_atom_per_ns"] = nd["fp64"]["drift_eV_per_atom_per_ns"]
+        out["mixed"]["nve_drift_eV_per_atom_per_ns"] = nd["mixed"]["drift_eV_per_atom_per_ns"]
+        out["drift_ratio_mixed_over_fp64"] = nd["ratio_mixed_over_fp64"]
     if not args.no_fit:
         out["fit"] = fit_leg(args.fit_snapshots, reps=3)
     if args.extra:
         out["extra"] = {"lif_200k_nhc3": extra_leg("lif", 5), "mgo_pim_12k_nhc": extra_leg("mgo", 10)}
     if not args.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(s, args.cpu_threads)
+            out["cpu_baseline"] = cpu_baseline(s, args.cpu_threads, r64["f0"], r64["t0"])
         except Exception as e:  # the GPU result stands on its own
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out))

@@ -52,7 +52,9 @@ typedef enum mdff_status {
   MDFF_OK = 0,
   MDFF_ERR_BAD_ARG = 1,            /* invalid descriptor / argument                 */
   MDFF_ERR_LIST_OVERFLOW = 2,      /* Verlet list capacity exceeded (tools.f90:337) */
-  MDFF_ERR_SCF_NOT_CONVERGED = 3,  /* reported only when strict convergence is requested */
+  MDFF_ERR_SCF_NOT_CONVERGED = 3,  /* reserved: like the reference, the SCF stops at
+                                      max_scf_pol_iter and the call succeeds
+                                      (src/moment_from_pola.f90:365) */
   MDFF_ERR_POLAR_CATASTROPHE = 4,  /* some mu component > 10 e.A (moment_from_pola.f90:176) */
   MDFF_ERR_HIP = 5,                /* HIP runtime error                             */
   MDFF_ERR_NO_DEVICE = 6,          /* no usable gfx950 device                       */
@@ -187,6 +189,12 @@ int mdff_get_pairs(mdff_ctx *ctx, int32_t which, int32_t *point, int32_t *list,
  * non-NULL fx/fy/fz to have them copied.  terms may be NULL. */
 int mdff_engforce(mdff_ctx *ctx, double *fx, double *fy, double *fz, mdff_terms *terms);
 
+/* Energies, virials and stress of the last force evaluation, whether made by
+ * mdff_engforce or by the last step of mdff_md_vv / mdff_md_nhcn: the module
+ * globals the reference's calc_thermo reads after each step
+ * (src/thermo.f90:122-211). */
+int mdff_last_terms(mdff_ctx *ctx, mdff_terms *terms);
+
 /* Total dipoles mu_t and field ef_t of the last engforce (src/coulomb.f90:75-78),
  * AoS 3 x natm.  Either pointer may be NULL. */
 int mdff_get_dipoles(mdff_ctx *ctx, double *mu, double *ef);
@@ -219,6 +227,12 @@ int mdff_md_nhcn(mdff_ctx *ctx, int32_t nsteps, double dt, double temp,
  * *conint accumulates the conserved-quantity integral (src/thermo.f90:190-193). */
 int mdff_csvr_rescale(mdff_ctx *ctx, double temp, double taut, double r1,
                       double sumnoise, double *conint);
+
+/* rescale_velocities with the factor given (src/kinetic.f90:122-198): v *=
+ * lambda, then the mean velocity is removed.  For the Berendsen rescale the
+ * caller forms lambda = sqrt(1 + dt/tauTberendsen (T0/T - 1)) (or the
+ * annealing factor) from mdff_calc_temp. */
+int mdff_rescale_velocities(mdff_ctx *ctx, double lambda);
 
 /* Kinetic energy and temperature (src/kinetic.f90:527-552). */
 int mdff_calc_temp(mdff_ctx *ctx, double boltz_unit, double *temp, double *ekin);

@@ -163,11 +163,13 @@ def _bind(lib: C.CDLL, prefix: str) -> None:
         "get_pairs": ([vp, C.c_int32, ip, ip, C.c_int64, _P(C.c_int64)], C.c_int),
         "engforce": ([vp, dp, dp, dp, _P(MdffTerms)], C.c_int),
         "get_dipoles": ([vp, dp, dp], C.c_int),
+        "last_terms": ([vp, _P(MdffTerms)], C.c_int),
         "md_vv": ([vp, C.c_int32, C.c_double, dp, dp], C.c_int),
         "md_nhcn": ([vp, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32,
                      C.c_int32, dp, dp, dp, dp, dp], C.c_int),
         "csvr_rescale": ([vp, C.c_double, C.c_double, C.c_double, C.c_double, dp], C.c_int),
         "calc_temp": ([vp, C.c_double, dp, dp], C.c_int),
+        "rescale_velocities": ([vp, C.c_double], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, prefix + name)
@@ -273,6 +275,11 @@ class Engine:
         self._call("engforce", _dp(f[0]), _dp(f[1]), _dp(f[2]), C.byref(t))
         return f.T.copy(), t.as_dict()
 
+    def last_terms(self) -> dict:
+        t = MdffTerms()
+        self._call("last_terms", C.byref(t))
+        return t.as_dict()
+
     def get_dipoles(self) -> tuple[np.ndarray, np.ndarray]:
         mu = np.zeros((self.n, 3))
         ef = np.zeros((self.n, 3))
@@ -300,6 +307,9 @@ class Engine:
         self._call("csvr_rescale", C.c_double(temp), C.c_double(taut), C.c_double(r1),
                    C.c_double(sumnoise), C.byref(ci))
         return ci.value
+
+    def rescale_velocities(self, lam: float) -> None:
+        self._call("rescale_velocities", C.c_double(lam))
 
     def calc_temp(self, boltz_unit: float = BOLTZ_UNIT) -> tuple[float, float]:
         t, e = C.c_double(), C.c_double()

@@ -95,6 +95,7 @@ constexpr int RAW_PA = 0;        // 15 per-atom column sums of the pair kernel
 constexpr int RAW_EREC = 15;     // 7 reciprocal sums: energy, stress (xx yy zz xy xz yz)
 constexpr int RAW_MOM = 22;      // musq, mutot(3), qtot(3)
 constexpr int RAW_EKIN = 29;     // sum m v^2
+constexpr int RAW_ENVT = 30;     // extended-system energy of a Nose-Hoover chain step
 constexpr int RAW_SCRATCH = 32;  // 7 moments of the SCF field (slot 0 only)
 constexpr int RAW_N = 40;
 constexpr int RAW_SLOTS = 1024;  // evaluations held before the sums go to the host
@@ -262,6 +263,8 @@ struct mdff_ctx {
   double chk_seq = 0.0;
   double* red_part = nullptr;             // reduction block partials
   double* fin_part = nullptr;             // block partials of the pair sums [0, 1024*16) and kinetic energy
+  double* nhc_part = nullptr;             // Nose-Hoover chain steps: two sets of kinetic block partials
+  double* nhc_st = nullptr;               // Nose-Hoover chain state (xi, vxi, Q, scale)
   double* rec_blk = nullptr;              // k-block partials of the reciprocal energy and stress
   FinalSum fin[kMaxFinals] = {};          // queued final sums
   RecFinal rfin = {};                     // reciprocal forces left to the force kernel
@@ -270,6 +273,7 @@ struct mdff_ctx {
   int efield_zero = 0;                    // efield holds zeros (no field, dipoles or surface term)
   PairParam* d_pp = nullptr;
   int itime = 0;
+  mdff_terms last_terms = {};             // terms of the last force evaluation (engforce or MD step)
   int32_t* polar = nullptr;               // 1 for polarisable ions
   int npol = 0;                           // number of polarisable ions
   double qsq = 0.0;                       // sum of q^2 (self term)
@@ -285,8 +289,19 @@ struct mdff_ctx {
   int32_t* nl_count = nullptr;            // n (+1): row lengths
   int32_t* nl_start = nullptr;            // n + 1: row starts
   int32_t* nl_j = nullptr;                // capacity entries
-  uint8_t* nl_flag = nullptr;             // bit0: r2 <= (rc_vdw+skin)^2, bit1: <= (rc_coul+skin)^2
+  uint8_t* nl_flag = nullptr;             // bit0: r2 <= (rc_vdw+skin)^2, bit1: <= (rc_coul+skin)^2,
+                                          // bit2: r2 <= (rc_inner+skin)^2 (inner entry, mixed form)
   int64_t nl_cap = 0;
+  // Mixed form: a row holds its inner entries (within rc_inner + skin at the
+  // build) first, nl_count_in of them, then the rest.  The inner entries run
+  // through the FP64 pair kernel, the rest through the FP32 one: the FP32
+  // rounding of the large forces of close pairs would otherwise dominate the
+  // force error.  The sweep writes the rows into nl_j_tmp / nl_flag_tmp and a
+  // partition pass reorders them into nl_j / nl_flag.  rc_inner = 0: no split.
+  double rc_inner = 0.0;
+  int32_t* nl_count_in = nullptr;         // n
+  int32_t* nl_j_tmp = nullptr;            // capacity entries (split only)
+  uint8_t* nl_flag_tmp = nullptr;
   int nl_row_w = 0;                       // fixed row width of the one-pass build (0: rows from a count)
   int ncell[3] = {1, 1, 1};
   int nb_lo[3] = {0, 0, 0}, nb_hi[3] = {0, 0, 0}, nb_all[3] = {1, 1, 1};  // neighbour cell offsets
@@ -326,6 +341,20 @@ struct mdff_ctx {
   int h3_sk = 0;                          // split-half structure factor as well
   int h3_ch = 1, h3_nsh = 0, h3_ct = 1, h3_splits = 1, h3_mt8 = 0;
   float h3_pscale = 1.f;                  // charge scale of the structure factor's B operand
+  float* mu_pscale = nullptr;             // device: scales of the two dipole passes (k_mu_pscale)
+  // Mixed form, charge-only evaluations: the k-vectors of the inner box
+  // |nx| <= kin[0], |ny| <= kin[1], |nz| <= kin[2] (every k with |k| <=
+  // 2.5 alpha) carry most of the weight of the force sums and run in FP64
+  // (k_in_sk, k_in_reduce, k_in_gather); the main kernels see A = 0 there
+  // (kA_mask).  kin_on = 0: no inner pass.
+  int kin_on = 0;
+  int kin[3] = {0, 0, 0};
+  int nb_in = 0, in_chunk = 0, in_nch = 0, kin_tile = 0, kin_msplit = 0, kin_splits = 0;
+  double* kA_mask = nullptr;              // kA with the inner box zeroed (nk_box)
+  double* kA_in = nullptr;                // A of the inner box, 0 outside the half space (nb_in)
+  double* kvec_in = nullptr;              // kx, ky, kz, kcoe of the inner box (4 x nb_in)
+  double* sk_in_part = nullptr;           // [chunk][2 nb_in] FP64 partials
+  double* wk_in = nullptr;                // W = A conj(S) of the inner box (2 x nb_in)
   void* amat_h3 = nullptr;                // gather A operand, split halves
   float* ascale_h3 = nullptr;             // inverse scale per gather m-tile
   double* kO_ws = nullptr;                // scf_kO_v2 work vectors (11 x 3n)

@@ -326,6 +326,136 @@ __global__ void k_kin(int n, const double* __restrict__ m, const double* vx, con
 
 inline int nblk(int n) { return div_up(n, 256); }
 
+// ---- Nose-Hoover chain steps on the device (nose_hoover_chain_n,
+// src/integration.f90:538-701).  The chain state lives in device memory
+// (NHC_XI, NHC_VXI, NHC_Q: nhc_n entries each; NHC_S: the last scale); the
+// kinetic energy reaches the chain as block partials of sum m v^2.
+constexpr int NHC_XI = 0, NHC_VXI = 16, NHC_Q = 32, NHC_S = 48, NHC_N = 64;
+
+struct NhcArgs {
+  int nhc, yosh, mults, nb;
+  double L, temp, dt;
+  double w[9];                   // Yoshida weights (get_yoshida_weigth)
+  const double* part;            // nb block partials of sum m v^2
+  double* st;                    // chain state (NHC_N doubles)
+  double* envt;                  // e_nvt of the step (null: not wanted)
+};
+
+// chain_nhcn (src/integration.f90:619-701): the thermostat update from the
+// kinetic energy, in one thread after the block's sum of the partials; the
+// velocity scale s goes to st[NHC_S]
+__global__ void __launch_bounds__(256) k_nhc_chain(NhcArgs a) {
+  __shared__ double sh[4 * kRedMaxCols];
+  __shared__ double tot[1];
+  sum_block_partials(a.nb, 1, a.part, tot, sh);
+  if (threadIdx.x != 0) return;
+  const double kin = tot[0] * 0.5;
+  double* xi = a.st + NHC_XI;
+  double* vxi = a.st + NHC_VXI;
+  const double* Q = a.st + NHC_Q;
+  const int nhc = a.nhc;
+  double G[16];
+  for (int h = 0; h < 16; ++h) G[h] = 0.0;
+  double s = 1.0;
+  G[0] = 2.0 * kin - a.L * a.temp;
+  for (int k = 0; k < a.mults; ++k)
+    for (int j = 0; j < a.yosh; ++j) {
+      const double dts = (a.w[j] * a.dt) / (double)a.mults, dts2 = dts * 0.5, dts4 = dts2 * 0.5, dts8 = dts4 * 0.5;
+      G[nhc - 1] = (vxi[nhc - 2] * vxi[nhc - 2] / Q[nhc - 2] - a.temp);
+      vxi[nhc - 1] = vxi[nhc - 1] + G[nhc - 1] * dts4;
+      for (int h = nhc - 2; h >= 0; --h) {
+        vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
+        vxi[h] = vxi[h] + G[h] * dts4;
+        vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
+      }
+      s = s * exp(-vxi[0] * dts2 / Q[0]);
+      for (int h = 0; h < nhc; ++h) xi[h] = xi[h] + vxi[h] * dts2 / Q[h];
+      G[0] = 2.0 * kin * s * s - a.L * a.temp;
+      for (int h = 0; h < nhc - 1; ++h) {
+        vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
+        vxi[h] = vxi[h] + G[h] * dts4;
+        vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
+        G[h + 1] = (vxi[h] * vxi[h] / Q[h] - a.temp);
+      }
+      vxi[nhc - 1] = vxi[nhc - 1] + G[nhc - 1] * dts4;
+    }
+  a.st[NHC_S] = s;
+  if (a.envt) {
+    // conserved quantity of the extended system (src/integration.f90:582-593)
+    double e3 = 0.0, e4 = 0.0;
+    for (int h = 1; h < nhc; ++h) {
+      e3 = e3 + vxi[h] * vxi[h] * 0.5 / Q[h];
+      e4 = e4 + xi[h];
+    }
+    a.envt[0] = a.L * a.temp * xi[0] + vxi[0] * vxi[0] * 0.5 / Q[0] + e3 + a.temp * e4;
+  }
+}
+
+// v = s v with the chain's scale (scale != 0), then the block partials of
+// sum m v^2
+__global__ void __launch_bounds__(256) k_nhc_scale_kin(int n, int scale, const double* __restrict__ st,
+                                                       const double* __restrict__ m, double* vx, double* vy,
+                                                       double* vz, double* __restrict__ part) {
+  __shared__ double sh[4 * kRedMaxCols];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double kv[kRedMaxCols] = {};
+  if (i < n) {
+    double a = vx[i], b = vy[i], cc = vz[i];
+    if (scale) {
+      const double s = st[NHC_S];
+      a = s * a;
+      b = s * b;
+      cc = s * cc;
+      vx[i] = a;
+      vy[i] = b;
+      vz[i] = cc;
+    }
+    kv[0] = (a * a + b * b + cc * cc) * m[i];
+  }
+  const double r = block_sum_cols(kv, 1, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// v = s v, then prop_pos_vel_verlet's half kick and drift (:420-427)
+__global__ void k_nhc_scale_kick_drift(int n, const double* __restrict__ st, double dt2, double dt,
+                                       const double* __restrict__ m, double* rx, double* ry, double* rz, double* vx,
+                                       double* vy, double* vz, const double* fx, const double* fy, const double* fz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double s = st[NHC_S];
+  const double a = s * vx[i], b = s * vy[i], cc = s * vz[i];
+  const double ax = a + fx[i] * dt2 / m[i];
+  const double by = b + fy[i] * dt2 / m[i];
+  const double cz = cc + fz[i] * dt2 / m[i];
+  vx[i] = ax;
+  vy[i] = by;
+  vz[i] = cz;
+  rx[i] = rx[i] + ax * dt;
+  ry[i] = ry[i] + by * dt;
+  rz[i] = rz[i] + cz * dt;
+}
+
+// the second half kick of prop_pos_vel_verlet (:436-442), with the block
+// partials of sum m v^2
+__global__ void __launch_bounds__(256) k_nhc_kick_kin(int n, double dt2, const double* __restrict__ m, double* vx,
+                                                      double* vy, double* vz, const double* fx, const double* fy,
+                                                      const double* fz, double* __restrict__ part) {
+  __shared__ double sh[4 * kRedMaxCols];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double kv[kRedMaxCols] = {};
+  if (i < n) {
+    const double a = vx[i] + fx[i] * dt2 / m[i];
+    const double b = vy[i] + fy[i] * dt2 / m[i];
+    const double cc = vz[i] + fz[i] * dt2 / m[i];
+    vx[i] = a;
+    vy[i] = b;
+    vz[i] = cc;
+    kv[0] = (a * a + b * b + cc * cc) * m[i];
+  }
+  const double r = block_sum_cols(kv, 1, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ helpers
@@ -565,6 +695,17 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   DMALLOC(c->cell_atoms_in, n);
   DMALLOC(c->nl_count, n + 1);
   DMALLOC(c->nl_start, n + 1);
+  DMALLOC(c->nl_count_in, n + 1);
+  // mixed form: pairs within 0.4 of the largest cutoff (plus the skin, at
+  // the build) run in FP64; MDFF_MIXED_INNER=<fraction> sets another
+  // fraction, 0 none
+  c->rc_inner = 0.0;
+  if (d->precision == MDFF_PREC_MIXED) {
+    double frac = 0.4;
+    const char* ev = getenv("MDFF_MIXED_INNER");
+    if (ev && ev[0]) frac = atof(ev);
+    if (frac > 0.0) c->rc_inner = frac * std::max(c->rc_vdw, c->rc_coul);
+  }
   DMALLOC(c->d_flag, 4);
   DMALLOC(c->cx, n); DMALLOC(c->cy, n); DMALLOC(c->cz, n);
   // written by the check kernels over the bus, read by the host after the
@@ -588,6 +729,8 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   DMALLOC(c->red_part, nred);
   DMALLOC(c->red, 64);
   DMALLOC(c->fin_part, 1024 * 16 + (size_t)nblk(n) + 16);
+  DMALLOC(c->nhc_part, 2 * (size_t)nblk(n));
+  DMALLOC(c->nhc_st, NHC_N);
   DMALLOC(c->raw, (size_t)RAW_N * RAW_SLOTS);
   HIP_TRY(c, hipHostMalloc((void**)&c->h_raw, sizeof(double) * RAW_N * RAW_SLOTS));
   HIP_TRY(c, hipMemcpy(c->itype, it0.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
@@ -1044,6 +1187,7 @@ static int engforce(mdff_ctx* c, mdff_terms* t) {
   MDFF_TRY(engforce_launch(c, c->raw, t, false));
   MDFF_TRY(fetch_raw(c, 1));
   engforce_finish(c, c->h_raw, t);
+  c->last_terms = *t;
   return MDFF_OK;
 }
 
@@ -1131,7 +1275,8 @@ int mdff_ctx_destroy(mdff_ctx* c) {
                 c->pol, c->invpol, c->dipia, c->mu, c->mu_ind, c->efield, c->e_stat, c->e_tot, c->e_ind,
                 c->hist, c->frec, c->zeros, c->arec, c->ameta, c->peratom, c->red, c->red_part, c->d_pp, c->cell_of, c->cell_key,
                 c->cell_atoms, c->cell_atoms_in, c->cell_start, c->nl_count, c->nl_start, c->nl_j,
-                c->nl_flag, c->sort_tmp, c->d_flag, c->raw, c->cx, c->cy, c->cz, c->fin_part};
+                c->nl_flag, c->nl_count_in, c->nl_j_tmp, c->nl_flag_tmp, c->sort_tmp, c->d_flag, c->raw,
+                c->cx, c->cy, c->cz, c->fin_part, c->nhc_part, c->nhc_st};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c->polar) (void)hipFree(c->polar);
@@ -1236,6 +1381,12 @@ int mdff_engforce(mdff_ctx* c, double* fx, double* fy, double* fz, mdff_terms* t
   return MDFF_OK;
 }
 
+int mdff_last_terms(mdff_ctx* c, mdff_terms* terms) {
+  if (!c || !terms) return MDFF_ERR_BAD_ARG;
+  *terms = c->last_terms;
+  return MDFF_OK;
+}
+
 int mdff_get_dipoles(mdff_ctx* c, double* mu, double* ef) {
   if (!c) return MDFF_ERR_BAD_ARG;
   const size_t b = 3 * sizeof(double) * c->n;
@@ -1275,6 +1426,7 @@ int mdff_md_vv(mdff_ctx* c, int32_t nsteps, double dt, double* ekin, double* upo
       for (int k = 0; k <= slot; ++k) {
         const double* h = c->h_raw + (size_t)RAW_N * k;
         engforce_finish(c, h, &held[k]);
+        c->last_terms = held[k];
         if (ekin) ekin[st0 + k] = 0.5 * h[RAW_EKIN];
         if (upot) upot[st0 + k] = upot_of(held[k]);
       }
@@ -1318,76 +1470,86 @@ static void yoshida_weights(int order, double w[9]) {
   }
 }
 
-// chain_nhcn (src/integration.f90:619-701): thermostat variables on the host,
-// the velocity scale s applied on the device
-static int chain(mdff_ctx* c, double* kin, double* vxi, double* xi, const double* Q, double L, double temp,
-                 double dt, int nhc, int yosh, int mults) {
-  double w[9];
-  yoshida_weights(yosh, w);
-  double G[16] = {0};
-  double s = 1.0;
-  G[0] = 2.0 * (*kin) - L * temp;
-  for (int k = 0; k < mults; ++k)
-    for (int j = 0; j < yosh; ++j) {
-      double dts = (w[j] * dt) / (double)mults, dts2 = dts * 0.5, dts4 = dts2 * 0.5, dts8 = dts4 * 0.5;
-      G[nhc - 1] = (vxi[nhc - 2] * vxi[nhc - 2] / Q[nhc - 2] - temp);
-      vxi[nhc - 1] = vxi[nhc - 1] + G[nhc - 1] * dts4;
-      for (int h = nhc - 2; h >= 0; --h) {
-        vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
-        vxi[h] = vxi[h] + G[h] * dts4;
-        vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
-      }
-      s = s * exp(-vxi[0] * dts2 / Q[0]);
-      for (int h = 0; h < nhc; ++h) xi[h] = xi[h] + vxi[h] * dts2 / Q[h];
-      G[0] = 2.0 * (*kin) * s * s - L * temp;
-      for (int h = 0; h < nhc - 1; ++h) {
-        vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
-        vxi[h] = vxi[h] + G[h] * dts4;
-        vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
-        G[h + 1] = (vxi[h] * vxi[h] / Q[h] - temp);
-      }
-      vxi[nhc - 1] = vxi[nhc - 1] + G[nhc - 1] * dts4;
-    }
-  hipLaunchKernelGGL(k_scale_v, dim3(nblk(c->n)), dim3(256), 0, c->stream, c->n, s, 0.0, 0.0, 0.0, c->vx, c->vy,
-                     c->vz);
-  return kinetic(c, kin, nullptr);
-}
-
 int mdff_md_nhcn(mdff_ctx* c, int32_t nsteps, double dt, double temp, double timesca, int32_t nhc, int32_t yosh,
                  int32_t mults, double* xi, double* vxi, double* ekin, double* upot, double* e_nvt) {
-  if (!c || nhc < 2 || nhc > 16 || !xi || !vxi || (yosh != 1 && yosh != 3 && yosh != 5 && yosh != 7 && yosh != 9) || mults < 1)
+  if (!c || nhc < 2 || nhc > 16 || !xi || !vxi || (yosh != 1 && yosh != 3 && yosh != 5 && yosh != 7 && yosh != 9) ||
+      mults < 1 || nsteps < 0)
     return MDFF_ERR_BAD_ARG;
-  const int n = c->n;
+  const int n = c->n, nb = nblk(n);
+  // thermostat masses (src/integration.f90:563-566) and the chain state
+  double st[NHC_N] = {0};
   const double L = 3.0 * (double)n;
-  double Q[16];
-  for (int h = 0; h < nhc; ++h) Q[h] = pow(timesca, 2.0) * temp;
-  Q[0] = Q[0] * L;
-  for (int st = 0; st < nsteps; ++st) {
-    double kin;
-    MDFF_TRY(kinetic(c, &kin, nullptr));
-    MDFF_TRY(chain(c, &kin, vxi, xi, Q, L, temp, dt, nhc, yosh, mults));
-    hipLaunchKernelGGL(k_kick_drift, dim3(nblk(n)), dim3(256), 0, c->stream, n, dt * 0.5, dt, c->mass, c->rx, c->ry,
-                       c->rz, c->vx, c->vy, c->vz, c->fx, c->fy, c->fz);
-    mdff_terms t;
+  for (int h = 0; h < nhc; ++h) {
+    st[NHC_XI + h] = xi[h];
+    st[NHC_VXI + h] = vxi[h];
+    st[NHC_Q + h] = pow(timesca, 2.0) * temp;
+  }
+  st[NHC_Q] = st[NHC_Q] * L;
+  HIP_TRY(c, hipMemcpyAsync(c->nhc_st, st, sizeof st, hipMemcpyHostToDevice, c->stream));
+  NhcArgs a{};
+  a.nhc = nhc;
+  a.yosh = yosh;
+  a.mults = mults;
+  a.nb = nb;
+  a.L = L;
+  a.temp = temp;
+  a.dt = dt;
+  yoshida_weights(yosh, a.w);
+  a.st = c->nhc_st;
+  double* pa = c->nhc_part;        // after the second half kick
+  double* pb = c->nhc_part + nb;   // after the second chain's scale (the step's kinetic energy)
+  // kinetic energy of the starting velocities (calc_temp, :569)
+  MDFF_TRY(claim_partials(c, pb));
+  hipLaunchKernelGGL(k_nhc_scale_kin, dim3(nb), dim3(256), 0, c->stream, n, 0, c->nhc_st, c->mass, c->vx, c->vy,
+                     c->vz, pb);
+  std::vector<mdff_terms> held(RAW_SLOTS);
+  int st0 = 0;
+  for (int step = 0; step < nsteps; ++step) {
+    const int slot = step - st0;
+    double* raw = c->raw + (size_t)RAW_N * slot;
+    // chain_nhcn, then prop_pos_vel_verlet: v = s v, half kick, drift
+    a.part = pb;
+    a.envt = nullptr;
+    hipLaunchKernelGGL(k_nhc_chain, dim3(1), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_nhc_scale_kick_drift, dim3(nb), dim3(256), 0, c->stream, n, c->nhc_st, dt * 0.5, dt,
+                       c->mass, c->rx, c->ry, c->rz, c->vx, c->vy, c->vz, c->fx, c->fy, c->fz);
     c->itime++;   // md%itime of this step (src/runmd.f90:257)
-    MDFF_TRY(engforce(c, &t));
-    hipLaunchKernelGGL(k_kick, dim3(nblk(n)), dim3(256), 0, c->stream, n, dt * 0.5, c->mass, c->vx, c->vy, c->vz,
-                       c->fx, c->fy, c->fz);
-    MDFF_TRY(kinetic(c, &kin, nullptr));
-    MDFF_TRY(chain(c, &kin, vxi, xi, Q, L, temp, dt, nhc, yosh, mults));
-    if (ekin) ekin[st] = kin;
-    if (upot) upot[st] = upot_of(t);
-    if (e_nvt) {
-      double e3 = 0.0, e4 = 0.0;
-      for (int h = 1; h < nhc; ++h) { e3 = e3 + vxi[h] * vxi[h] * 0.5 / Q[h]; e4 = e4 + xi[h]; }
-      e_nvt[st] = L * temp * xi[0] + vxi[0] * vxi[0] * 0.5 / Q[0] + e3 + temp * e4;
-    }
-    if (c->d.lvnlist) {
-      int rb = 0;
-      MDFF_TRY(nl_check(c, &rb));
+    MDFF_TRY(engforce_launch(c, raw, &held[slot], true));
+    hipLaunchKernelGGL(k_nhc_kick_kin, dim3(nb), dim3(256), 0, c->stream, n, dt * 0.5, c->mass, c->vx, c->vy,
+                       c->vz, c->fx, c->fy, c->fz, pa);
+    // chain_nhcn again; its scale and the step's kinetic energy; e_nvt
+    a.part = pa;
+    a.envt = raw + RAW_ENVT;
+    hipLaunchKernelGGL(k_nhc_chain, dim3(1), dim3(256), 0, c->stream, a);
+    MDFF_TRY(claim_partials(c, pb));
+    hipLaunchKernelGGL(k_nhc_scale_kin, dim3(nb), dim3(256), 0, c->stream, n, 1, c->nhc_st, c->mass, c->vx,
+                       c->vy, c->vz, pb);
+    MDFF_TRY(queue_final(c, pb, nb, 1, raw + RAW_EKIN));
+    // vnlistcheck after the step (src/runmd.f90:294-304), finished by the
+    // next step's force evaluation
+    if (c->d.lvnlist) MDFF_TRY(nl_check_launch(c));
+    if (slot == RAW_SLOTS - 1 || step == nsteps - 1) {
+      MDFF_TRY(flush_finals(c));
+      MDFF_TRY(fetch_raw(c, slot + 1));
+      for (int k = 0; k <= slot; ++k) {
+        const double* h = c->h_raw + (size_t)RAW_N * k;
+        engforce_finish(c, h, &held[k]);
+        c->last_terms = held[k];
+        if (ekin) ekin[st0 + k] = 0.5 * h[RAW_EKIN];
+        if (upot) upot[st0 + k] = upot_of(held[k]);
+        if (e_nvt) e_nvt[st0 + k] = h[RAW_ENVT];
+      }
+      st0 = step + 1;
     }
   }
+  int rb = 0;
+  MDFF_TRY(nl_check_finish(c, &rb));
+  HIP_TRY(c, hipMemcpyAsync(st, c->nhc_st, sizeof st, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (int h = 0; h < nhc; ++h) {
+    xi[h] = st[NHC_XI + h];
+    vxi[h] = st[NHC_VXI + h];
+  }
   return MDFF_OK;
 }
 
@@ -1403,6 +1565,19 @@ int mdff_csvr_rescale(mdff_ctx* c, double temp, double taut, double r1, double s
                       2.0 * r1 * sqrt(ekin * sigma / ndeg * (1.0 - factor) * factor);
   if (conint) *conint = *conint - (knew - ekin);
   const double lambda = sqrt(knew / ekin);
+  hipLaunchKernelGGL(k_scale_v, dim3(nblk(n)), dim3(256), 0, c->stream, n, lambda, 0.0, 0.0, 0.0, c->vx, c->vy,
+                     c->vz);
+  double k2, vs[3];
+  MDFF_TRY(kinetic(c, &k2, vs));
+  hipLaunchKernelGGL(k_scale_v, dim3(nblk(n)), dim3(256), 0, c->stream, n, 1.0, vs[0] / n, vs[1] / n, vs[2] / n,
+                     c->vx, c->vy, c->vz);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return MDFF_OK;
+}
+
+int mdff_rescale_velocities(mdff_ctx* c, double lambda) {
+  if (!c || !(lambda >= 0.0)) return MDFF_ERR_BAD_ARG;
+  const int n = c->n;
   hipLaunchKernelGGL(k_scale_v, dim3(nblk(n)), dim3(256), 0, c->stream, n, lambda, 0.0, 0.0, 0.0, c->vx, c->vy,
                      c->vz);
   double k2, vs[3];

@@ -114,6 +114,9 @@ struct NLArgs {
   int n, nx, ny, nz;
   double a11, a21, a31, a12, a22, a32, a13, a23, a33;  // simu_cell%A, A(i,j)
   double cut2_max, cut2_v, cut2_c;
+  // inner entries (flag bit 2): r^2 <= cut2_in at the build; counted per row
+  // into count_in when it is set (cut2_in < 0: none)
+  double cut2_in;
   const double *sx, *sy, *sz;             // fractional coords, original order
   const double *px, *py, *pz;             // fractional coords, cell order
   const float4* pf;                       // wrapped single-precision copy, cell order
@@ -124,11 +127,12 @@ struct NLArgs {
   // rounding) when within marg of a cutoff or when prefilter == 0
   int prefilter;
   float a11f, a21f, a31f, a12f, a22f, a32f, a13f, a23f, a33f;
-  float c2f_max, c2f_v, c2f_c, marg;
+  float c2f_max, c2f_v, c2f_c, c2f_in, marg;
   const int32_t* cell_atoms;
   const int32_t* cell_start;
   const int32_t* cell_of;
   int32_t* count;
+  int32_t* count_in;
   const int32_t* row_start;
   // row_w > 0: rows of fixed width row_w (row i at i * row_w), written in one
   // pass that also stores the row's start (row_out) and length (count);
@@ -228,7 +232,7 @@ __global__ void __launch_bounds__(256) k_nl_sweep(NLArgs a) {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const int* pref = rpref[wl];
   const int* st = rst[wl];
-  int found = 0;
+  int found = 0, found_in = 0;
   // chunk loads: candidate sorted position, its atom and coordinates (clamped)
   int qn = 0, jn = -1;
   float4 fn = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -246,7 +250,7 @@ __global__ void __launch_bounds__(256) k_nl_sweep(NLArgs a) {
       jn = a.cell_atoms[qn];
       fn = a.pf[qn];
     }
-    bool ok = false;
+    bool ok = false, inner = false;
     uint8_t flag = 0;
     if (live && j != i) {
       float dx = sif - f.x, dy = tif - f.y, dz = uif - f.z;
@@ -262,13 +266,17 @@ __global__ void __launch_bounds__(256) k_nl_sweep(NLArgs a) {
       if (!a.prefilter || near) {
         const double r2 = min_image_r2(a, si - a.px[q], ti - a.py[q], ui - a.pz[q]);
         ok = (r2 <= a.cut2_max);
+        inner = r2 <= a.cut2_in;
         flag = (uint8_t)((r2 <= a.cut2_v ? 1 : 0) | (r2 <= a.cut2_c ? 2 : 0));
       } else {
         ok = (r2f <= a.c2f_max);
+        inner = r2f <= a.c2f_in;
         flag = (uint8_t)((r2f <= a.c2f_v ? 1 : 0) | (r2f <= a.c2f_c ? 2 : 0));
       }
+      if (inner) flag |= 4;
     }
     const uint64_t m = __ballot(ok);
+    found_in += __popcll(__ballot(ok && inner));
     if (FILL && ok) {
       const int off = found + __popcll(m & ((1ull << lane) - 1ull));
       const int64_t e = base + off;
@@ -281,8 +289,46 @@ __global__ void __launch_bounds__(256) k_nl_sweep(NLArgs a) {
   }
   if (lane == 0) {
     if (!FILL || a.row_w) a.count[i] = found;
+    if (FILL && a.count_in) a.count_in[i] = found_in;
     if (FILL && a.row_w) a.row_out[i] = (int32_t)base;
     if (FILL && base + found > lim) atomicExch(a.overflow, 1);
+  }
+}
+
+// Mixed form: reorder each row so that its inner entries (flag bit 2) come
+// first, count_in of them, both parts in their sweep order.  One wave per
+// row; the rows are read from (sj, sf) and written to (dj, df) at the same
+// row starts.
+__global__ void __launch_bounds__(256) k_nl_partition(int n, const int32_t* __restrict__ start,
+                                                      const int32_t* __restrict__ count,
+                                                      const int32_t* __restrict__ count_in,
+                                                      const int32_t* __restrict__ sj, const uint8_t* __restrict__ sf,
+                                                      int32_t* __restrict__ dj, uint8_t* __restrict__ df) {
+  const int lane = threadIdx.x & 63;
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (i >= n) return;                      // whole waves
+  const int64_t base = start[i];
+  const int cnt = count[i], nin = count_in[i];
+  const uint64_t below = (1ull << lane) - 1ull;
+  int oin = 0, oout = 0;
+  for (int v0 = 0; v0 < cnt; v0 += 64) {
+    const int e = v0 + lane;
+    const bool live = e < cnt;
+    int32_t j = 0;
+    uint8_t f = 0;
+    if (live) {
+      j = sj[base + e];
+      f = sf[base + e];
+    }
+    const bool in = live && (f & 4);
+    const uint64_t mi = __ballot(in), mo = __ballot(live && !in);
+    if (live) {
+      const int64_t d = in ? base + oin + __popcll(mi & below) : base + nin + oout + __popcll(mo & below);
+      dj[d] = j;
+      df[d] = f;
+    }
+    oin += __popcll(mi);
+    oout += __popcll(mo);
   }
 }
 
@@ -310,6 +356,11 @@ static NLArgs make_args(mdff_ctx* c) {
   a.cut2_v = rv * rv;
   a.cut2_c = rcl * rcl;
   a.cut2_max = a.cut2_v > a.cut2_c ? a.cut2_v : a.cut2_c;
+  a.cut2_in = -1.0;
+  if (c->rc_inner > 0.0) {
+    const double rin = c->rc_inner + c->d.skindiff;
+    a.cut2_in = rin * rin;
+  }
   a.sx = c->sx; a.sy = c->sy; a.sz = c->sz;
   a.pf = c->bf;
   a.ox0 = c->nb_lo[0]; a.ox1 = c->nb_hi[0];
@@ -339,11 +390,13 @@ static NLArgs make_args(mdff_ctx* c) {
     a.c2f_max = (float)a.cut2_max;
     a.c2f_v = (float)a.cut2_v;
     a.c2f_c = (float)a.cut2_c;
+    a.c2f_in = (float)a.cut2_in;
   }
   a.cell_atoms = c->cell_atoms;
   a.cell_start = c->cell_start;
   a.cell_of = c->cell_of;
   a.count = c->nl_count;
+  a.count_in = c->rc_inner > 0.0 ? c->nl_count_in : nullptr;
   a.row_start = c->nl_start;
   a.row_out = c->nl_start;
   a.row_w = 0;
@@ -408,6 +461,19 @@ static int nl_build_from(mdff_ctx* c, const double* px, const double* py, const 
   a.sx = px; a.sy = py; a.sz = pz;
   a.px = c->bx; a.py = c->by; a.pz = c->bz;
   dim3 grid(div_up((int64_t)n * 64, 256));
+  // with the inner split the sweep fills the staging rows and the partition
+  // pass writes the final ones
+  const bool split = c->rc_inner > 0.0;
+  auto partition = [&]() {
+    if (split)
+      PROF_LAUNCH(c, PK_NLIST, hipLaunchKernelGGL(k_nl_partition, grid, dim3(256), 0, c->stream, n, c->nl_start,
+                                                  c->nl_count, c->nl_count_in, c->nl_j_tmp, c->nl_flag_tmp,
+                                                  c->nl_j, c->nl_flag));
+  };
+  if (split) {
+    a.nl_j = c->nl_j_tmp;
+    a.nl_flag = c->nl_flag_tmp;
+  }
   if (c->nl_row_w > 0) {
     // one pass into rows of the width sized at an earlier build; a row that
     // does not fit sends the build through the counting pass below
@@ -417,6 +483,8 @@ static int nl_build_from(mdff_ctx* c, const double* px, const double* py, const 
     HIP_TRY(c, hipMemcpyAsync(c->h_flag + 1, c->d_flag + 1, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (c->h_flag[1] == 0) {
+      partition();
+      HIP_TRY(c, hipGetLastError());
       c->lists_valid = 1;
       return MDFF_OK;
     }
@@ -449,17 +517,26 @@ static int nl_build_from(mdff_ctx* c, const double* px, const double* py, const 
   if (need > c->nl_cap) {
     if (c->nl_j) HIP_TRY(c, hipFree(c->nl_j));
     if (c->nl_flag) HIP_TRY(c, hipFree(c->nl_flag));
+    if (c->nl_j_tmp) HIP_TRY(c, hipFree(c->nl_j_tmp));
+    if (c->nl_flag_tmp) HIP_TRY(c, hipFree(c->nl_flag_tmp));
+    c->nl_j_tmp = nullptr;
+    c->nl_flag_tmp = nullptr;
     HIP_TRY(c, hipMalloc((void**)&c->nl_j, sizeof(int32_t) * need));
     HIP_TRY(c, hipMalloc((void**)&c->nl_flag, need));
+    if (split) {
+      HIP_TRY(c, hipMalloc((void**)&c->nl_j_tmp, sizeof(int32_t) * need));
+      HIP_TRY(c, hipMalloc((void**)&c->nl_flag_tmp, need));
+    }
     c->nl_cap = need;
   }
-  a.nl_j = c->nl_j;
-  a.nl_flag = c->nl_flag;
+  a.nl_j = split ? c->nl_j_tmp : c->nl_j;
+  a.nl_flag = split ? c->nl_flag_tmp : c->nl_flag;
   a.cap = c->nl_cap;
   a.row_w = (int)w;
   c->nl_row_w = (int)w;
   HIP_TRY(c, hipMemsetAsync(c->d_flag + 1, 0, sizeof(int32_t), c->stream));
   PROF_LAUNCH(c, PK_NLIST, hipLaunchKernelGGL(k_nl_sweep<true>, grid, dim3(256), 0, c->stream, a));
+  partition();
   HIP_TRY(c, hipGetLastError());
   c->lists_valid = 1;
   return MDFF_OK;

@@ -43,6 +43,11 @@ struct PairArgs {
   const PairParam* pp;
   const int32_t* row;                // row starts (n)
   const int32_t* cnt;                // row lengths (n)
+  // seg 0: whole rows; 1: the leading cnt_in[i] entries of a row (its inner
+  // entries); 2: the rest.  accum: add to the per-atom outputs already
+  // written (by the launch over the other part) instead of storing
+  const int32_t* cnt_in;
+  int seg, accum;
   double coul;                       // coul_unit
   const int32_t* nl_j;               // sorted positions of the neighbours
   double* fsr;                       // 3n AoS out: f_sr + coul_unit f_dir
@@ -111,7 +116,9 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
   using EA = T;
   EA usr = 0, ts[6] = {0, 0, 0, 0, 0, 0}, ucl = 0, tc[6] = {0, 0, 0, 0, 0, 0};
   const T coul = (T)a.coul;
-  const int r0 = a.row[i], r1 = r0 + a.cnt[i];
+  int r0 = a.row[i], r1 = r0 + a.cnt[i];
+  if (a.seg == 1) r1 = r0 + a.cnt_in[i];
+  else if (a.seg == 2) r0 += a.cnt_in[i];
   // two-stage prefetch of the neighbour chain: the row entry two steps ahead
   // and the record of the next entry are in flight while a pair is computed
   int pj_next = (r0 + sub + LPA < r1) ? a.nl_j[r0 + sub + LPA] : 0;
@@ -321,6 +328,20 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
     for (int k = 0; k < 6; ++k) cols[9 + k] = group_sum<LPA>((double)tc[k]);
   }
   if (sub == 0) {
+    if (a.accum) {
+      f0 = a.fsr[3 * i] + f0;
+      f1 = a.fsr[3 * i + 1] + f1;
+      f2 = a.fsr[3 * i + 2] + f2;
+      if (a.do_field) {
+        e0 = a.ef[3 * i] + e0;
+        e1 = a.ef[3 * i + 1] + e1;
+        e2 = a.ef[3 * i + 2] + e2;
+      }
+      if (a.want_stress) {
+#pragma unroll
+        for (int k = 0; k < NPA; ++k) cols[k] = a.peratom[(size_t)k * a.n + i] + cols[k];
+      }
+    }
     a.fsr[3 * i] = f0;
     a.fsr[3 * i + 1] = f1;
     a.fsr[3 * i + 2] = f2;
@@ -510,6 +531,17 @@ int realspace_force_impl(mdff_ctx* c, double* fsr, double* fcl, double* ef, bool
   const bool dip = (a.qd || a.dd);
   const bool fp32 = c->d.precision == MDFF_PREC_MIXED;
   if (fp32) {
+    if (c->rc_inner > 0.0) {
+      // the inner entries of the rows in FP64, then the rest in FP32 added
+      // to them
+      a.cnt_in = c->nl_count_in;
+      a.seg = 1;
+      a.accum = 0;
+      if (dip) launch_force<double, true>(c, a);
+      else launch_force<double, false>(c, a);
+      a.seg = 2;
+      a.accum = 1;
+    }
     if (dip) launch_force<float, true>(c, a);
     else launch_force<float, false>(c, a);
   } else {

@@ -876,6 +876,41 @@ __device__ __forceinline__ uint32_t pack_h(_Float16 a, _Float16 b) {
   return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
 }
 
+// Power-of-two scales of the dipole passes of the split-half structure
+// factor, from the current dipoles: the largest |P| an atom can give is
+// brought into [256, 512), so that the FP16 halves of the products stay
+// normal numbers (a lo half below 2^-14 loses bits).  Mode 1:
+// |q| + 2 pi (Kx |m.b1| + Ky |m.b2|); mode 2: |m.b3|.  One block.
+__global__ void __launch_bounds__(256) k_mu_pscale(int n, int Kx, int Ky, const double* __restrict__ q,
+                                                   const double* __restrict__ mu, double b0, double b1, double b2,
+                                                   double b3, double b4, double b5, double b6, double b7,
+                                                   double b8, float* __restrict__ out) {
+  __shared__ double s1[256], s2[256];
+  double m1 = 0.0, m2 = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double u0 = mu[3 * i], u1 = mu[3 * i + 1], u2 = mu[3 * i + 2];
+    const double c1 = fabs(u0 * b0 + u1 * b1 + u2 * b2), c2 = fabs(u0 * b3 + u1 * b4 + u2 * b5);
+    const double c3 = fabs(u0 * b6 + u1 * b7 + u2 * b8);
+    m1 = fmax(m1, fabs(q[i]) + 2.0 * MDFF_PI * (Kx * c1 + Ky * c2));
+    m2 = fmax(m2, c3);
+  }
+  s1[threadIdx.x] = m1;
+  s2[threadIdx.x] = m2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + w]);
+      s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int e = 0;
+    out[0] = s1[0] > 0.0 ? (float)ldexp(1.0, (frexp(s1[0], &e), 9 - e)) : 1.f;
+    out[1] = s2[0] > 0.0 ? (float)ldexp(1.0, (frexp(s2[0], &e), 9 - e)) : 1.f;
+  }
+}
+
 // ---- structure factor: S^T(nz, m) = sum_i Z_i(nz) P_i(m), split halves ----
 // Rows R = (nz, Re/Im) in 4 tiles of 32 (16 nz each; 64 nz per block,
 // blockIdx.z picks the group); columns = 32 values of m per wave; K = (atom,
@@ -883,10 +918,13 @@ __device__ __forceinline__ uint32_t pack_h(_Float16 a, _Float16 b) {
 // split once per staged tile into LDS; the B operand P_i(m) = a_i X_i[nx]
 // Y_i[ny] is formed and split per lane.  Modes as k_sk_mma2; the charges
 // are scaled by pscale (a power of two) and the sums written times
-// 1/pscale, FP32 per atom chunk.
+// 1/pscale, FP32 per atom chunk.  With dipoles the scale of the pass comes
+// from device memory (pscale_dev[0]: mode 1, [1]: mode 2), set by
+// k_mu_pscale from the current dipoles.
 struct Sh3Args {
   int n, Kx, Ky, Kz, NY, NZ, M, chunk, mode, nk_box;
   float pscale;
+  const float* pscale_dev;
   const double* q;
   const double* mu;
   double b[9];
@@ -897,6 +935,7 @@ struct Sh3Args {
 template <int TA, int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_sk_h3(Sh3Args a) {
   constexpr bool CQ = MODE != 1;
+  const float psc = a.pscale_dev ? a.pscale_dev[MODE == 2 ? 1 : 0] : a.pscale;
   // CQ: P = c X Y with one coefficient per atom (modes 0 and 2): c X is
   // formed once per tile when X is staged; otherwise (mode 1) P =
   // (c0 - i (nx c1 + ny c2)) X Y is formed per column
@@ -994,7 +1033,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       r0[129 * RW] = pack_h(il, rl);
     }
     float cs = 1.f;                                       // c of atom sia (CQ)
-    if (CQ) cs = MODE == 0 ? (float)(qd * a.pscale) : (float)(a.pscale * (m0 * a.b[6] + m1 * a.b[7] + m2 * a.b[8]));
+    if (CQ) cs = MODE == 0 ? (float)(qd * psc) : (float)(psc * (m0 * a.b[6] + m1 * a.b[7] + m2 * a.b[8]));
 #pragma unroll
     for (int u = 0; u < XU; ++u) {
       const int cidx = scg + CG * u;
@@ -1013,12 +1052,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     if (!CQ && threadIdx.x < TA) {
       float c0 = 0.f, c1 = 0.f, c2 = 0.f;
       if ((int)threadIdx.x < nt) {
-        if (MODE <= 1) c0 = (float)(qd * a.pscale);
+        if (MODE <= 1) c0 = (float)(qd * psc);
         if (MODE == 1) {
-          c1 = (float)(2.0 * MDFF_PI * a.pscale * (m0 * a.b[0] + m1 * a.b[1] + m2 * a.b[2]));
-          c2 = (float)(2.0 * MDFF_PI * a.pscale * (m0 * a.b[3] + m1 * a.b[4] + m2 * a.b[5]));
+          c1 = (float)(2.0 * MDFF_PI * psc * (m0 * a.b[0] + m1 * a.b[1] + m2 * a.b[2]));
+          c2 = (float)(2.0 * MDFF_PI * psc * (m0 * a.b[3] + m1 * a.b[4] + m2 * a.b[5]));
         } else if (MODE == 2) {
-          c0 = (float)(a.pscale * (m0 * a.b[6] + m1 * a.b[7] + m2 * a.b[8]));
+          c0 = (float)(psc * (m0 * a.b[6] + m1 * a.b[7] + m2 * a.b[8]));
         }
       }
       sq[threadIdx.x] = c0;
@@ -1082,7 +1121,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   // rt -> nz index zoff + (32 rt + R) / 2, part (32 rt + R) % 2
   if (!bok) return;
   float* out = a.part + (size_t)by * 2 * a.nk_box + 2 * (size_t)m * a.NZ;
-  const float inv = 1.0f / a.pscale;
+  const float inv = 1.0f / psc;
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -1391,6 +1430,210 @@ k_gather_h3(Gh3Args a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Inner k-box in FP64 (mixed form, charge-only evaluations).
+//
+// The weight of a k-vector in the force sums, |k| A(k), falls off as
+// exp(-k^2 / 4 alpha^2) / |k|, so the few k-vectors of small |k| carry the
+// large, mutually cancelling contributions; their FP32-class structure
+// factors (24-bit accumulation over thousands of atoms) set the mixed form's
+// force error.  The box |nx| <= Kx', |ny| <= Ky', |nz| <= Kz' around k = 0
+// (nx >= 0, entries outside the half space and k = 0 carry A = 0) is summed
+// here in FP64 and masked out of the main kernels (A = 0 there): per k the
+// same terms as the main path, so the split changes which precision sums
+// a k-vector, not the sum.
+//   k_in_sk      S(k) partials per chunk of atoms, one thread per k, FP64
+//                phase tables of the chunk's atoms staged in LDS
+//   k_in_reduce  S(k) over the chunks in order; W = A conj(S); the energy
+//                and stress block partials (appended to the main blocks)
+//   k_in_gather  one thread per atom: the moments sum_k n_d Im(W e^{ik.r}),
+//                written as one more split of the gather's partials
+// ---------------------------------------------------------------------------
+struct InArgs {
+  int n, Kx, Ky, Kz, NY, NZ, nb, chunk, tile, msplit;
+  const double *sx, *sy, *sz, *q;
+  const double* kA;               // A of the inner box (nb)
+  const double* kvec;             // kx, ky, kz, kcoe (4 nb)
+  double* part;                   // [chunk][2 nb]
+  double* wk;                     // 2 nb
+  double* blk;                    // energy / stress block partials [block][7]
+  double* out;                    // gather moments [split][3][n], one split per m range
+};
+
+// e^{2 pi i k s} for k = 0..K into dst[0..K] (stride st): sincospi of the
+// reduced angle for k = 1, then products (each within a few units of 2^-53)
+__device__ __forceinline__ void phase_row(double s, int K, cplx<double>* dst, int st) {
+  double t1 = s - rint(s);
+  double s1, c1;
+  sincospi(2.0 * t1, &s1, &c1);
+  double cs = 1.0, sn = 0.0;
+  dst[0] = {1.0, 0.0};
+  for (int k = 1; k <= K; ++k) {
+    const double c2 = cs * c1 - sn * s1, s2 = cs * s1 + sn * c1;
+    cs = c2;
+    sn = s2;
+    dst[(size_t)k * st] = {cs, sn};
+  }
+}
+
+// grid (k blocks of 256, atom chunks); the chunk's atoms pass through LDS
+// in tiles of a.tile atoms (FP64 tables and charges)
+__global__ void __launch_bounds__(256) k_in_sk(InArgs a) {
+  extern __shared__ unsigned char smem_raw[];
+  const int ncol = a.Kx + a.Ky + a.Kz + 3;
+  cplx<double>* tb = reinterpret_cast<cplx<double>*>(smem_raw);          // [atom][ncol]
+  double* qs = reinterpret_cast<double*>(tb + (size_t)a.tile * ncol);    // [atom]
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = k < a.nb;
+  const int kk = act ? k : 0;
+  const int nx = kk / (a.NY * a.NZ);
+  const int ny = (kk / a.NZ) % a.NY - a.Ky;
+  const int nz = kk % a.NZ - a.Kz;
+  const int cx = nx, cy = a.Kx + 1 + (ny < 0 ? -ny : ny), cz = a.Kx + a.Ky + 2 + (nz < 0 ? -nz : nz);
+  const double sy = ny < 0 ? -1.0 : 1.0, sz = nz < 0 ? -1.0 : 1.0;   // negative n: conjugate
+  double re = 0.0, im = 0.0;
+  const int c0 = blockIdx.y * a.chunk, c1 = min(a.n, c0 + a.chunk);
+  for (int t0 = c0; t0 < c1; t0 += a.tile) {
+    const int na = min(a.tile, c1 - t0);
+    __syncthreads();
+    // one (atom, direction) row per thread
+    for (int r = threadIdx.x; r < 3 * na; r += blockDim.x) {
+      const int at = r / 3, d = r - 3 * (r / 3);
+      const int i = t0 + at;
+      const double sv = d == 0 ? a.sx[i] : (d == 1 ? a.sy[i] : a.sz[i]);
+      const int K = d == 0 ? a.Kx : (d == 1 ? a.Ky : a.Kz);
+      const int col = d == 0 ? 0 : (d == 1 ? a.Kx + 1 : a.Kx + a.Ky + 2);
+      phase_row(sv, K, tb + (size_t)at * ncol + col, 1);
+      if (d == 0) qs[at] = a.q[i];
+    }
+    __syncthreads();
+    if (act) {
+      for (int at = 0; at < na; ++at) {
+        const cplx<double>* row = tb + (size_t)at * ncol;
+        const cplx<double> X = row[cx];
+        cplx<double> Y = row[cy], Z = row[cz];
+        Y.im *= sy;
+        Z.im *= sz;
+        const cplx<double> e = cmul(cmul(X, Y), Z);
+        const double qa = qs[at];
+        re += qa * e.re;
+        im += qa * e.im;
+      }
+    }
+  }
+  if (!act) return;
+  a.part[(size_t)blockIdx.y * 2 * a.nb + 2 * k] = re;
+  a.part[(size_t)blockIdx.y * 2 * a.nb + 2 * k + 1] = im;
+}
+
+__global__ void __launch_bounds__(256) k_in_reduce(InArgs a, int nchunk) {
+  __shared__ double sh[4 * kRedMaxCols];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[kRedMaxCols];
+#pragma unroll
+  for (int c = 0; c < kRedMaxCols; ++c) v[c] = 0.0;
+  if (k < a.nb) {
+    double re = 0.0, im = 0.0;
+    for (int c = 0; c < nchunk; ++c) {
+      re += a.part[(size_t)c * 2 * a.nb + 2 * k];
+      im += a.part[(size_t)c * 2 * a.nb + 2 * k + 1];
+    }
+    const double Ak = a.kA[k];
+    a.wk[2 * k] = Ak * re;
+    a.wk[2 * k + 1] = -Ak * im;
+    const double str = (re * re + im * im) * Ak;
+    const double kx = a.kvec[4 * k], ky = a.kvec[4 * k + 1], kz = a.kvec[4 * k + 2], kc = a.kvec[4 * k + 3];
+    v[0] = str;
+    v[1] = (1.0 - kc * kx * kx) * str;
+    v[2] = (1.0 - kc * ky * ky) * str;
+    v[3] = (1.0 - kc * kz * kz) * str;
+    v[4] = -kc * kx * ky * str;
+    v[5] = -kc * kx * kz * str;
+    v[6] = -kc * ky * kz * str;
+  }
+  const double r = block_sum_cols(v, 7, sh);
+  if (threadIdx.x < 7) a.blk[blockIdx.x * 7 + threadIdx.x] = r;
+}
+
+// one thread per atom and one range of m = (nx, ny) values per blockIdx.y
+// (a.msplit values each; the ranges' moments are separate splits of the
+// output): W of the range through LDS in tiles of kInMT values of m
+// (wave-uniform reads); X, Y and Z of the atom by recurrence from
+// e^{2 pi i s} (each product within a few units of 2^-53)
+constexpr int kInMT = 8;
+__global__ void __launch_bounds__(256) k_in_gather(InArgs a) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<double>* ws = reinterpret_cast<cplx<double>*>(smem_raw);   // [kInMT][NZ]
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = i < a.n;
+  const int ii = act ? i : 0;
+  double x1r, x1i, y1r, y1i, z1r, z1i;
+  {
+    double t = a.sx[ii] - rint(a.sx[ii]);
+    sincospi(2.0 * t, &x1i, &x1r);
+    t = a.sy[ii] - rint(a.sy[ii]);
+    sincospi(2.0 * t, &y1i, &y1r);
+    t = a.sz[ii] - rint(a.sz[ii]);
+    sincospi(2.0 * t, &z1i, &z1r);
+  }
+  const cplx<double> x1{x1r, x1i}, y1{y1r, y1i}, zz1{z1r, z1i};
+  // Y(-Ky) and Z(-Kz): conjugate powers
+  cplx<double> y0{1.0, 0.0}, z0{1.0, 0.0};
+  for (int k = 0; k < a.Ky; ++k) y0 = cmul(y0, cplx<double>{y1r, -y1i});
+  for (int k = 0; k < a.Kz; ++k) z0 = cmul(z0, cplx<double>{z1r, -z1i});
+  const int M = (a.Kx + 1) * a.NY;
+  const int mb = blockIdx.y * a.msplit, me = min(M, mb + a.msplit);
+  // X and Y at the first m of the range
+  cplx<double> X{1.0, 0.0}, Y = y0;
+  {
+    const int nx0 = mb / a.NY, iy0 = mb - (mb / a.NY) * a.NY;
+    for (int k = 0; k < nx0; ++k) X = cmul(X, x1);
+    for (int k = 0; k < iy0; ++k) Y = cmul(Y, y1);
+  }
+  double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+  for (int mt = mb; mt < me; mt += kInMT) {
+    const int nm = min(kInMT, me - mt);
+    __syncthreads();
+    for (int x = threadIdx.x; x < nm * a.NZ; x += blockDim.x)
+      ws[x] = {a.wk[2 * ((size_t)mt * a.NZ + x)], a.wk[2 * ((size_t)mt * a.NZ + x) + 1]};
+    __syncthreads();
+    for (int mm = 0; mm < nm; ++mm) {
+      const int m = mt + mm;
+      const int nx = m / a.NY, ny = m - (m / a.NY) * a.NY - a.Ky;
+      const cplx<double> XY = cmul(X, Y);
+      const cplx<double>* wrow = ws + (size_t)mm * a.NZ;
+      double s0r = 0.0, s0i = 0.0, s1r = 0.0, s1i = 0.0;
+      cplx<double> Z = z0;
+      for (int t = 0; t < a.NZ; ++t) {
+        const cplx<double> w = wrow[t];
+        const double tr = w.re * Z.re - w.im * Z.im;
+        const double ti = w.re * Z.im + w.im * Z.re;
+        const double fz = (double)(t - a.Kz);
+        s0r += tr; s0i += ti;
+        s1r += fz * tr; s1i += fz * ti;
+        Z = cmul(Z, zz1);
+      }
+      const double i0 = XY.re * s0i + XY.im * s0r;
+      const double i1 = XY.re * s1i + XY.im * s1r;
+      u0 += (double)nx * i0;
+      u1 += (double)ny * i0;
+      u2 += i1;
+      // next m: ny + 1, or the next nx with ny = -Ky
+      if (ny == a.Ky) {
+        X = cmul(X, x1);
+        Y = y0;
+      } else {
+        Y = cmul(Y, y1);
+      }
+    }
+  }
+  if (!act) return;
+  double* o = a.out + (size_t)blockIdx.y * 3 * a.n;
+  o[i] = u0;
+  o[(size_t)a.n + i] = u1;
+  o[(size_t)2 * a.n + i] = u2;
+}
+
 }  // namespace
 
 // K-step counts compiled for the matrix-core gather (Kz + 1 rounded up)
@@ -1509,6 +1752,75 @@ static const void* sk_reduce_h3_kernel(int nsh) {
 
 // k-mesh on the box with the reference's k vectors and A(n), kcoe
 // (src/kspace.f90:162-191); allocation of the reciprocal buffers.
+// The inner k-box of the mixed form (charges only): half-widths from the
+// cutoff |k| <= 2.5 alpha (|n_d| <= |k| |a_d| / 2 pi, a_d lattice vector d),
+// within the main box; A and k of its entries as the main mesh's; the main
+// mesh's A copied with the box zeroed; chunking of k_in_sk.
+static int recip_inner_setup(mdff_ctx* c, const std::vector<double>& Amain) {
+  c->kin_on = 0;
+  const bool fp32 = c->d.precision == MDFF_PREC_MIXED;
+  if (!fp32 || !c->task_qq || c->task_qd || c->task_dd) return MDFF_OK;
+  const double kcut = 2.5 * c->alpha;
+  int kin[3];
+  for (int d = 0; d < 3; ++d) {
+    const double* a = c->A + 3 * d;
+    const double len = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    kin[d] = std::min(c->K[d], (int)ceil(kcut * len / (2.0 * MDFF_PI)));
+    if (kin[d] < 1) return MDFF_OK;
+  }
+  for (int d = 0; d < 3; ++d) c->kin[d] = kin[d];
+  const int NY = 2 * kin[1] + 1, NZ = 2 * kin[2] + 1;
+  const int nbi = (kin[0] + 1) * NY * NZ;
+  c->nb_in = nbi;
+  std::vector<double> Ai(nbi, 0.0), kv(4 * (size_t)nbi, 0.0), Am(Amain);
+  const int Ky = c->K[1], Kz = c->K[2];
+  for (int nx = 0; nx <= kin[0]; ++nx)
+    for (int ny = -kin[1]; ny <= kin[1]; ++ny)
+      for (int nz = -kin[2]; nz <= kin[2]; ++nz) {
+        const int im = (nx * c->NYk + (ny + Ky)) * c->NZk + (nz + Kz);
+        const int ii = (nx * NY + (ny + kin[1])) * NZ + (nz + kin[2]);
+        Ai[ii] = Amain[im];
+        Am[im] = 0.0;
+      }
+  // k vectors and kcoe of the inner entries: the main mesh's values
+  std::vector<double> kvm(4 * (size_t)c->nk_box);
+  HIP_TRY(c, hipMemcpy(kvm.data(), c->kvec, sizeof(double) * 4 * c->nk_box, hipMemcpyDeviceToHost));
+  for (int nx = 0; nx <= kin[0]; ++nx)
+    for (int ny = -kin[1]; ny <= kin[1]; ++ny)
+      for (int nz = -kin[2]; nz <= kin[2]; ++nz) {
+        const int im = (nx * c->NYk + (ny + Ky)) * c->NZk + (nz + Kz);
+        const int ii = (nx * NY + (ny + kin[1])) * NZ + (nz + kin[2]);
+        for (int u = 0; u < 4; ++u) kv[4 * (size_t)ii + u] = kvm[4 * (size_t)im + u];
+      }
+  HIP_TRY(c, hipMalloc((void**)&c->kA_in, sizeof(double) * nbi));
+  HIP_TRY(c, hipMalloc((void**)&c->kvec_in, sizeof(double) * 4 * nbi));
+  HIP_TRY(c, hipMalloc((void**)&c->kA_mask, sizeof(double) * c->nk_box));
+  HIP_TRY(c, hipMalloc((void**)&c->wk_in, sizeof(double) * 2 * nbi));
+  HIP_TRY(c, hipMemcpy(c->kA_in, Ai.data(), sizeof(double) * nbi, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->kvec_in, kv.data(), sizeof(double) * 4 * nbi, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->kA_mask, Am.data(), sizeof(double) * c->nk_box, hipMemcpyHostToDevice));
+  // k_in_sk: about 2048 blocks; tiles of atoms within 48 KiB of LDS
+  const int ncol = kin[0] + kin[1] + kin[2] + 3;
+  int tile = (int)std::min<size_t>(64, 49152 / (16 * (size_t)ncol + 8));
+  if (tile < 1) return MDFF_OK;
+  const int kblocks = div_up(nbi, 256);
+  const int want = std::max(1, 2048 / kblocks);
+  int chunk = div_up(c->n, want);
+  chunk = div_up(chunk, tile) * tile;
+  c->in_chunk = chunk;
+  c->in_nch = div_up(c->n, chunk);
+  c->kin_tile = tile;
+  HIP_TRY(c, hipMalloc((void**)&c->sk_in_part, sizeof(double) * 2 * (size_t)nbi * c->in_nch));
+  // k_in_gather: m ranges of whole kInMT tiles, about 2048 blocks
+  const int M = (kin[0] + 1) * NY;
+  const int tiles = div_up(M, kInMT);
+  const int sp = std::max(1, std::min(tiles, div_up(2048, div_up(c->n, 256))));
+  c->kin_msplit = div_up(tiles, sp) * kInMT;
+  c->kin_splits = div_up(M, c->kin_msplit);
+  c->kin_on = 1;
+  return MDFF_OK;
+}
+
 int recip_setup(mdff_ctx* c) {
   const int Kx = c->d.kES[0], Ky = c->d.kES[1], Kz = c->d.kES[2];
   c->K[0] = Kx; c->K[1] = Ky; c->K[2] = Kz;
@@ -1597,8 +1909,10 @@ int recip_setup(mdff_ctx* c) {
   }
   HIP_TRY(c, hipMalloc(&c->amat, tsz / 2 * (size_t)c->mma_mtiles * c->mma_nks * 64));
   HIP_TRY(c, hipMemset(c->amat, 0, tsz / 2 * (size_t)c->mma_mtiles * c->mma_nks * 64));
-  // block partials of the energy / stress: k-blocks of 256, or m-tiles of 8
-  const size_t nblk_rec = std::max<size_t>(div_up(nb, 256), div_up((int64_t)c->NXk * c->NYk, 8));
+  // block partials of the energy / stress: k-blocks of 256, or m-tiles of 8,
+  // then those of the inner box
+  const size_t nblk_rec = std::max<size_t>(div_up(nb, 256), div_up((int64_t)c->NXk * c->NYk, 8)) +
+                          div_up((int64_t)(Kx + 1) * (2 * Ky + 1) * (2 * Kz + 1), 256);
   HIP_TRY(c, hipMalloc((void**)&c->rec_blk, sizeof(double) * 7 * nblk_rec));
   // mixed form: split-half kernels when their shapes fit (MDFF_RECIP_F32MMA=1
   // keeps the FP32 matrix-core kernels)
@@ -1643,19 +1957,23 @@ int recip_setup(mdff_ctx* c) {
       }
       HIP_TRY(c, hipMalloc(&c->amat_h3, 16 * 128 * (size_t)nsh * c->h3_mt8));
       HIP_TRY(c, hipMalloc((void**)&c->ascale_h3, sizeof(float) * (size_t)c->h3_mt8));
+      HIP_TRY(c, hipMalloc((void**)&c->mu_pscale, sizeof(float) * 2));
     }
   }
   HIP_TRY(c, hipMalloc(&c->sk_part, sizeof(double) * 4 * (size_t)std::max({c->sk_chunks, c->sk_mma_ch, c->h3_ch}) * nb));
+  MDFF_TRY(recip_inner_setup(c, A));
   splits = std::max(splits, c->mma_splits);
   if (c->h3) splits = std::max(splits, c->h3_splits);
   c->rec_splits = splits;
-  HIP_TRY(c, hipMalloc((void**)&c->rec_part, sizeof(double) * 9 * (size_t)splits * c->n));
+  // moments of every split, plus (3 moments each) the inner box's
+  HIP_TRY(c, hipMalloc((void**)&c->rec_part, sizeof(double) * (9 * (size_t)splits + 3 * (size_t)c->kin_splits) * c->n));
   return MDFF_OK;
 }
 
 void recip_free(mdff_ctx* c) {
   void* ps[] = {c->kA, c->kvec, c->tab, c->sk_part, c->sk, c->wk, c->wk_t, c->rec_part, c->amat, c->rec_blk,
-                c->amat_h3, c->ascale_h3};
+                c->amat_h3, c->ascale_h3, c->kA_mask, c->kA_in, c->kvec_in, c->sk_in_part, c->wk_in,
+                c->mu_pscale};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -1775,7 +2093,14 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     float* p1 = reinterpret_cast<float*>(c->sk_part);
     sh.part = p1;
     sh.mode = use_mu ? 1 : 0;
-    sh.pscale = use_mu ? 1.f : c->h3_pscale;
+    sh.pscale = c->h3_pscale;
+    sh.pscale_dev = nullptr;
+    if (use_mu) {
+      const double* b = c->B;
+      hipLaunchKernelGGL(k_mu_pscale, dim3(1), dim3(256), 0, c->stream, c->n, c->K[0], c->K[1], sh.q, mu, b[0],
+                         b[1], b[2], b[3], b[4], b[5], b[6], b[7], b[8], c->mu_pscale);
+      sh.pscale_dev = c->mu_pscale;
+    }
     const size_t lds = sk_h3_lds(c->K[0], c->K[1], kH3TA);
     dim3 gs(div_up(div_up(s.M, 32), 4), ch, div_up(c->NZk, 64));
     if (use_mu) PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, 1>), gs, dim3(256), lds, c->stream, sh));
@@ -1834,6 +2159,28 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     dim3 g1(div_up((int64_t)s.M * s.NG, 256), chunks);
     PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk<T, GZ>), g1, dim3(256), shm, c->stream, s));
   }
+  // inner k-box in FP64 (mixed form, charges only): its structure factor
+  // here; the main kernels below see A = 0 on the box
+  const bool inner = sizeof(T) == 4 && c->kin_on && use_q && !use_mu;
+  const double* kA = inner ? c->kA_mask : c->kA;
+  InArgs ia{};
+  if (inner) {
+    ia.n = c->n;
+    ia.Kx = c->kin[0]; ia.Ky = c->kin[1]; ia.Kz = c->kin[2];
+    ia.NY = 2 * c->kin[1] + 1; ia.NZ = 2 * c->kin[2] + 1;
+    ia.nb = c->nb_in;
+    ia.chunk = c->in_chunk;
+    ia.tile = c->kin_tile;
+    ia.sx = c->sx; ia.sy = c->sy; ia.sz = c->sz;
+    ia.q = q;
+    ia.kA = c->kA_in;
+    ia.kvec = c->kvec_in;
+    ia.part = c->sk_in_part;
+    ia.wk = c->wk_in;
+    const size_t lds = (size_t)ia.tile * (16 * (size_t)(ia.Kx + ia.Ky + ia.Kz + 3) + 8);
+    PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL(k_in_sk, dim3(div_up(ia.nb, 256), c->in_nch), dim3(256), lds,
+                                             c->stream, ia));
+  }
   // the 7 energy / stress sums over the k-blocks: block partials here, the
   // final sum queued into e_out (device memory)
   const bool dipf = forces && use_mu;
@@ -1843,28 +2190,33 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
   T* amat = mma_gather ? reinterpret_cast<T*>(c->amat) : nullptr;
   const int nb = h3_gather ? c->h3_mt8 : div_up(c->nk_box, 256);
   MDFF_TRY(claim_partials(c, c->rec_blk));
+  const int nbi = inner ? div_up(c->nb_in, 256) : 0;
+  if (inner) {
+    ia.blk = c->rec_blk + (size_t)7 * nb;     // after the main blocks
+    PROF_LAUNCH(c, PK_SKRED, hipLaunchKernelGGL(k_in_reduce, dim3(nbi), dim3(256), 0, c->stream, ia, c->in_nch));
+  }
   PROF_LAUNCH(c, PK_SKRED,
               if (h3_gather) {
                 const void* prt = c->sk_part;        // FP32 (split-half) or FP64 (FP32 matrix-core) partials
                 float* wkt = reinterpret_cast<float*>(c->wk_t);
                 u32x4* am = reinterpret_cast<u32x4*>(c->amat_h3);
-                void* rargs[] = {(void*)&c->nk_box, (void*)&chunks, (void*)&prt, (void*)&c->kA, (void*)&c->kvec,
+                void* rargs[] = {(void*)&c->nk_box, (void*)&chunks, (void*)&prt, (void*)&kA, (void*)&c->kvec,
                                  (void*)&c->sk, (void*)&c->wk, (void*)&wkt, (void*)&c->rec_blk, (void*)&c->NZk,
                                  (void*)&c->K[2], (void*)&am, (void*)&c->ascale_h3};
                 const void* fn = sk_fp64 ? sk_reduce_h3_kernel<double>(c->h3_nsh) : sk_reduce_h3_kernel<float>(c->h3_nsh);
                 (void)hipLaunchKernel(fn, dim3(nb), dim3(256), rargs, 0, c->stream);
               } else if (sk_fp64) {
                 hipLaunchKernelGGL((k_sk_reduce<T, double>), dim3(nb), dim3(256), 0, c->stream, c->nk_box, chunks,
-                                   reinterpret_cast<const double*>(c->sk_part), c->kA, c->kvec, c->sk, c->wk,
+                                   reinterpret_cast<const double*>(c->sk_part), kA, c->kvec, c->sk, c->wk,
                                    reinterpret_cast<T*>(c->wk_t), c->rec_blk, amat, c->NZk, c->K[2],
                                    c->mma_nks);
               } else {
                 hipLaunchKernelGGL((k_sk_reduce<T, T>), dim3(nb), dim3(256), 0, c->stream, c->nk_box, chunks,
-                                   reinterpret_cast<const T*>(c->sk_part), c->kA, c->kvec, c->sk, c->wk,
+                                   reinterpret_cast<const T*>(c->sk_part), kA, c->kvec, c->sk, c->wk,
                                    reinterpret_cast<T*>(c->wk_t), c->rec_blk, amat, c->NZk, c->K[2],
                                    c->mma_nks);
               });
-  if (e_out) MDFF_TRY(queue_final(c, c->rec_blk, nb, 7, e_out));
+  if (e_out) MDFF_TRY(queue_final(c, c->rec_blk, nb + nbi, 7, e_out));
   if (!forces && !field) return MDFF_OK;
   int nsplit = 0, moments = dipf ? 9 : 3;
   if (h3_gather) {
@@ -1924,6 +2276,15 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     dim3 g2(div_up(c->n, nthr), nsplit);
     if (dipf) PROF_LAUNCH(c, PK_GATHER, hipLaunchKernelGGL((k_gather<T, 9>), g2, dim3(nthr), shm2, c->stream, g));
     else PROF_LAUNCH(c, PK_GATHER, hipLaunchKernelGGL((k_gather<T, 3>), g2, dim3(nthr), shm2, c->stream, g));
+  }
+  if (inner) {
+    // the inner box's moments: kin_splits more splits after the main ones
+    ia.out = c->rec_part + (size_t)nsplit * 3 * c->n;
+    ia.msplit = c->kin_msplit;
+    const size_t lds = (size_t)kInMT * ia.NZ * 16;
+    PROF_LAUNCH(c, PK_GATHER, hipLaunchKernelGGL(k_in_gather, dim3(div_up(c->n, 256), c->kin_splits), dim3(256),
+                                                 lds, c->stream, ia));
+    nsplit += c->kin_splits;
   }
   RecFinal rf{};
   rf.n = c->n;

@@ -107,6 +107,15 @@ program mdff_gpu_driver
   logical :: sym_nb, sym_pim
   real(dp) :: ekin, tempi, upot, etot, conint, r1, sumnoise, kT
   real(dp), allocatable :: ek(:), up(:), env(:), xi(:), vxi(:)
+  ! ---- thermodynamic (src/thermo.f90:37-110, calc_thermo :122-211)
+  real(dp) :: B(3, 3), omega_c, anorm(3)
+  real(dp) :: e_kin, temp_r, e_nvt, u_tot, e_tot, h_tot, u_vdw, pvirial_vdw, pvirial_coul, pressure_tot
+  real(dp) :: pvirial_tot
+  integer :: oszcall, itime0, itime1, u_osz, u_traj
+  ! ---- RESTART (src/restart.f90:34-478)
+  integer, parameter :: vnlmax = 2000
+  integer :: itime_rst = 1
+  real(dp), allocatable :: xi_rst(:), vxi_rst(:)
 
   ctx = c_null_ptr
   if (command_argument_count() < 1) then
@@ -119,7 +128,11 @@ program mdff_gpu_driver
 
   call default_tags()
   call read_tags()
-  call read_posff()
+  if (full_restart) then
+    call read_restart()
+  else
+    call read_posff()
+  end if
   call fill_desc()
 
   st = mdff_ctx_create(d, itype, ctx)
@@ -127,12 +140,13 @@ program mdff_gpu_driver
   write (*, '(a,a)') 'engforce library : ', mdff_version_string()
   st = mdff_set_positions(ctx, rx, ry, rz)
   call mdff_check(st, ctx, 'mdff_set_positions')
-  call init_velocities()
+  if (.not. full_restart) call init_velocities()
   st = mdff_set_velocities(ctx, vx, vy, vz)
   call mdff_check(st, ctx, 'mdff_set_velocities')
 
   ! ---- t = 0 force call
   st = mdff_engforce(ctx, fx, fy, fz, t)
+  if (st == MDFF_ERR_POLAR_CATASTROPHE) call write_dipff()
   call mdff_check(st, ctx, 'engforce')
   call print_terms(t)
   open (newunit=u, file='FORCES', status='replace')
@@ -141,60 +155,8 @@ program mdff_gpu_driver
   end do
   close (u)
 
-  ! ---- MD
-  if (npas > 0 .and. trim(calc) == 'md') then
-    if (trim(integrator) /= 'nve-vv' .and. trim(integrator) /= 'nvt-nhcn') then
-      write (*, '(a,a)') 'ERROR integrator not available on this path: ', trim(integrator)
-      stop 1
-    end if
-    if (nequil > 0 .and. trim(integrator) == 'nve-vv' .and. .not. lcsvr) then
-      write (*, '(a)') 'ERROR Berendsen equilibration rescale not available on this path (use lcsvr)'
-      stop 1
-    end if
-    write (*, '(a)') '      step        time(ps)              etot              ekin              upot' // &
-                     '              temp'
-    allocate (ek(max(nprint, 1)), up(max(nprint, 1)), env(max(nprint, 1)))
-    allocate (xi(nhc_n), vxi(nhc_n))
-    xi = 0.0_dp
-    vxi = 0.0_dp
-    conint = 0.0_dp
-    kT = temp * boltz_unit
-    itime = 0
-    do while (itime < npas)
-      ! one step at a time while the equilibration rescale may apply, else a
-      ! chunk up to the next print
-      if (lcsvr .and. itime + 1 <= nequil) then
-        nchunk = 1
-      else
-        nchunk = min(max(nprint, 1) - mod(itime, max(nprint, 1)), npas - itime)
-      end if
-      if (trim(integrator) == 'nve-vv') then
-        st = mdff_md_vv(ctx, int(nchunk, c_int32_t), dt * time_unit, ek, up)
-      else
-        st = mdff_md_nhcn(ctx, int(nchunk, c_int32_t), dt * time_unit, kT, timesca_thermo * time_unit, &
-                          int(nhc_n, c_int32_t), int(nhc_yosh_order, c_int32_t), &
-                          int(nhc_mults, c_int32_t), xi, vxi, ek, up, env)
-      end if
-      call mdff_check(st, ctx, 'md step')
-      itime = itime + nchunk
-      ! equilibration rescale after the step and its skin check (src/runmd.f90:316-330)
-      if (lcsvr .and. itime <= nequil .and. mod(itime, max(nequil_period, 1)) == 0 .and. &
-          itime /= 1 .and. itime /= npas) then
-        call gauss(r1)
-        sumnoise = sum_noises(3 * natm - 4)
-        st = mdff_csvr_rescale(ctx, kT, taucsvr * time_unit, r1, sumnoise, conint)
-        call mdff_check(st, ctx, 'csvr rescale')
-      end if
-      if (mod(itime, max(nprint, 1)) == 0 .or. itime == npas) then
-        st = mdff_calc_temp(ctx, boltz_unit, tempi, ekin)
-        call mdff_check(st, ctx, 'calc_temp')
-        upot = up(nchunk)
-        etot = ekin + upot
-        write (*, '(i10,f16.6,4es18.9)') itime, itime * dt, etot, ekin, upot, tempi
-      end if
-    end do
-    call write_contff()
-  end if
+  ! ---- MD (md_run, src/runmd.f90:63-504)
+  if (trim(calc) == 'md') call md_run()
 
   st = mdff_ctx_destroy(ctx)
 
@@ -533,6 +495,755 @@ contains
     write (*, '(a,9es25.16e3)') 'tau_coul     = ', tt%tau_coul
     write (*, '(a,i0)') 'n_scf        = ', tt%n_scf
   end subroutine print_terms
+
+  ! lattice (src/lattice.f90:60-168): B = reciprocal basis / omega, the
+  ! cell volume and the lattice vector lengths
+  subroutine lattice_b()
+    integer :: j
+    B(1, 1) = A(2, 2) * A(3, 3) - A(3, 2) * A(2, 3)
+    B(2, 1) = A(3, 2) * A(1, 3) - A(1, 2) * A(3, 3)
+    B(3, 1) = A(1, 2) * A(2, 3) - A(2, 2) * A(1, 3)
+    B(1, 2) = A(2, 3) * A(3, 1) - A(3, 3) * A(2, 1)
+    B(2, 2) = A(3, 3) * A(1, 1) - A(1, 3) * A(3, 1)
+    B(3, 2) = A(1, 3) * A(2, 1) - A(2, 3) * A(1, 1)
+    B(1, 3) = A(2, 1) * A(3, 2) - A(3, 1) * A(2, 2)
+    B(2, 3) = A(3, 1) * A(1, 2) - A(1, 1) * A(3, 2)
+    B(3, 3) = A(1, 1) * A(2, 2) - A(2, 1) * A(1, 2)
+    omega_c = B(1, 1) * A(1, 1) + B(2, 1) * A(2, 1) + B(3, 1) * A(3, 1)
+    B = B / omega_c
+    do j = 1, 3
+      anorm(j) = sqrt(sum(A(:, j) * A(:, j)))
+    end do
+  end subroutine lattice_b
+
+  ! calc_thermo (src/thermo.f90:122-211) from the terms of the last force
+  ! evaluation and the kinetic energy of the step
+  subroutine calc_thermo(tt)
+    type(mdff_terms), intent(in) :: tt
+    real(dp) :: pv_lj, pv_morse, pv_bmhft
+    temp_r = 2.0_dp * e_kin / (3.0_dp * real(natm, dp) * boltz_unit)
+    u_tot = tt%u_lj + tt%u_coul + tt%u_morse + tt%u_bmhft + tt%u_tail
+    e_tot = u_tot + e_kin
+    u_vdw = tt%u_lj + tt%u_bmhft + tt%u_morse
+    pv_lj = tt%vir_lj / omega_c
+    pv_morse = tt%vir_morse / omega_c
+    pv_bmhft = tt%vir_bmhft / omega_c
+    pvirial_coul = tt%pvirial_coul
+    pvirial_tot = pv_lj + pvirial_coul + pv_morse + pv_bmhft
+    pvirial_vdw = pv_lj + pv_morse + pv_bmhft
+    pressure_tot = (pvirial_tot + real(natm, dp) * temp_r * boltz_unit / omega_c) / press_unit
+    if (trim(integrator) == 'nvt-nhcn') then
+      h_tot = e_tot + e_nvt
+    else
+      h_tot = e_tot
+      if (lcsvr) h_tot = e_tot + conint
+    end if
+  end subroutine calc_thermo
+
+  ! write_thermo (src/thermo.f90:340-424): 'osz' lines to OSZIFF (a header
+  ! every 10 calls), 'std' block to the standard output
+  subroutine write_thermo(step, key)
+    integer, intent(in) :: step
+    character(len=3), intent(in) :: key
+    real(dp) :: time, density
+    time = real(step, dp) * dt
+    density = real(natm, dp) / omega_c
+    if (key == 'osz') then
+      if (mod(oszcall, 10) == 0) then
+        write (u_osz, '(a)') repeat('-', 110)
+        write (u_osz, '(a)') '     step      Time/Temp   Etot/Pressure    Ekin/Pvir_nb  Utot/Pvir_coul' // &
+                             '      U_nb/Omega     U_coul/Htot'
+        write (u_osz, '(a)') repeat('-', 110)
+      end if
+      write (u_osz, '(I9,2X,6E16.8)') step, time, e_tot, e_kin, u_tot, u_vdw, t%u_coul
+      write (u_osz, '(I9,2X,6E16.8)') step, temp_r, pressure_tot, pvirial_vdw, pvirial_coul, omega_c, h_tot
+      oszcall = oszcall + 1
+    else
+      write (*, 300) step, time, e_kin, temp_r, u_tot, u_vdw, t%u_coul, pressure_tot, pvirial_vdw, &
+                     pvirial_coul, pvirial_tot, omega_c, anorm(1), anorm(2), anorm(3), density, e_tot, h_tot
+    end if
+300 format(/ '  Thermodynamic information ' / '  ---------------------------------------------' / &
+           '  step                  = ', I9 / '  time                  = ', E19.12 / &
+           '  Ekin                  = ', E19.12 / '  Temp                  = ', E19.12 / &
+           '  Utot                  = ', E19.12 / '  U_vdw                 = ', E19.12 / &
+           '  U_coul                = ', E19.12 / '  Pressure              = ', E19.12 / &
+           '  Pvir_vdw              = ', E19.12 / '  Pvir_coul             = ', E19.12 / &
+           '  Pvir_tot              = ', E19.12 / '  volume                = ', E19.12 / &
+           '  a cell                = ', E19.12 / '  b cell                = ', E19.12 / &
+           '  c cell                = ', E19.12 / '  density               = ', E19.12 / &
+           '  ---------------------------------------------' / '  Etot                  = ', E19.12 / &
+           '  Htot                  = ', E19.12)
+  end subroutine write_thermo
+
+  ! write_trajff_xyz (src/config.f90:543-630): positions folded into the
+  ! cell (kardir, periodicbc with nint, dirkar), formatted or unformatted
+  subroutine write_trajff()
+    real(dp), allocatable :: xx(:), yy(:), zz(:)
+    real(dp) :: v1, v2, v3
+    integer :: i, j
+    st = mdff_get_positions(ctx, rx, ry, rz)
+    call mdff_check(st, ctx, 'get_positions')
+    st = mdff_get_velocities(ctx, vx, vy, vz)
+    call mdff_check(st, ctx, 'get_velocities')
+    st = mdff_get_forces(ctx, fx, fy, fz)
+    call mdff_check(st, ctx, 'get_forces')
+    allocate (xx(natm), yy(natm), zz(natm))
+    do ia = 1, natm
+      v1 = rx(ia) * B(1, 1) + ry(ia) * B(2, 1) + rz(ia) * B(3, 1)
+      v2 = rx(ia) * B(1, 2) + ry(ia) * B(2, 2) + rz(ia) * B(3, 2)
+      v3 = rx(ia) * B(1, 3) + ry(ia) * B(2, 3) + rz(ia) * B(3, 3)
+      v1 = v1 - nint(v1); v2 = v2 - nint(v2); v3 = v3 - nint(v3)
+      xx(ia) = v1 * A(1, 1) + v2 * A(1, 2) + v3 * A(1, 3)
+      yy(ia) = v1 * A(2, 1) + v2 * A(2, 2) + v3 * A(2, 3)
+      zz(ia) = v1 * A(3, 1) + v2 * A(3, 2) + v3 * A(3, 3)
+    end do
+    if (itraj_format /= 0) then
+      write (u_traj, '(i6)') natm
+      write (u_traj, '(a)') system_name
+      do i = 1, 3
+        write (u_traj, '(3f20.12)') (A(i, j), j=1, 3)
+      end do
+      write (u_traj, '(i4)') ntype
+      write (u_traj, *) (atypei(it), it=1, ntype)
+      write (u_traj, *) (natmi(it), it=1, ntype)
+      write (u_traj, '(A)') 'Cartesian'
+      do ia = 1, natm
+        select case (trajff_data)
+        case ('rvf')
+          write (u_traj, '(a,9e20.12)') atype(ia), xx(ia), yy(ia), zz(ia), vx(ia), vy(ia), vz(ia), &
+                                        fx(ia), fy(ia), fz(ia)
+        case ('rvn')
+          write (u_traj, '(a,9e20.12)') atype(ia), xx(ia), yy(ia), zz(ia), vx(ia), vy(ia), vz(ia)
+        case ('rnf')
+          write (u_traj, '(a,9e20.12)') atype(ia), xx(ia), yy(ia), zz(ia), fx(ia), fy(ia), fz(ia)
+        case ('rnn')
+          write (u_traj, '(a,9e20.12)') atype(ia), xx(ia), yy(ia), zz(ia)
+        end select
+      end do
+    else
+      write (u_traj) natm
+      write (u_traj) system_name
+      do i = 1, 3
+        write (u_traj) (A(i, j), j=1, 3)
+      end do
+      write (u_traj) ntype
+      write (u_traj) (atypei(it), it=1, ntype)
+      write (u_traj) (natmi(it), it=1, ntype)
+      write (u_traj) 'C'
+      select case (trajff_data)
+      case ('rvf')
+        write (u_traj) xx, yy, zz, vx, vy, vz, fx, fy, fz
+      case ('rvn')
+        write (u_traj) xx, yy, zz, vx, vy, vz
+      case ('rnf')
+        write (u_traj) xx, yy, zz, fx, fy, fz
+      case ('rnn')
+        write (u_traj) xx, yy, zz
+      end select
+    end if
+    deallocate (xx, yy, zz)
+  end subroutine write_trajff
+
+  ! write_DIPFF (src/field.f90:300-340): the total dipoles mu_t
+  subroutine write_dipff()
+    integer :: uu
+    real(dp), allocatable :: mu(:, :), ef(:, :)
+    integer(c_int) :: s2
+    allocate (mu(3, natm), ef(3, natm))
+    s2 = mdff_get_dipoles(ctx, mu, ef)
+    open (newunit=uu, file='DIPFF', status='unknown')
+    write (uu, *) natm
+    write (uu, *) system_name
+    write (uu, *) A(1, 1), A(2, 1), A(3, 1)
+    write (uu, *) A(1, 2), A(2, 2), A(3, 2)
+    write (uu, *) A(1, 3), A(2, 3), A(3, 3)
+    write (uu, *) ntype
+    write (uu, *) (atypei(it), it=1, ntype)
+    write (uu, *) (natmi(it), it=1, ntype)
+    write (uu, '(a)') '      ia type                   mux                  muy                 muz'
+    do ia = 1, natm
+      write (uu, '(i8,2x,a3,6e24.16)') ia, atype(ia), mu(1, ia), mu(2, ia), mu(3, ia)
+    end do
+    close (uu)
+    deallocate (mu, ef)
+  end subroutine write_dipff
+
+  ! write_EFALL (src/field.f90:455-497): the electric field ef_t
+  subroutine write_efall()
+    integer :: uu
+    real(dp), allocatable :: mu(:, :), ef(:, :)
+    allocate (mu(3, natm), ef(3, natm))
+    st = mdff_get_dipoles(ctx, mu, ef)
+    call mdff_check(st, ctx, 'get_dipoles')
+    open (newunit=uu, file='EFALL', status='unknown')
+    write (uu, *) natm
+    write (uu, *) system_name
+    write (uu, *) A(1, 1), A(2, 1), A(3, 1)
+    write (uu, *) A(1, 2), A(2, 2), A(3, 2)
+    write (uu, *) A(1, 3), A(2, 3), A(3, 3)
+    write (uu, *) ntype
+    write (uu, *) (atypei(it), it=1, ntype)
+    write (uu, *) (natmi(it), it=1, ntype)
+    write (uu, '(a)') '      ia type                   Ex                      Ey                      Ez'
+    do ia = 1, natm
+      write (uu, '(i8,2x,a3,3e24.16)') ia, atype(ia), ef(1, ia), ef(2, ia), ef(3, ia)
+    end do
+    close (uu)
+    deallocate (mu, ef)
+  end subroutine write_efall
+
+  ! the next step after `now` at which the host acts: the equilibration
+  ! rescale, a trajectory frame, thermo output, CONTFF, or the last step
+  integer function next_event(now) result(nx)
+    integer, intent(in) :: now
+    nx = now
+    do
+      nx = nx + 1
+      if (nx >= itime1) exit
+      if (nx <= nequil .and. mod(nx, max(nequil_period, 1)) == 0) exit
+      if (ltraj .and. nx > itraj_start .and. mod(nx, max(itraj_period, 1)) == 0) exit
+      if (mod(nx, max(nprint, 1)) == 0 .or. mod(nx, max(fprint, 1)) == 0 .or. mod(nx, max(spas, 1)) == 0) exit
+    end do
+  end function next_event
+
+  ! md_run (src/runmd.f90:63-504) on the device: steps between host actions
+  ! run in one call of the library's MD entry point; at each step the host
+  ! acts on, in the reference's order: the step (with its skin check), the
+  ! equilibration rescale, the trajectory frame, calc_thermo and the
+  ! standard-output / OSZIFF / CONTFF writes
+  subroutine md_run()
+    integer :: nchunk, nmax
+    real(dp) :: lambda
+    if (trim(integrator) /= 'nve-vv' .and. trim(integrator) /= 'nvt-nhcn') then
+      write (*, '(a,a)') 'ERROR integrator not available on this path: ', trim(integrator)
+      stop 1
+    end if
+    call lattice_b()
+    ! md_init (src/md.f90:308-309): itime0 = itime = 1, itime1 = itime0 - 1 +
+    ! npas; after a full restart restart_init sets itime0 = itime (the step
+    ! after the last one written), itime1 = itime0 + npas
+    ! (src/restart.f90:196-197)
+    if (full_restart) then
+      itime0 = itime_rst
+      itime1 = itime0 + npas
+    else
+      itime0 = 1
+      itime1 = itime0 - 1 + npas
+    end if
+    oszcall = 0
+    open (newunit=u_osz, file='OSZIFF', status='unknown')
+    if (itraj_format /= 0) open (newunit=u_traj, file='TRAJFF')
+    if (itraj_format == 0) open (newunit=u_traj, file='TRAJFF', form='unformatted')
+    allocate (xi(nhc_n), vxi(nhc_n))
+    xi = 0.0_dp
+    vxi = 0.0_dp
+    if (full_restart .and. allocated(xi_rst)) then
+      if (size(xi_rst) == nhc_n) then
+        xi = xi_rst
+        vxi = vxi_rst
+      end if
+    end if
+    conint = 0.0_dp
+    e_nvt = 0.0_dp
+    kT = temp * boltz_unit
+    ! thermo of the configuration at t = 0 (src/runmd.f90:160-183)
+    st = mdff_calc_temp(ctx, boltz_unit, tempi, e_kin)
+    call mdff_check(st, ctx, 'calc_temp')
+    call calc_thermo(t)
+    call write_thermo(itime0 - 1, 'std')
+    call write_thermo(itime0 - 1, 'osz')
+    nmax = max(1, itime1 - itime0 + 1)
+    allocate (ek(nmax), up(nmax), env(nmax))
+    env = 0.0_dp
+    itime = itime0 - 1
+    do while (itime < itime1)
+      nchunk = next_event(itime) - itime
+      if (trim(integrator) == 'nve-vv') then
+        st = mdff_md_vv(ctx, int(nchunk, c_int32_t), dt * time_unit, ek, up)
+      else
+        st = mdff_md_nhcn(ctx, int(nchunk, c_int32_t), dt * time_unit, kT, timesca_thermo * time_unit, &
+                          int(nhc_n, c_int32_t), int(nhc_yosh_order, c_int32_t), &
+                          int(nhc_mults, c_int32_t), xi, vxi, ek, up, env)
+      end if
+      if (st == MDFF_ERR_POLAR_CATASTROPHE) call write_dipff()
+      call mdff_check(st, ctx, 'md step')
+      itime = itime + nchunk
+      e_kin = ek(nchunk)
+      e_nvt = env(nchunk)
+      st = mdff_last_terms(ctx, t)
+      call mdff_check(st, ctx, 'last_terms')
+      ! equilibration rescale of the NVE integrators (src/runmd.f90:316-330,
+      ! rescale_velocities src/kinetic.f90:122-198)
+      if (trim(integrator) == 'nve-vv' .and. itime <= nequil .and. mod(itime, max(nequil_period, 1)) == 0 .and. &
+          itime /= itime0 .and. itime /= itime1) then
+        if (lcsvr) then
+          call gauss(r1)
+          sumnoise = sum_noises(3 * natm - 4)
+          st = mdff_csvr_rescale(ctx, kT, taucsvr * time_unit, r1, sumnoise, conint)
+          call mdff_check(st, ctx, 'csvr rescale')
+        else
+          st = mdff_calc_temp(ctx, boltz_unit, tempi, ekin)
+          call mdff_check(st, ctx, 'calc_temp')
+          lambda = (1.0_dp + (dt / tauTberendsen) * ((kT / tempi / boltz_unit) - 1.0_dp))**0.5_dp
+          if (annealing /= 1.0_dp) lambda = annealing
+          st = mdff_rescale_velocities(ctx, lambda)
+          call mdff_check(st, ctx, 'rescale velocities')
+        end if
+      end if
+      if (ltraj .and. itime > itraj_start .and. mod(itime, max(itraj_period, 1)) == 0) call write_trajff()
+      call calc_thermo(t)
+      if (mod(itime, max(nprint, 1)) == 0) call write_thermo(itime, 'std')
+      if (mod(itime, max(fprint, 1)) == 0) call write_thermo(itime, 'osz')
+      if (mod(itime, max(spas, 1)) == 0) call write_contff()
+    end do
+    close (u_osz)
+    close (u_traj)
+    ! final configuration (src/mdff.f90:337-345); the loop index of the
+    ! reference ends one past itime1, the step a restart starts from
+    call write_contff()
+    if (doefield) call write_efall()
+    if (lwrite_restart) call write_restart(itime1 + 1)
+  end subroutine md_run
+
+  ! write_RESTART (src/restart.f90:295-478): the reference's unformatted
+  ! record sequence, with the reference's types and sizes (character
+  ! lengths, ntypemax-sized tables, natm*vnlmax list arrays); md times and
+  ! temperatures in internal units as the reference holds them.  Quantities
+  ! outside this path (leap-frog / Beeman / barostat state, quadrupoles,
+  ! Wannier flags) carry their reference defaults.
+  subroutine write_restart(itime_next)
+    integer, intent(in) :: itime_next
+    integer :: uu, j
+    character(len=3) :: atypei0(0:ntypemax)
+    integer :: natmi0(0:ntypemax)
+    integer, allocatable :: lst(:), pnt(:)
+    integer(c_int64_t) :: nl
+    real(dp), allocatable :: zero1(:), sfx(:), sfy(:), sfz(:), dipia(:, :), q0(:), m0(:), pold(:, :, :), &
+                             ipold(:, :, :), mu_t(:, :), ef_t(:, :), quad3(:, :, :), quad4(:, :, :, :)
+    logical, allocatable :: ipolar(:)
+    logical :: task_coul(6), lfalse
+    real(dp) :: vxib(max(nhc_n, 1)), xib(max(nhc_n, 1)), xiw(max(nhc_n, 1)), vxiw(max(nhc_n, 1)), det
+    integer :: nth
+    st = mdff_get_positions(ctx, rx, ry, rz)
+    call mdff_check(st, ctx, 'get_positions')
+    st = mdff_get_velocities(ctx, vx, vy, vz)
+    call mdff_check(st, ctx, 'get_velocities')
+    st = mdff_get_forces(ctx, fx, fy, fz)
+    call mdff_check(st, ctx, 'get_forces')
+    atypei0 = '   '
+    atypei0(1:ntype) = atypei(1:ntype)
+    atypei0(0) = 'ALL'
+    natmi0 = 0
+    natmi0(1:ntype) = natmi(1:ntype)
+    natmi0(0) = natm
+    allocate (zero1(natm), sfx(natm), sfy(natm), sfz(natm), dipia(3, natm), q0(natm), m0(natm), &
+              pold(3, 3, natm), ipold(3, 3, natm), mu_t(3, natm), ef_t(3, natm), quad3(3, 3, natm), &
+              quad4(3, 3, 3, natm), ipolar(natm), lst(natm * vnlmax), pnt(natm + 1))
+    zero1 = 0.0_dp
+    quad3 = 0.0_dp
+    quad4 = 0.0_dp
+    pold = 0.0_dp
+    ipold = 0.0_dp
+    do ia = 1, natm
+      it = itype(ia)
+      q0(ia) = qch(it)
+      m0(ia) = mass(it)
+      dipia(:, ia) = dip(it, :)
+      ipolar(ia) = ldip_polar(it)
+      if (ldip_polar(it)) then
+        pold(:, :, ia) = poldip(it, :, :)
+        ! 3x3 inverse (typeinfo_init's DGETRF/DGETRI, src/read_pos.f90:203-212)
+        det = pold(1, 1, ia) * (pold(2, 2, ia) * pold(3, 3, ia) - pold(2, 3, ia) * pold(3, 2, ia)) - &
+              pold(1, 2, ia) * (pold(2, 1, ia) * pold(3, 3, ia) - pold(2, 3, ia) * pold(3, 1, ia)) + &
+              pold(1, 3, ia) * (pold(2, 1, ia) * pold(3, 2, ia) - pold(2, 2, ia) * pold(3, 1, ia))
+        ipold(1, 1, ia) = (pold(2, 2, ia) * pold(3, 3, ia) - pold(2, 3, ia) * pold(3, 2, ia)) / det
+        ipold(1, 2, ia) = (pold(1, 3, ia) * pold(3, 2, ia) - pold(1, 2, ia) * pold(3, 3, ia)) / det
+        ipold(1, 3, ia) = (pold(1, 2, ia) * pold(2, 3, ia) - pold(1, 3, ia) * pold(2, 2, ia)) / det
+        ipold(2, 1, ia) = (pold(2, 3, ia) * pold(3, 1, ia) - pold(2, 1, ia) * pold(3, 3, ia)) / det
+        ipold(2, 2, ia) = (pold(1, 1, ia) * pold(3, 3, ia) - pold(1, 3, ia) * pold(3, 1, ia)) / det
+        ipold(2, 3, ia) = (pold(1, 3, ia) * pold(2, 1, ia) - pold(1, 1, ia) * pold(2, 3, ia)) / det
+        ipold(3, 1, ia) = (pold(2, 1, ia) * pold(3, 2, ia) - pold(2, 2, ia) * pold(3, 1, ia)) / det
+        ipold(3, 2, ia) = (pold(1, 2, ia) * pold(3, 1, ia) - pold(1, 1, ia) * pold(3, 2, ia)) / det
+        ipold(3, 3, ia) = (pold(1, 1, ia) * pold(2, 2, ia) - pold(1, 2, ia) * pold(2, 1, ia)) / det
+      end if
+      ! fractional coordinates of the current positions (the saved list
+      ! positions of vnlistcheck after a rebuild here)
+      sfx(ia) = rx(ia) * B(1, 1) + ry(ia) * B(2, 1) + rz(ia) * B(3, 1)
+      sfy(ia) = rx(ia) * B(1, 2) + ry(ia) * B(2, 2) + rz(ia) * B(3, 2)
+      sfz(ia) = rx(ia) * B(1, 3) + ry(ia) * B(2, 3) + rz(ia) * B(3, 3)
+    end do
+    mu_t = 0.0_dp
+    if (lcoulomb) then
+      st = mdff_get_dipoles(ctx, mu_t, ef_t)
+      call mdff_check(st, ctx, 'get_dipoles')
+    end if
+    task_coul = .false.
+    lfalse = .false.
+    nth = max(nhc_n, 1)
+    if (trim(integrator) /= 'nvt-nhcn') nth = 1
+    xiw = 0.0_dp
+    vxiw = 0.0_dp
+    vxib = 0.0_dp
+    xib = 0.0_dp
+    if (trim(integrator) == 'nvt-nhcn') then
+      xiw(1:nhc_n) = xi(1:nhc_n)
+      vxiw(1:nhc_n) = vxi(1:nhc_n)
+    end if
+    open (newunit=uu, file='RESTART', form='unformatted', status='replace')
+    write (uu) lnmlj
+    write (uu) lcoulomb
+    write (uu) lmorse
+    write (uu) lbmhft
+    write (uu) lbmhftd
+    write (uu) lsurf
+    write (uu) lcsvr
+    write (uu) lharm
+    write (uu) ltraj
+    write (uu) lvnlist
+    write (uu) lstatic
+    write (uu) lreduced
+    write (uu) lreducedN
+    write (uu) ltest
+    write (uu) lmsd
+    write (uu) lvacf
+    write (uu) lwrite_restart
+    write (uu) cutlongrange
+    write (uu) cutshortrange
+    write (uu) calc
+    write (uu) dgauss
+    write (uu) longrange
+    write (uu) itraj_start
+    write (uu) itraj_period
+    write (uu) itraj_format
+    write (uu) trajff_data
+    write (uu) iscff_format
+    write (uu) iscff_data
+    write (uu) iefall_format
+    write (uu) iefgall_format
+    write (uu) idipall_format
+    write (uu) posff_data
+    write (uu) skindiff
+    write (uu) natm
+    write (uu) system_name
+    write (uu) A
+    write (uu) ntype
+    write (uu) ntype
+    write (uu) atypei0
+    write (uu) natmi0
+    write (uu) rx, ry, rz
+    write (uu) vx, vy, vz
+    write (uu) fx, fy, fz
+    write (uu) zero1, zero1, zero1
+    write (uu) zero1, zero1, zero1
+    write (uu) sfx, sfy, sfz
+    write (uu) atype
+    write (uu) itype
+    do j = 0, 1
+      lst = 0
+      pnt = 0
+      if (lvnlist) then
+        st = mdff_get_pairs(ctx, int(j, c_int32_t), pnt, lst, int(size(lst), c_int64_t), nl)
+        call mdff_check(st, ctx, 'get_pairs')
+      end if
+      write (uu) lst
+      write (uu) pnt
+    end do
+    write (uu) cutshortrange
+    write (uu) cutlongrange
+    write (uu) q0
+    write (uu) m0
+    write (uu) zero1
+    write (uu) dipia
+    write (uu) quad3
+    write (uu) pold
+    write (uu) quad4
+    write (uu) ipold
+    write (uu) ipolar
+    write (uu) dt * time_unit
+    write (uu) npas
+    write (uu) itime_next
+    write (uu) nequil
+    write (uu) nequil_period
+    write (uu) spas
+    write (uu) npropr
+    write (uu) npropr_start
+    write (uu) nprint
+    write (uu) fprint
+    write (uu) 0
+    write (uu) setvel
+    write (uu) temp * boltz_unit
+    write (uu) press * press_unit
+    write (uu) timesca_thermo * time_unit
+    write (uu) timesca_baro * time_unit
+    write (uu) tauTberendsen * time_unit
+    write (uu) tauPberendsen * time_unit
+    write (uu) taucsvr * time_unit
+    write (uu) nuandersen
+    write (uu) annealing
+    write (uu) .true.
+    write (uu) integrator
+    write (uu) nhc_n
+    write (uu) nhc_yosh_order
+    write (uu) nhc_mults
+    write (uu) vxiw(1:nth)
+    write (uu) xiw(1:nth)
+    write (uu) vxib(1:nth)
+    write (uu) xib(1:nth)
+    write (uu) 0.0_dp
+    write (uu) 0.0_dp
+    write (uu) 0.0_dp
+    write (uu) lKA
+    write (uu) ctrunc
+    write (uu) 2
+    write (uu) kES
+    write (uu) alphaES
+    write (uu) qlj
+    write (uu) plj
+    write (uu) sigmalj
+    write (uu) epslj
+    write (uu) sigmamor
+    write (uu) epsmor
+    write (uu) rhomor
+    write (uu) mass
+    write (uu) doefield
+    write (uu) doefg
+    write (uu) qch
+    write (uu) quad_nuc
+    write (uu) dip
+    write (uu) quad
+    write (uu) poldip
+    write (uu) poldip_iso
+    write (uu) polquad
+    write (uu) polquad_iso
+    write (uu) pol_damp_b
+    write (uu) pol_damp_c
+    write (uu) pol_damp_k
+    write (uu) extrapolate_order
+    write (uu) conv_tol_ind
+    write (uu) min_scf_pol_iter
+    write (uu) max_scf_pol_iter
+    write (uu) algo_ext_dipole
+    write (uu) thole_functions
+    write (uu) thole_function_type
+    write (uu) thole_param
+    write (uu) omegakO
+    write (uu) epsw
+    write (uu) lautoES
+    write (uu) lwrite_dip_wfc
+    write (uu) lfalse
+    write (uu) lfalse
+    write (uu) lfalse
+    write (uu) lfalse
+    write (uu) ldip_wfc
+    write (uu) rcut_wfc
+    write (uu) ldip_polar
+    write (uu) ldip_damping
+    write (uu) lquad_polar
+    write (uu) Abmhftd
+    write (uu) Bbmhftd
+    write (uu) Cbmhftd
+    write (uu) Dbmhftd
+    write (uu) BDbmhftd
+    write (uu) task_coul
+    write (uu) algo_moment_from_pola
+    write (uu) mu_t
+    close (uu)
+  end subroutine write_restart
+
+  ! restart_init (src/restart.f90:34-292): every record in the order
+  ! written; the control, md and force-field variables it holds replace
+  ! those of the control file, times and temperatures converted back to
+  ! the input units the rest of the driver uses
+  subroutine read_restart()
+    integer :: uu, j, ntype2, nth, itmp, ncd
+    character(len=3) :: atypei0(0:ntypemax)
+    integer :: natmi0(0:ntypemax)
+    integer, allocatable :: lst(:), pnt(:)
+    real(dp), allocatable :: r1(:), r2(:), r3(:), d2(:, :), d3(:, :, :), d4(:, :, :, :)
+    logical, allocatable :: l1(:)
+    logical :: ltmp, task_coul(6)
+    real(dp) :: rtmp, vc
+    open (newunit=uu, file='RESTART', form='unformatted', status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+      write (*, '(a)') 'ERROR full_restart: cannot open RESTART'
+      stop 1
+    end if
+    read (uu) lnmlj
+    read (uu) lcoulomb
+    read (uu) lmorse
+    read (uu) lbmhft
+    read (uu) lbmhftd
+    read (uu) lsurf
+    read (uu) lcsvr
+    read (uu) lharm
+    read (uu) ltraj
+    read (uu) lvnlist
+    read (uu) lstatic
+    read (uu) lreduced
+    read (uu) lreducedN
+    read (uu) ltest
+    read (uu) lmsd
+    read (uu) lvacf
+    read (uu) lwrite_restart
+    read (uu) cutlongrange
+    read (uu) cutshortrange
+    read (uu) calc
+    read (uu) dgauss
+    read (uu) longrange
+    read (uu) itraj_start
+    read (uu) itraj_period
+    read (uu) itraj_format
+    read (uu) trajff_data
+    read (uu) iscff_format
+    read (uu) iscff_data
+    read (uu) iefall_format
+    read (uu) iefgall_format
+    read (uu) idipall_format
+    read (uu) posff_data
+    read (uu) skindiff
+    read (uu) natm
+    read (uu) system_name
+    read (uu) A
+    read (uu) ntype
+    read (uu) ntype2
+    read (uu) atypei0
+    read (uu) natmi0
+    atypei(1:ntype) = atypei0(1:ntype)
+    natmi(1:ntype) = natmi0(1:ntype)
+    allocate (atype(natm), itype(natm), rx(natm), ry(natm), rz(natm), vx(natm), vy(natm), vz(natm), &
+              fx(natm), fy(natm), fz(natm), r1(natm), r2(natm), r3(natm), lst(natm * vnlmax), pnt(natm + 1), &
+              d2(3, natm), d3(3, 3, natm), d4(3, 3, 3, natm), l1(natm))
+    read (uu) rx, ry, rz
+    read (uu) vx, vy, vz
+    read (uu) fx, fy, fz
+    read (uu) r1, r2, r3
+    read (uu) r1, r2, r3
+    read (uu) r1, r2, r3
+    read (uu) atype
+    read (uu) itype
+    do j = 0, 1
+      read (uu) lst
+      read (uu) pnt
+    end do
+    read (uu) rtmp
+    read (uu) rtmp
+    read (uu) r1
+    read (uu) r1
+    read (uu) r1
+    read (uu) d2
+    read (uu) d3
+    read (uu) d3
+    read (uu) d4
+    read (uu) d3
+    read (uu) l1
+    read (uu) dt
+    dt = dt / time_unit
+    read (uu) npas
+    read (uu) itime_rst
+    read (uu) nequil
+    read (uu) nequil_period
+    read (uu) spas
+    read (uu) npropr
+    read (uu) npropr_start
+    read (uu) nprint
+    read (uu) fprint
+    read (uu) itmp
+    read (uu) setvel
+    read (uu) temp
+    temp = temp / boltz_unit
+    read (uu) press
+    press = press / press_unit
+    read (uu) timesca_thermo
+    timesca_thermo = timesca_thermo / time_unit
+    read (uu) timesca_baro
+    timesca_baro = timesca_baro / time_unit
+    read (uu) tauTberendsen
+    tauTberendsen = tauTberendsen / time_unit
+    read (uu) tauPberendsen
+    tauPberendsen = tauPberendsen / time_unit
+    read (uu) taucsvr
+    taucsvr = taucsvr / time_unit
+    read (uu) nuandersen
+    read (uu) annealing
+    read (uu) ltmp
+    read (uu) integrator
+    read (uu) nhc_n
+    read (uu) nhc_yosh_order
+    read (uu) nhc_mults
+    nth = 1
+    if (trim(integrator) == 'nvt-nhcn' .or. trim(integrator) == 'npt-nhcnp') nth = nhc_n
+    allocate (xi_rst(nth), vxi_rst(nth))
+    read (uu) vxi_rst
+    read (uu) xi_rst
+    read (uu) r1(1:nth)
+    read (uu) r1(1:nth)
+    read (uu) vc
+    read (uu) vc
+    read (uu) vc
+    read (uu) lKA
+    read (uu) ctrunc
+    read (uu) ncd
+    read (uu) kES
+    read (uu) alphaES
+    read (uu) qlj
+    read (uu) plj
+    read (uu) sigmalj
+    read (uu) epslj
+    read (uu) sigmamor
+    read (uu) epsmor
+    read (uu) rhomor
+    read (uu) mass
+    read (uu) doefield
+    read (uu) doefg
+    read (uu) qch
+    read (uu) quad_nuc
+    read (uu) dip
+    read (uu) quad
+    read (uu) poldip
+    read (uu) poldip_iso
+    read (uu) polquad
+    read (uu) polquad_iso
+    read (uu) pol_damp_b
+    read (uu) pol_damp_c
+    read (uu) pol_damp_k
+    read (uu) extrapolate_order
+    read (uu) conv_tol_ind
+    read (uu) min_scf_pol_iter
+    read (uu) max_scf_pol_iter
+    read (uu) algo_ext_dipole
+    read (uu) thole_functions
+    read (uu) thole_function_type
+    read (uu) thole_param
+    read (uu) omegakO
+    read (uu) epsw
+    read (uu) lautoES
+    read (uu) lwrite_dip_wfc
+    read (uu) ltmp
+    read (uu) ltmp
+    read (uu) ltmp
+    read (uu) ltmp
+    read (uu) ldip_wfc
+    read (uu) rcut_wfc
+    read (uu) ldip_polar
+    read (uu) ldip_damping
+    read (uu) lquad_polar
+    read (uu) Abmhftd
+    read (uu) Bbmhftd
+    read (uu) Cbmhftd
+    read (uu) Dbmhftd
+    read (uu) BDbmhftd
+    read (uu) task_coul
+    read (uu) algo_moment_from_pola
+    read (uu) d2
+    close (uu)
+    ! the tables were stored in full: no symmetrisation or automatic Ewald
+    ! parameters again (restart_init calls initialize_param_non_bonded /
+    ! initialize_coulomb on the stored values)
+    sym_nb = .false.
+    sym_pim = .false.
+    lautoES = .false.
+    poldip_iso = 0.0_dp
+  end subroutine read_restart
 
   ! CONTFF (src/config.f90:208-252): Cartesian positions, velocities, forces
   subroutine write_contff()

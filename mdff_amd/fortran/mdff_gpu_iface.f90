@@ -69,8 +69,8 @@ module mdff_gpu
   public :: mdff_desc_defaults, mdff_ctx_create, mdff_ctx_destroy, mdff_last_error_c, mdff_version_c
   public :: mdff_set_positions, mdff_get_positions, mdff_set_velocities, mdff_get_velocities
   public :: mdff_get_forces, mdff_set_cell, mdff_vnlist_build, mdff_vnlistcheck, mdff_get_pairs
-  public :: mdff_engforce, mdff_get_dipoles, mdff_set_itime, mdff_md_vv, mdff_md_nhcn
-  public :: mdff_csvr_rescale, mdff_calc_temp, mdff_last_timings
+  public :: mdff_engforce, mdff_get_dipoles, mdff_set_itime, mdff_md_vv, mdff_md_nhcn, mdff_last_terms
+  public :: mdff_csvr_rescale, mdff_calc_temp, mdff_last_timings, mdff_rescale_velocities
   public :: mdff_check, mdff_error_string, mdff_version_string
 
   interface
@@ -163,6 +163,12 @@ module mdff_gpu
       type(mdff_terms), intent(out) :: terms
     end function
 
+    integer(c_int) function mdff_last_terms(ctx, terms) bind(C, name="mdff_last_terms")
+      import :: c_int, c_ptr, mdff_terms
+      type(c_ptr), value :: ctx
+      type(mdff_terms), intent(out) :: terms
+    end function
+
     integer(c_int) function mdff_get_dipoles(ctx, mu, ef) bind(C, name="mdff_get_dipoles")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
@@ -199,6 +205,12 @@ module mdff_gpu
       type(c_ptr), value :: ctx
       real(c_double), value :: temp, taut, r1, sumnoise
       real(c_double), intent(inout) :: conint
+    end function
+
+    integer(c_int) function mdff_rescale_velocities(ctx, lambda) bind(C, name="mdff_rescale_velocities")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: lambda
     end function
 
     integer(c_int) function mdff_calc_temp(ctx, boltz_unit, temp, ekin) bind(C, name="mdff_calc_temp")

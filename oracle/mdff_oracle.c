@@ -74,6 +74,7 @@ static const double VNLMAX = 2000.0;       /* src/config.f90:43 */
 
 struct orc_ctx {
   mdff_desc d;
+  mdff_terms last_terms;
   int n;
   int32_t *itype;                /* 1-based */
   double A[9], B[9], omega, W[3];
@@ -1263,6 +1264,14 @@ int orc_engforce(orc_ctx *c, double *fx, double *fy, double *fz, mdff_terms *t) 
   free(s);
   if (fx) orc_get_forces(c, fx, fy, fz);
   if (t) *t = tm;
+  c->last_terms = tm;
+  return MDFF_OK;
+}
+
+/* terms of the last force evaluation (engforce or an MD step) */
+int orc_last_terms(orc_ctx *c, mdff_terms *t) {
+  if (!c || !t) return MDFF_ERR_BAD_ARG;
+  *t = c->last_terms;
   return MDFF_OK;
 }
 
@@ -1443,6 +1452,19 @@ int orc_csvr_rescale(orc_ctx *c, double temp, double taut, double r1, double sum
                 2.0 * r1 * sqrt(ekin * sigma / ndeg * (1.0 - factor) * factor);
   if (conint) *conint = *conint - (knew - ekin);
   double lambda = sqrt(knew / ekin);
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int i = 0; i < n; ++i) {
+    c->vx[i] *= lambda; c->vy[i] *= lambda; c->vz[i] *= lambda;
+  }
+  for (int i = 0; i < n; ++i) { sx += c->vx[i]; sy += c->vy[i]; sz += c->vz[i]; }
+  sx /= (double)n; sy /= (double)n; sz /= (double)n;
+  for (int i = 0; i < n; ++i) { c->vx[i] -= sx; c->vy[i] -= sy; c->vz[i] -= sz; }
+  return MDFF_OK;
+}
+
+/* rescale_velocities with a given factor (src/kinetic.f90:155-180) */
+int orc_rescale_velocities(orc_ctx *c, double lambda) {
+  int n = c->n;
   double sx = 0.0, sy = 0.0, sz = 0.0;
   for (int i = 0; i < n; ++i) {
     c->vx[i] *= lambda; c->vy[i] *= lambda; c->vz[i] *= lambda;

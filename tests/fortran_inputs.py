@@ -18,7 +18,7 @@ def _r(x) -> str:
 
 
 def write_inputs(workdir: Path, s, names: dict[int, str], npas: int = 0, dt_ps: float = 0.001,
-                 extra_control: list[str] | None = None) -> Path:
+                 extra_control: list[str] | None = None, extra_md: list[str] | None = None) -> Path:
     """control.F and POSFF (Cartesian, rvn) for a Buckingham/BMH + Ewald system."""
     d = s.desc
     ntype = int(d.ntype)
@@ -39,6 +39,7 @@ def write_inputs(workdir: Path, s, names: dict[int, str], npas: int = 0, dt_ps: 
         f" npas = {npas}",
         f" dt = {_r(dt_ps)}",
         " nprint = 1",
+        *(extra_md or []),
         "&end",
         "&non_bondedtag",
     ]
@@ -90,21 +91,57 @@ def parse_terms(stdout: str) -> dict:
     return out
 
 
-def parse_thermo(stdout: str) -> np.ndarray:
-    rows = []
-    seen = False
+THERMO_KEYS = ["step", "time", "Ekin", "Temp", "Utot", "U_vdw", "U_coul", "Pressure", "Pvir_vdw",
+               "Pvir_coul", "Pvir_tot", "volume", "a cell", "b cell", "c cell", "density", "Etot", "Htot"]
+
+
+def parse_thermo(stdout: str) -> dict:
+    """The 'Thermodynamic information' blocks of the standard output
+    (write_thermo 'std', src/thermo.f90:395-420): one array per quantity."""
+    out = {k: [] for k in THERMO_KEYS}
     for line in stdout.splitlines():
-        if line.strip().startswith("step"):
-            seen = True
+        if "=" not in line:
             continue
-        if seen:
-            parts = line.split()
-            if len(parts) == 6:
-                rows.append([float(x) for x in parts])
-    return np.array(rows)
+        k, v = line.split("=", 1)
+        k = k.strip()
+        if k in out:
+            out[k].append(float(v))
+    return {k: np.array(v) for k, v in out.items()}
+
+
+def parse_osziff(path: Path) -> np.ndarray:
+    """OSZIFF (write_thermo 'osz', src/thermo.f90:367-383): pairs of lines
+    (I9,2X,6E16.8) under a dashed header every 10 records; rows of
+    step, time, Etot, Ekin, Utot, U_vdw, U_coul, Temp, Pressure, Pvir_vdw,
+    Pvir_coul, Omega, Htot."""
+    recs = []
+    lines = [ln for ln in path.read_text().splitlines() if ln.strip() and not ln.startswith("-")
+             and "step" not in ln]
+    for a, b in zip(lines[0::2], lines[1::2]):
+        ia, ib = int(a[:9]), int(b[:9])
+        assert ia == ib
+        fa = [float(a[11 + 16 * j:11 + 16 * (j + 1)]) for j in range(6)]
+        fb = [float(b[11 + 16 * j:11 + 16 * (j + 1)]) for j in range(6)]
+        recs.append([ia] + fa + fb)
+    return np.array(recs)
 
 
 def read_contff(path: Path, natm: int) -> np.ndarray:
     lines = path.read_text().splitlines()
     data = np.array([[float(x) for x in ln.split()[1:]] for ln in lines[9:9 + natm]])
     return data   # natm x 9: r, v, f
+
+
+def read_trajff(path: Path, natm: int, ncol: int) -> list[np.ndarray]:
+    """Formatted TRAJFF frames (write_trajff_xyz, src/config.f90:565-595):
+    9 header lines, then natm lines of a type label and ncol reals."""
+    lines = path.read_text().splitlines()
+    frames = []
+    k = 0
+    while k + 9 <= len(lines):
+        n = int(lines[k])
+        assert n == natm
+        rows = lines[k + 9:k + 9 + natm]
+        frames.append(np.array([[float(x) for x in ln.split()[1:1 + ncol]] for ln in rows]))
+        k += 9 + natm
+    return frames

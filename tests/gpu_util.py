@@ -37,7 +37,16 @@ def pair(system, precision=capi.PREC_FP64, **over):
 
 
 def rel_force_err(f, fref):
+    """max |dF| / max |F| over components (the FP64 parity metric)."""
     return float(np.abs(f - fref).max() / max(np.abs(fref).max(), 1e-300))
+
+
+def rms_force_err(f, fref):
+    """max_i |F_i - F_i^ref| / RMS_i |F_i^ref| (the mixed form's force-error
+    metric: the worst atom's error against the typical force)."""
+    dF = np.linalg.norm(np.asarray(f) - np.asarray(fref), axis=1)
+    rms = float(np.sqrt(np.mean(np.sum(np.asarray(fref) ** 2, axis=1))))
+    return float(dF.max() / max(rms, 1e-300))
 
 
 def rel(a, b):

@@ -81,14 +81,96 @@ def test_driver_engforce_and_nve_match_oracle(tmp_path):
     assert rel(fdrv, fo) < 1e-10
 
     thermo = FI.parse_thermo(r.stdout)
-    assert thermo.shape == (nsteps, 6)
+    assert list(thermo["step"]) == list(range(nsteps + 1))
     ek, up = [], []
     for _ in range(nsteps):
         e, u = o.md_vv(1, 0.001 * capi.TIME_UNIT)
         ek.append(e[0])
         up.append(u[0])
-    assert np.allclose(thermo[:, 4], up, rtol=1e-9, atol=0)
-    assert np.allclose(thermo[:, 3], ek, rtol=1e-8, atol=0)
+    assert np.allclose(thermo["Utot"][1:], up, rtol=1e-9, atol=0)
+    assert np.allclose(thermo["Ekin"][1:], ek, rtol=1e-8, atol=0)
+    osz = FI.parse_osziff(tmp_path / "OSZIFF")
+    assert osz.shape == (nsteps + 1, 13)
+    # the same thermo in both outputs (E16.8 in OSZIFF)
+    assert np.allclose(osz[:, 3], thermo["Ekin"], rtol=1e-7)
+    assert np.allclose(osz[:, 4], thermo["Utot"], rtol=1e-7)
+    assert np.allclose(osz[:, 12], thermo["Htot"], rtol=1e-7)
     cont = FI.read_contff(tmp_path / "CONTFF", s.natm)
     assert rel(cont[:, 0:3], o.get_positions()) < 1e-10
     assert rel(cont[:, 3:6], o.get_velocities()) < 1e-8
+
+
+def _fold(pos, A):
+    """kardir, periodicbc (nint, half away from zero), dirkar
+    (src/lattice.f90:183-295): positions folded into the cell as TRAJFF
+    writes them."""
+    B = np.linalg.inv(A)          # rows: reciprocal vectors / omega
+    sfr = pos @ B.T
+    sfr = sfr - np.where(sfr >= 0, np.floor(sfr + 0.5), np.ceil(sfr - 0.5))
+    return sfr @ A.T
+
+
+@pytest.mark.gpu
+def test_driver_trajff_and_osziff(tmp_path):
+    """TRAJFF frames at the steps itime > itraj_start with itime % itraj_period
+    == 0 (src/runmd.f90:349-353), positions folded into the cell, rvf data;
+    OSZIFF records every fprint steps plus t = 0; the last frame is the
+    final configuration of CONTFF."""
+    from gpu_util import note, rel
+    s = configs.nacl(ncell=2, seed=4)
+    FI.write_inputs(tmp_path, s, {1: "Na", 2: "Cl"}, npas=6,
+                    extra_control=[" ltraj = .true.", " itraj_start = 1", " itraj_period = 2",
+                                   " trajff_data = 'rvf'"],
+                    extra_md=[" fprint = 3", " nprint = 2"])
+    note("fortran driver trajff / osziff")
+    r = _run(tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    frames = FI.read_trajff(tmp_path / "TRAJFF", s.natm, 9)
+    assert len(frames) == 3                       # steps 2, 4, 6
+    cont = FI.read_contff(tmp_path / "CONTFF", s.natm)
+    A = np.array(s.desc.cell[:]).reshape(3, 3).T  # columns = lattice vectors
+    assert rel(frames[-1][:, 0:3], _fold(cont[:, 0:3], A)) < 1e-10
+    assert rel(frames[-1][:, 3:9], cont[:, 3:9]) < 1e-10
+    osz = FI.parse_osziff(tmp_path / "OSZIFF")
+    assert list(osz[:, 0].astype(int)) == [0, 3, 6]
+    thermo = FI.parse_thermo(r.stdout)
+    assert list(thermo["step"].astype(int)) == [0, 2, 4, 6]
+
+
+@pytest.mark.gpu
+def test_driver_restart_full_vs_split(tmp_path):
+    """ex14 (examples/ex14/run_ex14.sh:13-51): a run of 2 n + 1 steps against
+    a run of n steps that writes RESTART (lwrite_restart) followed by a
+    full_restart run, which takes every parameter and the state from
+    RESTART and runs from step n + 1 to step 2 n + 1 (itime1 = itime0 + npas,
+    src/restart.f90:196-197).  The final configurations agree, and so do
+    the thermo records of the steps both runs make."""
+    from gpu_util import note, rel
+    s = configs.nacl(ncell=2, seed=8, temp_K=600.0)
+    n = 6
+    full = tmp_path / "full"
+    split = tmp_path / "split"
+    FI.write_inputs(full, s, {1: "Na", 2: "Cl"}, npas=2 * n + 1)
+    FI.write_inputs(split, s, {1: "Na", 2: "Cl"}, npas=n, extra_control=[" lwrite_restart = .true."])
+    note("fortran driver restart: full run")
+    r = _run(full)
+    assert r.returncode == 0, r.stdout + r.stderr
+    note("fortran driver restart: split run 1")
+    r1 = _run(split)
+    assert r1.returncode == 0, r1.stdout + r1.stderr
+    assert (split / "RESTART").exists()
+    FI.write_inputs(split, s, {1: "Na", 2: "Cl"}, npas=0, extra_control=[" full_restart = .true."])
+    (split / "POSFF").unlink()                    # everything comes from RESTART
+    note("fortran driver restart: split run 2")
+    r2 = _run(split)
+    assert r2.returncode == 0, r2.stdout + r2.stderr
+    cf = FI.read_contff(full / "CONTFF", s.natm)
+    cs = FI.read_contff(split / "CONTFF", s.natm)
+    assert rel(cs[:, 0:3], cf[:, 0:3]) < 1e-9
+    assert rel(cs[:, 3:6], cf[:, 3:6]) < 1e-7
+    tf = FI.parse_thermo(r.stdout)
+    t2 = FI.parse_thermo(r2.stdout)
+    assert list(t2["step"].astype(int)) == list(range(n, 2 * n + 2))
+    sel = [int(k) for k in t2["step"]]
+    assert np.allclose(t2["Utot"], tf["Utot"][sel], rtol=1e-9)
+    assert np.allclose(t2["Ekin"], tf["Ekin"][sel], rtol=1e-8)
