@@ -345,11 +345,17 @@ struct mdff_ctx {
   // Mixed form, charge-only evaluations: the k-vectors of the inner box
   // |nx| <= kin[0], |ny| <= kin[1], |nz| <= kin[2] (every k with |k| <=
   // 2.5 alpha) carry most of the weight of the force sums and run in FP64
-  // (k_in_sk, k_in_reduce, k_in_gather); the main kernels see A = 0 there
+  // (k_in_phase, k_in_sk_s, k_in_reduce_r, k_in_gather_s, or the generic
+  // k_in_sk, k_in_reduce, k_in_gather); the main kernels see A = 0 there
   // (kA_mask).  kin_on = 0: no inner pass.
   int kin_on = 0;
   int kin[3] = {0, 0, 0};
   int nb_in = 0, in_chunk = 0, in_nch = 0, kin_tile = 0, kin_msplit = 0, kin_splits = 0;
+  int kin_reg = 0;                        // 1: the scalar-operand forms (Kz' small enough)
+  int kin_ts = 0, kin_sg = 0;             // k_in_sk_s: threads per subgroup, subgroups
+  size_t kin_lds = 0;                     // LDS bytes of the structure-factor kernel
+  void* tab_in = nullptr;                 // per atom X, Y, q Z rows (FP64 complex)
+  double* wt_in = nullptr;                // per (m, j) sum / difference table of W
   double* kA_mask = nullptr;              // kA with the inner box zeroed (nk_box)
   double* kA_in = nullptr;                // A of the inner box, 0 outside the half space (nb_in)
   double* kvec_in = nullptr;              // kx, ky, kz, kcoe of the inner box (4 x nb_in)

@@ -17,6 +17,7 @@
 //            field = 2 (4 pi/V) 2 pi B U1; force = 2 (4 pi/V) [q_i 2 pi B U1
 //            + (2 pi)^2 B M m_i]                               (:196-233, :300-358)
 #include <type_traits>
+#include <utility>
 
 #include "common.h"
 #include "reduce.h"
@@ -1442,6 +1443,8 @@ k_gather_h3(Gh3Args a) {
 // here in FP64 and masked out of the main kernels (A = 0 there): per k the
 // same terms as the main path, so the split changes which precision sums
 // a k-vector, not the sum.
+// Generic forms (any Kz'; the scalar-operand forms below replace them when
+// Kz' <= kInKZ):
 //   k_in_sk      S(k) partials per chunk of atoms, one thread per k, FP64
 //                phase tables of the chunk's atoms staged in LDS
 //   k_in_reduce  S(k) over the chunks in order; W = A conj(S); the energy
@@ -1450,7 +1453,7 @@ k_gather_h3(Gh3Args a) {
 //                written as one more split of the gather's partials
 // ---------------------------------------------------------------------------
 struct InArgs {
-  int n, Kx, Ky, Kz, NY, NZ, nb, chunk, tile, msplit;
+  int n, Kx, Ky, Kz, NY, NZ, nb, chunk, tile, msplit, ts;
   const double *sx, *sy, *sz, *q;
   const double* kA;               // A of the inner box (nb)
   const double* kvec;             // kx, ky, kz, kcoe (4 nb)
@@ -1458,21 +1461,24 @@ struct InArgs {
   double* wk;                     // 2 nb
   double* blk;                    // energy / stress block partials [block][7]
   double* out;                    // gather moments [split][3][n], one split per m range
+  cplx<double>* tab;              // per atom: X(0..Kx'), Y(0..Ky'), q Z(0..Kz')
+  double* wt;                     // per (m, j = 0..Kz'): sums / differences of W(+-j)
 };
 
 // e^{2 pi i k s} for k = 0..K into dst[0..K] (stride st): sincospi of the
 // reduced angle for k = 1, then products (each within a few units of 2^-53)
-__device__ __forceinline__ void phase_row(double s, int K, cplx<double>* dst, int st) {
+// (times w: the products are formed unscaled, each entry scaled once)
+__device__ __forceinline__ void phase_row(double s, int K, cplx<double>* dst, int st, double w = 1.0) {
   double t1 = s - rint(s);
   double s1, c1;
   sincospi(2.0 * t1, &s1, &c1);
   double cs = 1.0, sn = 0.0;
-  dst[0] = {1.0, 0.0};
+  dst[0] = {w, 0.0};
   for (int k = 1; k <= K; ++k) {
     const double c2 = cs * c1 - sn * s1, s2 = cs * s1 + sn * c1;
     cs = c2;
     sn = s2;
-    dst[(size_t)k * st] = {cs, sn};
+    dst[(size_t)k * st] = {w * cs, w * sn};
   }
 }
 
@@ -1526,18 +1532,46 @@ __global__ void __launch_bounds__(256) k_in_sk(InArgs a) {
   a.part[(size_t)blockIdx.y * 2 * a.nb + 2 * k + 1] = im;
 }
 
+// 64 k-vectors per block: wave w sums the chunks [w q, (w + 1) q) in order,
+// the four wave sums are added in wave order; then W = A conj(S) and the
+// energy / stress block partials
 __global__ void __launch_bounds__(256) k_in_reduce(InArgs a, int nchunk) {
   __shared__ double sh[4 * kRedMaxCols];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  double v[kRedMaxCols];
-#pragma unroll
-  for (int c = 0; c < kRedMaxCols; ++c) v[c] = 0.0;
+  __shared__ double ws[4][64][2];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int k = blockIdx.x * 64 + lane;
+  const int q = (nchunk + 3) / 4;
+  const int c0 = w * q, c1 = min(nchunk, c0 + q);
+  double re = 0.0, im = 0.0;
   if (k < a.nb) {
-    double re = 0.0, im = 0.0;
-    for (int c = 0; c < nchunk; ++c) {
+    int c = c0;
+    for (; c + 4 <= c1; c += 4) {
+      double vr[4], vi[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        vr[u] = a.part[(size_t)(c + u) * 2 * a.nb + 2 * k];
+        vi[u] = a.part[(size_t)(c + u) * 2 * a.nb + 2 * k + 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        re += vr[u];
+        im += vi[u];
+      }
+    }
+    for (; c < c1; ++c) {
       re += a.part[(size_t)c * 2 * a.nb + 2 * k];
       im += a.part[(size_t)c * 2 * a.nb + 2 * k + 1];
     }
+  }
+  ws[w][lane][0] = re;
+  ws[w][lane][1] = im;
+  __syncthreads();
+  double v[kRedMaxCols];
+#pragma unroll
+  for (int c = 0; c < kRedMaxCols; ++c) v[c] = 0.0;
+  if (w == 0 && k < a.nb) {
+    re = ((ws[0][lane][0] + ws[1][lane][0]) + ws[2][lane][0]) + ws[3][lane][0];
+    im = ((ws[0][lane][1] + ws[1][lane][1]) + ws[2][lane][1]) + ws[3][lane][1];
     const double Ak = a.kA[k];
     a.wk[2 * k] = Ak * re;
     a.wk[2 * k + 1] = -Ak * im;
@@ -1553,6 +1587,222 @@ __global__ void __launch_bounds__(256) k_in_reduce(InArgs a, int nchunk) {
   }
   const double r = block_sum_cols(v, 7, sh);
   if (threadIdx.x < 7) a.blk[blockIdx.x * 7 + threadIdx.x] = r;
+}
+
+// Forms for boxes with Kz' <= kInKZ (KZ = Kz' a template argument), where
+// the operands shared by a wave come through scalar loads rather than LDS:
+//   k_in_phase     per atom the FP64 rows X(0..Kx'), Y(0..Ky'), q Z(0..Kz')
+//   k_in_sk_s      one thread per m = (nx, ny) and subgroup of atoms, the
+//                  S(m, +-nz) sums in registers; q Z of the wave's atom is
+//                  wave-uniform (scalar loads), X and Y from LDS; the
+//                  subgroups' sums added in subgroup order through LDS
+//   k_in_reduce_r  whole rows m per block: S over the chunks, W, the
+//                  energy / stress partials, and per (m, j) the sums and
+//                  differences of W(m, +j) and W(m, -j) that the gather reads
+//   k_in_gather_s  one thread per atom, Z(0..Kz') in registers, the (m, j)
+//                  table through scalar loads
+constexpr int kInKZ = 12;
+
+__global__ void __launch_bounds__(256) k_in_phase(InArgs a) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= 3 * a.n) return;
+  const int i = r / 3, d = r - 3 * (r / 3);
+  const int ncol = a.Kx + a.Ky + a.Kz + 3;
+  const double sv = d == 0 ? a.sx[i] : (d == 1 ? a.sy[i] : a.sz[i]);
+  const int K = d == 0 ? a.Kx : (d == 1 ? a.Ky : a.Kz);
+  const int col = d == 0 ? 0 : (d == 1 ? a.Kx + 1 : a.Kx + a.Ky + 2);
+  phase_row(sv, K, a.tab + (size_t)i * ncol + col, 1, d == 2 ? a.q[i] : 1.0);
+}
+
+// grid: one block per chunk of atoms; block: SG subgroups of a.ts threads
+// (a.ts a multiple of 64, >= M); subgroup g takes atoms g, g + SG, ... of
+// each tile
+template <int KZ>
+__global__ void __launch_bounds__(1024) k_in_sk_s(InArgs a) {
+  extern __shared__ unsigned char smem_raw[];
+  const int nxy = a.Kx + a.Ky + 2;
+  const int ncol = nxy + KZ + 1;
+  cplx<double>* xy = reinterpret_cast<cplx<double>*>(smem_raw);   // [tile][nxy]
+  const int M = (a.Kx + 1) * a.NY;
+  const int SG = blockDim.x / a.ts;
+  const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / a.ts);
+  const int m = threadIdx.x - g * a.ts;
+  const int mm = m < M ? m : 0;
+  const int nx = mm / a.NY, ny = mm - (mm / a.NY) * a.NY - a.Ky;
+  const int cy = a.Kx + 1 + (ny < 0 ? -ny : ny);
+  const double sy = ny < 0 ? -1.0 : 1.0;
+  const cplx<double>* __restrict__ tab = a.tab;
+  double pr[KZ + 1], pi[KZ + 1], qr[KZ + 1], qi[KZ + 1];   // nz = +j, nz = -j
+#pragma unroll
+  for (int j = 0; j <= KZ; ++j) pr[j] = pi[j] = qr[j] = qi[j] = 0.0;
+  const int c0 = blockIdx.x * a.chunk, c1 = min(a.n, c0 + a.chunk);
+  for (int t0 = c0; t0 < c1; t0 += a.tile) {
+    const int na = min(a.tile, c1 - t0);
+    __syncthreads();
+    for (int x = threadIdx.x; x < na * nxy; x += blockDim.x) {
+      const int at = x / nxy;
+      xy[x] = tab[(size_t)(t0 + at) * ncol + (x - at * nxy)];
+    }
+    __syncthreads();
+    // q Z of the next atom loaded while the current one is summed (the
+    // last atom's row is loaded again at the end: every address in range)
+    cplx<double> zn[KZ + 1];
+    if (g < na) {
+      const cplx<double>* zq = tab + (size_t)(t0 + g) * ncol + nxy;
+#pragma unroll
+      for (int j = 0; j <= KZ; ++j) zn[j] = zq[j];
+    }
+    for (int at = g; at < na; at += SG) {
+      cplx<double> z[KZ + 1];
+#pragma unroll
+      for (int j = 0; j <= KZ; ++j) z[j] = zn[j];
+      {
+        const cplx<double>* zq = tab + (size_t)(t0 + min(at + SG, na - 1)) * ncol + nxy;
+#pragma unroll
+        for (int j = 0; j <= KZ; ++j) zn[j] = zq[j];
+      }
+      const cplx<double> X = xy[at * nxy + nx];
+      cplx<double> Y = xy[at * nxy + cy];
+      Y.im *= sy;
+      const cplx<double> P = cmul(X, Y);
+      pr[0] = fma(P.re, z[0].re, pr[0]);
+      pr[0] = fma(-P.im, z[0].im, pr[0]);
+      pi[0] = fma(P.re, z[0].im, pi[0]);
+      pi[0] = fma(P.im, z[0].re, pi[0]);
+#pragma unroll
+      for (int j = 1; j <= KZ; ++j) {
+        const double zr = z[j].re, zi = z[j].im;
+        pr[j] = fma(P.re, zr, pr[j]);
+        pr[j] = fma(-P.im, zi, pr[j]);
+        pi[j] = fma(P.re, zi, pi[j]);
+        pi[j] = fma(P.im, zr, pi[j]);
+        qr[j] = fma(P.re, zr, qr[j]);
+        qr[j] = fma(P.im, zi, qr[j]);
+        qi[j] = fma(P.im, zr, qi[j]);
+        qi[j] = fma(-P.re, zi, qi[j]);
+      }
+    }
+  }
+  // subgroups 1..SG-1 into subgroup 0, in order, one j at a time
+  double* red = reinterpret_cast<double*>(smem_raw);            // [SG - 1][ts][4]
+#pragma unroll
+  for (int j = 0; j <= KZ; ++j) {
+    __syncthreads();
+    if (g > 0) {
+      double* r = red + ((size_t)(g - 1) * a.ts + m) * 4;
+      r[0] = pr[j]; r[1] = pi[j]; r[2] = qr[j]; r[3] = qi[j];
+    }
+    __syncthreads();
+    if (g == 0) {
+      for (int h = 1; h < SG; ++h) {
+        const double* r = red + ((size_t)(h - 1) * a.ts + m) * 4;
+        pr[j] += r[0]; pi[j] += r[1]; qr[j] += r[2]; qi[j] += r[3];
+      }
+    }
+  }
+  if (g != 0 || m >= M) return;
+  double* out = a.part + (size_t)blockIdx.x * 2 * a.nb + 2 * (size_t)m * a.NZ;
+#pragma unroll
+  for (int j = 0; j <= KZ; ++j) {
+    out[2 * (KZ + j)] = pr[j];
+    out[2 * (KZ + j) + 1] = pi[j];
+    if (j > 0) {
+      out[2 * (KZ - j)] = qr[j];
+      out[2 * (KZ - j) + 1] = qi[j];
+    }
+  }
+}
+
+// R = floor(64 / NZ) rows m per block, lane l the k = m0 NZ + l; wave w of
+// 16 sums the chunks [w q, (w + 1) q) in order, the 16 sums are added in
+// wave order
+__global__ void __launch_bounds__(1024) k_in_reduce_r(InArgs a, int nchunk) {
+  __shared__ double ws[16][64][2];
+  __shared__ double wl[64][2];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int R = 64 / a.NZ;
+  const int M = (a.Kx + 1) * a.NY;
+  const int m0 = blockIdx.x * R;
+  const int rows = min(R, M - m0);
+  const bool act = lane < rows * a.NZ;
+  const int k = m0 * a.NZ + (act ? lane : 0);
+  const int q = (nchunk + 15) / 16;
+  const int c0 = w * q, c1 = min(nchunk, c0 + q);
+  double re = 0.0, im = 0.0;
+  if (act) {
+    int c = c0;
+    for (; c + 4 <= c1; c += 4) {
+      double vr[4], vi[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        vr[u] = a.part[(size_t)(c + u) * 2 * a.nb + 2 * k];
+        vi[u] = a.part[(size_t)(c + u) * 2 * a.nb + 2 * k + 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        re += vr[u];
+        im += vi[u];
+      }
+    }
+    for (; c < c1; ++c) {
+      re += a.part[(size_t)c * 2 * a.nb + 2 * k];
+      im += a.part[(size_t)c * 2 * a.nb + 2 * k + 1];
+    }
+  }
+  ws[w][lane][0] = re;
+  ws[w][lane][1] = im;
+  __syncthreads();
+  if (w == 0) {
+    re = 0.0;
+    im = 0.0;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      re += ws[v][lane][0];
+      im += ws[v][lane][1];
+    }
+    double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double wr = 0.0, wi = 0.0;
+    if (act) {
+      const double Ak = a.kA[k];
+      wr = Ak * re;
+      wi = -Ak * im;
+      a.wk[2 * k] = wr;
+      a.wk[2 * k + 1] = wi;
+      const double str = (re * re + im * im) * Ak;
+      const double kx = a.kvec[4 * k], ky = a.kvec[4 * k + 1], kz = a.kvec[4 * k + 2], kc = a.kvec[4 * k + 3];
+      v[0] = str;
+      v[1] = (1.0 - kc * kx * kx) * str;
+      v[2] = (1.0 - kc * ky * ky) * str;
+      v[3] = (1.0 - kc * kz * kz) * str;
+      v[4] = -kc * kx * ky * str;
+      v[5] = -kc * kx * kz * str;
+      v[6] = -kc * ky * kz * str;
+    }
+    wl[lane][0] = wr;
+    wl[lane][1] = wi;
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+      double x = v[c];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+      if (lane == 0) a.blk[blockIdx.x * 7 + c] = x;
+    }
+  }
+  __syncthreads();
+  // (m, j): W(+j) + W(-j) and W(+j) - W(-j); j = 0: W(0)
+  const int nj = a.Kz + 1;
+  const int t = threadIdx.x;
+  if (t < rows * nj) {
+    const int r = t / nj, j = t - (t / nj) * nj;
+    double* o = a.wt + ((size_t)(m0 + r) * nj + j) * 4;
+    const double* p = wl[r * a.NZ + a.Kz + j];
+    const double* n = wl[r * a.NZ + a.Kz - j];
+    if (j == 0) {
+      o[0] = p[0]; o[1] = p[1]; o[2] = 0.0; o[3] = 0.0;
+    } else {
+      o[0] = p[0] + n[0]; o[1] = p[1] + n[1]; o[2] = p[0] - n[0]; o[3] = p[1] - n[1];
+    }
+  }
 }
 
 // one thread per atom and one range of m = (nx, ny) values per blockIdx.y
@@ -1633,6 +1883,105 @@ __global__ void __launch_bounds__(256) k_in_gather(InArgs a) {
   o[(size_t)a.n + i] = u1;
   o[(size_t)2 * a.n + i] = u2;
 }
+
+// one thread per atom, one range of a.msplit values of m per blockIdx.y;
+// per m: s0 = sum_nz W Z, s1 = sum_nz nz W Z from the (m, j) table,
+// Z(+j) = z_j, Z(-j) = conj(z_j):
+//   s0 = W(0) + sum_j [(Sr c - Di s) + i (Dr s + Si c)]
+//   s1 = sum_j [(Dr jc - Si js) + i (Sr js + Di jc)]
+// with S = W(+j) + W(-j), D = W(+j) - W(-j), z_j = c + i s, (jc, js) = j z_j
+template <int KZ>
+__global__ void __launch_bounds__(256) k_in_gather_s(InArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = i < a.n;
+  const int ii = act ? i : 0;
+  double x1r, x1i, y1r, y1i;
+  double zc[KZ + 1], zs[KZ + 1];
+  {
+    double t = a.sx[ii] - rint(a.sx[ii]);
+    sincospi(2.0 * t, &x1i, &x1r);
+    t = a.sy[ii] - rint(a.sy[ii]);
+    sincospi(2.0 * t, &y1i, &y1r);
+    double s1, c1;
+    t = a.sz[ii] - rint(a.sz[ii]);
+    sincospi(2.0 * t, &s1, &c1);
+    zc[0] = 1.0;
+    zs[0] = 0.0;
+#pragma unroll
+    for (int k = 1; k <= KZ; ++k) {
+      zc[k] = zc[k - 1] * c1 - zs[k - 1] * s1;
+      zs[k] = zc[k - 1] * s1 + zs[k - 1] * c1;
+    }
+  }
+  double jc[KZ + 1], js[KZ + 1];          // j z_j
+#pragma unroll
+  for (int k = 0; k <= KZ; ++k) {
+    jc[k] = (double)k * zc[k];
+    js[k] = (double)k * zs[k];
+  }
+  const cplx<double> x1{x1r, x1i}, y1{y1r, y1i};
+  cplx<double> y0{1.0, 0.0};
+  for (int k = 0; k < a.Ky; ++k) y0 = cmul(y0, cplx<double>{y1r, -y1i});
+  const int M = (a.Kx + 1) * a.NY;
+  const int mb = blockIdx.y * a.msplit, me = min(M, mb + a.msplit);
+  cplx<double> X{1.0, 0.0}, Y = y0;
+  {
+    const int nx0 = mb / a.NY, iy0 = mb - (mb / a.NY) * a.NY;
+    for (int k = 0; k < nx0; ++k) X = cmul(X, x1);
+    for (int k = 0; k < iy0; ++k) Y = cmul(Y, y1);
+  }
+  const double* __restrict__ wt = a.wt;
+  double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+  for (int m = mb; m < me; ++m) {
+    const int nx = m / a.NY, ny = m - (m / a.NY) * a.NY - a.Ky;
+    const double* e = wt + (size_t)m * (KZ + 1) * 4;
+    double s0r = e[0], s0i = e[1], s1r = 0.0, s1i = 0.0;
+#pragma unroll
+    for (int j = 1; j <= KZ; ++j) {
+      const double sr = e[4 * j], si = e[4 * j + 1], dr = e[4 * j + 2], di = e[4 * j + 3];
+      s0r = fma(sr, zc[j], s0r);
+      s0r = fma(-di, zs[j], s0r);
+      s0i = fma(dr, zs[j], s0i);
+      s0i = fma(si, zc[j], s0i);
+      s1r = fma(dr, jc[j], s1r);
+      s1r = fma(-si, js[j], s1r);
+      s1i = fma(sr, js[j], s1i);
+      s1i = fma(di, jc[j], s1i);
+    }
+    const cplx<double> XY = cmul(X, Y);
+    const double i0 = XY.re * s0i + XY.im * s0r;
+    const double i1 = XY.re * s1i + XY.im * s1r;
+    u0 += (double)nx * i0;
+    u1 += (double)ny * i0;
+    u2 += i1;
+    if (ny == a.Ky) {
+      X = cmul(X, x1);
+      Y = y0;
+    } else {
+      Y = cmul(Y, y1);
+    }
+  }
+  if (!act) return;
+  double* o = a.out + (size_t)blockIdx.y * 3 * a.n;
+  o[i] = u0;
+  o[(size_t)a.n + i] = u1;
+  o[(size_t)2 * a.n + i] = u2;
+}
+
+using InKernel = void (*)(InArgs);
+template <int... I>
+InKernel pick_in_sk(int kz, std::integer_sequence<int, I...>) {
+  static const InKernel t[] = {k_in_sk_s<I + 1>...};
+  return t[kz - 1];
+}
+template <int... I>
+InKernel pick_in_gather(int kz, std::integer_sequence<int, I...>) {
+  static const InKernel t[] = {k_in_gather_s<I + 1>...};
+  return t[kz - 1];
+}
+// the register-blocked kernels for Kz' = kz (1 <= kz <= kInKZ)
+InKernel in_sk_kernel(int kz) { return pick_in_sk(kz, std::make_integer_sequence<int, kInKZ>{}); }
+InKernel in_gather_kernel(int kz) { return pick_in_gather(kz, std::make_integer_sequence<int, kInKZ>{}); }
 
 }  // namespace
 
@@ -1799,24 +2148,50 @@ static int recip_inner_setup(mdff_ctx* c, const std::vector<double>& Amain) {
   HIP_TRY(c, hipMemcpy(c->kA_in, Ai.data(), sizeof(double) * nbi, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->kvec_in, kv.data(), sizeof(double) * 4 * nbi, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->kA_mask, Am.data(), sizeof(double) * c->nk_box, hipMemcpyHostToDevice));
-  // k_in_sk: about 2048 blocks; tiles of atoms within 48 KiB of LDS
   const int ncol = kin[0] + kin[1] + kin[2] + 3;
-  int tile = (int)std::min<size_t>(64, 49152 / (16 * (size_t)ncol + 8));
-  if (tile < 1) return MDFF_OK;
-  const int kblocks = div_up(nbi, 256);
-  const int want = std::max(1, 2048 / kblocks);
-  int chunk = div_up(c->n, want);
-  chunk = div_up(chunk, tile) * tile;
-  c->in_chunk = chunk;
-  c->in_nch = div_up(c->n, chunk);
-  c->kin_tile = tile;
-  HIP_TRY(c, hipMalloc((void**)&c->sk_in_part, sizeof(double) * 2 * (size_t)nbi * c->in_nch));
-  // k_in_gather: m ranges of whole kInMT tiles, about 2048 blocks
   const int M = (kin[0] + 1) * NY;
-  const int tiles = div_up(M, kInMT);
-  const int sp = std::max(1, std::min(tiles, div_up(2048, div_up(c->n, 256))));
-  c->kin_msplit = div_up(tiles, sp) * kInMT;
-  c->kin_splits = div_up(M, c->kin_msplit);
+  const int ts = div_up(M, 64) * 64;
+  c->kin_reg = (kin[2] <= kInKZ && ts <= 1024) ? 1 : 0;
+  if (c->kin_reg) {
+    // k_in_sk_s: SG subgroups of ts threads (at most 4, at most 1024
+    // threads), one block per chunk, about 256 chunks (one per CU); the
+    // chunk's X / Y rows through LDS in tiles within 48 KiB.
+    // k_in_gather_s: m ranges for about 768 blocks of 256 atoms.
+    const int sg = std::min(4, 1024 / ts);
+    const int nxy = kin[0] + kin[1] + 2;
+    int chunk = std::max(div_up(c->n, 256), 4 * sg);
+    const int tile = std::min(chunk, 49152 / (16 * nxy));
+    chunk = div_up(chunk, tile) * tile;
+    c->kin_ts = ts;
+    c->kin_sg = sg;
+    c->in_chunk = chunk;
+    c->in_nch = div_up(c->n, chunk);
+    c->kin_tile = tile;
+    c->kin_lds = std::max<size_t>((size_t)tile * nxy * 16, (size_t)(sg - 1) * ts * 32);
+    const int sp = std::max(1, std::min(M, div_up(768, div_up(c->n, 256))));
+    c->kin_msplit = div_up(M, sp);
+    c->kin_splits = div_up(M, c->kin_msplit);
+    HIP_TRY(c, hipMalloc((void**)&c->tab_in, 16 * (size_t)ncol * c->n));
+    HIP_TRY(c, hipMalloc((void**)&c->wt_in, sizeof(double) * 4 * (size_t)M * (kin[2] + 1)));
+  } else {
+    // k_in_sk: about 2048 blocks; tiles of atoms within 48 KiB of LDS.
+    // k_in_gather: m ranges of whole kInMT tiles, about 2048 blocks
+    const int tile = (int)std::min<size_t>(64, 49152 / (16 * (size_t)ncol + 8));
+    if (tile < 1) return MDFF_OK;
+    const int kblocks = div_up(nbi, 256);
+    const int want = std::max(1, 2048 / kblocks);
+    int chunk = div_up(c->n, want);
+    chunk = div_up(chunk, tile) * tile;
+    c->in_chunk = chunk;
+    c->in_nch = div_up(c->n, chunk);
+    c->kin_tile = tile;
+    c->kin_lds = (size_t)tile * (16 * (size_t)ncol + 8);
+    const int tiles = div_up(M, kInMT);
+    const int sp = std::max(1, std::min(tiles, div_up(2048, div_up(c->n, 256))));
+    c->kin_msplit = div_up(tiles, sp) * kInMT;
+    c->kin_splits = div_up(M, c->kin_msplit);
+  }
+  HIP_TRY(c, hipMalloc((void**)&c->sk_in_part, sizeof(double) * 2 * (size_t)nbi * c->in_nch));
   c->kin_on = 1;
   return MDFF_OK;
 }
@@ -1910,9 +2285,10 @@ int recip_setup(mdff_ctx* c) {
   HIP_TRY(c, hipMalloc(&c->amat, tsz / 2 * (size_t)c->mma_mtiles * c->mma_nks * 64));
   HIP_TRY(c, hipMemset(c->amat, 0, tsz / 2 * (size_t)c->mma_mtiles * c->mma_nks * 64));
   // block partials of the energy / stress: k-blocks of 256, or m-tiles of 8,
-  // then those of the inner box
+  // then those of the inner box (k-blocks of 64, or of whole rows m, at
+  // least 46 k each)
   const size_t nblk_rec = std::max<size_t>(div_up(nb, 256), div_up((int64_t)c->NXk * c->NYk, 8)) +
-                          div_up((int64_t)(Kx + 1) * (2 * Ky + 1) * (2 * Kz + 1), 256);
+                          div_up((int64_t)(Kx + 1) * (2 * Ky + 1) * (2 * Kz + 1), 32);
   HIP_TRY(c, hipMalloc((void**)&c->rec_blk, sizeof(double) * 7 * nblk_rec));
   // mixed form: split-half kernels when their shapes fit (MDFF_RECIP_F32MMA=1
   // keeps the FP32 matrix-core kernels)
@@ -1972,7 +2348,7 @@ int recip_setup(mdff_ctx* c) {
 
 void recip_free(mdff_ctx* c) {
   void* ps[] = {c->kA, c->kvec, c->tab, c->sk_part, c->sk, c->wk, c->wk_t, c->rec_part, c->amat, c->rec_blk,
-                c->amat_h3, c->ascale_h3, c->kA_mask, c->kA_in, c->kvec_in, c->sk_in_part, c->wk_in,
+                c->amat_h3, c->ascale_h3, c->kA_mask, c->kA_in, c->kvec_in, c->sk_in_part, c->wk_in, c->tab_in, c->wt_in,
                 c->mu_pscale};
   for (void* p : ps)
     if (p) (void)hipFree(p);
@@ -2177,9 +2553,18 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     ia.kvec = c->kvec_in;
     ia.part = c->sk_in_part;
     ia.wk = c->wk_in;
-    const size_t lds = (size_t)ia.tile * (16 * (size_t)(ia.Kx + ia.Ky + ia.Kz + 3) + 8);
-    PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL(k_in_sk, dim3(div_up(ia.nb, 256), c->in_nch), dim3(256), lds,
-                                             c->stream, ia));
+    const size_t lds = c->kin_lds;
+    if (c->kin_reg) {
+      ia.ts = c->kin_ts;
+      ia.tab = reinterpret_cast<cplx<double>*>(c->tab_in);
+      ia.wt = c->wt_in;
+      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL(k_in_phase, dim3(div_up(3 * c->n, 256)), dim3(256), 0, c->stream, ia));
+      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL(in_sk_kernel(ia.Kz), dim3(c->in_nch), dim3(c->kin_ts * c->kin_sg), lds,
+                                               c->stream, ia));
+    } else {
+      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL(k_in_sk, dim3(div_up(ia.nb, 256), c->in_nch), dim3(256), lds,
+                                               c->stream, ia));
+    }
   }
   // the 7 energy / stress sums over the k-blocks: block partials here, the
   // final sum queued into e_out (device memory)
@@ -2190,10 +2575,15 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
   T* amat = mma_gather ? reinterpret_cast<T*>(c->amat) : nullptr;
   const int nb = h3_gather ? c->h3_mt8 : div_up(c->nk_box, 256);
   MDFF_TRY(claim_partials(c, c->rec_blk));
-  const int nbi = inner ? div_up(c->nb_in, 256) : 0;
+  const int rows_in = inner && c->kin_reg ? 64 / ia.NZ : 0;
+  const int nbi = !inner ? 0 : c->kin_reg ? div_up((ia.Kx + 1) * ia.NY, rows_in) : div_up(c->nb_in, 64);
   if (inner) {
     ia.blk = c->rec_blk + (size_t)7 * nb;     // after the main blocks
-    PROF_LAUNCH(c, PK_SKRED, hipLaunchKernelGGL(k_in_reduce, dim3(nbi), dim3(256), 0, c->stream, ia, c->in_nch));
+    if (c->kin_reg) {
+      PROF_LAUNCH(c, PK_SKRED, hipLaunchKernelGGL(k_in_reduce_r, dim3(nbi), dim3(1024), 0, c->stream, ia, c->in_nch));
+    } else {
+      PROF_LAUNCH(c, PK_SKRED, hipLaunchKernelGGL(k_in_reduce, dim3(nbi), dim3(256), 0, c->stream, ia, c->in_nch));
+    }
   }
   PROF_LAUNCH(c, PK_SKRED,
               if (h3_gather) {
@@ -2282,8 +2672,12 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     ia.out = c->rec_part + (size_t)nsplit * 3 * c->n;
     ia.msplit = c->kin_msplit;
     const size_t lds = (size_t)kInMT * ia.NZ * 16;
-    PROF_LAUNCH(c, PK_GATHER, hipLaunchKernelGGL(k_in_gather, dim3(div_up(c->n, 256), c->kin_splits), dim3(256),
-                                                 lds, c->stream, ia));
+    const dim3 gg(div_up(c->n, 256), c->kin_splits);
+    if (c->kin_reg) {
+      PROF_LAUNCH(c, PK_GATHER, hipLaunchKernelGGL(in_gather_kernel(ia.Kz), gg, dim3(256), 0, c->stream, ia));
+    } else {
+      PROF_LAUNCH(c, PK_GATHER, hipLaunchKernelGGL(k_in_gather, gg, dim3(256), lds, c->stream, ia));
+    }
     nsplit += c->kin_splits;
   }
   RecFinal rf{};
