@@ -237,6 +237,21 @@ int mdff_rescale_velocities(mdff_ctx *ctx, double lambda);
 /* Kinetic energy and temperature (src/kinetic.f90:527-552). */
 int mdff_calc_temp(mdff_ctx *ctx, double boltz_unit, double *temp, double *ekin);
 
+/* mdff_calc_temp for velocities already in device memory, with no allocation
+ * of any kind.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops).  n atoms; mass, vx, vy, vz: device arrays of n doubles; out: device
+ * memory for 2 doubles, receiving the kinetic energy (eV) and the
+ * temperature (K, with boltz_unit); stream: a hipStream_t (NULL: the null
+ * stream).  Asynchronous on that stream; the same sums, in the same order,
+ * as mdff_calc_temp.  The block partials live in a fixed device array of
+ * the library (1,026 doubles per device, kept until the process ends), so
+ * any natm fits (at most 1,024 blocks, each thread striding over the atoms);
+ * calls that overlap on one device race on that array.  Returns
+ * MDFF_ERR_BAD_ARG for n <= 0, null pointers or boltz_unit <= 0. */
+int mdff_calc_temp_noalloc(int32_t n, const double *mass, const double *vx, const double *vy,
+                           const double *vz, double boltz_unit, void *stream, double *out);
+
 /* Timing of the last engforce on the context stream (HIP events), ms:
  * [0] total, [1] list build/check, [2] short-range+real-space, [3] reciprocal,
  * [4] SCF loop.  n = number of entries written (<= 8); the events are

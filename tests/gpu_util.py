@@ -3,6 +3,8 @@ library and through the oracle, and measure the differences."""
 from __future__ import annotations
 
 import copy
+import ctypes as C
+import os
 import sys
 
 import numpy as np
@@ -18,6 +20,24 @@ def note(msg: str) -> None:
     """Name the case before a GPU step (flushed, so a crash still shows it)."""
     print(f"[gpu-case] {msg}", flush=True)
     sys.stdout.flush()
+
+
+def hip_runtime():
+    """The HIP runtime the library links, as loaded with it (no second
+    runtime in the process): device buffers for the entry points that take
+    device pointers."""
+    capi.gpu_lib()
+    try:
+        hip = C.CDLL("libamdhip64.so.7", mode=os.RTLD_NOLOAD | os.RTLD_GLOBAL)
+    except OSError:
+        hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipDeviceSynchronize.argtypes = []
+    for fn in (hip.hipMalloc, hip.hipMemcpy, hip.hipFree, hip.hipDeviceSynchronize):
+        fn.restype = C.c_int
+    return hip
 
 
 def pair(system, precision=capi.PREC_FP64, **over):
