@@ -45,7 +45,7 @@ FLOPS_PER_UNIT = {
 # the kernel behind each profiler class of the mixed form, and the unit it
 # is bound by: the reciprocal sums run on split-half FP16 matrix cores
 # (three FP16 products per FP32-class product), the pair kernel on FP32 VALU
-KERNEL_OF = {"k_sk": "k_sk_h3<32,0>", "k_gather": "k_gather_h3<CT,NSH>", "k_pair_force": "k_pair_force<float,false,16>"}
+KERNEL_OF = {"k_sk": "k_sk_h3<32,0>", "k_gather": "k_gather_h3<CT,NSH>", "k_pair_force": "k_pair_force<float,false,16,2>"}
 BOUND_OF = {"k_sk": ("mfma", "fp16"), "k_gather": ("mfma", "fp16"), "k_pair_force": ("valu", "fp32")}
 # HBM bytes per launch of the mixed-form kernels on this workload: rocprofv3
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (KiB per dispatch x 1024; the factors
@@ -71,7 +71,7 @@ def mfma_issued(kind: str, natm: int, kmax) -> float:
     return 0.0
 
 
-NKINDS = 9   # kernel classes of the library's event profiler (mdff_profile_name)
+NKINDS = 10  # kernel classes of the library's event profiler (mdff_profile_name)
 
 
 def build_system(natm: int, relax_iters: int):

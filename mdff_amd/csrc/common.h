@@ -84,7 +84,7 @@ struct DevBuf {
 };
 
 // kernel classes timed by the in-library event profiler
-enum ProfKind { PK_PAIR = 0, PK_FIELD, PK_TABLES, PK_SK, PK_SKRED, PK_GATHER, PK_GFINAL, PK_NLIST, PK_AMAT, PK_COUNT };
+enum ProfKind { PK_PAIR = 0, PK_FIELD, PK_TABLES, PK_SK, PK_SKRED, PK_GATHER, PK_GFINAL, PK_NLIST, PK_AMAT, PK_INNER, PK_COUNT };
 struct ProfRec {
   int kind;
   hipEvent_t a, b;

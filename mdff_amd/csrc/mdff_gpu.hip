@@ -1217,7 +1217,7 @@ hipEvent_t prof_event(mdff_ctx* c) {
 
 static const char* kProfNames[PK_COUNT] = {"k_pair_force", "k_pair_field", "k_tables",       "k_sk",
                                            "k_sk_reduce",  "k_gather",     "k_gather_final", "k_nl_sweep",
-                                           "k_amat"};
+                                           "k_amat",       "k_inner_box"};
 
 // ====================================================================== C-ABI
 

@@ -43,11 +43,10 @@ struct PairArgs {
   const PairParam* pp;
   const int32_t* row;                // row starts (n)
   const int32_t* cnt;                // row lengths (n)
-  // seg 0: whole rows; 1: the leading cnt_in[i] entries of a row (its inner
-  // entries); 2: the rest.  accum: add to the per-atom outputs already
-  // written (by the launch over the other part) instead of storing
+  // seg 0: whole rows; 1 (mixed form): the leading cnt_in[i] entries of a
+  // row (its inner entries) in FP64, the rest in FP32
   const int32_t* cnt_in;
-  int seg, accum;
+  int seg;
   double coul;                       // coul_unit
   const int32_t* nl_j;               // sorted positions of the neighbours
   double* fsr;                       // 3n AoS out: f_sr + coul_unit f_dir
@@ -82,60 +81,61 @@ __device__ __forceinline__ double group_sum(double v) {
   return v;
 }
 
-template <typename T, bool DIP, int LPA>
-__global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
-  __shared__ PairParam spp[64];
-  for (int k = threadIdx.x; k < a.ntype * a.ntype; k += blockDim.x) spp[k] = a.pp[k];
-  __syncthreads();
-  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int pi = gtid / LPA;             // atoms in cell order: neighbouring groups share neighbours
-  const int sub = gtid % LPA;
-  if (pi >= a.n) return;  // whole groups exit together
-  const AtomRec ri = a.rec[pi];
-  const int2 mi = a.meta[pi];
-  const int i = mi.y;
-  const double si = ri.x, ti = ri.y, ui = ri.z;
-  const int ita = mi.x;
-  const T qi = (T)ri.q;
+// The lane's sums over its share (entries r0 + sub, r0 + sub + LPA, ...) of
+// one row segment [r0, r1): pair functions in T, forces and fields summed in
+// FP64 per pair, energies and stresses in T over the lane's share
+template <typename T>
+struct RowSums {
+  double f0 = 0, f1 = 0, f2 = 0, e0 = 0, e1 = 0, e2 = 0;
+  T usr = 0, ucl = 0;
+  T ts[6] = {0, 0, 0, 0, 0, 0}, tc[6] = {0, 0, 0, 0, 0, 0};
+};
+
+// atom i of a row: fractional coordinates, charge, type, dipole
+struct AtomI {
+  double si, ti, ui, q, m0, m1, m2;
+  int ita;
+  bool dip;
+};
+
+template <typename T, bool DIP, int LPA, int SRM, bool PREF = true>
+__device__ __forceinline__ void pair_row(const PairArgs& a, const PairParam* spp, const AtomI& ai, int r0, int r1,
+                                         int sub, RowSums<T>& s) {
+  const double si = ai.si, ti = ai.ti, ui = ai.ui;
+  const int ita = ai.ita;
+  const T qi = (T)ai.q;
   const T alpha = (T)a.alpha, a2 = alpha * alpha;
   const T isqpi = T(0.56418958354775628695);  // 1/sqrt(pi)
-  V3<T> mui{T(0), T(0), T(0)};
-  bool dip_i = false;
-  if (DIP) {
-    double m0 = a.mu[3 * i], m1 = a.mu[3 * i + 1], m2 = a.mu[3 * i + 2];
-    dip_i = (m0 != 0.0 || m1 != 0.0 || m2 != 0.0);
-    mui = {(T)m0, (T)m1, (T)m2};
-  }
-  // f: the pair forces on i, short range + coul_unit x real-space Ewald,
-  // combined per pair in T
-  double f0 = 0, f1 = 0, f2 = 0, e0 = 0, e1 = 0, e2 = 0;
-  // energies and stresses: summed in T over the lane's share of the row,
-  // then in FP64 across the lanes and atoms (each lane holds a few dozen
-  // pairs, so the T rounding of these global sums stays far below the
-  // force error of the mixed form; the forces are summed in FP64 per pair)
-  using EA = T;
-  EA usr = 0, ts[6] = {0, 0, 0, 0, 0, 0}, ucl = 0, tc[6] = {0, 0, 0, 0, 0, 0};
+  const V3<T> mui{(T)ai.m0, (T)ai.m1, (T)ai.m2};
+  const bool dip_i = ai.dip;
   const T coul = (T)a.coul;
-  int r0 = a.row[i], r1 = r0 + a.cnt[i];
-  if (a.seg == 1) r1 = r0 + a.cnt_in[i];
-  else if (a.seg == 2) r0 += a.cnt_in[i];
-  // two-stage prefetch of the neighbour chain: the row entry two steps ahead
-  // and the record of the next entry are in flight while a pair is computed
-  int pj_next = (r0 + sub + LPA < r1) ? a.nl_j[r0 + sub + LPA] : 0;
+  const int srm = SRM;   // short-range model, a compile-time constant of the kernel
+  // PREF: two-stage prefetch of the neighbour chain: the row entry two steps
+  // ahead and the record of the next entry are in flight while a pair is
+  // computed (short segments: each entry loaded when it is reached)
+  int pj_next = (PREF && r0 + sub + LPA < r1) ? a.nl_j[r0 + sub + LPA] : 0;
   AtomRec rj_next{};
   int2 mj_next{};
-  if (r0 + sub < r1) {
+  if (PREF && r0 + sub < r1) {
     const int pj0 = a.nl_j[r0 + sub];
     rj_next = a.rec[pj0];
     mj_next = a.meta[pj0];
   }
   for (int e = r0 + sub; e < r1; e += LPA) {
-    const AtomRec rj = rj_next;
-    const int2 mj = mj_next;
-    if (e + LPA < r1) {
-      rj_next = a.rec[pj_next];
-      mj_next = a.meta[pj_next];
-      if (e + 2 * LPA < r1) pj_next = a.nl_j[e + 2 * LPA];
+    AtomRec rj;
+    int2 mj;
+    if (PREF) {
+      rj = rj_next;
+      mj = mj_next;
+      if (e + LPA < r1) {
+        rj_next = a.rec[pj_next];
+        mj_next = a.meta[pj_next];
+        if (e + 2 * LPA < r1) pj_next = a.nl_j[e + 2 * LPA];
+      }
+    } else {
+      const int pj = a.nl_j[e];
+      rj = a.rec[pj];
+      mj = a.meta[pj];
     }
     const int j = mj.y;
     double dxs = si - rj.x, dys = ti - rj.y, dzs = ui - rj.z;
@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
     const double r2d = rxd * rxd + ryd * ryd + rzd * rzd;
     const int jta = mj.x;
     const PairParam& P = spp[ita + a.ntype * jta];
-    const bool in_sr = a.sr_model != 0 && r2d < P.rcutsq && !(a.sr_model == 2 && P.A == 0.0);
+    const bool in_sr = srm != 0 && r2d < P.rcutsq && !(srm == 2 && P.A == 0.0);
     const bool in_cl = (a.qq | a.qd | a.dd) && !(r2d > a.cutsq_c);
     if (!in_sr && !in_cl) continue;
     const T rx = (T)rxd, ry = (T)ryd, rz = (T)rzd, r2 = (T)r2d;
@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
     T pfx = T(0), pfy = T(0), pfz = T(0);
     if (in_sr) {
       T w = T(0), u = T(0);
-      if (a.sr_model == 2) {
+      if (srm == 2) {
         const T erh = (T)P.A * exp(-(T)P.B * d);
         T ir6 = dm2 * dm2 * dm2;
         T ir8 = ir6 * dm2;
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
         }
         u = erh - ir6d - ir8d;
         w = ((T)P.B * erh - T(6) * ir6d * dm1 - T(8) * ir8d * dm1 + ir6 * fd6 + ir8 * fd8) * dm1;
-      } else if (a.sr_model == 1) {
+      } else if (srm == 1) {
         const T sr2 = (T)P.sigsq * dm2;
         const T srp = pow(sr2, (T)(0.5 * P.pl));
         const T srq = pow(sr2, (T)(0.5 * P.ql));
@@ -205,13 +205,13 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
       pfx = fx;
       pfy = fy;
       pfz = fz;
-      usr += (EA)u;
-      ts[0] += (EA)(rx * fx);
-      ts[1] += (EA)(ry * fy);
-      ts[2] += (EA)(rz * fz);
-      ts[3] += (EA)(rx * fy);
-      ts[4] += (EA)(rx * fz);
-      ts[5] += (EA)(ry * fz);
+      s.usr += u;
+      s.ts[0] += (rx * fx);
+      s.ts[1] += (ry * fy);
+      s.ts[2] += (rz * fz);
+      s.ts[3] += (rx * fy);
+      s.ts[4] += (rx * fz);
+      s.ts[5] += (ry * fz);
     }
     if (in_cl) {
       const T qj = (T)rj.q;
@@ -238,9 +238,9 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
         if (a.do_field) {
           T g = qj * F1_dm3;
           if (ldamp) g -= qj * (T(1) - fdamp) * dm3;
-          e0 += (double)(g * rx);
-          e1 += (double)(g * ry);
-          e2 += (double)(g * rz);
+          s.e0 += (double)(g * rx);
+          s.e1 += (double)(g * ry);
+          s.e2 += (double)(g * rz);
         }
       }
       if (DIP) {
@@ -279,9 +279,9 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
             const T mm = mui.x * muj.x + mui.y * muj.y + mui.z * muj.z;
             u -= F2_dm5 * rmi * rmj - F1_dm3 * mm;
             if (a.do_field) {
-              e0 += (double)(F2_dm5 * rx * rmj - F1_dm3 * muj.x);
-              e1 += (double)(F2_dm5 * ry * rmj - F1_dm3 * muj.y);
-              e2 += (double)(F2_dm5 * rz * rmj - F1_dm3 * muj.z);
+              s.e0 += (double)(F2_dm5 * rx * rmj - F1_dm3 * muj.x);
+              s.e1 += (double)(F2_dm5 * ry * rmj - F1_dm3 * muj.y);
+              s.e2 += (double)(F2_dm5 * rz * rmj - F1_dm3 * muj.z);
             }
             const T d5 = d3 * r2;
             const T F3 = F2 + T(8.0 / 15.0) * alpha * a2 * a2 * d5 * expon;
@@ -296,17 +296,78 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
       pfx -= coul * fij0;
       pfy -= coul * fij1;
       pfz -= coul * fij2;
-      ucl += (EA)u;
-      tc[0] -= (EA)(T(2) * rx * fij0);
-      tc[1] -= (EA)(T(2) * ry * fij1);
-      tc[2] -= (EA)(T(2) * rz * fij2);
-      tc[3] -= (EA)(rx * fij1 + ry * fij0);
-      tc[4] -= (EA)(rx * fij2 + rz * fij0);
-      tc[5] -= (EA)(ry * fij2 + rz * fij1);
+      s.ucl += u;
+      s.tc[0] -= (T(2) * rx * fij0);
+      s.tc[1] -= (T(2) * ry * fij1);
+      s.tc[2] -= (T(2) * rz * fij2);
+      s.tc[3] -= (rx * fij1 + ry * fij0);
+      s.tc[4] -= (rx * fij2 + rz * fij0);
+      s.tc[5] -= (ry * fij2 + rz * fij1);
     }
-    f0 += (double)pfx;
-    f1 += (double)pfy;
-    f2 += (double)pfz;
+    s.f0 += (double)pfx;
+    s.f1 += (double)pfy;
+    s.f2 += (double)pfz;
+  }
+}
+
+// seg 0: the whole row in T; seg 1 (T = float): the row's leading
+// cnt_in[i] entries (its inner entries) in FP64, the rest in FP32, one pass.
+// The lane's FP64 and FP32 sums are added in FP64 before the group sums.
+template <typename T, bool DIP, int LPA, int SRM>
+__global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
+  __shared__ PairParam spp[64];
+  for (int k = threadIdx.x; k < a.ntype * a.ntype; k += blockDim.x) spp[k] = a.pp[k];
+  __syncthreads();
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int pi = gtid / LPA;             // atoms in cell order: neighbouring groups share neighbours
+  const int sub = gtid % LPA;
+  if (pi >= a.n) return;  // whole groups exit together
+  const AtomRec ri = a.rec[pi];
+  const int2 mi = a.meta[pi];
+  const int i = mi.y;
+  AtomI ai{ri.x, ri.y, ri.z, ri.q, 0.0, 0.0, 0.0, mi.x, false};
+  if (DIP) {
+    ai.m0 = a.mu[3 * i];
+    ai.m1 = a.mu[3 * i + 1];
+    ai.m2 = a.mu[3 * i + 2];
+    ai.dip = (ai.m0 != 0.0 || ai.m1 != 0.0 || ai.m2 != 0.0);
+  }
+  int r0 = a.row[i];
+  const int r1 = r0 + a.cnt[i];
+  // energies and stresses: summed in T over the lane's share of the row,
+  // then in FP64 across the lanes and atoms (each lane holds a few dozen
+  // pairs, so the T rounding of these global sums stays far below the
+  // force error of the mixed form)
+  double f0 = 0, f1 = 0, f2 = 0, e0 = 0, e1 = 0, e2 = 0;
+  double usr = 0, ucl = 0, ts[6] = {0, 0, 0, 0, 0, 0}, tc[6] = {0, 0, 0, 0, 0, 0};
+  if constexpr (sizeof(T) == 4) {
+    if (a.seg == 1) {
+      const int rm = r0 + a.cnt_in[i];
+      RowSums<double> sd;
+      pair_row<double, DIP, LPA, SRM, false>(a, spp, ai, r0, rm, sub, sd);
+      f0 = sd.f0; f1 = sd.f1; f2 = sd.f2;
+      e0 = sd.e0; e1 = sd.e1; e2 = sd.e2;
+      usr = sd.usr; ucl = sd.ucl;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        ts[k] = sd.ts[k];
+        tc[k] = sd.tc[k];
+      }
+      r0 = rm;
+    }
+  }
+  {
+    RowSums<T> st;
+    pair_row<T, DIP, LPA, SRM>(a, spp, ai, r0, r1, sub, st);
+    f0 += st.f0; f1 += st.f1; f2 += st.f2;
+    e0 += st.e0; e1 += st.e1; e2 += st.e2;
+    usr += (double)st.usr;
+    ucl += (double)st.ucl;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      ts[k] += (double)st.ts[k];
+      tc[k] += (double)st.tc[k];
+    }
   }
   f0 = group_sum<LPA>(f0);
   f1 = group_sum<LPA>(f1);
@@ -318,30 +379,16 @@ __global__ void __launch_bounds__(256) k_pair_force(PairArgs a) {
   }
   double cols[NPA];
   if (a.want_stress) {
-    cols[0] = group_sum<LPA>((double)usr);
+    cols[0] = group_sum<LPA>(usr);
     cols[1] = 0.0;                        // set from the stress trace below
 #pragma unroll
-    for (int k = 0; k < 6; ++k) cols[2 + k] = group_sum<LPA>((double)ts[k]);
+    for (int k = 0; k < 6; ++k) cols[2 + k] = group_sum<LPA>(ts[k]);
     cols[1] = (cols[2] + cols[3]) + cols[4];  // virial w r^2 = trace of r (w r)
-    cols[8] = group_sum<LPA>((double)ucl);
+    cols[8] = group_sum<LPA>(ucl);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) cols[9 + k] = group_sum<LPA>((double)tc[k]);
+    for (int k = 0; k < 6; ++k) cols[9 + k] = group_sum<LPA>(tc[k]);
   }
   if (sub == 0) {
-    if (a.accum) {
-      f0 = a.fsr[3 * i] + f0;
-      f1 = a.fsr[3 * i + 1] + f1;
-      f2 = a.fsr[3 * i + 2] + f2;
-      if (a.do_field) {
-        e0 = a.ef[3 * i] + e0;
-        e1 = a.ef[3 * i + 1] + e1;
-        e2 = a.ef[3 * i + 2] + e2;
-      }
-      if (a.want_stress) {
-#pragma unroll
-        for (int k = 0; k < NPA; ++k) cols[k] = a.peratom[(size_t)k * a.n + i] + cols[k];
-      }
-    }
     a.fsr[3 * i] = f0;
     a.fsr[3 * i + 1] = f1;
     a.fsr[3 * i + 2] = f2;
@@ -501,7 +548,21 @@ template <typename T, bool DIP>
 void launch_force(mdff_ctx* c, const PairArgs& a) {
   constexpr int LPA = 16;
   dim3 grid(div_up((int64_t)c->n * LPA, 256));
-  PROF_LAUNCH(c, PK_PAIR, hipLaunchKernelGGL((k_pair_force<T, DIP, LPA>), grid, dim3(256), 0, c->stream, a));
+  // one kernel per short-range model (0 none, 1 n-m LJ, 2 BMH family, 3 Morse)
+  switch (a.sr_model) {
+    case 1:
+      PROF_LAUNCH(c, PK_PAIR, hipLaunchKernelGGL((k_pair_force<T, DIP, LPA, 1>), grid, dim3(256), 0, c->stream, a));
+      break;
+    case 2:
+      PROF_LAUNCH(c, PK_PAIR, hipLaunchKernelGGL((k_pair_force<T, DIP, LPA, 2>), grid, dim3(256), 0, c->stream, a));
+      break;
+    case 3:
+      PROF_LAUNCH(c, PK_PAIR, hipLaunchKernelGGL((k_pair_force<T, DIP, LPA, 3>), grid, dim3(256), 0, c->stream, a));
+      break;
+    default:
+      PROF_LAUNCH(c, PK_PAIR, hipLaunchKernelGGL((k_pair_force<T, DIP, LPA, 0>), grid, dim3(256), 0, c->stream, a));
+      break;
+  }
 }
 
 template <typename T, int MODE>
@@ -532,15 +593,9 @@ int realspace_force_impl(mdff_ctx* c, double* fsr, double* fcl, double* ef, bool
   const bool fp32 = c->d.precision == MDFF_PREC_MIXED;
   if (fp32) {
     if (c->rc_inner > 0.0) {
-      // the inner entries of the rows in FP64, then the rest in FP32 added
-      // to them
+      // the inner entries of the rows in FP64, the rest in FP32
       a.cnt_in = c->nl_count_in;
       a.seg = 1;
-      a.accum = 0;
-      if (dip) launch_force<double, true>(c, a);
-      else launch_force<double, false>(c, a);
-      a.seg = 2;
-      a.accum = 1;
     }
     if (dip) launch_force<float, true>(c, a);
     else launch_force<float, false>(c, a);

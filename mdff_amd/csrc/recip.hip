@@ -16,6 +16,7 @@
 //   phase 2  per atom: U1_d = sum_n n_d Im(W e_i), M_de = sum_n n_d n_e Re(W e_i)
 //            field = 2 (4 pi/V) 2 pi B U1; force = 2 (4 pi/V) [q_i 2 pi B U1
 //            + (2 pi)^2 B M m_i]                               (:196-233, :300-358)
+#include <cstdlib>
 #include <type_traits>
 #include <utility>
 
@@ -1138,7 +1139,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 // One block per m-tile (k = m NZ + t for its 8 values of m, every nz): S(k)
 // as k_sk_reduce (chunks in order), W = A conj(S), the energy / stress block
 // partials, and the split A operand of the gather:
-// amat[((mt * NSH + s) * 64 + lane) * 2 + {0: hi, 1: lo}] = 8 halves: row
+// amat[(mt * NSH + s) * 128 + {0: hi, 1: lo} * 64 + lane] = 8 halves (hi and
+// lo planes apart, so a wave's 16-byte reads are consecutive in LDS): row
 // l%32 = (m_local = row / 4, comp = row % 4), K = 16 s + 8 (l/32) + j =
 // (nz index t, Re/Im cc): comp 0 (Re U0): (wr, -wi), comp 1 (Im U0):
 // (wi, wr), comp 2/3 the same times nz.  Values times 2^(14 - e), e the
@@ -1253,9 +1255,9 @@ __global__ void __launch_bounds__(256) k_sk_reduce_h3(int nk_box, int nchunk, co
         split_h(val[u][2 * comp + cc] * sc, h, l);
         const int K = 2 * t + cc, row = 4 * ml + comp;
         const int ln = 32 * ((K >> 3) & 1) + row;
-        const int at = (((K >> 4) * 64 + ln) * 2) * 8 + (K & 7);
+        const int at = ((K >> 4) * 128 + ln) * 8 + (K & 7);
         th[at] = __builtin_bit_cast(uint16_t, h);
-        th[at + 8] = __builtin_bit_cast(uint16_t, l);
+        th[at + 64 * 8] = __builtin_bit_cast(uint16_t, l);
       }
   }
   __syncthreads();
@@ -1278,6 +1280,9 @@ struct Gh3Args {
   const float* ascale;
   double* out;                   // [split][3][n]
 };
+
+// word 3 of a raw buffer resource on gfx950 (32-bit data, no swizzle)
+constexpr int kBufWord3 = 0x00020000;
 
 template <int CT, int NSH>
 constexpr int gather_h3_waves() {
@@ -1338,6 +1343,14 @@ k_gather_h3(Gh3Args a) {
   }
   __syncthreads();
   const float inv_ny = 1.0f / (float)a.NY;
+  // the phase tables as a buffer resource: 32-bit byte offsets, no 64-bit
+  // address arithmetic per load (tables below 2 GiB: checked at set-up)
+  const __amdgpu_buffer_rsrc_t trs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<void*>(a.tab), (short)0, (int)(8 * (size_t)(a.Kx + a.Ky + a.Kz + 3) * a.n), kBufWord3);
+  const int rowb = 8 * a.n;                           // bytes per table column
+  int ioff[CT];
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct) ioff[ct] = 8 * iac[ct];
   for (int mt = mt0; mt < mt1; ++mt) {
     const int buf = (mt - mt0) & 1;
     if (mt + 1 < mt1) {
@@ -1359,12 +1372,12 @@ k_gather_h3(Gh3Args a) {
       fx[q] = (float)nxx;
       fy[q] = (float)ny;
       ysg[q] = ny < 0 ? -1.f : 1.f;
-      const size_t xrow = (size_t)nxx * a.n;
-      const size_t yrow = (size_t)(a.Kx + 1 + (ny >= 0 ? ny : -ny)) * a.n;
+      const int xrow = nxx * rowb;
+      const int yrow = (a.Kx + 1 + (ny >= 0 ? ny : -ny)) * rowb;
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) {
-        Xv[ct][q] = tab[xrow + iac[ct]];
-        Yv[ct][q] = tab[yrow + iac[ct]];
+        Xv[ct][q] = __builtin_bit_cast(cplx<float>, __builtin_amdgcn_raw_buffer_load_b64(trs, xrow + ioff[ct], 0, 0));
+        Yv[ct][q] = __builtin_bit_cast(cplx<float>, __builtin_amdgcn_raw_buffer_load_b64(trs, yrow + ioff[ct], 0, 0));
       }
     }
     f32x16 acc[CT];
@@ -1374,8 +1387,8 @@ k_gather_h3(Gh3Args a) {
       for (int r = 0; r < 16; ++r) acc[ct][r] = 0.f;
 #pragma unroll
     for (int s = 0; s < NSH; ++s) {
-      const half8 ah = __builtin_bit_cast(half8, as[buf][(s * 64 + lane) * 2]);
-      const half8 al = __builtin_bit_cast(half8, as[buf][(s * 64 + lane) * 2 + 1]);
+      const half8 ah = __builtin_bit_cast(half8, as[buf][s * 128 + lane]);
+      const half8 al = __builtin_bit_cast(half8, as[buf][s * 128 + 64 + lane]);
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) {
         acc[ct] = mma_h(ah, bh[ct][s], acc[ct]);
@@ -2105,11 +2118,23 @@ static const void* sk_reduce_h3_kernel(int nsh) {
 // cutoff |k| <= 2.5 alpha (|n_d| <= |k| |a_d| / 2 pi, a_d lattice vector d),
 // within the main box; A and k of its entries as the main mesh's; the main
 // mesh's A copied with the box zeroed; chunking of k_in_sk.
+// a positive number from the environment, or def
+static double env_pos(const char* name, double def) {
+  const char* ev = getenv(name);
+  if (!ev || !*ev) return def;
+  const double v = atof(ev);
+  return v > 0.0 ? v : def;
+}
+
+// Tuning knobs read here (context creation): MDFF_KIN_ALPHA, the box's |k|
+// bound in units of alpha (2.5); MDFF_KIN_SG, subgroups of the structure
+// factor's blocks; MDFF_KIN_CHUNKS, its atom chunks (256); MDFF_KIN_GBLOCKS,
+// the gather's target block count (768).
 static int recip_inner_setup(mdff_ctx* c, const std::vector<double>& Amain) {
   c->kin_on = 0;
   const bool fp32 = c->d.precision == MDFF_PREC_MIXED;
   if (!fp32 || !c->task_qq || c->task_qd || c->task_dd) return MDFF_OK;
-  const double kcut = 2.5 * c->alpha;
+  const double kcut = env_pos("MDFF_KIN_ALPHA", 2.5) * c->alpha;
   int kin[3];
   for (int d = 0; d < 3; ++d) {
     const double* a = c->A + 3 * d;
@@ -2157,9 +2182,9 @@ static int recip_inner_setup(mdff_ctx* c, const std::vector<double>& Amain) {
     // threads), one block per chunk, about 256 chunks (one per CU); the
     // chunk's X / Y rows through LDS in tiles within 48 KiB.
     // k_in_gather_s: m ranges for about 768 blocks of 256 atoms.
-    const int sg = std::min(4, 1024 / ts);
+    const int sg = std::min((int)env_pos("MDFF_KIN_SG", 4), 1024 / ts);
     const int nxy = kin[0] + kin[1] + 2;
-    int chunk = std::max(div_up(c->n, 256), 4 * sg);
+    int chunk = std::max(div_up(c->n, (int)env_pos("MDFF_KIN_CHUNKS", 256)), 4 * sg);
     const int tile = std::min(chunk, 49152 / (16 * nxy));
     chunk = div_up(chunk, tile) * tile;
     c->kin_ts = ts;
@@ -2168,7 +2193,7 @@ static int recip_inner_setup(mdff_ctx* c, const std::vector<double>& Amain) {
     c->in_nch = div_up(c->n, chunk);
     c->kin_tile = tile;
     c->kin_lds = std::max<size_t>((size_t)tile * nxy * 16, (size_t)(sg - 1) * ts * 32);
-    const int sp = std::max(1, std::min(M, div_up(768, div_up(c->n, 256))));
+    const int sp = std::max(1, std::min(M, div_up((int)env_pos("MDFF_KIN_GBLOCKS", 768), div_up(c->n, 256))));
     c->kin_msplit = div_up(M, sp);
     c->kin_splits = div_up(M, c->kin_msplit);
     HIP_TRY(c, hipMalloc((void**)&c->tab_in, 16 * (size_t)ncol * c->n));
@@ -2300,7 +2325,9 @@ int recip_setup(mdff_ctx* c) {
     const bool sk_off = ev && ev[0] == '2';         // FP32 matrix-core structure factor only
     const int nxy = Kx + Ky + 2;
     const int nsh = h3_bucket(div_up(2 * c->NZk, 16));
-    if (fp32 && c->use_mma && !off && nsh > 0) {
+    // the gather addresses the FP32 phase tables with 32-bit byte offsets
+    const bool tab_ok = 8.0 * (Kx + Ky + Kz + 3) * (double)c->n < 2147483648.0;
+    if (fp32 && c->use_mma && !off && nsh > 0 && tab_ok) {
       // the gather for any kz its K steps cover; the structure factor where
       // its staged columns fit (else the FP32 matrix-core one feeds it)
       c->h3 = 1;
@@ -2558,11 +2585,11 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
       ia.ts = c->kin_ts;
       ia.tab = reinterpret_cast<cplx<double>*>(c->tab_in);
       ia.wt = c->wt_in;
-      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL(k_in_phase, dim3(div_up(3 * c->n, 256)), dim3(256), 0, c->stream, ia));
-      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL(in_sk_kernel(ia.Kz), dim3(c->in_nch), dim3(c->kin_ts * c->kin_sg), lds,
+      PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(k_in_phase, dim3(div_up(3 * c->n, 256)), dim3(256), 0, c->stream, ia));
+      PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(in_sk_kernel(ia.Kz), dim3(c->in_nch), dim3(c->kin_ts * c->kin_sg), lds,
                                                c->stream, ia));
     } else {
-      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL(k_in_sk, dim3(div_up(ia.nb, 256), c->in_nch), dim3(256), lds,
+      PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(k_in_sk, dim3(div_up(ia.nb, 256), c->in_nch), dim3(256), lds,
                                                c->stream, ia));
     }
   }
@@ -2580,9 +2607,9 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
   if (inner) {
     ia.blk = c->rec_blk + (size_t)7 * nb;     // after the main blocks
     if (c->kin_reg) {
-      PROF_LAUNCH(c, PK_SKRED, hipLaunchKernelGGL(k_in_reduce_r, dim3(nbi), dim3(1024), 0, c->stream, ia, c->in_nch));
+      PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(k_in_reduce_r, dim3(nbi), dim3(1024), 0, c->stream, ia, c->in_nch));
     } else {
-      PROF_LAUNCH(c, PK_SKRED, hipLaunchKernelGGL(k_in_reduce, dim3(nbi), dim3(256), 0, c->stream, ia, c->in_nch));
+      PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(k_in_reduce, dim3(nbi), dim3(256), 0, c->stream, ia, c->in_nch));
     }
   }
   PROF_LAUNCH(c, PK_SKRED,
@@ -2674,9 +2701,9 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     const size_t lds = (size_t)kInMT * ia.NZ * 16;
     const dim3 gg(div_up(c->n, 256), c->kin_splits);
     if (c->kin_reg) {
-      PROF_LAUNCH(c, PK_GATHER, hipLaunchKernelGGL(in_gather_kernel(ia.Kz), gg, dim3(256), 0, c->stream, ia));
+      PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(in_gather_kernel(ia.Kz), gg, dim3(256), 0, c->stream, ia));
     } else {
-      PROF_LAUNCH(c, PK_GATHER, hipLaunchKernelGGL(k_in_gather, gg, dim3(256), lds, c->stream, ia));
+      PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(k_in_gather, gg, dim3(256), lds, c->stream, ia));
     }
     nsplit += c->kin_splits;
   }
