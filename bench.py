@@ -48,10 +48,10 @@ FLOPS_PER_UNIT = {
 KERNEL_OF = {"k_sk": "k_sk_h3<32,0>", "k_gather": "k_gather_h3<CT,NSH>", "k_pair_force": "k_pair_force<float,false,16,2>"}
 BOUND_OF = {"k_sk": ("mfma", "fp16"), "k_gather": ("mfma", "fp16"), "k_pair_force": ("valu", "fp32")}
 # HBM bytes per launch of the mixed-form kernels on this workload: rocprofv3
-# FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (KiB per dispatch x 1024; the factors
+# FETCH_SIZE x 1.987 + WRITE_SIZE x 0.996 (KiB per dispatch x 1024; the factors
 # calibrate the counters on a 256 MiB device-to-device copy), measured with
 # mdff_amd.profile_workload (profiles/engforce_sio2_24k_hbm_traffic.md)
-TRAFFIC = {"k_gather": 65.12e6, "k_sk": 40.82e6, "k_pair_force": 63.18e6}
+TRAFFIC = {"k_gather": 64.46e6, "k_sk": 40.60e6, "k_pair_force": 63.92e6}
 
 
 def mfma_issued(kind: str, natm: int, kmax) -> float:
@@ -263,7 +263,24 @@ def fit_leg(nsnap: int, reps: int) -> dict:
     sec64 = time.perf_counter() - t0
     P.close()
     natm = sum(s.natm for s in snaps)
+    # roofline of k_fit: the reciprocal (k, atom) terms of every dipole-field
+    # operator application (SCF and adjoint iterations) and the four other
+    # passes (static field, forces, h, its reciprocal part), at the
+    # structure-factor and gather figures of the engforce (11 + 10 flops per
+    # (k, atom)); pairs left out (under a tenth of the work).  FP32 VALU peak.
+    def nk_of(k):
+        return k[2] + k[1] * (2 * k[2] + 1) + k[0] * (2 * k[1] + 1) * (2 * k[2] + 1)
+    nkn = np.array([s.natm * nk_of(list(s.desc.kES)) for s in snaps], float)
+    passes = r32.nscf[:, 0] + r32.nscf[:, 1] + 4
+    fit_flops = float(np.sum(passes * nkn) * 21.0)
+    fit_roof = {"kernel": "k_fit<float,LB>", "bound": "valu", "achieved": round(fit_flops / sec / 1e12, 3),
+                "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": round(fit_flops / sec / 1e12 / PEAK["fp32"], 4),
+                "algorithmic_flops_per_launch": fit_flops, "flops_per_unit": 21.0,
+                "units_per_launch": float(np.sum(passes * nkn)), "traffic": None,
+                "note": "one launch per evaluation; max/mean of atoms x k-vectors over snapshots "
+                        f"{nkn.max() / nkn.mean():.2f} (one workgroup per snapshot)"}
     return {"metric": "fit-gradients/s", "value": round(1.0 / sec, 3), "seconds_per_gradient": sec,
+            "roofline": fit_roof,
             "fp64_seconds_per_gradient": sec64, "snapshots": nsnap, "ions": natm,
             "sizes": [int(min(s.natm for s in snaps)), int(max(s.natm for s in snaps))],
             "params": len(params), "launch": {"block": tune["block"], "grid": tune["grid"]},

@@ -779,6 +779,8 @@ contains
         if (lcsvr) then
           call gauss(r1)
           sumnoise = sum_noises(3 * natm - 4)
+          ! the deviates of this rescale, so that a run can be replayed
+          write (*, '(a,i10,2es26.17)') ' csvr noise', itime, r1, sumnoise
           st = mdff_csvr_rescale(ctx, kT, taucsvr * time_unit, r1, sumnoise, conint)
           call mdff_check(st, ctx, 'csvr rescale')
         else

@@ -3,9 +3,13 @@ profiling with rocprofv3: steepest-descent relaxation, then `--steps`
 velocity-Verlet steps of one precision form, then a device-to-device copy of
 `--copy-mb` MiB (hipMemcpy device to device) whose counter readings
 calibrate FETCH_SIZE / WRITE_SIZE (the copy reads and writes exactly that
-many bytes).
+many bytes).  `--fit N` instead runs N forward + gradient evaluations of
+the batched fit (256 PIM oxide snapshots of 96-1536 ions, ten parameters,
+the bench's fit leg without its autotuner: 1024-thread workgroups, one per
+snapshot).
 
     python -m mdff_amd.profile_workload --steps 20 --precision mixed
+    python -m mdff_amd.profile_workload --fit 1 --precision mixed
 """
 from __future__ import annotations
 
@@ -16,6 +20,25 @@ import sys
 from . import capi, configs
 
 
+def fit_workload(args) -> int:
+    import numpy as np
+    from . import fit as F
+    params = [("A", 1, 2), ("RHO", 1, 2), ("A", 2, 2), ("RHO", 2, 2), ("C", 2, 2), ("ALPHA", 2, 0),
+              ("TTB", 2, 1), ("TTC", 2, 1), ("TTB", 2, 2), ("TTC", 2, 2)]
+    snaps = configs.fit_snapshots(256)
+    theta = F.theta_of(params, configs.FIT_THETA_STAR)
+    natm = sum(s.natm for s in snaps)
+    P = F.FitProblem(snaps, params)
+    P.set_targets(np.zeros((natm, 3)), np.zeros((len(snaps), 9)), 0.05)
+    P.set_launch(1024, len(snaps))
+    prec = capi.PREC_MIXED if args.precision == "mixed" else capi.PREC_FP64
+    for _ in range(args.fit):
+        P.eval(theta, prec, grad=True)
+    P.close()
+    print(f"profile_workload: {args.fit} {args.precision} fit evaluations on {len(snaps)} snapshots done", flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=20)
@@ -23,7 +46,10 @@ def main(argv=None) -> int:
     ap.add_argument("--relax", type=int, default=150)
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--copy-mb", type=int, default=256)
+    ap.add_argument("--fit", type=int, default=0, help="batched-fit evaluations instead of MD steps")
     args = ap.parse_args(argv)
+    if args.fit > 0:
+        return fit_workload(args)
     s = configs.sio2(natm=args.natm)
     if args.relax > 0:
         eng = capi.gpu_engine(copy.deepcopy(s.desc), s.itype)

@@ -18,7 +18,8 @@ def _r(x) -> str:
 
 
 def write_inputs(workdir: Path, s, names: dict[int, str], npas: int = 0, dt_ps: float = 0.001,
-                 extra_control: list[str] | None = None, extra_md: list[str] | None = None) -> Path:
+                 extra_control: list[str] | None = None, extra_md: list[str] | None = None,
+                 extra_coulomb: list[str] | None = None) -> Path:
     """control.F and POSFF (Cartesian, rvn) for a Buckingham/BMH + Ewald system."""
     d = s.desc
     ntype = int(d.ntype)
@@ -53,6 +54,7 @@ def write_inputs(workdir: Path, s, names: dict[int, str], npas: int = 0, dt_ps: 
         "&coulombtag",
         f" alphaES = {_r(d.alphaES)}",
         f" kES = {d.kES[0]}, {d.kES[1]}, {d.kES[2]}",
+        *(extra_coulomb or []),
         "&end",
         "&fieldtag",
         " mass = " + ", ".join(_r(d.mass[k]) for k in range(ntype)),

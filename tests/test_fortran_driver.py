@@ -174,3 +174,104 @@ def test_driver_restart_full_vs_split(tmp_path):
     sel = [int(k) for k in t2["step"]]
     assert np.allclose(t2["Utot"], tf["Utot"][sel], rtol=1e-9)
     assert np.allclose(t2["Ekin"], tf["Ekin"][sel], rtol=1e-8)
+
+
+@pytest.mark.gpu
+def test_driver_nhcn_matches_oracle(tmp_path):
+    """integrator 'nvt-nhcn' (src/integration.f90:538-701) with a chain of 3
+    at 600 K, timesca_thermo 0.05 ps: the driver's steps against the
+    oracle's chain from the same start (xi = vxi = 0): per-step potential and
+    kinetic energies and the final positions and velocities."""
+    import oracle_lib as O
+    from gpu_util import note, rel
+    s = configs.nacl(ncell=2, seed=5, temp_K=600.0)
+    nsteps = 6
+    FI.write_inputs(tmp_path, s, {1: "Na", 2: "Cl"}, npas=nsteps,
+                    extra_md=[" integrator = 'nvt-nhcn'", " temp = 600.0", " timesca_thermo = 0.05",
+                              " nhc_n = 3"])
+    note("fortran driver nvt-nhcn")
+    r = _run(tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = O.oracle(s.desc, s.itype)
+    o.set_positions(s.pos)
+    o.set_velocities(s.vel)
+    o.engforce()
+    ek, up, env, xi, vxi = o.md_nhcn(nsteps, 0.001 * capi.TIME_UNIT, 600.0 * capi.BOLTZ_UNIT,
+                                      0.05 * capi.TIME_UNIT, np.zeros(3), np.zeros(3))
+    thermo = FI.parse_thermo(r.stdout)
+    assert list(thermo["step"].astype(int)) == list(range(nsteps + 1))
+    assert np.allclose(thermo["Utot"][1:], up, rtol=1e-9, atol=0)
+    assert np.allclose(thermo["Ekin"][1:], ek, rtol=1e-8, atol=0)
+    cont = FI.read_contff(tmp_path / "CONTFF", s.natm)
+    assert rel(cont[:, 0:3], o.get_positions()) < 1e-10
+    assert rel(cont[:, 3:6], o.get_velocities()) < 1e-8
+
+
+@pytest.mark.gpu
+def test_driver_csvr_equilibration_replayed_on_oracle(tmp_path):
+    """nve-vv with lcsvr and nequil = npas (src/runmd.f90:320-331): the driver
+    prints the deviates of every CSVR rescale; none at itime0 = 1 and
+    itime1 = npas (src/md.f90:308-309); replaying them through the oracle's
+    rescale (src/kinetic.f90:122-198) between its VV steps gives the same
+    energies and the same final state.  The step's thermo line carries the
+    kinetic energy of the VV step, from before the rescale: rescale_velocities
+    does not update e_kin (src/integration.f90:188, src/kinetic.f90:188)."""
+    import re
+
+    import oracle_lib as O
+    from gpu_util import note, rel
+    s = configs.nacl(ncell=2, seed=6, temp_K=300.0)
+    nsteps = 8
+    FI.write_inputs(tmp_path, s, {1: "Na", 2: "Cl"}, npas=nsteps, extra_control=[" lcsvr = .true."],
+                    extra_md=[" nequil = 8", " nequil_period = 1", " temp = 900.0", " taucsvr = 0.01"])
+    note("fortran driver csvr equilibration")
+    r = _run(tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    noise = {int(t): (float(a), float(b))
+             for t, a, b in re.findall(r"csvr noise\s+(\d+)\s+(\S+)\s+(\S+)", r.stdout)}
+    assert sorted(noise) == list(range(2, nsteps))
+    thermo = FI.parse_thermo(r.stdout)
+    o = O.oracle(s.desc, s.itype)
+    o.set_positions(s.pos)
+    o.set_velocities(s.vel)
+    o.engforce()
+    ci = 0.0
+    for it in range(1, nsteps + 1):
+        e, u = o.md_vv(1, 0.001 * capi.TIME_UNIT)
+        assert abs(thermo["Utot"][it] - u[0]) <= 1e-9 * abs(u[0]), it
+        assert abs(thermo["Ekin"][it] - e[0]) <= 1e-8 * e[0], it
+        if it in noise:
+            ci = o.csvr_rescale(900.0 * capi.BOLTZ_UNIT, 0.01 * capi.TIME_UNIT, *noise[it], ci)
+    cont = FI.read_contff(tmp_path / "CONTFF", s.natm)
+    assert rel(cont[:, 0:3], o.get_positions()) < 1e-10
+    assert rel(cont[:, 3:6], o.get_velocities()) < 1e-8
+
+
+@pytest.mark.gpu
+def test_driver_lautoES_matches_oracle(tmp_path):
+    """lautoES (src/coulomb.f90:363-410, dl_poly form): the driver's alpha,
+    kmax and clamped cutoff from epsw = 1e-7 give the terms of the oracle run
+    at the parameters the same formula gives (configs.auto_ewald)."""
+    import copy
+
+    import oracle_lib as O
+    from gpu_util import note
+    s = configs.nacl(ncell=2, seed=9)
+    FI.write_inputs(tmp_path, s, {1: "Na", 2: "Cl"}, npas=0,
+                    extra_coulomb=[" lautoES = .true.", " epsw = 1e-7"])
+    note("fortran driver lautoES")
+    r = _run(tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    terms = FI.parse_terms(r.stdout)
+    rows = np.array(s.desc.cell[:]).reshape(3, 3)       # lattice vectors
+    rc, alpha, kmax = configs.auto_ewald(rows, float(s.desc.cutlongrange), 1e-7)
+    assert (alpha, kmax) != (float(s.desc.alphaES), list(s.desc.kES))
+    d = copy.deepcopy(s.desc)
+    d.alphaES = alpha
+    d.kES[:] = kmax
+    d.cutlongrange = rc
+    o = O.oracle(d, s.itype)
+    o.set_positions(s.pos)
+    _, to = o.engforce()
+    for k in ("u_coul", "u_dir", "u_rec", "u_self"):
+        assert abs(terms[k] - to[k]) <= 1e-10 * abs(to[k]), (k, terms[k], to[k])
