@@ -263,7 +263,7 @@ def fit_leg(nsnap: int, reps: int) -> dict:
     sec64 = time.perf_counter() - t0
     P.close()
     natm = sum(s.natm for s in snaps)
-    # roofline of k_fit: the reciprocal (k, atom) terms of every dipole-field
+    # roofline of the fit: the reciprocal (k, atom) terms of every dipole-field
     # operator application (SCF and adjoint iterations) and the four other
     # passes (static field, forces, h, its reciprocal part), at the
     # structure-factor and gather figures of the engforce (11 + 10 flops per
@@ -273,17 +273,17 @@ def fit_leg(nsnap: int, reps: int) -> dict:
     nkn = np.array([s.natm * nk_of(list(s.desc.kES)) for s in snaps], float)
     passes = r32.nscf[:, 0] + r32.nscf[:, 1] + 4
     fit_flops = float(np.sum(passes * nkn) * 21.0)
-    fit_roof = {"kernel": "k_fit<float,LB>", "bound": "valu", "achieved": round(fit_flops / sec / 1e12, 3),
+    fit_roof = {"kernel": "kf_* phase kernels (FP32)", "bound": "valu", "achieved": round(fit_flops / sec / 1e12, 3),
                 "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": round(fit_flops / sec / 1e12 / PEAK["fp32"], 4),
                 "algorithmic_flops_per_launch": fit_flops, "flops_per_unit": 21.0,
                 "units_per_launch": float(np.sum(passes * nkn)), "traffic": None,
-                "note": "one launch per evaluation; max/mean of atoms x k-vectors over snapshots "
-                        f"{nkn.max() / nkn.mean():.2f} (one workgroup per snapshot)"}
+                "note": "phase launches over (snapshot, block of atoms or k-vectors); launches per "
+                        "evaluation: 12 + 5 per SCF or adjoint iteration of the slowest snapshot"}
     return {"metric": "fit-gradients/s", "value": round(1.0 / sec, 3), "seconds_per_gradient": sec,
             "roofline": fit_roof,
             "fp64_seconds_per_gradient": sec64, "snapshots": nsnap, "ions": natm,
             "sizes": [int(min(s.natm for s in snaps)), int(max(s.natm for s in snaps))],
-            "params": len(params), "launch": {"block": tune["block"], "grid": tune["grid"]},
+            "params": len(params), "launch": {"block": tune["block"]},
             "grad_rel_diff_fp32_vs_fp64": float(np.linalg.norm(r32.grad - r64.grad) / np.linalg.norm(r64.grad)),
             "loss_at_start": r64.loss, "scf_iters_max": int(r32.nscf[:, 0].max()),
             "adjoint_iters_max": int(r32.nscf[:, 1].max()), "plain_targets_seconds": t_targets}

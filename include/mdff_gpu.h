@@ -303,7 +303,9 @@ int mdff_fit_create(const mdff_desc *d, int32_t nsnap, const int32_t *natm,
                     const int32_t *spec, mdff_fit **out);
 /* targets: forces AoS 3 x total atoms (eV/A), stress 9 x nsnap (GPa), weight w */
 int mdff_fit_set_targets(mdff_fit *f, const double *fref, const double *tauref, double w_stress);
-/* workgroup size (32, 64, 128, 256, 512 or 1024 threads per snapshot) and number of workgroups */
+/* block: threads per workgroup of the phase launches (a power of two, 8 to
+ * 256); every snapshot's atoms and k-vectors are cut into blocks of that
+ * many.  grid (>= 1) is not used: the workgroup counts follow the blocks. */
 int mdff_fit_set_launch(mdff_fit *f, int32_t block, int32_t grid);
 /* Evaluate at theta: loss; grad[nparam] if want_grad; optional forces
  * (3 x total atoms), stress (9 x nsnap) and iteration counts nscf[2s] (SCF),

@@ -3,11 +3,15 @@
 // of a force/stress loss with respect to the potential parameters through the
 // converged dipole SCF (implicit adjoint), in one persistent launch.
 //
-// Work layout: one workgroup per snapshot, snapshots taken from a list sorted
-// by estimated cost (largest first), grid-stride over the list.  Inside a
-// workgroup every phase is a loop over atoms, pairs or k-vectors separated by
-// barriers; nothing is shared between workgroups, so there is no grid-wide
-// synchronisation and every wave reaches every barrier.
+// Work layout: one launch per phase over all snapshots at once.  The atoms of
+// every snapshot are cut into blocks of blockDim atoms and its k-vectors into
+// blocks of blockDim k-vectors (host-made maps: block -> snapshot and range),
+// so a launch spreads the large and the small snapshots alike over the GPU.
+// Per snapshot sums (SCF residuals, stress, loss, gradients) are block
+// partials added in block order by a one-block control kernel, which also
+// advances the SCF / adjoint iterations of each snapshot and counts those
+// still iterating; the host repeats an iteration's launches while any is.
+// A snapshot that has converged skips the later iterations' work.
 //
 // Physics (the same terms as the single-configuration path, per snapshot):
 //   Buckingham  V = A exp(-r/rho) - C/r^6       (src/non_bonded.f90:1156-1189)
@@ -80,7 +84,6 @@ struct FitArgs {
   int nsnap, ntype, want_grad, min_iter, max_iter, ncol_max;
   double tol, tol_adj, coul_unit, press_unit, w_stress;
   const FitSnap* snaps;
-  const int* order;
   const FitPair* pairs;                 // ntype x ntype, [it + ntype jt]
   double alpha_t[FIT_NT];               // polarisability per type (0: none)
   int s_alpha[FIT_NT];                  // slot of alpha per type
@@ -101,7 +104,36 @@ struct FitArgs {
   double* loss;                         // per snap
   double* grad;                         // FIT_MAXP x snaps
   int* nscf;                            // 2 x snaps: SCF and adjoint iterations
+  // blocks: atom blocks (snapshot, first atom, end) and k blocks (snapshot,
+  // first k, end); per snapshot the first atom / k block (nsnap + 1 each)
+  const int4* ablk;
+  const int4* kblk;
+  const int* snap_ab0;
+  const int* snap_kb0;
+  double* pa;                           // atom-block partials [block][FIT_PA]
+  double* pk;                           // k-block partials [block][FIT_PK]
+  int* active;                          // per snap: still iterating
+  int* iter;                            // per snap: iterations made
+  double* hn;                           // per snap: RMS of h (adjoint tolerance)
+  double* Gs;                           // per snap: symmetric stress weight (9)
+  double* gsave;                        // per snap: gradient partial sums of the pair pass (FIT_MAXP)
+  int* count;                           // snapshots still iterating (control kernel)
+  const int4* gblk;                     // gather blocks: (snapshot, first atom, first k, k chunk)
+  const int* snap_g0;                   // per snapshot: first gather record (atom x k chunk)
+  const int* snap_nch;                  // per snapshot: k chunks
+  double* gp;                           // gather records [record][FIT_GC]
+  const int4* sblk;                     // structure-factor blocks: (snapshot, first k, first atom, chunk)
+  const int* snap_s0;                   // per snapshot: first structure-factor record (k x atom chunk)
+  const int* snap_sch;                  // per snapshot: atom chunks
+  double* sp;                           // structure-factor records [record][2]
+  int ws_dst, ws_src;                   // kf_update: ws slots (3 n each) dst = alpha src
 };
+
+constexpr int FIT_PA = 40;              // atom-block partial slots
+constexpr int FIT_GC = 12;              // gather record: im3, re3, re6
+constexpr int FIT_KC = 512;             // k-vectors per gather chunk
+constexpr int FIT_AC = 512;             // atoms per structure-factor chunk
+constexpr int FIT_PK = 8;               // k-block partial slots
 
 // per-atom scratch: seven 3-vectors (static field, mu, total field, g, h,
 // lambda, tmp), stored per snapshot as seven 3 x n blocks
@@ -201,22 +233,23 @@ __device__ __forceinline__ c2<T> phase(const c2<T>* tab, const FitSnap& S, int j
 // k's three factors from LDS).  Called by the whole workgroup (barriers).
 constexpr int FIT_CA = 32;
 template <typename T>
-__device__ void struct_factor(const FitArgs& a, const FitSnap& S, const c2<T>* tab, int mode, const double* v,
-                              const double* g, double* out /* [k] 2 doubles, stride FIT_KWS */, c2<T>* lt,
-                              double* la) {
+__device__ void struct_factor(const FitArgs& a, const FitSnap& S, int kb0, int kb1, int ja0, int ja1,
+                              const c2<T>* tab, int mode, const double* v, const double* g,
+                              double* out /* per k of the snapshot: 2 doubles at out + ostride k */, int ostride,
+                              c2<T>* lt, double* la) {
   const double* q = a.q + S.a0;
   const int ncol = S.ncol;
-  for (int k0 = 0; k0 < S.nk; k0 += blockDim.x) {
+  for (int k0 = kb0; k0 < kb1; k0 += blockDim.x) {
     const int k = k0 + threadIdx.x;
-    const bool kok = k < S.nk;
-    const int kg = S.k0 + (kok ? k : 0);
+    const bool kok = k < kb1;
+    const int kg = S.k0 + (kok ? k : kb0);
     const double kx = a.kv[5 * kg], ky = a.kv[5 * kg + 1], kz = a.kv[5 * kg + 2];
     const int nx = a.kn[3 * kg], ny = a.kn[3 * kg + 1], nz = a.kn[3 * kg + 2];
     const int cy = S.Kx + 1 + (ny < 0 ? -ny : ny), cz = S.Kx + S.Ky + 2 + (nz < 0 ? -nz : nz);
     const T sy = ny < 0 ? T(-1) : T(1), sz = nz < 0 ? T(-1) : T(1);
     double sr = 0.0, si = 0.0;
-    for (int j0 = 0; j0 < S.n; j0 += FIT_CA) {
-      const int nc = min(FIT_CA, S.n - j0);
+    for (int j0 = ja0; j0 < ja1; j0 += FIT_CA) {
+      const int nc = min(FIT_CA, ja1 - j0);
       __syncthreads();
       for (int x = threadIdx.x; x < ncol * nc; x += blockDim.x) {
         const int col = x / nc, jj = x % nc;
@@ -261,8 +294,8 @@ __device__ void struct_factor(const FitArgs& a, const FitSnap& S, const c2<T>* t
       }
     }
     if (kok) {
-      out[(size_t)FIT_KWS * (S.k0 + k)] = sr;
-      out[(size_t)FIT_KWS * (S.k0 + k) + 1] = si;
+      out[(size_t)ostride * k] = sr;
+      out[(size_t)ostride * k + 1] = si;
     }
   }
   __syncthreads();
@@ -271,11 +304,12 @@ __device__ void struct_factor(const FitArgs& a, const FitSnap& S, const c2<T>* t
 // per atom j: im3 = sum_k k Im(C1_k e^{-ik r_j}), re3 = sum_k k Re(C2_k e^{-ik r_j}),
 // re6 = sum_k k k Re(C3_k e^{-ik r_j}) (xx yy zz xy xz yz); flags pick the sums
 template <typename T>
-__device__ void gather(const FitArgs& a, const FitSnap& S, const c2<T>* tab, int j, int c1, int c2i, int c3,
-                       double im3[3], double re3[3], double re6[6]) {
+__device__ void gather(const FitArgs& a, const FitSnap& S, const c2<T>* tab, int j, int kb0, int kb1, int c1,
+                       int c2i, int c3, double im3[3], double re3[3], double re6[6]) {
   for (int c = 0; c < 3; ++c) im3[c] = re3[c] = 0.0;
   for (int c = 0; c < 6; ++c) re6[c] = 0.0;
-  for (int k = 0; k < S.nk; ++k) {
+#pragma unroll 4
+  for (int k = kb0; k < kb1; ++k) {
     const int kg = S.k0 + k;
     const double kx = a.kv[5 * kg], ky = a.kv[5 * kg + 1], kz = a.kv[5 * kg + 2];
     const int nx = a.kn[3 * kg], ny = a.kn[3 * kg + 1], nz = a.kn[3 * kg + 2];
@@ -312,14 +346,15 @@ __device__ void gather(const FitArgs& a, const FitSnap& S, const c2<T>* tab, int
 // static field of the charges at polarisable atoms, TT-damped (SCF), plus
 // the lambda . dE_stat/d(b, c) gradient terms when lam != nullptr
 template <typename T>
-__device__ void field_static(const FitArgs& a, const FitSnap& S, double* Es, const double* lam, double* gacc) {
+__device__ void field_static(const FitArgs& a, const FitSnap& S, int ib0, int ib1, double* Es, const double* lam,
+                             double* gacc) {
   const double* sx = a.sx + S.a0;
   const double* sy = a.sy + S.a0;
   const double* sz = a.sz + S.a0;
   const int* it = a.itype + S.a0;
   const double* q = a.q + S.a0;
   const T alpha = (T)S.alpha;
-  for (int i = threadIdx.x; i < S.n; i += blockDim.x) {
+  for (int i = ib0 + threadIdx.x; i < ib1; i += blockDim.x) {
     double e0 = 0, e1 = 0, e2 = 0;
     const int ti = it[i];
     if (a.alpha_t[ti] != 0.0) {
@@ -358,13 +393,13 @@ __device__ void field_static(const FitArgs& a, const FitSnap& S, double* Es, con
 // field of a dipole set v (undamped, real space only), at polarisable atoms,
 // from polarisable atoms (the mu-mu pairs of src/ewald_dir.f90:557-593)
 template <typename T>
-__device__ void field_dip_real(const FitArgs& a, const FitSnap& S, const double* v, double* out) {
+__device__ void field_dip_real(const FitArgs& a, const FitSnap& S, int ib0, int ib1, const double* v, double* out) {
   const double* sx = a.sx + S.a0;
   const double* sy = a.sy + S.a0;
   const double* sz = a.sz + S.a0;
   const int* it = a.itype + S.a0;
   const T alpha = (T)S.alpha;
-  for (int i = threadIdx.x; i < S.n; i += blockDim.x) {
+  for (int i = ib0 + threadIdx.x; i < ib1; i += blockDim.x) {
     double e0 = 0, e1 = 0, e2 = 0;
     if (a.alpha_t[it[i]] != 0.0) {
       for (int e = a.row[S.row0 + i]; e < a.row[S.row0 + i + 1]; ++e) {
@@ -389,36 +424,6 @@ __device__ void field_dip_real(const FitArgs& a, const FitSnap& S, const double*
   }
 }
 
-// full dipole-field operator: out = T v (real + reciprocal + self), at
-// polarisable atoms; uses the S slot of the k scratch
-template <typename T>
-__device__ void dip_operator(const FitArgs& a, const FitSnap& S, const c2<T>* tab, const double* v, double* out,
-                             c2<T>* lt, double* la) {
-  field_dip_real<T>(a, S, v, out);
-  struct_factor<T>(a, S, tab, 1, v, nullptr, a.kws + K_S, lt, la);
-  __syncthreads();
-  for (int k = threadIdx.x; k < S.nk; k += blockDim.x) {
-    double* K = a.kws + (size_t)FIT_KWS * (S.k0 + k);
-    const double Ak = a.kv[5 * (S.k0 + k) + 3];
-    K[K_C1] = Ak * K[K_S];
-    K[K_C1 + 1] = Ak * K[K_S + 1];
-  }
-  __syncthreads();
-  const double pre = -8.0 * MDFF_PI / S.omega;
-  const double self = 4.0 * S.alpha * S.alpha * S.alpha / (3.0 * sqrt(MDFF_PI));
-  const int* it = a.itype + S.a0;
-  for (int j = threadIdx.x; j < S.n; j += blockDim.x) {
-    if (a.alpha_t[it[j]] == 0.0) {
-      out[3 * j] = out[3 * j + 1] = out[3 * j + 2] = 0.0;
-      continue;
-    }
-    double im3[3], re3[3], re6[6];
-    gather<T>(a, S, tab, j, K_C1, -1, -1, im3, re3, re6);
-    for (int c = 0; c < 3; ++c) out[3 * j + c] += pre * im3[c] + self * v[3 * j + c];
-  }
-  __syncthreads();
-}
-
 // ---------------------------------------------------------------------------
 // Forces, stress and the adjoint pair pass
 // ---------------------------------------------------------------------------
@@ -426,8 +431,9 @@ __device__ void dip_operator(const FitArgs& a, const FitSnap& S, const c2<T>* ta
 // One pass over the rows.  FORCE: F (eV/A) and the real-space stress sums.
 // ADJ: h_real (into h), explicit Buckingham and TT gradients (into gacc).
 template <typename T, bool ADJ>
-__device__ void pair_pass(const FitArgs& a, const FitSnap& S, const double* mu, const double* gv, const double Gs[9],
-                          double* Fout, double* h, double* gacc, double st_nonb[6], double st_dir[6]) {
+__device__ void pair_pass(const FitArgs& a, const FitSnap& S, int ib0, int ib1, const double* mu, const double* gv,
+                          const double Gs[9], double* Fout, double* h, double* gacc, double st_nonb[6],
+                          double st_dir[6]) {
   const double* sx = a.sx + S.a0;
   const double* sy = a.sy + S.a0;
   const double* sz = a.sz + S.a0;
@@ -436,7 +442,7 @@ __device__ void pair_pass(const FitArgs& a, const FitSnap& S, const double* mu, 
   const T alpha = (T)S.alpha;
   const double cu = a.coul_unit;
   const double sgs = 1.0 / (S.omega * a.press_unit);
-  for (int i = threadIdx.x; i < S.n; i += blockDim.x) {
+  for (int i = ib0 + threadIdx.x; i < ib1; i += blockDim.x) {
     const int ti = it[i];
     const T qi = (T)q[i];
     const T mix = (T)mu[3 * i], miy = (T)mu[3 * i + 1], miz = (T)mu[3 * i + 2];
@@ -610,192 +616,127 @@ __device__ void pair_pass(const FitArgs& a, const FitSnap& S, const double* mu, 
   }
 }
 
-template <typename T, int LB>
-__global__ void __launch_bounds__(LB) k_fit(FitArgs a) {
+// ---------------------------------------------------------------------------
+// Phase kernels.  Atom-block kernels: blockIdx.x -> ablk[b] = (snapshot,
+// first atom, end atom, -); k-block kernels: kblk[b] = (snapshot, first k,
+// end k, -).  The per-snapshot scratch: seven 3 x n blocks (ws) and ten
+// doubles per k (kws), as listed at FIT_WS / FIT_KWS.
+// ---------------------------------------------------------------------------
+enum { W_ES = 0, W_MU = 1, W_ET = 2, W_G = 3, W_H = 4, W_LAM = 5, W_TMP = 6 };
+
+__device__ __forceinline__ double* ws_of(const FitArgs& a, const FitSnap& S, int slot) {
+  return a.ws + (size_t)FIT_WS * S.a0 + (size_t)slot * 3 * S.n;
+}
+
+// per-block sums of v[0..n) into the block's partial slots
+__device__ __forceinline__ void block_partials(double* v, int n, double* red, double* out) {
+  for (int c = 0; c < n; ++c) {
+    const double r = block_sum(v[c], red);
+    if (threadIdx.x == 0) out[c] = r;
+  }
+}
+
+// static field of the charges (real space, damped) at the block's atoms
+template <typename T>
+__global__ void __launch_bounds__(256) kf_static(FitArgs a) {
+  const int4 B = a.ablk[blockIdx.x];
+  const FitSnap S = a.snaps[B.x];
+  field_static<T>(a, S, B.y, B.z, ws_of(a, S, W_ES), nullptr, nullptr);
+}
+
+// dst = alpha src at the block's atoms, for snapshots still iterating
+__global__ void __launch_bounds__(256) kf_update(FitArgs a) {
+  const int4 B = a.ablk[blockIdx.x];
+  if (!a.active[B.x]) return;
+  const FitSnap S = a.snaps[B.x];
+  const int* it = a.itype + S.a0;
+  double* dst = ws_of(a, S, a.ws_dst);
+  const double* src = ws_of(a, S, a.ws_src);
+  for (int j = B.y + threadIdx.x; j < B.z; j += blockDim.x) {
+    const double al = a.alpha_t[it[j]];
+    for (int c = 0; c < 3; ++c) dst[3 * j + c] = al * src[3 * j + c];
+  }
+}
+
+// real-space dipole field of v = ws[slot] into ws[W_TMP] (iterating snapshots)
+template <typename T>
+__global__ void __launch_bounds__(256) kf_dipreal(FitArgs a, int slot) {
+  const int4 B = a.ablk[blockIdx.x];
+  if (!a.active[B.x]) return;
+  const FitSnap S = a.snaps[B.x];
+  field_dip_real<T>(a, S, B.y, B.z, ws_of(a, S, slot), ws_of(a, S, W_TMP));
+}
+
+// structure factors of the block's k-vectors and their per-k follow-ups.
+// mode 0: q -> K_SQ, C1 = A S; 1: i k.v -> K_S, C1 = A S (iterating
+// snapshots only); 2: q + i k.mu -> K_S, C1 = A S and the reciprocal stress
+// partials; 3: (k.g)(q + i k.mu) -> K_C2 = A S3, C3 = (tr Gs - kcoe k.Gs.k)
+// A S.  v / g: ws slots (-1: none).
+// one block of k-vectors over one chunk of FIT_AC atoms: the chunk's
+// structure-factor partial sums (sblk[b] = (snapshot, first k, first atom,
+// chunk)); a large snapshot's sums are split over several workgroups and
+// added in chunk order by kf_sf
+template <typename T>
+__global__ void __launch_bounds__(256) kf_sfpart(FitArgs a, int mode, int vslot, int gslot) {
   extern __shared__ double red[];                 // blockDim doubles, then staging
   c2<T>* lt = reinterpret_cast<c2<T>*>(red + blockDim.x);            // FIT_CA x ncol_max
   double* la = reinterpret_cast<double*>(lt + (size_t)FIT_CA * a.ncol_max);   // FIT_CA x 7
-  const c2<T>* tab_all = reinterpret_cast<const c2<T>*>(a.tab);
-  for (int w = blockIdx.x; w < a.nsnap; w += gridDim.x) {
-    const int s = a.order[w];
-    const FitSnap S = a.snaps[s];
-    const c2<T>* tab = tab_all + S.tab0;
-    const size_t na3 = 3 * (size_t)S.n;
-    // per-snapshot scratch views (3 x n each, contiguous per snapshot)
-    double* base = a.ws + (size_t)FIT_WS * S.a0;
-    double* vES = base;
-    double* vMU = base + na3;
-    double* vET = base + 2 * na3;
-    double* vG = base + 3 * na3;
-    double* vH = base + 4 * na3;
-    double* vLAM = base + 5 * na3;
-    double* vTMP = base + 6 * na3;
-    const int* it = a.itype + S.a0;
-    const double* q = a.q + S.a0;
-    const double pre = -8.0 * MDFF_PI / S.omega;
+  const int4 B = a.sblk[blockIdx.x];
+  if (mode == 1 && !a.active[B.x]) return;
+  const FitSnap S = a.snaps[B.x];
+  const c2<T>* tab = reinterpret_cast<const c2<T>*>(a.tab) + S.tab0;
+  const double* v = vslot >= 0 ? ws_of(a, S, vslot) : nullptr;
+  const double* g = gslot >= 0 ? ws_of(a, S, gslot) : nullptr;
+  double* out = a.sp + 2 * ((size_t)a.snap_s0[B.x] + (size_t)B.w * S.nk);
+  struct_factor<T>(a, S, B.y, min(S.nk, B.y + (int)blockDim.x), B.z, min(S.n, B.z + FIT_AC), tab, mode, v, g, out, 2,
+                   lt, la);
+}
 
-    // ---- phase 1: static field E_s = real (damped) + reciprocal of the charges
-    field_static<T>(a, S, vES, nullptr, nullptr);
-    struct_factor<T>(a, S, tab, 0, nullptr, nullptr, a.kws + K_SQ, lt, la);
-    __syncthreads();
-    for (int k = threadIdx.x; k < S.nk; k += blockDim.x) {
-      double* K = a.kws + (size_t)FIT_KWS * (S.k0 + k);
-      const double Ak = a.kv[5 * (S.k0 + k) + 3];
+// the block's structure factors (chunk partials in chunk order) and their
+// per-k follow-ups
+template <typename T>
+__global__ void __launch_bounds__(256) kf_sf(FitArgs a, int mode) {
+  extern __shared__ double red[];
+  const int4 B = a.kblk[blockIdx.x];
+  if (mode == 1 && !a.active[B.x]) return;
+  const FitSnap S = a.snaps[B.x];
+  const int slot = mode == 0 ? K_SQ : (mode == 3 ? K_C2 : K_S);
+  const int nch = a.snap_sch[B.x];
+  for (int k = B.y + threadIdx.x; k < B.z; k += blockDim.x) {
+    double sr = 0.0, si = 0.0;
+    for (int ch = 0; ch < nch; ++ch) {
+      const double* p = a.sp + 2 * ((size_t)a.snap_s0[B.x] + (size_t)ch * S.nk + k);
+      sr += p[0];
+      si += p[1];
+    }
+    double* K = a.kws + (size_t)FIT_KWS * (S.k0 + k);
+    K[slot] = sr;
+    K[slot + 1] = si;
+  }
+  double strec[6] = {0, 0, 0, 0, 0, 0};
+  const double* Gs = a.Gs + 9 * (size_t)B.x;
+  for (int k = B.y + threadIdx.x; k < B.z; k += blockDim.x) {
+    const int kg = S.k0 + k;
+    double* K = a.kws + (size_t)FIT_KWS * kg;
+    const double kx = a.kv[5 * kg], ky = a.kv[5 * kg + 1], kz = a.kv[5 * kg + 2], Ak = a.kv[5 * kg + 3],
+                 kc = a.kv[5 * kg + 4];
+    if (mode == 0) {
       K[K_C1] = Ak * K[K_SQ];
       K[K_C1 + 1] = Ak * K[K_SQ + 1];
-    }
-    __syncthreads();
-    for (int j = threadIdx.x; j < S.n; j += blockDim.x) {
-      if (a.alpha_t[it[j]] == 0.0) continue;
-      double im3[3], re3[3], re6[6];
-      gather<T>(a, S, tab, j, K_C1, -1, -1, im3, re3, re6);
-      for (int c = 0; c < 3; ++c) vES[3 * j + c] += pre * im3[c];
-    }
-    __syncthreads();
-
-    // ---- phase 2: SCF  mu = alpha (E_s + T mu)  (src/moment_from_pola.f90:365-439)
-    for (int j = threadIdx.x; j < S.n; j += blockDim.x)
-      for (int c = 0; c < 3; ++c) {
-        vET[3 * j + c] = vES[3 * j + c];
-        vMU[3 * j + c] = 0.0;
-      }
-    __syncthreads();
-    int iscf = 0;
-    double rmsd = 1e300;
-    while ((iscf < a.max_iter && rmsd > a.tol) || iscf < a.min_iter) {
-      ++iscf;
-      for (int j = threadIdx.x; j < S.n; j += blockDim.x) {
-        const double al = a.alpha_t[it[j]];
-        for (int c = 0; c < 3; ++c) vMU[3 * j + c] = al * vET[3 * j + c];
-      }
-      __syncthreads();
-      dip_operator<T>(a, S, tab, vMU, vTMP, lt, la);
-      double r2 = 0.0;
-      for (int j = threadIdx.x; j < S.n; j += blockDim.x) {
-        const double al = a.alpha_t[it[j]];
-        for (int c = 0; c < 3; ++c) {
-          const double et = vES[3 * j + c] + vTMP[3 * j + c];
-          vET[3 * j + c] = et;
-          if (al != 0.0) {
-            const double rr = vMU[3 * j + c] / al - et;
-            r2 += rr * rr;
-          }
-        }
-      }
-      rmsd = sqrt(block_sum(r2, red) / (double)max(S.npol, 1));
-      if (S.npol == 0) break;
-    }
-
-    // ---- phase 3: forces and stress at the converged dipoles
-    double stn[6] = {0, 0, 0, 0, 0, 0}, std_[6] = {0, 0, 0, 0, 0, 0};
-    double* Fs = a.F + 3 * (size_t)S.a0;
-    const double Gdummy[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    pair_pass<T, false>(a, S, vMU, nullptr, Gdummy, Fs, nullptr, nullptr, stn, std_);
-    struct_factor<T>(a, S, tab, 2, vMU, nullptr, a.kws + K_S, lt, la);
-    __syncthreads();
-    double strec[6] = {0, 0, 0, 0, 0, 0};
-    for (int k = threadIdx.x; k < S.nk; k += blockDim.x) {
-      const int kg = S.k0 + k;
-      double* K = a.kws + (size_t)FIT_KWS * kg;
-      const double kx = a.kv[5 * kg], ky = a.kv[5 * kg + 1], kz = a.kv[5 * kg + 2], Ak = a.kv[5 * kg + 3],
-                   kc = a.kv[5 * kg + 4];
+    } else if (mode == 1 || mode == 2) {
       K[K_C1] = Ak * K[K_S];
       K[K_C1 + 1] = Ak * K[K_S + 1];
-      const double str = (K[K_S] * K[K_S] + K[K_S + 1] * K[K_S + 1]) * Ak;
-      strec[0] += (1.0 - kc * kx * kx) * str;
-      strec[1] += (1.0 - kc * ky * ky) * str;
-      strec[2] += (1.0 - kc * kz * kz) * str;
-      strec[3] -= kc * kx * ky * str;
-      strec[4] -= kc * kx * kz * str;
-      strec[5] -= kc * ky * kz * str;
-    }
-    __syncthreads();
-    for (int j = threadIdx.x; j < S.n; j += blockDim.x) {
-      double im3[3], re3[3], re6[6];
-      gather<T>(a, S, tab, j, K_C1, -1, K_C1, im3, re3, re6);
-      const double m0 = vMU[3 * j], m1 = vMU[3 * j + 1], m2 = vMU[3 * j + 2];
-      // F_rec = -(8 pi / V) [q im3 - re6 mu]
-      const double r0 = re6[0] * m0 + re6[3] * m1 + re6[4] * m2;
-      const double r1 = re6[3] * m0 + re6[1] * m1 + re6[5] * m2;
-      const double r2 = re6[4] * m0 + re6[5] * m1 + re6[2] * m2;
-      Fs[3 * j] += a.coul_unit * pre * (q[j] * im3[0] - r0);
-      Fs[3 * j + 1] += a.coul_unit * pre * (q[j] * im3[1] - r1);
-      Fs[3 * j + 2] += a.coul_unit * pre * (q[j] * im3[2] - r2);
-    }
-    // stress (GPa): tau_nonb + tau_coul
-    double tau[9];
-    {
-      double sn[6], sd[6], sr[6];
-      for (int c = 0; c < 6; ++c) {
-        sn[c] = block_sum(stn[c], red);
-        sd[c] = block_sum(std_[c], red);
-        sr[c] = block_sum(strec[c], red);
+      if (mode == 2) {
+        const double str = (K[K_S] * K[K_S] + K[K_S + 1] * K[K_S + 1]) * Ak;
+        strec[0] += (1.0 - kc * kx * kx) * str;
+        strec[1] += (1.0 - kc * ky * ky) * str;
+        strec[2] += (1.0 - kc * kz * kz) * str;
+        strec[3] -= kc * kx * ky * str;
+        strec[4] -= kc * kx * kz * str;
+        strec[5] -= kc * ky * kz * str;
       }
-      const double om = S.omega;
-      const double fn = 0.5 / om / a.press_unit;
-      const double fc = a.coul_unit / a.press_unit;
-      const double tpi_V = 2.0 * MDFF_PI / om;
-      const int ix[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
-      for (int c = 0; c < 9; ++c) {
-        const double nb = fn * sn[ix[c]];
-        const double dir = 0.25 * sd[ix[c]] / om;
-        const double rec = 2.0 * sr[ix[c]] * tpi_V / om;
-        tau[c] = nb + (dir + rec) * fc;
-      }
-    }
-    // loss, g = 2 (F - F*), G = 2 w (tau - tau*)
-    double l = 0.0;
-    for (int j = threadIdx.x; j < S.n; j += blockDim.x)
-      for (int c = 0; c < 3; ++c) {
-        const double dF = Fs[3 * j + c] - a.fref[3 * ((size_t)S.a0 + j) + c];
-        l += dF * dF;
-        vG[3 * j + c] = 2.0 * dF;
-      }
-    l = block_sum(l, red);
-    double Gs[9];
-    {
-      double lt = 0.0, G[9];
-      for (int c = 0; c < 9; ++c) {
-        const double dt = tau[c] - a.tauref[9 * s + c];
-        lt += dt * dt;
-        G[c] = 2.0 * a.w_stress * dt;
-      }
-      l += a.w_stress * lt;
-      // d(G:tau)/d tau_coul-like symmetric contraction uses Gs = (G + G^T)/2; the
-      // coulomb part of tau carries coul_unit/press_unit, folded into the pair pass
-      for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) Gs[r + 3 * c] = 0.5 * (G[r + 3 * c] + G[c + 3 * r]);
-    }
-    if (threadIdx.x == 0) {
-      a.loss[s] = l;
-      for (int c = 0; c < 9; ++c) a.tau[9 * s + c] = tau[c];
-      a.nscf[2 * s] = iscf;
-    }
-    if (!a.want_grad) {
-      __syncthreads();
-      continue;
-    }
-
-    // ---- phase 4: explicit pair gradients and real-space h
-    double gacc[FIT_MAXP];
-    for (int p = 0; p < FIT_MAXP; ++p) gacc[p] = 0.0;
-    // the non-bonded stress carries 1/press_unit, the Coulomb one coul_unit/press_unit:
-    // the pair pass scales its virtual displacement by 1/(omega pu) and the
-    // Coulomb terms by coul_unit itself
-    pair_pass<T, true>(a, S, vMU, vG, Gs, nullptr, vH, gacc, stn, std_);
-    __syncthreads();
-
-    // ---- phase 5: reciprocal h
-    //   h_j += -cu (8pi/V) [re3(A P) - re6(A S) g_j] + (cu/pu)(8pi/V^2) im3(w S),
-    //   P = sum (k.g)(q + i k.mu) e^{ikr}, w = (tr Gs - kcoe k.Gs.k) A
-    struct_factor<T>(a, S, tab, 3, vMU, vG, a.kws + K_C2, lt, la);
-    __syncthreads();
-    for (int k = threadIdx.x; k < S.nk; k += blockDim.x) {
-      const int kg = S.k0 + k;
-      double* K = a.kws + (size_t)FIT_KWS * kg;
-      const double kx = a.kv[5 * kg], ky = a.kv[5 * kg + 1], kz = a.kv[5 * kg + 2], Ak = a.kv[5 * kg + 3],
-                   kc = a.kv[5 * kg + 4];
-      // K_C1 already holds A S (phase 3)
+    } else {
+      // K_C1 already holds A S (forces)
       K[K_C2] *= Ak;
       K[K_C2 + 1] *= Ak;
       const double kGk = kx * (Gs[0] * kx + Gs[3] * ky + Gs[6] * kz) + ky * (Gs[1] * kx + Gs[4] * ky + Gs[7] * kz) +
@@ -804,17 +745,138 @@ __global__ void __launch_bounds__(LB) k_fit(FitArgs a) {
       K[K_C3] = wk * K[K_S];
       K[K_C3 + 1] = wk * K[K_S + 1];
     }
-    __syncthreads();
-    {
-      const double cu = a.coul_unit;
-      const double pv2 = cu / a.press_unit * 8.0 * MDFF_PI / (S.omega * S.omega);
-      for (int j = threadIdx.x; j < S.n; j += blockDim.x) {
-        if (a.alpha_t[it[j]] == 0.0) {
-          vH[3 * j] = vH[3 * j + 1] = vH[3 * j + 2] = 0.0;
-          continue;
+  }
+  if (mode == 2) block_partials(strec, 6, red, a.pk + (size_t)FIT_PK * blockIdx.x);
+}
+
+// gather sums of one chunk of k-vectors at the block's atoms into the
+// (atom, chunk) records: the sums of a large snapshot are split over several
+// workgroups, and the records are added in chunk order by kf_gather.
+// PH_STATIC / PH_ITER: im3 of C1 at polarisable atoms; PH_FORCE: im3 and
+// re6 of C1 at every atom; PH_H: im3 of C3, re3 of C2, re6 of C1 at
+// polarisable atoms.
+template <typename T, int PH>
+__global__ void __launch_bounds__(256) kf_gpart(FitArgs a) {
+  const int4 B = a.gblk[blockIdx.x];
+  if (PH == 1 && !a.active[B.x]) return;
+  const FitSnap S = a.snaps[B.x];
+  const c2<T>* tab = reinterpret_cast<const c2<T>*>(a.tab) + S.tab0;
+  const int* it = a.itype + S.a0;
+  const int j = B.y + threadIdx.x;
+  if (j >= min(S.n, B.y + (int)blockDim.x)) return;
+  if (PH != 2 && a.alpha_t[it[j]] == 0.0) return;
+  const int k1 = min(S.nk, B.z + FIT_KC);
+  double im3[3], re3[3], re6[6];
+  if (PH == 0 || PH == 1) gather<T>(a, S, tab, j, B.z, k1, K_C1, -1, -1, im3, re3, re6);
+  else if (PH == 2) gather<T>(a, S, tab, j, B.z, k1, K_C1, -1, K_C1, im3, re3, re6);
+  else gather<T>(a, S, tab, j, B.z, k1, K_C3, K_C2, K_C1, im3, re3, re6);
+  double* r = a.gp + ((size_t)a.snap_g0[B.x] + (size_t)B.w * S.n + j) * FIT_GC;
+  for (int c = 0; c < 3; ++c) r[c] = im3[c];
+  if (PH == 3)
+    for (int c = 0; c < 3; ++c) r[3 + c] = re3[c];
+  if (PH >= 2)
+    for (int c = 0; c < 6; ++c) r[6 + c] = re6[c];
+}
+
+// the gather sums of atom j: its chunk records added in chunk order
+__device__ __forceinline__ void gather_sum(const FitArgs& a, int s, const FitSnap& S, int j, int ncol,
+                                           double im3[3], double re3[3], double re6[6]) {
+  for (int c = 0; c < 3; ++c) im3[c] = re3[c] = 0.0;
+  for (int c = 0; c < 6; ++c) re6[c] = 0.0;
+  const int nch = a.snap_nch[s];
+  for (int ch = 0; ch < nch; ++ch) {
+    const double* r = a.gp + ((size_t)a.snap_g0[s] + (size_t)ch * S.n + j) * FIT_GC;
+    for (int c = 0; c < 3; ++c) im3[c] += r[c];
+    if (ncol == 12)
+      for (int c = 0; c < 3; ++c) re3[c] += r[3 + c];
+    if (ncol >= 9)
+      for (int c = 0; c < 6; ++c) re6[c] += r[6 + c];
+  }
+}
+
+// gathers at the block's atoms and their per-atom follow-ups.
+//   PH_STATIC: E_s += reciprocal field of the charges; E_t = E_s, mu = 0
+//   PH_ITER:   tmp += reciprocal + self field of v = ws[vslot]; E_t = base +
+//              tmp; residual |v / alpha - E_t|^2 partial (iterating only);
+//              base = ws[bslot] (E_s for the SCF, h for the adjoint)
+//   PH_FORCE:  F += reciprocal force; g = 2 (F - F*), loss partial
+//   PH_H:      h += reciprocal h; E_t = h; |h|^2 partial
+enum { PH_STATIC = 0, PH_ITER = 1, PH_FORCE = 2, PH_H = 3 };
+template <typename T, int PH>
+__global__ void __launch_bounds__(256) kf_gather(FitArgs a, int vslot, int bslot) {
+  extern __shared__ double red[];
+  const int4 B = a.ablk[blockIdx.x];
+  if (PH == PH_ITER && !a.active[B.x]) return;
+  const FitSnap S = a.snaps[B.x];
+  const c2<T>* tab = reinterpret_cast<const c2<T>*>(a.tab) + S.tab0;
+  const int* it = a.itype + S.a0;
+  const double* q = a.q + S.a0;
+  const double pre = -8.0 * MDFF_PI / S.omega;
+  double part = 0.0;
+  for (int j = B.y + threadIdx.x; j < B.z; j += blockDim.x) {
+    const double al = a.alpha_t[it[j]];
+    double im3[3], re3[3], re6[6];
+    if (PH == PH_STATIC) {
+      double* vES = ws_of(a, S, W_ES);
+      if (al != 0.0) {
+        gather_sum(a, B.x, S, j, 3, im3, re3, re6);
+        for (int c = 0; c < 3; ++c) vES[3 * j + c] += pre * im3[c];
+      }
+      double* vET = ws_of(a, S, W_ET);
+      double* vMU = ws_of(a, S, W_MU);
+      for (int c = 0; c < 3; ++c) {
+        vET[3 * j + c] = vES[3 * j + c];
+        vMU[3 * j + c] = 0.0;
+      }
+    } else if (PH == PH_ITER) {
+      const double* v = ws_of(a, S, vslot);
+      const double* base = ws_of(a, S, bslot);
+      double* tmp = ws_of(a, S, W_TMP);
+      double* vET = ws_of(a, S, W_ET);
+      const double self = 4.0 * S.alpha * S.alpha * S.alpha / (3.0 * sqrt(MDFF_PI));
+      if (al == 0.0) {
+        tmp[3 * j] = tmp[3 * j + 1] = tmp[3 * j + 2] = 0.0;
+      } else {
+        gather_sum(a, B.x, S, j, 3, im3, re3, re6);
+        for (int c = 0; c < 3; ++c) tmp[3 * j + c] += pre * im3[c] + self * v[3 * j + c];
+      }
+      for (int c = 0; c < 3; ++c) {
+        const double et = base[3 * j + c] + tmp[3 * j + c];
+        vET[3 * j + c] = et;
+        if (al != 0.0) {
+          const double rr = v[3 * j + c] / al - et;
+          part += rr * rr;
         }
-        double im3[3], re3[3], re6[6];
-        gather<T>(a, S, tab, j, K_C3, K_C2, K_C1, im3, re3, re6);
+      }
+    } else if (PH == PH_FORCE) {
+      const double* vMU = ws_of(a, S, W_MU);
+      double* vG = ws_of(a, S, W_G);
+      double* Fs = a.F + 3 * (size_t)S.a0;
+      gather_sum(a, B.x, S, j, 9, im3, re3, re6);
+      const double m0 = vMU[3 * j], m1 = vMU[3 * j + 1], m2 = vMU[3 * j + 2];
+      // F_rec = -(8 pi / V) [q im3 - re6 mu]
+      const double r0 = re6[0] * m0 + re6[3] * m1 + re6[4] * m2;
+      const double r1 = re6[3] * m0 + re6[1] * m1 + re6[5] * m2;
+      const double r2 = re6[4] * m0 + re6[5] * m1 + re6[2] * m2;
+      Fs[3 * j] += a.coul_unit * pre * (q[j] * im3[0] - r0);
+      Fs[3 * j + 1] += a.coul_unit * pre * (q[j] * im3[1] - r1);
+      Fs[3 * j + 2] += a.coul_unit * pre * (q[j] * im3[2] - r2);
+      for (int c = 0; c < 3; ++c) {
+        const double dF = Fs[3 * j + c] - a.fref[3 * ((size_t)S.a0 + j) + c];
+        part += dF * dF;
+        vG[3 * j + c] = 2.0 * dF;
+      }
+    } else {
+      //   h_j += -cu (8pi/V) [re3(A P) - re6(A S) g_j] + (cu/pu)(8pi/V^2) im3(w S)
+      double* vH = ws_of(a, S, W_H);
+      const double* vG = ws_of(a, S, W_G);
+      double* vET = ws_of(a, S, W_ET);
+      if (al == 0.0) {
+        vH[3 * j] = vH[3 * j + 1] = vH[3 * j + 2] = 0.0;
+      } else {
+        const double cu = a.coul_unit;
+        const double pv2 = cu / a.press_unit * 8.0 * MDFF_PI / (S.omega * S.omega);
+        gather_sum(a, B.x, S, j, 12, im3, re3, re6);
         const double g0 = vG[3 * j], g1 = vG[3 * j + 1], g2 = vG[3 * j + 2];
         const double e0 = re6[0] * g0 + re6[3] * g1 + re6[4] * g2;
         const double e1 = re6[3] * g0 + re6[1] * g1 + re6[5] * g2;
@@ -823,58 +885,156 @@ __global__ void __launch_bounds__(LB) k_fit(FitArgs a) {
         vH[3 * j + 1] += cu * pre * (re3[1] - e1) + pv2 * im3[1];
         vH[3 * j + 2] += cu * pre * (re3[2] - e2) + pv2 * im3[2];
       }
-    }
-    __syncthreads();
-
-    // ---- phase 6: adjoint solve (1/alpha - T) lambda = h with the SCF iteration
-    for (int j = threadIdx.x; j < S.n; j += blockDim.x)
-      for (int c = 0; c < 3; ++c) vET[3 * j + c] = vH[3 * j + c];
-    __syncthreads();
-    double hn2 = 0.0;
-    for (int j = threadIdx.x; j < S.n; j += blockDim.x)
-      for (int c = 0; c < 3; ++c) hn2 += vH[3 * j + c] * vH[3 * j + c];
-    const double hn = sqrt(block_sum(hn2, red) / (double)max(S.npol, 1));
-    int iadj = 0;
-    double radj = 1e300;
-    while ((iadj < a.max_iter && radj > a.tol_adj * hn) || iadj < a.min_iter) {
-      ++iadj;
-      for (int j = threadIdx.x; j < S.n; j += blockDim.x) {
-        const double al = a.alpha_t[it[j]];
-        for (int c = 0; c < 3; ++c) vLAM[3 * j + c] = al * vET[3 * j + c];
+      for (int c = 0; c < 3; ++c) {
+        vET[3 * j + c] = vH[3 * j + c];
+        part += vH[3 * j + c] * vH[3 * j + c];
       }
-      __syncthreads();
-      dip_operator<T>(a, S, tab, vLAM, vTMP, lt, la);
-      double r2 = 0.0;
-      for (int j = threadIdx.x; j < S.n; j += blockDim.x) {
-        const double al = a.alpha_t[it[j]];
-        for (int c = 0; c < 3; ++c) {
-          const double et = vH[3 * j + c] + vTMP[3 * j + c];
-          vET[3 * j + c] = et;
-          if (al != 0.0) {
-            const double rr = vLAM[3 * j + c] / al - et;
-            r2 += rr * rr;
-          }
-        }
-      }
-      radj = sqrt(block_sum(r2, red) / (double)max(S.npol, 1));
-      if (S.npol == 0) break;
     }
+  }
+  // slot 12 for the force loss (the pair pass holds 0..11), else slot 0
+  if (PH != PH_STATIC) block_partials(&part, 1, red, a.pa + (size_t)FIT_PA * blockIdx.x + (PH == PH_FORCE ? 12 : 0));
+}
 
-    // ---- phase 7: - lambda . dR/dtheta: alpha (lambda.mu / alpha^2), TT b c (lambda . dE_s/dtheta)
-    for (int j = threadIdx.x; j < S.n; j += blockDim.x) {
-      const int tj = it[j];
-      const double al = a.alpha_t[tj];
-      if (al == 0.0 || a.s_alpha[tj] < 0) continue;
-      const double lm = vLAM[3 * j] * vMU[3 * j] + vLAM[3 * j + 1] * vMU[3 * j + 1] + vLAM[3 * j + 2] * vMU[3 * j + 2];
-      gacc[a.s_alpha[tj]] += lm / (al * al);
+// pair pass at the block's atoms.  !ADJ: forces (ws-independent) and the
+// real-space stress partials (slots 0..11); ADJ: h (real space, overwritten)
+// and the explicit gradient partials (slots 0..FIT_MAXP-1)
+template <typename T, bool ADJ>
+__global__ void __launch_bounds__(256) kf_pair(FitArgs a) {
+  extern __shared__ double red[];
+  const int4 B = a.ablk[blockIdx.x];
+  const FitSnap S = a.snaps[B.x];
+  double stn[6] = {0, 0, 0, 0, 0, 0}, std_[6] = {0, 0, 0, 0, 0, 0};
+  double* out = a.pa + (size_t)FIT_PA * blockIdx.x;
+  if (!ADJ) {
+    const double Gdummy[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    pair_pass<T, false>(a, S, B.y, B.z, ws_of(a, S, W_MU), nullptr, Gdummy, a.F + 3 * (size_t)S.a0, nullptr,
+                        nullptr, stn, std_);
+    double v[12];
+    for (int c = 0; c < 6; ++c) {
+      v[c] = stn[c];
+      v[6 + c] = std_[c];
     }
-    field_static<T>(a, S, nullptr, vLAM, gacc);
-    for (int p = 0; p < FIT_MAXP; ++p) {
-      const double v = block_sum(gacc[p], red);
-      if (threadIdx.x == 0) a.grad[(size_t)FIT_MAXP * s + p] = v;
+    block_partials(v, 12, red, out);
+  } else {
+    double gacc[FIT_MAXP];
+    for (int p = 0; p < FIT_MAXP; ++p) gacc[p] = 0.0;
+    double Gs[9];
+    for (int c = 0; c < 9; ++c) Gs[c] = a.Gs[9 * (size_t)B.x + c];
+    // the non-bonded stress carries 1/press_unit, the Coulomb one coul_unit/press_unit:
+    // the pair pass scales its virtual displacement by 1/(omega pu) and the
+    // Coulomb terms by coul_unit itself
+    pair_pass<T, true>(a, S, B.y, B.z, ws_of(a, S, W_MU), ws_of(a, S, W_G), Gs, nullptr, ws_of(a, S, W_H), gacc,
+                       stn, std_);
+    block_partials(gacc, FIT_MAXP, red, out);
+  }
+}
+
+// - lambda . dR/dtheta at the block's atoms: alpha (lambda.mu / alpha^2) and
+// TT b, c (lambda . dE_s/dtheta); gradient partials (slots 0..FIT_MAXP-1)
+template <typename T>
+__global__ void __launch_bounds__(256) kf_lambda(FitArgs a) {
+  extern __shared__ double red[];
+  const int4 B = a.ablk[blockIdx.x];
+  const FitSnap S = a.snaps[B.x];
+  const int* it = a.itype + S.a0;
+  const double* vLAM = ws_of(a, S, W_LAM);
+  const double* vMU = ws_of(a, S, W_MU);
+  double gacc[FIT_MAXP];
+  for (int p = 0; p < FIT_MAXP; ++p) gacc[p] = 0.0;
+  for (int j = B.y + threadIdx.x; j < B.z; j += blockDim.x) {
+    const int tj = it[j];
+    const double al = a.alpha_t[tj];
+    if (al == 0.0 || a.s_alpha[tj] < 0) continue;
+    const double lm = vLAM[3 * j] * vMU[3 * j] + vLAM[3 * j + 1] * vMU[3 * j + 1] + vLAM[3 * j + 2] * vMU[3 * j + 2];
+    gacc[a.s_alpha[tj]] += lm / (al * al);
+  }
+  field_static<T>(a, S, B.y, B.z, nullptr, vLAM, gacc);
+  block_partials(gacc, FIT_MAXP, red, a.pa + (size_t)FIT_PA * blockIdx.x);
+}
+
+// Control of every snapshot (one block, thread-strided over snapshots; each
+// snapshot's block partials added in block order):
+//   C_START:  iterations to 0, every snapshot iterating
+//   C_SCF:    SCF residual of the iteration; rmsd, iteration count, continue?
+//             (src/moment_from_pola.f90:365-439)
+//   C_FORCE:  stress (GPa), loss, Gs = sym(2 w (tau - tau*)); SCF count out
+//   C_PAIRG:  keep the pair pass's gradient partial sums
+//   C_HN:     RMS of h, adjoint iterations to 0, every snapshot iterating
+//   C_ADJ:    adjoint residual; continue while above tol_adj |h|
+//   C_GRAD:   gradient = pair-pass sums + lambda-pass sums; adjoint count out
+// count = snapshots still iterating after C_START / C_SCF / C_HN / C_ADJ.
+enum { C_START = 0, C_SCF = 1, C_FORCE = 2, C_PAIRG = 3, C_HN = 4, C_ADJ = 5, C_GRAD = 6 };
+__global__ void __launch_bounds__(64) kf_ctl(FitArgs a, int what) {
+  __shared__ int cnt[64];
+  int mine = 0;
+  for (int s = threadIdx.x; s < a.nsnap; s += blockDim.x) {
+    const FitSnap S = a.snaps[s];
+    const int b0 = a.snap_ab0[s], b1 = a.snap_ab0[s + 1];
+    auto asum = [&](int c) {
+      double t = 0.0;
+      for (int b = b0; b < b1; ++b) t += a.pa[(size_t)FIT_PA * b + c];
+      return t;
+    };
+    if (what == C_START || what == C_HN) {
+      if (what == C_HN) a.hn[s] = sqrt(asum(0) / (double)max(S.npol, 1));
+      a.iter[s] = 0;
+      a.active[s] = 1;
+    } else if (what == C_SCF || what == C_ADJ) {
+      if (a.active[s]) {
+        const double rms = sqrt(asum(0) / (double)max(S.npol, 1));
+        const int n = ++a.iter[s];
+        const double tol = what == C_SCF ? a.tol : a.tol_adj * a.hn[s];
+        a.active[s] = (S.npol > 0 && ((n < a.max_iter && rms > tol) || n < a.min_iter)) ? 1 : 0;
+      }
+    } else if (what == C_FORCE) {
+      double sn[6], sd[6], sr[6];
+      for (int c = 0; c < 6; ++c) {
+        sn[c] = asum(c);
+        sd[c] = asum(6 + c);
+        double t = 0.0;
+        for (int b = a.snap_kb0[s]; b < a.snap_kb0[s + 1]; ++b) t += a.pk[(size_t)FIT_PK * b + c];
+        sr[c] = t;
+      }
+      // the force loss: the gather's partials (slot 12)
+      double l = 0.0;
+      for (int b = b0; b < b1; ++b) l += a.pa[(size_t)FIT_PA * b + 12];
+      const double om = S.omega;
+      const double fn = 0.5 / om / a.press_unit;
+      const double fc = a.coul_unit / a.press_unit;
+      const double tpi_V = 2.0 * MDFF_PI / om;
+      const int ix[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+      double tau[9], G[9];
+      double lt = 0.0;
+      for (int c = 0; c < 9; ++c) {
+        const double nb = fn * sn[ix[c]];
+        const double dir = 0.25 * sd[ix[c]] / om;
+        const double rec = 2.0 * sr[ix[c]] * tpi_V / om;
+        tau[c] = nb + (dir + rec) * fc;
+        const double dt = tau[c] - a.tauref[9 * s + c];
+        lt += dt * dt;
+        G[c] = 2.0 * a.w_stress * dt;
+      }
+      l += a.w_stress * lt;
+      // d(G:tau)/d tau_coul-like symmetric contraction uses Gs = (G + G^T)/2
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) a.Gs[9 * (size_t)s + r + 3 * c] = 0.5 * (G[r + 3 * c] + G[c + 3 * r]);
+      a.loss[s] = l;
+      for (int c = 0; c < 9; ++c) a.tau[9 * s + c] = tau[c];
+      a.nscf[2 * s] = a.iter[s];
+    } else if (what == C_PAIRG) {
+      for (int p = 0; p < FIT_MAXP; ++p) a.gsave[(size_t)FIT_MAXP * s + p] = asum(p);
+    } else if (what == C_GRAD) {
+      for (int p = 0; p < FIT_MAXP; ++p) a.grad[(size_t)FIT_MAXP * s + p] = a.gsave[(size_t)FIT_MAXP * s + p] + asum(p);
+      a.nscf[2 * s + 1] = a.iter[s];
     }
-    if (threadIdx.x == 0) a.nscf[2 * s + 1] = iadj;
-    __syncthreads();
+    mine += a.active[s];
+  }
+  cnt[threadIdx.x] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int x = 0; x < (int)blockDim.x; ++x) t += cnt[x];
+    a.count[0] = t;
   }
 }
 
@@ -918,15 +1078,26 @@ struct mdff_fit {
   int nsnap = 0, ntype = 0, natm = 0, nk = 0, nparam = 0;
   std::vector<int> spec;          // 3 x nparam
   std::vector<FitSnap> snaps;
-  std::vector<int> order;
   std::vector<FitPair> pairs0;    // base pair table (desc values)
   double alpha0[FIT_NT] = {0, 0, 0, 0};
   mdff_desc base;
-  int block = 256, grid = 0, ncol_max = 0;
+  int block = 256, ncol_max = 0;
   hipStream_t stream = nullptr;
   // device
   FitSnap* d_snaps = nullptr;
-  int* d_order = nullptr;
+  // block maps for the current block size
+  int nab = 0, nkb = 0;
+  int4 *d_ablk = nullptr, *d_kblk = nullptr;
+  int *d_ab0 = nullptr, *d_kb0 = nullptr, *d_active = nullptr, *d_iter = nullptr, *d_count = nullptr;
+  int ngb = 0;
+  int4* d_gblk = nullptr;
+  int nsb = 0;
+  int4* d_sblk = nullptr;
+  int *d_s0 = nullptr, *d_sch = nullptr;
+  double* d_sp = nullptr;
+  int *d_g0 = nullptr, *d_nch = nullptr;
+  double* d_gp = nullptr;
+  double *d_pa = nullptr, *d_pk = nullptr, *d_hn = nullptr, *d_Gs = nullptr, *d_gsave = nullptr;
   FitPair* d_pairs = nullptr;
   int *d_itype = nullptr, *d_row = nullptr, *d_nl = nullptr, *d_kn = nullptr, *d_nscf = nullptr;
   double *d_q = nullptr, *d_sx = nullptr, *d_sy = nullptr, *d_sz = nullptr, *d_kv = nullptr;
@@ -945,6 +1116,12 @@ struct mdff_fit {
       (f)->err = std::string("HIP: ") + hipGetErrorString(_e) + " at " #expr; \
       return MDFF_ERR_HIP;                                                     \
     }                                                                          \
+  } while (0)
+
+#define FIT_RC(call)                   \
+  do {                                 \
+    const int _rc = (call);            \
+    if (_rc != MDFF_OK) return _rc;    \
   } while (0)
 
 static void fit_lattice(const double* A, double* B, double* omega, double* W) {
@@ -978,6 +1155,139 @@ static void fit_lattice(const double* A, double* B, double* omega, double* W) {
 static int fit_fail(mdff_fit* f, int code, const std::string& m) {
   f->err = m;
   return code;
+}
+
+// Block maps for blocks of `block` threads: every snapshot's atoms in blocks
+// of `block` atoms and its k-vectors in blocks of `block` k-vectors, in
+// snapshot order; per snapshot the first block of each kind.
+static int fit_blocks(mdff_fit* f, int block) {
+  std::vector<int4> ab, kb, gb, sb;
+  std::vector<int> ab0(1, 0), kb0(1, 0), g0, nch, s0, sch;
+  size_t nrec = 0, nsrec = 0;
+  for (int s = 0; s < f->nsnap; ++s) {
+    const FitSnap& S = f->snaps[s];
+    for (int i = 0; i < S.n; i += block) ab.push_back(make_int4(s, i, std::min(S.n, i + block), 0));
+    for (int k = 0; k < S.nk; k += block) kb.push_back(make_int4(s, k, std::min(S.nk, k + block), 0));
+    ab0.push_back((int)ab.size());
+    kb0.push_back((int)kb.size());
+    // gather blocks: atom blocks x chunks of FIT_KC k-vectors
+    const int nc = std::max(1, (S.nk + FIT_KC - 1) / FIT_KC);
+    for (int c = 0; c < nc; ++c)
+      for (int i = 0; i < S.n; i += block) gb.push_back(make_int4(s, i, c * FIT_KC, c));
+    g0.push_back((int)nrec);
+    nch.push_back(nc);
+    nrec += (size_t)nc * S.n;
+    // structure-factor blocks: k blocks x chunks of FIT_AC atoms
+    const int na = std::max(1, (S.n + FIT_AC - 1) / FIT_AC);
+    for (int c = 0; c < na; ++c)
+      for (int k = 0; k < S.nk; k += block) sb.push_back(make_int4(s, k, c * FIT_AC, c));
+    s0.push_back((int)nsrec);
+    sch.push_back(na);
+    nsrec += (size_t)na * S.nk;
+  }
+  if (sb.empty()) sb.push_back(make_int4(0, 0, 0, 0));
+  if (kb.empty()) kb.push_back(make_int4(0, 0, 0, 0));
+  void* old[] = {f->d_ablk, f->d_kblk, f->d_ab0, f->d_kb0, f->d_pa, f->d_pk, f->d_gblk, f->d_g0, f->d_nch, f->d_gp,
+                 f->d_sblk, f->d_s0, f->d_sch, f->d_sp};
+  for (void* p : old)
+    if (p) (void)hipFree(p);
+  f->nab = (int)ab.size();
+  f->nkb = (int)kb.size();
+  f->block = block;
+  FIT_TRY(f, hipMalloc((void**)&f->d_ablk, sizeof(int4) * ab.size()));
+  FIT_TRY(f, hipMemcpy(f->d_ablk, ab.data(), sizeof(int4) * ab.size(), hipMemcpyHostToDevice));
+  FIT_TRY(f, hipMalloc((void**)&f->d_kblk, sizeof(int4) * kb.size()));
+  FIT_TRY(f, hipMemcpy(f->d_kblk, kb.data(), sizeof(int4) * kb.size(), hipMemcpyHostToDevice));
+  FIT_TRY(f, hipMalloc((void**)&f->d_ab0, sizeof(int) * ab0.size()));
+  FIT_TRY(f, hipMemcpy(f->d_ab0, ab0.data(), sizeof(int) * ab0.size(), hipMemcpyHostToDevice));
+  FIT_TRY(f, hipMalloc((void**)&f->d_kb0, sizeof(int) * kb0.size()));
+  FIT_TRY(f, hipMemcpy(f->d_kb0, kb0.data(), sizeof(int) * kb0.size(), hipMemcpyHostToDevice));
+  FIT_TRY(f, hipMalloc((void**)&f->d_pa, sizeof(double) * FIT_PA * ab.size()));
+  FIT_TRY(f, hipMalloc((void**)&f->d_pk, sizeof(double) * FIT_PK * kb.size()));
+  f->ngb = (int)gb.size();
+  FIT_TRY(f, hipMalloc((void**)&f->d_gblk, sizeof(int4) * gb.size()));
+  FIT_TRY(f, hipMemcpy(f->d_gblk, gb.data(), sizeof(int4) * gb.size(), hipMemcpyHostToDevice));
+  FIT_TRY(f, hipMalloc((void**)&f->d_g0, sizeof(int) * g0.size()));
+  FIT_TRY(f, hipMemcpy(f->d_g0, g0.data(), sizeof(int) * g0.size(), hipMemcpyHostToDevice));
+  FIT_TRY(f, hipMalloc((void**)&f->d_nch, sizeof(int) * nch.size()));
+  FIT_TRY(f, hipMemcpy(f->d_nch, nch.data(), sizeof(int) * nch.size(), hipMemcpyHostToDevice));
+  FIT_TRY(f, hipMalloc((void**)&f->d_gp, sizeof(double) * FIT_GC * std::max<size_t>(nrec, 1)));
+  f->nsb = (int)sb.size();
+  FIT_TRY(f, hipMalloc((void**)&f->d_sblk, sizeof(int4) * sb.size()));
+  FIT_TRY(f, hipMemcpy(f->d_sblk, sb.data(), sizeof(int4) * sb.size(), hipMemcpyHostToDevice));
+  FIT_TRY(f, hipMalloc((void**)&f->d_s0, sizeof(int) * s0.size()));
+  FIT_TRY(f, hipMemcpy(f->d_s0, s0.data(), sizeof(int) * s0.size(), hipMemcpyHostToDevice));
+  FIT_TRY(f, hipMalloc((void**)&f->d_sch, sizeof(int) * sch.size()));
+  FIT_TRY(f, hipMemcpy(f->d_sch, sch.data(), sizeof(int) * sch.size(), hipMemcpyHostToDevice));
+  FIT_TRY(f, hipMalloc((void**)&f->d_sp, sizeof(double) * 2 * std::max<size_t>(nsrec, 1)));
+  return MDFF_OK;
+}
+
+// One forward (+ adjoint gradient) evaluation: the phase kernels in order;
+// the SCF and the adjoint solve repeat their iteration's launches while the
+// control kernel counts a snapshot still iterating (read back per iteration).
+template <typename T>
+static int fit_run(mdff_fit* f, FitArgs a, size_t tsz) {
+  hipStream_t st = f->stream;
+  const dim3 ga(f->nab), gk(f->nkb), gg(f->ngb), gs(f->nsb), b(f->block);
+  const size_t red = sizeof(double) * f->block;
+  const size_t shm_sf = red + tsz * FIT_CA * f->ncol_max + sizeof(double) * 7 * FIT_CA;
+  int cnt = 0;
+  auto active_count = [&]() -> int {
+    FIT_TRY(f, hipMemcpyAsync(&cnt, f->d_count, sizeof(int), hipMemcpyDeviceToHost, st));
+    FIT_TRY(f, hipStreamSynchronize(st));
+    return MDFF_OK;
+  };
+  auto iterate = [&](int vslot, int bslot, int what) -> int {
+    for (int it = 0; it < a.max_iter; ++it) {
+      a.ws_dst = vslot;
+      a.ws_src = W_ET;
+      hipLaunchKernelGGL(kf_update, ga, b, 0, st, a);
+      hipLaunchKernelGGL((kf_dipreal<T>), ga, b, 0, st, a, vslot);
+      hipLaunchKernelGGL((kf_sfpart<T>), gs, b, shm_sf, st, a, 1, vslot, -1);
+      hipLaunchKernelGGL((kf_sf<T>), gk, b, red, st, a, 1);
+      hipLaunchKernelGGL((kf_gpart<T, PH_ITER>), gg, b, 0, st, a);
+      hipLaunchKernelGGL((kf_gather<T, PH_ITER>), ga, b, red, st, a, vslot, bslot);
+      hipLaunchKernelGGL(kf_ctl, dim3(1), dim3(64), 0, st, a, what);
+      FIT_TRY(f, hipGetLastError());
+      const int rc = active_count();
+      if (rc != MDFF_OK) return rc;
+      if (cnt == 0) break;
+    }
+    return MDFF_OK;
+  };
+  // static field, SCF
+  hipLaunchKernelGGL(kf_ctl, dim3(1), dim3(64), 0, st, a, (int)C_START);
+  hipLaunchKernelGGL((kf_static<T>), ga, b, 0, st, a);
+  hipLaunchKernelGGL((kf_sfpart<T>), gs, b, shm_sf, st, a, 0, -1, -1);
+  hipLaunchKernelGGL((kf_sf<T>), gk, b, red, st, a, 0);
+  hipLaunchKernelGGL((kf_gpart<T, PH_STATIC>), gg, b, 0, st, a);
+  hipLaunchKernelGGL((kf_gather<T, PH_STATIC>), ga, b, red, st, a, -1, -1);
+  FIT_TRY(f, hipGetLastError());
+  FIT_RC(iterate(W_MU, W_ES, C_SCF));
+  // forces, stress, loss
+  hipLaunchKernelGGL((kf_pair<T, false>), ga, b, red, st, a);
+  hipLaunchKernelGGL((kf_sfpart<T>), gs, b, shm_sf, st, a, 2, (int)W_MU, -1);
+  hipLaunchKernelGGL((kf_sf<T>), gk, b, red, st, a, 2);
+  hipLaunchKernelGGL((kf_gpart<T, PH_FORCE>), gg, b, 0, st, a);
+  hipLaunchKernelGGL((kf_gather<T, PH_FORCE>), ga, b, red, st, a, -1, -1);
+  hipLaunchKernelGGL(kf_ctl, dim3(1), dim3(64), 0, st, a, (int)C_FORCE);
+  FIT_TRY(f, hipGetLastError());
+  if (!a.want_grad) return MDFF_OK;
+  // explicit pair gradients and h; adjoint solve; lambda terms
+  hipLaunchKernelGGL((kf_pair<T, true>), ga, b, red, st, a);
+  hipLaunchKernelGGL(kf_ctl, dim3(1), dim3(64), 0, st, a, (int)C_PAIRG);
+  hipLaunchKernelGGL((kf_sfpart<T>), gs, b, shm_sf, st, a, 3, (int)W_MU, (int)W_G);
+  hipLaunchKernelGGL((kf_sf<T>), gk, b, red, st, a, 3);
+  hipLaunchKernelGGL((kf_gpart<T, PH_H>), gg, b, 0, st, a);
+  hipLaunchKernelGGL((kf_gather<T, PH_H>), ga, b, red, st, a, -1, -1);
+  hipLaunchKernelGGL(kf_ctl, dim3(1), dim3(64), 0, st, a, (int)C_HN);
+  FIT_TRY(f, hipGetLastError());
+  FIT_RC(iterate(W_LAM, W_H, C_ADJ));
+  hipLaunchKernelGGL((kf_lambda<T>), ga, b, red, st, a);
+  hipLaunchKernelGGL(kf_ctl, dim3(1), dim3(64), 0, st, a, (int)C_GRAD);
+  FIT_TRY(f, hipGetLastError());
+  return MDFF_OK;
 }
 
 extern "C" {
@@ -1140,15 +1450,6 @@ int mdff_fit_create(const mdff_desc* d, int32_t nsnap, const int32_t* natm, cons
   f->natm = a0;
   f->nk = k0;
   f->tab_elems = tab0;
-  // cost order: largest first
-  f->order.resize(nsnap);
-  for (int s = 0; s < nsnap; ++s) f->order[s] = s;
-  std::vector<double> cost(nsnap);
-  for (int s = 0; s < nsnap; ++s) {
-    const FitSnap& S = f->snaps[s];
-    cost[s] = (double)S.n * S.nk + 30.0 * (double)(row[S.row0 + S.n] - row[S.row0]);
-  }
-  std::stable_sort(f->order.begin(), f->order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
   FIT_TRY(f, hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
 #define FUP(dst, vec)                                                                                   \
   do {                                                                                                  \
@@ -1157,7 +1458,6 @@ int mdff_fit_create(const mdff_desc* d, int32_t nsnap, const int32_t* natm, cons
       FIT_TRY(f, hipMemcpy((dst), (vec).data(), sizeof((vec)[0]) * (vec).size(), hipMemcpyHostToDevice)); \
   } while (0)
   FUP(f->d_snaps, f->snaps);
-  FUP(f->d_order, f->order);
   FUP(f->d_itype, itv);
   FUP(f->d_q, q);
   FUP(f->d_sx, sx);
@@ -1182,6 +1482,12 @@ int mdff_fit_create(const mdff_desc* d, int32_t nsnap, const int32_t* natm, cons
   FIT_TRY(f, hipMalloc((void**)&f->d_loss, sizeof(double) * (size_t)nsnap));
   FIT_TRY(f, hipMalloc((void**)&f->d_grad, sizeof(double) * FIT_MAXP * (size_t)nsnap));
   FIT_TRY(f, hipMalloc((void**)&f->d_nscf, sizeof(int) * 2 * (size_t)nsnap));
+  FIT_TRY(f, hipMalloc((void**)&f->d_active, sizeof(int) * (size_t)nsnap));
+  FIT_TRY(f, hipMalloc((void**)&f->d_iter, sizeof(int) * (size_t)nsnap));
+  FIT_TRY(f, hipMalloc((void**)&f->d_count, sizeof(int)));
+  FIT_TRY(f, hipMalloc((void**)&f->d_hn, sizeof(double) * (size_t)nsnap));
+  FIT_TRY(f, hipMalloc((void**)&f->d_Gs, sizeof(double) * 9 * (size_t)nsnap));
+  FIT_TRY(f, hipMalloc((void**)&f->d_gsave, sizeof(double) * FIT_MAXP * (size_t)nsnap));
   const int tg = std::min(nsnap, 1024);
   hipLaunchKernelGGL((k_fit_tables<float>), dim3(tg), dim3(256), 0, f->stream, nsnap, f->d_snaps, f->d_sx,
                      f->d_sy, f->d_sz, f->d_tab32);
@@ -1189,11 +1495,7 @@ int mdff_fit_create(const mdff_desc* d, int32_t nsnap, const int32_t* natm, cons
                      f->d_sy, f->d_sz, f->d_tab64);
   FIT_TRY(f, hipGetLastError());
   FIT_TRY(f, hipStreamSynchronize(f->stream));
-  int cus = 256;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, 0) == hipSuccess) cus = prop.multiProcessorCount;
-  f->grid = std::min(nsnap, cus);
-  return MDFF_OK;
+  return fit_blocks(f, 256);
 }
 
 int mdff_fit_set_targets(mdff_fit* f, const double* fref, const double* tauref, double w_stress) {
@@ -1205,12 +1507,8 @@ int mdff_fit_set_targets(mdff_fit* f, const double* fref, const double* tauref, 
 }
 
 int mdff_fit_set_launch(mdff_fit* f, int32_t block, int32_t grid) {
-  if (!f || (block != 32 && block != 64 && block != 128 && block != 256 && block != 512 && block != 1024) ||
-      grid < 1)
-    return MDFF_ERR_BAD_ARG;
-  f->block = block;
-  f->grid = std::min<int>(grid, f->nsnap);
-  return MDFF_OK;
+  if (!f || block < 8 || block > 256 || (block & (block - 1)) || grid < 1) return MDFF_ERR_BAD_ARG;
+  return fit_blocks(f, block);
 }
 
 int mdff_fit_eval(mdff_fit* f, const double* theta, int32_t precision, int32_t want_grad, double* loss,
@@ -1259,7 +1557,26 @@ int mdff_fit_eval(mdff_fit* f, const double* theta, int32_t precision, int32_t w
   a.press_unit = f->base.press_unit;
   a.w_stress = f->w_stress;
   a.snaps = f->d_snaps;
-  a.order = f->d_order;
+  a.ablk = f->d_ablk;
+  a.kblk = f->d_kblk;
+  a.snap_ab0 = f->d_ab0;
+  a.snap_kb0 = f->d_kb0;
+  a.pa = f->d_pa;
+  a.pk = f->d_pk;
+  a.active = f->d_active;
+  a.iter = f->d_iter;
+  a.hn = f->d_hn;
+  a.Gs = f->d_Gs;
+  a.gsave = f->d_gsave;
+  a.count = f->d_count;
+  a.gblk = f->d_gblk;
+  a.snap_g0 = f->d_g0;
+  a.snap_nch = f->d_nch;
+  a.gp = f->d_gp;
+  a.sblk = f->d_sblk;
+  a.snap_s0 = f->d_s0;
+  a.snap_sch = f->d_sch;
+  a.sp = f->d_sp;
   a.pairs = f->d_pairs;
   for (int t = 0; t < FIT_NT; ++t) a.alpha_t[t] = alpha_t[t];
   a.itype = f->d_itype;
@@ -1284,16 +1601,12 @@ int mdff_fit_eval(mdff_fit* f, const double* theta, int32_t precision, int32_t w
   // relative tolerance of the adjoint solve: |residual| <= tol |h| (FP32: not
   // below 2e-6, the resolution of its field sums)
   a.tol_adj = precision == MDFF_PREC_MIXED ? std::max(a.tol, 2e-6) : a.tol;
-  const size_t tsz = precision == MDFF_PREC_MIXED ? sizeof(c2<float>) : sizeof(c2<double>);
-  const size_t shm = sizeof(double) * f->block + tsz * FIT_CA * f->ncol_max + sizeof(double) * 7 * FIT_CA;
   if (precision == MDFF_PREC_MIXED) {
     a.tab = f->d_tab32;
-    if (f->block <= 256) hipLaunchKernelGGL((k_fit<float, 256>), dim3(f->grid), dim3(f->block), shm, f->stream, a);
-    else hipLaunchKernelGGL((k_fit<float, 1024>), dim3(f->grid), dim3(f->block), shm, f->stream, a);
+    FIT_RC(fit_run<float>(f, a, sizeof(c2<float>)));
   } else {
     a.tab = f->d_tab64;
-    if (f->block <= 256) hipLaunchKernelGGL((k_fit<double, 256>), dim3(f->grid), dim3(f->block), shm, f->stream, a);
-    else hipLaunchKernelGGL((k_fit<double, 1024>), dim3(f->grid), dim3(f->block), shm, f->stream, a);
+    FIT_RC(fit_run<double>(f, a, sizeof(c2<double>)));
   }
   FIT_TRY(f, hipGetLastError());
   std::vector<double> ls(f->nsnap), gr((size_t)FIT_MAXP * f->nsnap);
@@ -1323,7 +1636,8 @@ int mdff_fit_eval(mdff_fit* f, const double* theta, int32_t precision, int32_t w
 int mdff_fit_destroy(mdff_fit* f) {
   if (!f) return MDFF_ERR_BAD_ARG;
   if (f->stream) (void)hipStreamSynchronize(f->stream);
-  void* ptrs[] = {f->d_snaps, f->d_order, f->d_pairs, f->d_itype, f->d_row, f->d_nl, f->d_kn, f->d_nscf,
+  void* ptrs[] = {f->d_snaps, f->d_sblk, f->d_s0, f->d_sch, f->d_sp, f->d_gblk, f->d_g0, f->d_nch, f->d_gp, f->d_ablk, f->d_kblk, f->d_ab0, f->d_kb0, f->d_pa, f->d_pk, f->d_active, f->d_iter,
+                  f->d_count, f->d_hn, f->d_Gs, f->d_gsave, f->d_pairs, f->d_itype, f->d_row, f->d_nl, f->d_kn, f->d_nscf,
                   f->d_q, f->d_sx, f->d_sy, f->d_sz, f->d_kv, f->d_tab32, f->d_tab64, f->d_fref,
                   f->d_tauref, f->d_ws, f->d_kws, f->d_F, f->d_tau, f->d_loss, f->d_grad};
   for (void* p : ptrs)

@@ -143,28 +143,24 @@ class FitProblem:
                          F.reshape(-1, 3) if outputs else None, tau.reshape(-1, 9) if outputs else None,
                          ns.reshape(-1, 2) if outputs else None)
 
-    def autotune(self, theta, precision: int = capi.PREC_MIXED, blocks=(64, 128, 256, 512, 1024),
-                 grids=None, reps: int = 2) -> dict:
-        """Pick the workgroup size and count with the lowest measured time of a
-        full forward + gradient evaluation; keeps the winner."""
+    def autotune(self, theta, precision: int = capi.PREC_MIXED, blocks=(64, 128, 256), reps: int = 2) -> dict:
+        """Pick the block size (atoms and k-vectors per workgroup of the phase
+        launches) with the lowest measured time of a full forward + gradient
+        evaluation; keeps the winner."""
         import time
-        ncu = 256
-        grids = grids or sorted({min(self.nsnap, g) for g in (ncu // 2, ncu, 2 * ncu)})
         best, table = None, {}
         for b in blocks:
-            for gsz in grids:
-                self.set_launch(b, gsz)
-                self.eval(theta, precision)               # warm
-                t0 = time.perf_counter()
-                for _ in range(reps):
-                    self.eval(theta, precision)
-                dt = (time.perf_counter() - t0) / reps
-                table[(b, gsz)] = dt
-                if best is None or dt < best[0]:
-                    best = (dt, b, gsz)
-        self.set_launch(best[1], best[2])
-        return {"block": best[1], "grid": best[2], "seconds": best[0],
-                "table": {f"{b}x{g}": v for (b, g), v in table.items()}}
+            self.set_launch(b, 1)
+            self.eval(theta, precision)                   # warm
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                self.eval(theta, precision)
+            dt = (time.perf_counter() - t0) / reps
+            table[b] = dt
+            if best is None or dt < best[0]:
+                best = (dt, b)
+        self.set_launch(best[1], 1)
+        return {"block": best[1], "seconds": best[0], "table": {str(b): v for b, v in table.items()}}
 
 
 def theta_of(params, table: dict) -> np.ndarray:

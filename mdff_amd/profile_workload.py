@@ -5,8 +5,8 @@ velocity-Verlet steps of one precision form, then a device-to-device copy of
 calibrate FETCH_SIZE / WRITE_SIZE (the copy reads and writes exactly that
 many bytes).  `--fit N` instead runs N forward + gradient evaluations of
 the batched fit (256 PIM oxide snapshots of 96-1536 ions, ten parameters,
-the bench's fit leg without its autotuner: 1024-thread workgroups, one per
-snapshot).
+the bench's fit leg without its autotuner: blocks of 256 atoms or
+k-vectors).
 
     python -m mdff_amd.profile_workload --steps 20 --precision mixed
     python -m mdff_amd.profile_workload --fit 1 --precision mixed
@@ -30,7 +30,7 @@ def fit_workload(args) -> int:
     natm = sum(s.natm for s in snaps)
     P = F.FitProblem(snaps, params)
     P.set_targets(np.zeros((natm, 3)), np.zeros((len(snaps), 9)), 0.05)
-    P.set_launch(1024, len(snaps))
+    P.set_launch(256, 1)
     prec = capi.PREC_MIXED if args.precision == "mixed" else capi.PREC_FP64
     for _ in range(args.fit):
         P.eval(theta, prec, grad=True)

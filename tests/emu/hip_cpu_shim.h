@@ -5,6 +5,7 @@
 // CPU; the library itself always runs on the GPU.
 #pragma once
 #include <atomic>
+#include <ucontext.h>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -22,25 +23,22 @@
 struct dim3 { unsigned x, y, z; dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {} };
 inline thread_local dim3 threadIdx, blockIdx;
 inline dim3 blockDim, gridDim;
-// sense-reversing barrier that yields while waiting (workgroup barriers are
-// frequent; a futex-based barrier makes the emulation system-call bound)
-struct YieldBarrier {
-  std::atomic<int> count{0};
-  std::atomic<int> sense{0};
-  int n = 1;
-  void wait() {
-    const int s = sense.load(std::memory_order_acquire);
-    if (count.fetch_add(1, std::memory_order_acq_rel) + 1 == n) {
-      count.store(0, std::memory_order_relaxed);
-      sense.store(s ^ 1, std::memory_order_release);
-    } else {
-      while (sense.load(std::memory_order_acquire) == s) std::this_thread::yield();
-    }
-  }
+// Lanes of a workgroup run as user-level contexts (ucontext) on one OS
+// thread: __syncthreads switches back to the workgroup's scheduler, which
+// resumes every lane in turn, so a barrier costs a context switch per lane.
+// Workgroups run in parallel on a few OS threads.
+struct EmuGroup {
+  ucontext_t sched;
+  std::vector<ucontext_t> ctx;
+  std::vector<char> done;
+  int cur = 0;
 };
-inline YieldBarrier* g_bar = nullptr;
-inline void __syncthreads() { g_bar->wait(); }
-inline double g_shared[16384];
+inline thread_local EmuGroup* g_grp = nullptr;
+inline void __syncthreads() { swapcontext(&g_grp->ctx[g_grp->cur], &g_grp->sched); }
+
+inline thread_local double* g_shared = nullptr;   // the workgroup's LDS
+struct int4 { int x, y, z, w; };
+inline int4 make_int4(int x, int y, int z, int w) { return int4{x, y, z, w}; }
 typedef int hipError_t;
 constexpr int hipSuccess = 0;
 typedef void* hipStream_t;
@@ -62,20 +60,62 @@ inline const char* hipGetErrorString(hipError_t) { return "emu"; }
 #define hipMemcpyDeviceToHost 2
 #define hipMemcpyDeviceToDevice 3
 inline void sincospi(double x, double* s, double* c) { *s = sin(M_PI * x); *c = cos(M_PI * x); }
-template <typename F, typename A>
-void emu_launch(F kern, dim3 g, dim3 b, A args) {
-  gridDim = g; blockDim = b;
-  for (unsigned bx = 0; bx < g.x; ++bx) {
-    YieldBarrier bar;
-    bar.n = (int)b.x;
-    g_bar = &bar;
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < b.x; ++t)
-      th.emplace_back([&, t, bx]() { threadIdx = dim3(t); blockIdx = dim3(bx); kern(args); });
-    for (auto& x : th) x.join();
+template <typename F>
+struct EmuLane {
+  static void entry(unsigned lo, unsigned hi) {
+    F* fn = reinterpret_cast<F*>(((uintptr_t)hi << 32) | (uintptr_t)lo);
+    (*fn)();
+    g_grp->done[g_grp->cur] = 1;
   }
+};
+
+// a launch: workgroups shared out over up to 8 OS threads; a workgroup's
+// lanes are contexts with 128 KiB stacks, resumed round-robin until all end;
+// fn is the kernel call with its arguments
+template <typename F>
+void emu_launch_f(F fn, dim3 g, dim3 b) {
+  gridDim = g; blockDim = b;
+  std::atomic<unsigned> next{0};
+  auto work = [&]() {
+    constexpr size_t kStack = 128 * 1024;
+    std::vector<char> stacks((size_t)b.x * kStack);
+    std::vector<double> lds(16384);
+    EmuGroup grp;
+    grp.ctx.resize(b.x);
+    grp.done.resize(b.x);
+    g_grp = &grp;
+    g_shared = lds.data();
+    for (unsigned bx; (bx = next.fetch_add(1)) < g.x;) {
+      blockIdx = dim3(bx);
+      const uintptr_t p = reinterpret_cast<uintptr_t>(&fn);
+      for (unsigned t = 0; t < b.x; ++t) {
+        getcontext(&grp.ctx[t]);
+        grp.ctx[t].uc_stack.ss_sp = stacks.data() + (size_t)t * kStack;
+        grp.ctx[t].uc_stack.ss_size = kStack;
+        grp.ctx[t].uc_link = &grp.sched;
+        makecontext(&grp.ctx[t], (void (*)())&EmuLane<F>::entry, 2, (unsigned)(p & 0xffffffffu),
+                    (unsigned)(p >> 32));
+        grp.done[t] = 0;
+      }
+      for (unsigned left = b.x; left > 0;) {
+        left = 0;
+        for (unsigned t = 0; t < b.x; ++t) {
+          if (grp.done[t]) continue;
+          grp.cur = (int)t;
+          threadIdx = dim3(t);
+          swapcontext(&grp.sched, &grp.ctx[t]);
+          if (!grp.done[t]) ++left;
+        }
+      }
+    }
+  };
+  const unsigned nth = std::min(8u, g.x);
+  std::vector<std::thread> th;
+  for (unsigned i = 1; i < nth; ++i) th.emplace_back(work);
+  work();
+  for (auto& x : th) x.join();
 }
-#define hipLaunchKernelGGL(K, G, B, SHM, ST, ...) emu_launch(K, G, B, __VA_ARGS__)
+#define hipLaunchKernelGGL(K, G, B, SHM, ST, ...) emu_launch_f([&]() { K(__VA_ARGS__); }, dim3(G), dim3(B))
 using std::min; using std::max;
 inline double nint_d(double x) { return x >= 0.0 ? floor(x + 0.5) : -floor(0.5 - x); }
 #define MDFF_PI 3.14159265358979323846264338327950

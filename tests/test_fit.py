@@ -142,21 +142,21 @@ def test_fit_gradient_matches_reference_loss_differences():
 
 @pytest.mark.gpu
 def test_fit_block_sizes_agree(fd_problem):
-    """Workgroups of 512 and 1024 threads per snapshot (the kernel compiled
-    for 1024) give the loss, gradient and forces of the 256-thread launch up
-    to the order of the block sums (1e-11 relative)."""
+    """Atom and k blocks of 64 and 128 give the loss, gradient and forces of
+    the 256-wide blocks up to the order of the per-snapshot block sums (1e-11
+    relative)."""
     from gpu_util import note, rel
     snaps, P, _, _ = fd_problem
-    note("fit block sizes 256 / 512 / 1024")
+    note("fit block sizes 64 / 128 / 256")
     runs = {}
     try:
-        for b in (256, 512, 1024):
-            P.set_launch(b, len(snaps))
+        for b in (256, 128, 64):
+            P.set_launch(b, 1)
             runs[b] = P.eval(THETA0, capi.PREC_FP64, grad=True, outputs=True)
     finally:
-        P.set_launch(256, len(snaps))
+        P.set_launch(256, 1)
     r0 = runs[256]
-    for b in (512, 1024):
+    for b in (128, 64):
         r = runs[b]
         assert abs(r.loss - r0.loss) <= 1e-11 * abs(r0.loss), b
         assert np.linalg.norm(r.grad - r0.grad) <= 1e-11 * np.linalg.norm(r0.grad), b

@@ -107,3 +107,27 @@ def test_fit_adjoint_gradient_cpu_build_matches_finite_differences(cpu_lib):
     g32 = P.eval(theta0, capi.PREC_MIXED, grad=True).grad
     assert np.linalg.norm(g32 - g) <= 1e-3 * np.linalg.norm(g)
     P.close()
+
+
+def test_fit_block_maps_cpu_build_agree(cpu_lib):
+    """Several snapshots of different sizes cut into atom and k blocks of 32
+    and of 64: the same loss, forces, iteration counts and gradient (FP64;
+    only the order of the per-snapshot block sums differs)."""
+    snaps = configs.fit_snapshots(3, nmin=96, nmax=216, seed=21)
+    for s in snaps:
+        s.desc.conv_tol_ind = 1e-12
+        s.desc.max_scf_pol_iter = 200
+    theta0 = F.theta_of(PP, configs.FIT_THETA_STAR) * (1.0 + 0.03 * PERTURB)
+    Fr, Tr = _oracle_targets(snaps)
+    P = F.FitProblem(snaps, PP, scf_tol=1e-12, max_iter=200, lib=cpu_lib)
+    P.set_targets(Fr, Tr, 0.05)
+    out = []
+    for b in (32, 64):
+        P.set_launch(b, 1)
+        out.append(P.eval(theta0, capi.PREC_FP64, grad=True, outputs=True))
+    r1, r2 = out
+    assert abs(r2.loss - r1.loss) <= 1e-12 * abs(r1.loss)
+    assert _rel(r2.forces, r1.forces) < 1e-12
+    assert np.array_equal(r2.nscf, r1.nscf)
+    assert np.linalg.norm(r2.grad - r1.grad) <= 1e-10 * np.linalg.norm(r1.grad)
+    P.close()
