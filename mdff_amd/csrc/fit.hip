@@ -308,12 +308,20 @@ __device__ void gather(const FitArgs& a, const FitSnap& S, const c2<T>* tab, int
                        int c2i, int c3, double im3[3], double re3[3], double re6[6]) {
   for (int c = 0; c < 3; ++c) im3[c] = re3[c] = 0.0;
   for (int c = 0; c < 6; ++c) re6[c] = 0.0;
-#pragma unroll 4
+  // the k list runs (nx, ny) rows with nz innermost: X Y of the atom is
+  // formed once per row (the same products as phase())
+  int pnx = -1, pny = 0;
+  c2<T> XY{T(1), T(0)};
   for (int k = kb0; k < kb1; ++k) {
     const int kg = S.k0 + k;
     const double kx = a.kv[5 * kg], ky = a.kv[5 * kg + 1], kz = a.kv[5 * kg + 2];
     const int nx = a.kn[3 * kg], ny = a.kn[3 * kg + 1], nz = a.kn[3 * kg + 2];
-    const c2<T> e = phase(tab, S, j, nx, ny, nz);   // e^{+ik r_j}
+    if (nx != pnx || ny != pny) {
+      XY = cm(tabv(tab, S.n, nx, j, false), tabv(tab, S.n, S.Kx + 1 + (ny < 0 ? -ny : ny), j, ny < 0));
+      pnx = nx;
+      pny = ny;
+    }
+    const c2<T> e = cm(XY, tabv(tab, S.n, S.Kx + S.Ky + 2 + (nz < 0 ? -nz : nz), j, nz < 0));   // e^{+ik r_j}
     const double* K = a.kws + (size_t)FIT_KWS * kg;
     if (c1 >= 0) {   // Im(C e^-) = Ci er - Cr ei
       const double v = (double)((T)K[c1 + 1] * e.re - (T)K[c1] * e.im);
