@@ -8,7 +8,7 @@
  * descriptor (mdff_desc) whose fields mirror the reference namelists:
  *   &controltag   src/control.f90:47-66        (lnmlj, lbmhft, lbmhftd, lcoulomb, lsurf, ...)
  *   &non_bondedtag src/non_bonded.f90:76-116    (LJ n-m, Morse, BMHFT(D) tables)
- *   &coulombtag   src/coulomb.f90:47-79        (alphaES, kES, doefield)
+ *   &coulombtag   src/coulomb.f90:47-79        (alphaES, kES, doefield, doefg)
  *   &fieldtag     src/field.f90:113-188        (mass, qch, dip)
  *   &pimtag       src/pim.f90:63-96            (poldip, TT damping tables, SCF controls)
  *   &mdtag        src/md.f90:37                (itime for the ASPC dipole predictor)
@@ -46,7 +46,7 @@ extern "C" {
 #endif
 
 #define MDFF_NTYPEMAX 16
-#define MDFF_ABI_VERSION 2
+#define MDFF_ABI_VERSION 3
 
 typedef enum mdff_status {
   MDFF_OK = 0,
@@ -124,6 +124,10 @@ typedef struct mdff_desc {
   int32_t extrapolate_order;       /* ASPC order 0..4, default 0                      */
   int32_t algo_moment_from_pola;   /* 0: 'scf' (default), 1: 'scf_kO_v2'              */
   double  omegakO;                 /* mixing of the scf_kO_v2 inner loop, default 0.7 */
+
+  /* &coulombtag, continued */
+  int32_t doefg;                   /* compute the electric field gradient (efg_t)     */
+  int32_t pad1;
 } mdff_desc;
 
 /* Energies, virials and stress of one engforce call (src/thermo.f90:41-46,
@@ -198,6 +202,21 @@ int mdff_last_terms(mdff_ctx *ctx, mdff_terms *terms);
 /* Total dipoles mu_t and field ef_t of the last engforce (src/coulomb.f90:75-78),
  * AoS 3 x natm.  Either pointer may be NULL. */
 int mdff_get_dipoles(mdff_ctx *ctx, double *mu, double *ef);
+
+/* Electric field gradient efg_t of the last force evaluation
+ * (src/coulomb.f90:78, set at src/engforce_driver.f90:171), 9 x natm in the
+ * reference layout efg(3,3,natm): component (a,b) of atom i at
+ * efg[9i + (a-1) + 3(b-1)], in eV/(e A^2) (x coul_unit, src/ewald.f90:231).
+ * As in the reference (src/ewald.f90:208-216,231; src/ewald_rec.f90:209-216):
+ *  - with doefg, real space (-q_j T2 for the q-q task; T3.mu_j for the mu-mu
+ *    task between two non-zero dipoles, src/ewald_dir.f90:543-546,577-582),
+ *    reciprocal space (upper triangle and diagonal only) and the self term
+ *    -2 selfa2 q_i on the diagonal;
+ *  - without doefg, the self term alone;
+ *  - without lcoulomb, zeros.
+ * The lower triangle therefore lacks the reciprocal part; the EFGALL file
+ * prints xx, yy, zz, xy, xz, yz (src/field.f90:423-425). */
+int mdff_get_efg(mdff_ctx *ctx, double *efg);
 
 /* MD step counter used by the ASPC predictor (md%itime, src/md.f90:37). */
 int mdff_set_itime(mdff_ctx *ctx, int32_t itime);
@@ -277,8 +296,8 @@ const char *mdff_profile_name(int32_t kind);
  *   L = sum_s [ sum_i |F_i - F*_i|^2 + w sum_ab (tau_ab - tau*_ab)^2 ]
  * and its gradient with respect to the parameter vector theta is computed
  * through the converged SCF by one adjoint solve per snapshot.  All
- * snapshots run in one persistent launch (one workgroup per snapshot, largest
- * first).
+ * snapshots run together in a sequence of launches, each over the blocks of
+ * atoms or k-vectors of every snapshot.
  * ---------------------------------------------------------------------- */
 typedef struct mdff_fit mdff_fit;
 

@@ -24,7 +24,7 @@ HIPCC = str(ROCM / "bin" / "hipcc")
 COMMON = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
           "-Wno-unused-variable", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 PER_FILE = {"nlist.hip": ["-ffp-contract=off"]}
-SOURCES = ["nlist.hip", "realspace.hip", "recip.hip", "mdff_gpu.hip", "fit.hip", "noalloc.hip"]
+SOURCES = ["nlist.hip", "realspace.hip", "recip.hip", "mdff_gpu.hip", "fit.hip", "noalloc.hip", "efg.hip"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:

@@ -16,7 +16,7 @@ from pathlib import Path
 import numpy as np
 
 NTYPEMAX = 16
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 STATUS = {
     0: "MDFF_OK",
@@ -62,6 +62,7 @@ class MdffDesc(C.Structure):
         ("conv_tol_ind", C.c_double), ("min_scf_pol_iter", C.c_int32),
         ("max_scf_pol_iter", C.c_int32), ("extrapolate_order", C.c_int32),
         ("algo_moment_from_pola", C.c_int32), ("omegakO", C.c_double),
+        ("doefg", C.c_int32), ("pad1", C.c_int32),
     ]
 
 
@@ -163,6 +164,7 @@ def _bind(lib: C.CDLL, prefix: str) -> None:
         "get_pairs": ([vp, C.c_int32, ip, ip, C.c_int64, _P(C.c_int64)], C.c_int),
         "engforce": ([vp, dp, dp, dp, _P(MdffTerms)], C.c_int),
         "get_dipoles": ([vp, dp, dp], C.c_int),
+        "get_efg": ([vp, dp], C.c_int),
         "last_terms": ([vp, _P(MdffTerms)], C.c_int),
         "md_vv": ([vp, C.c_int32, C.c_double, dp, dp], C.c_int),
         "md_nhcn": ([vp, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32,
@@ -285,6 +287,14 @@ class Engine:
         ef = np.zeros((self.n, 3))
         self._call("get_dipoles", _dp(mu), _dp(ef))
         return mu, ef
+
+    def get_efg(self) -> np.ndarray:
+        """efg_t of the last force evaluation as (natm, 3, 3): [i, a, b] is
+        component (a+1, b+1) of the reference array efg(3,3,natm)."""
+        g = np.zeros((self.n, 3, 3))
+        self._call("get_efg", _dp(g))
+        # the C layout efg[9i + a + 3b] is column-major per atom
+        return g.transpose(0, 2, 1).copy()
 
     def md_vv(self, nsteps: int, dt: float) -> tuple[np.ndarray, np.ndarray]:
         ek = np.zeros(nsteps)

@@ -367,6 +367,14 @@ struct mdff_ctx {
   AtomRec* arec = nullptr;                // atoms in cell-sorted order (refreshed every evaluation)
   int2* ameta = nullptr;                  // (type, original index) in the same order
 
+  // electric field gradient efg_t (9 x n, column-major per atom) and its
+  // work arrays: real-space part (6 x n), S(k) / reciprocal partials, W = A S
+  double* efg = nullptr;
+  double* efg_dir = nullptr;
+  double* efg_part = nullptr;
+  double* efg_w = nullptr;
+  int efg_sk_nch = 1, efg_sk_chunk = 1, efg_rec_nch = 1, efg_rec_chunk = 1;
+
   double last_ms[8] = {0};
   int n_last = 0;
 
@@ -443,3 +451,8 @@ int recip_tables_from_r(mdff_ctx* c, bool pack);
 int recip_eval(mdff_ctx* c, const double* q, const double* mu, bool use_q, bool use_mu,
                bool forces, bool field, double* ef_out, double* f_out, double* e_out);
 void recip_free(mdff_ctx* c);
+// electric field gradient (efg.hip; its real-space part in realspace.hip)
+int realspace_efg(mdff_ctx* c, double* out);
+int efg_alloc(mdff_ctx* c);
+int efg_eval(mdff_ctx* c);
+void efg_free(mdff_ctx* c);

@@ -1110,6 +1110,9 @@ static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t, bool defer_f
                          d.lsurf, raw + RAW_MOM, 0, fpi_3V, c->efield);
     }
     c->efield_zero = zero_field ? 1 : 0;
+    // efg_t (src/engforce_driver.f90:171): with doefg every evaluation;
+    // without it only the self term, the same every time
+    if (d.doefg || !c->efg) MDFF_TRY(efg_eval(c));
   }
   if (!defer_finals) MDFF_TRY(flush_finals(c));
   ev_rec(c, 2 * EVS_TOTAL + 1);
@@ -1281,6 +1284,7 @@ int mdff_ctx_destroy(mdff_ctx* c) {
     if (p) (void)hipFree(p);
   if (c->polar) (void)hipFree(c->polar);
   recip_free(c);
+  efg_free(c);
   if (c->h_flag) (void)hipHostFree(c->h_flag);
   if (c->h_red) (void)hipHostFree(c->h_red);
   if (c->h_raw) (void)hipHostFree(c->h_raw);
@@ -1393,6 +1397,20 @@ int mdff_get_dipoles(mdff_ctx* c, double* mu, double* ef) {
   if (mu) HIP_TRY(c, hipMemcpyAsync(mu, c->mu, b, hipMemcpyDeviceToHost, c->stream));
   if (ef) HIP_TRY(c, hipMemcpyAsync(ef, c->efield, b, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return MDFF_OK;
+}
+
+int mdff_get_efg(mdff_ctx* c, double* efg) {
+  if (!c || !efg) return MDFF_ERR_BAD_ARG;
+  const size_t b = 9 * sizeof(double) * c->n;
+  if (c->efg) {
+    HIP_TRY(c, hipMemcpyAsync(efg, c->efg, b, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+  } else {
+    // before the first Coulomb evaluation (or without lcoulomb): efg_t = 0
+    // as allocated (src/coulomb.f90:434-439)
+    memset(efg, 0, b);
+  }
   return MDFF_OK;
 }
 

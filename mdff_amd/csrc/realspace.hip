@@ -514,6 +514,89 @@ __global__ void __launch_bounds__(256) k_pair_field(PairArgs a) {
   }
 }
 
+// Real-space part of the electric field gradient at each atom
+// (src/ewald_dir.f90:543-546, 577-582), FP64 in both precision forms: the
+// q-q task adds -q_j T2(r_ij), the mu-mu task T3(r_ij).mu_j for pairs where
+// both dipoles are non-zero (the reference's dd block, :557; its q-mu block
+// adds nothing to the gradient).  Every pair is seen from both atoms; the
+// reference's j-side term -T3(r_ij).mu_i is T3(r_ji).mu_i (T3 is odd).
+// out: 6 per atom, (xx, yy, zz, xy, xz, yz).
+template <int LPA>
+__global__ void __launch_bounds__(256) k_pair_efg(PairArgs a, double* __restrict__ out) {
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int pi = gtid / LPA;
+  const int sub = gtid % LPA;
+  if (pi >= a.n) return;
+  const AtomRec ri = a.rec[pi];
+  const int i = a.meta[pi].y;
+  const double alpha = a.alpha, a2 = alpha * alpha, a3 = a2 * alpha, a5 = a3 * a2;
+  const double isqpi = 0.56418958354775628695;
+  const bool dd = a.dd && a.mu != nullptr &&
+                  (a.mu[3 * i] != 0.0 || a.mu[3 * i + 1] != 0.0 || a.mu[3 * i + 2] != 0.0);
+  double g[6] = {0, 0, 0, 0, 0, 0};
+  const int r0 = a.row[i], r1 = r0 + a.cnt[i];
+  for (int e = r0 + sub; e < r1; e += LPA) {
+    const int pj = a.nl_j[e];
+    const AtomRec rj = a.rec[pj];
+    const int j = a.meta[pj].y;
+    double dxs = ri.x - rj.x, dys = ri.y - rj.y, dzs = ri.z - rj.z;
+    if (a.exact_image) {
+      dxs -= nint_d(dxs);
+      dys -= nint_d(dys);
+      dzs -= nint_d(dzs);
+    } else {
+      dxs -= rint(dxs);
+      dys -= rint(dys);
+      dzs -= rint(dzs);
+    }
+    const double rx = dxs * a.a11 + dys * a.a12 + dzs * a.a13;
+    const double ry = dxs * a.a21 + dys * a.a22 + dzs * a.a23;
+    const double rz = dxs * a.a31 + dys * a.a32 + dzs * a.a33;
+    const double r2 = rx * rx + ry * ry + rz * rz;
+    if (r2 > a.cutsq_c) continue;
+    const double d = sqrt(r2), dm1 = 1.0 / d, dm3 = dm1 / r2, dm5 = dm3 / r2, dm7 = dm5 / r2;
+    const double d3 = r2 * d, d5 = d3 * r2;
+    const double expon = exp(-a2 * r2) * isqpi;
+    const double F0 = errfc_pos<double>(alpha * d);
+    const double F1 = F0 + 2.0 * alpha * d * expon;
+    const double F2 = F1 + 4.0 * a3 * d3 * expon / 3.0;
+    const double F2_dm5 = 3.0 * F2 * dm5;
+    const double F1_dm3 = F1 * dm3;
+    const double r[3] = {rx, ry, rz};
+    constexpr int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+    if (a.qq) {
+      const double qj = rj.q;
+#pragma unroll
+      for (int x = 0; x < 6; ++x) {
+        double t2 = r[ia[x]] * r[ib[x]] * F2_dm5;
+        if (x < 3) t2 -= F1_dm3;
+        g[x] -= qj * t2;
+      }
+    }
+    if (dd) {
+      const double m[3] = {a.mu[3 * j], a.mu[3 * j + 1], a.mu[3 * j + 2]};
+      if (m[0] != 0.0 || m[1] != 0.0 || m[2] != 0.0) {
+        const double F3 = F2 + 8.0 * a5 * d5 * expon / 15.0;
+        const double F3_dm7 = 15.0 * F3 * dm7;
+        const double rm = r[0] * m[0] + r[1] * m[1] + r[2] * m[2];
+#pragma unroll
+        for (int x = 0; x < 6; ++x) {
+          const int p = ia[x], q = ib[x];
+          double t3 = -r[p] * r[q] * rm * F3_dm7 + (m[p] * r[q] + r[p] * m[q]) * F2_dm5;
+          if (x < 3) t3 += rm * F2_dm5;
+          g[x] += t3;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 6; ++x) g[x] = group_sum<LPA>(g[x]);
+  if (sub == 0) {
+#pragma unroll
+    for (int x = 0; x < 6; ++x) out[6 * (size_t)i + x] = g[x];
+  }
+}
+
 PairArgs make_args(mdff_ctx* c) {
   PairArgs a{};
   a.n = c->n;
@@ -621,6 +704,19 @@ int realspace_field(mdff_ctx* c, int mode, const double* mu_in, double* ef_out, 
     if (fp32) launch_field<float, 1>(c, a);
     else launch_field<double, 1>(c, a);
   }
+  HIP_TRY(c, hipGetLastError());
+  return MDFF_OK;
+}
+
+// Real-space electric field gradient of the engforce task set into out
+// (6 per atom, FP64); the atoms and lists of the current evaluation.
+int realspace_efg(mdff_ctx* c, double* out) {
+  PairArgs a = make_args(c);
+  a.qq = c->task_qq;
+  a.dd = c->task_dd;
+  a.mu = c->mu;
+  constexpr int LPA = 16;
+  hipLaunchKernelGGL((k_pair_efg<LPA>), dim3(div_up((int64_t)c->n * LPA, 256)), dim3(256), 0, c->stream, a, out);
   HIP_TRY(c, hipGetLastError());
   return MDFF_OK;
 }

@@ -4,7 +4,8 @@
 ! same argument-less SUBROUTINE engforce.  The first call creates the GPU
 ! context from the reference's module globals; every call copies the
 ! positions in, runs mdff_engforce and copies forces, energies, virials,
-! stress tensors, dipoles and fields back into the module globals the rest
+! stress tensors, dipoles, fields and field gradients back into the module
+! globals the rest
 ! of the reference reads.  Build it in place of src/engforce_driver.f90,
 ! together with mdff_gpu_iface.f90, and link -lmdff_gpu.
 MODULE engforce_driver
@@ -27,7 +28,7 @@ SUBROUTINE engforce_gpu_init
                          cutshortrange, cutlongrange, skindiff
   USE non_bonded, ONLY : epslj, sigmalj, plj, qlj, epsmor, sigmamor, rhomor, &
                          Abmhftd, Bbmhftd, Cbmhftd, Dbmhftd, BDbmhftd, trunc
-  USE coulomb,    ONLY : alphaES, kES, doefield
+  USE coulomb,    ONLY : alphaES, kES, doefield, doefg
   USE pim,        ONLY : ldip_damping, pol_damp_b, pol_damp_c, pol_damp_k, conv_tol_ind, &
                          min_scf_pol_iter, max_scf_pol_iter, extrapolate_order, &
                          algo_moment_from_pola, omegakO
@@ -46,7 +47,7 @@ SUBROUTINE engforce_gpu_init
   d%epsmor = reshape(epsmor,[256]) ; d%sigmamor = reshape(sigmamor,[256]) ; d%rhomor = reshape(rhomor,[256])
   d%Abmhftd = reshape(Abmhftd,[256]) ; d%Bbmhftd = reshape(Bbmhftd,[256])
   d%Cbmhftd = reshape(Cbmhftd,[256]) ; d%Dbmhftd = reshape(Dbmhftd,[256]) ; d%BDbmhftd = reshape(BDbmhftd,[256])
-  d%alphaES = alphaES ; d%kES = kES ; d%doefield = merge(1,0,doefield)
+  d%alphaES = alphaES ; d%kES = kES ; d%doefield = merge(1,0,doefield) ; d%doefg = merge(1,0,doefg)
   d%mass = mass ; d%qch = qch ; d%dip = reshape(dip,[48])
   d%ldip_polar = merge(1,0,ldip_polar) ; d%poldip = reshape(poldip,[144])
   d%ldip_damping = reshape(merge(1,0,ldip_damping),[4096])
@@ -62,7 +63,7 @@ END SUBROUTINE engforce_gpu_init
 SUBROUTINE engforce
   USE config,        ONLY : natm, rx, ry, rz, fx, fy, fz, tau, tau_nonb, tau_coul
   USE control,       ONLY : lcoulomb
-  USE coulomb,       ONLY : mu_t, ef_t
+  USE coulomb,       ONLY : mu_t, ef_t, efg_t
   USE md,            ONLY : itime
   USE thermodynamic, ONLY : u_lj, u_bmhft, u_morse, u_coul, u_pol, u_tail, &
                             vir_lj, vir_bmhft, vir_morse, pvirial_coul
@@ -83,6 +84,8 @@ SUBROUTINE engforce
   tau_coul = reshape(t%tau_coul, [3,3])
   if (lcoulomb) then
     st = mdff_get_dipoles(gpu_ctx, mu_t, ef_t)
+    call mdff_check(st, gpu_ctx, 'engforce')
+    st = mdff_get_efg(gpu_ctx, efg_t)
     call mdff_check(st, gpu_ctx, 'engforce')
   endif
 END SUBROUTINE engforce

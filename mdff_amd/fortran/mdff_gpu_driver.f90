@@ -349,11 +349,11 @@ contains
         end do
       end if
     end do
-    if (any(lquad_polar) .or. any(quad /= 0.0_dp) .or. doefg .or. thole_functions .or. lwfc(1) /= 0 .or. &
+    if (any(lquad_polar) .or. any(quad /= 0.0_dp) .or. thole_functions .or. lwfc(1) /= 0 .or. &
         (trim(algo_moment_from_pola) /= 'scf' .and. trim(algo_moment_from_pola) /= 'scf_kO_v2') .or. &
         lmorse .and. (lnmlj .or. lbmhft .or. lbmhftd)) then
       write (*, '(a)') 'ERROR option outside the engforce path of this library ' // &
-                       '(quadrupoles, EFG, Thole, Wannier centres, SCF variants other than scf and scf_kO_v2)'
+                       '(quadrupoles, Thole, Wannier centres, SCF variants other than scf and scf_kO_v2)'
       stop 1
     end if
     if (lreduced) then
@@ -406,6 +406,7 @@ contains
     d%alphaES = alphaES
     d%kES = kES
     d%doefield = merge(1, 0, doefield)
+    d%doefg = merge(1, 0, doefg)
     d%mass = mass; d%qch = qch; d%dip = reshape(dip, [48])
     d%ldip_polar = merge(1, 0, ldip_polar)
     d%poldip = reshape(poldip, [144])
@@ -692,6 +693,46 @@ contains
     deallocate (mu, ef)
   end subroutine write_efall
 
+  ! write_EFGALL (src/field.f90:392-448): the field gradient efg_t, formatted
+  ! (upper triangle and diagonal) unless iefgall_format = 0 (the whole array)
+  subroutine write_efgall()
+    integer :: uu
+    real(dp), allocatable :: efg(:, :, :)
+    allocate (efg(3, 3, natm))
+    st = mdff_get_efg(ctx, efg)
+    call mdff_check(st, ctx, 'get_efg')
+    if (iefgall_format /= 0) then
+      open (newunit=uu, file='EFGALL', status='unknown')
+      write (uu, *) natm
+      write (uu, *) system_name
+      write (uu, *) A(1, 1), A(2, 1), A(3, 1)
+      write (uu, *) A(1, 2), A(2, 2), A(3, 2)
+      write (uu, *) A(1, 3), A(2, 3), A(3, 3)
+      write (uu, *) ntype
+      write (uu, *) (atypei(it), it=1, ntype)
+      write (uu, *) (natmi(it), it=1, ntype)
+      write (uu, '(a)') '      ia type                   vxx                     vyy                     vzz' // &
+                        '                     vxy                     vxz                     vyz'
+      do ia = 1, natm
+        write (uu, '(i8,2x,a3,6e24.16)') ia, atype(ia), efg(1, 1, ia), efg(2, 2, ia), efg(3, 3, ia), &
+                                         efg(1, 2, ia), efg(1, 3, ia), efg(2, 3, ia)
+      end do
+    else
+      open (newunit=uu, file='EFGALL', status='unknown', form='unformatted')
+      write (uu) natm
+      write (uu) system_name
+      write (uu) A(1, 1), A(2, 1), A(3, 1)
+      write (uu) A(1, 2), A(2, 2), A(3, 2)
+      write (uu) A(1, 3), A(2, 3), A(3, 3)
+      write (uu) ntype
+      write (uu) (atypei(it), it=1, ntype)
+      write (uu) (natmi(it), it=1, ntype)
+      write (uu) efg
+    end if
+    close (uu)
+    deallocate (efg)
+  end subroutine write_efgall
+
   ! the next step after `now` at which the host acts: the equilibration
   ! rescale, a trajectory frame, thermo output, CONTFF, or the last step
   integer function next_event(now) result(nx)
@@ -804,6 +845,7 @@ contains
     ! reference ends one past itime1, the step a restart starts from
     call write_contff()
     if (doefield) call write_efall()
+    if (doefg) call write_efgall()
     if (lwrite_restart) call write_restart(itime1 + 1)
   end subroutine md_run
 

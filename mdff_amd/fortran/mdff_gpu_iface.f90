@@ -17,7 +17,7 @@ module mdff_gpu
 
   integer, parameter, public :: dp = c_double
   integer(c_int32_t), parameter, public :: MDFF_NTYPEMAX = 16
-  integer(c_int32_t), parameter, public :: MDFF_ABI_VERSION = 2
+  integer(c_int32_t), parameter, public :: MDFF_ABI_VERSION = 3
 
   integer(c_int), parameter, public :: MDFF_OK = 0, MDFF_ERR_BAD_ARG = 1, MDFF_ERR_LIST_OVERFLOW = 2, &
        MDFF_ERR_SCF_NOT_CONVERGED = 3, MDFF_ERR_POLAR_CATASTROPHE = 4, MDFF_ERR_HIP = 5, &
@@ -51,6 +51,7 @@ module mdff_gpu
     integer(c_int32_t) :: min_scf_pol_iter, max_scf_pol_iter, extrapolate_order
     integer(c_int32_t) :: algo_moment_from_pola
     real(c_double)     :: omegakO
+    integer(c_int32_t) :: doefg, pad1
   end type mdff_desc
 
   ! struct mdff_terms
@@ -69,7 +70,7 @@ module mdff_gpu
   public :: mdff_desc_defaults, mdff_ctx_create, mdff_ctx_destroy, mdff_last_error_c, mdff_version_c
   public :: mdff_set_positions, mdff_get_positions, mdff_set_velocities, mdff_get_velocities
   public :: mdff_get_forces, mdff_set_cell, mdff_vnlist_build, mdff_vnlistcheck, mdff_get_pairs
-  public :: mdff_engforce, mdff_get_dipoles, mdff_set_itime, mdff_md_vv, mdff_md_nhcn, mdff_last_terms
+  public :: mdff_engforce, mdff_get_dipoles, mdff_get_efg, mdff_set_itime, mdff_md_vv, mdff_md_nhcn, mdff_last_terms
   public :: mdff_csvr_rescale, mdff_calc_temp, mdff_last_timings, mdff_rescale_velocities
   public :: mdff_check, mdff_error_string, mdff_version_string
 
@@ -167,6 +168,12 @@ module mdff_gpu
       import :: c_int, c_ptr, mdff_terms
       type(c_ptr), value :: ctx
       type(mdff_terms), intent(out) :: terms
+    end function
+
+    integer(c_int) function mdff_get_efg(ctx, efg) bind(C, name="mdff_get_efg")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: efg(3, 3, *)
     end function
 
     integer(c_int) function mdff_get_dipoles(ctx, mu, ef) bind(C, name="mdff_get_dipoles")

@@ -42,6 +42,14 @@
  *  ewald q-q, mu-mu- Nymand & Linse values (examples/ex04/run_ex4.sh:145,179);
  *                    NaCl Madelung constant; alpha independence of the total;
  *                    FD forces and stress.
+ *  efg             - Nymand & Linse V1xx, V1yy (examples/ex04/run_ex4.sh:145,
+ *                    179); upper triangle = dE_a/dr_b of the (pinned) field by
+ *                    central differences in a cubic cell; alpha independence;
+ *                    zero trace for a neutral system.  The lower triangle
+ *                    (real space + self only, src/ewald_rec.f90:209-216 fills
+ *                    the upper one) and the real-space mu-mu term only between
+ *                    two non-zero dipoles (src/ewald_dir.f90:557) follow the
+ *                    reference lines; no data of the reference pins them.
  *  scf             - converged mu against a dense solve (alpha^-1 - T) mu = E;
  *                    F = -dU/dr of the converged PIM energy (damped terms).
  *  damped q-mu     - an isolated charge + polarisable ion: mu -> pol q f_k(br)/r^2
@@ -100,6 +108,7 @@ struct orc_ctx {
   double *hist;                  /* dipia_ind_t(5,3,natm): hist[k*3n + 3*ia + a] */
   int itime;
   double *mu_t, *ef_t;
+  double *efg_t;                 /* efg(3,3,natm): efg_t[9*ia + a + 3*b] */
   double tau[9];
   int nthreads;
   char err[256];
@@ -389,7 +398,7 @@ int orc_create(const mdff_desc *d, const int32_t *itype, orc_ctx **out) {
   c->vx = dalloc(n); c->vy = dalloc(n); c->vz = dalloc(n);
   c->fx = dalloc(n); c->fy = dalloc(n); c->fz = dalloc(n);
   c->xs = dalloc(n); c->ys = dalloc(n); c->zs = dalloc(n);
-  c->mu_t = dalloc(3 * n); c->ef_t = dalloc(3 * n);
+  c->mu_t = dalloc(3 * n); c->ef_t = dalloc(3 * n); c->efg_t = dalloc(9 * (size_t)n);
   c->hist = dalloc(15 * (size_t)n);
   init_nonbonded(c);
   if (d->lcoulomb) {
@@ -408,7 +417,7 @@ void orc_destroy(orc_ctx *c) {
   if (!c) return;
   void *ps[] = {c->itype, c->q, c->mass, c->pol, c->invpol, c->dipia, c->rx, c->ry, c->rz,
                 c->vx, c->vy, c->vz, c->fx, c->fy, c->fz, c->xs, c->ys, c->zs, c->mu_t,
-                c->ef_t, c->hist, c->kx, c->ky, c->kz, c->kk, c->Ak, c->kcoe, c->pv,
+                c->ef_t, c->efg_t, c->hist, c->kx, c->ky, c->kz, c->kk, c->Ak, c->kcoe, c->pv,
                 c->lv, c->pc, c->lc};
   for (size_t i = 0; i < sizeof ps / sizeof ps[0]; ++i) free(ps[i]);
   free(c);
@@ -644,13 +653,15 @@ typedef struct {
   int efield, forces, stress;
   int norec;           /* do_rec = .false. */
   double alpha;        /* alphaES for this call when > 0 (the scf_kO_v2 inner loop) */
+  int efg;             /* do_efg */
 } es_task;
 
 /* multipole_ES_dir (src/ewald_dir.f90:40-798) restricted to charges and
  * dipoles (tasks 1-3; Thole off, src/coulomb.f90:173).  Outputs are
- * accumulated into ef (3n), f (3n, AoS), tau[9] (already /omega*0.5). */
+ * accumulated into ef (3n), efg (9n, only with tk.efg), f (3n, AoS), tau[9]
+ * (already /omega*0.5). */
 static void ewald_dir(orc_ctx *c, const double *s, const double *q, const double *mu, es_task tk,
-                      double *u_out, double *ef, double *f, double tau[9]) {
+                      double *u_out, double *ef, double *efg, double *f, double tau[9]) {
   const mdff_desc *d = &c->d;
   int n = c->n;
   double cutsq = c->rc_coul * c->rc_coul;
@@ -731,6 +742,12 @@ static void ewald_dir(orc_ctx *c, const double *s, const double *q, const double
               ef[3 * j + a] = ef[3 * j + a] - qi * T1d2[a];
             }
         }
+        /* src/ewald_dir.f90:543-546 */
+        if (tk.efg)
+          for (int x = 0; x < 9; ++x) {
+            efg[9 * i + x] = efg[9 * i + x] - qj * T2[x];
+            efg[9 * j + x] = efg[9 * j + x] - qi * T2[x];
+          }
         if (tk.forces)
           for (int a = 0; a < 3; ++a) fij[a] = fij[a] + qij * T1[a];
       }
@@ -743,6 +760,19 @@ static void ewald_dir(orc_ctx *c, const double *s, const double *q, const double
               ef[3 * i + a] = ef[3 * i + a] + T2[a + 3 * b] * muj[b];
               ef[3 * j + a] = ef[3 * j + a] + T2[a + 3 * b] * mui[b];
             }
+        /* src/ewald_dir.f90:577-582: efg(:,:,ia) += T3(:,:,k) muj(k),
+         * efg(:,:,ja) -= T3(:,:,k) mui(k), T3 as at :436-447 */
+        if (tk.efg)
+          for (int k = 0; k < 3; ++k)
+            for (int b = 0; b < 3; ++b)
+              for (int a = 0; a < 3; ++a) {
+                double t3 = -rij[a] * rij[b] * rij[k] * F3_dm7;
+                if (a == b) t3 += rij[k] * F2_dm5;
+                if (a == k) t3 += rij[b] * F2_dm5;
+                if (b == k) t3 += rij[a] * F2_dm5;
+                efg[9 * i + a + 3 * b] = efg[9 * i + a + 3 * b] + t3 * muj[k];
+                efg[9 * j + a + 3 * b] = efg[9 * j + a + 3 * b] - t3 * mui[k];
+              }
         if (tk.forces) {
           /* T3(a,b,k) for fixed pair (src/ewald_dir.f90:436-447) */
           for (int a = 0; a < 3; ++a)
@@ -796,15 +826,17 @@ static void ewald_dir(orc_ctx *c, const double *s, const double *q, const double
 }
 
 /* multipole_ES_rec (src/ewald_rec.f90:38-365) for charges and dipoles.
- * Parallel over k-vectors when nthreads > 1 (private accumulators). */
+ * Parallel over k-vectors when nthreads > 1 (private accumulators).  efg (9n)
+ * receives the upper triangle and the diagonal only (:209-216). */
 static void ewald_rec(orc_ctx *c, const double *q, const double *mu, es_task tk, double *u_out,
-                      double *ef, double *f, double tau[9]) {
+                      double *ef, double *efg, double *f, double tau[9]) {
   int n = c->n, nk = c->nk;
   int ldip = tk.qd || tk.dd;
   double tpi_V = 2.0 * PI / c->omega;
   double fpi_V = tpi_V * 2.0;
   int nth = c->nthreads;
-  double *acc = dalloc((size_t)nth * (6 * (size_t)n + 10));
+  const size_t per = 12 * (size_t)n + 10;   /* ef 3n, f 3n, u, tau 9, efg 6n */
+  double *acc = dalloc((size_t)nth * per);
 #ifdef _OPENMP
 #pragma omp parallel num_threads(nth)
 #endif
@@ -813,8 +845,9 @@ static void ewald_rec(orc_ctx *c, const double *q, const double *mu, es_task tk,
 #ifdef _OPENMP
     tid = omp_get_thread_num();
 #endif
-    double *my = acc + (size_t)tid * (6 * (size_t)n + 10);
+    double *my = acc + (size_t)tid * per;
     double *mef = my, *mf = my + 3 * n, *mu_rec = my + 6 * n, *mtau = my + 6 * n + 1;
+    double *mefg = my + 6 * n + 10;
     double *ckr = dalloc(n), *skr = dalloc(n);
 #ifdef _OPENMP
 #pragma omp for schedule(static)
@@ -845,6 +878,16 @@ static void ewald_rec(orc_ctx *c, const double *q, const double *mu, es_task tk,
         if (tk.efield) {
           mef[3 * ia] -= fxij; mef[3 * ia + 1] -= fyij; mef[3 * ia + 2] -= fzij;
         }
+        if (tk.efg) {
+          /* (1,1) (2,2) (3,3) (1,2) (1,3) (2,3), src/ewald_rec.f90:209-216 */
+          double *g = &mefg[6 * ia];
+          g[0] = g[0] + kx * kx * recarg2;
+          g[1] = g[1] + ky * ky * recarg2;
+          g[2] = g[2] + kz * kz * recarg2;
+          g[3] = g[3] + kx * ky * recarg2;
+          g[4] = g[4] + kx * kz * recarg2;
+          g[5] = g[5] + ky * kz * recarg2;
+        }
         if (tk.forces) {
           double qi = q[ia];
           mf[3 * ia] -= qi * fxij; mf[3 * ia + 1] -= qi * fyij; mf[3 * ia + 2] -= qi * fzij;
@@ -867,14 +910,18 @@ static void ewald_rec(orc_ctx *c, const double *q, const double *mu, es_task tk,
     free(skr);
   }
   double u = 0.0, tr[9] = {0};
+  static const int up[6] = {0, 4, 8, 3, 6, 7};   /* (1,1) (2,2) (3,3) (1,2) (1,3) (2,3) */
   for (int t = 0; t < nth; ++t) {
-    double *my = acc + (size_t)t * (6 * (size_t)n + 10);
+    double *my = acc + (size_t)t * per;
     u += my[6 * n];
     for (int k = 0; k < 9; ++k) tr[k] += my[6 * n + 1 + k];
     for (size_t k = 0; k < 3 * (size_t)n; ++k) {
       ef[k] += my[k] * 2.0 * fpi_V;
       f[k] += my[3 * n + k] * 2.0 * fpi_V;
     }
+    if (tk.efg)
+      for (int ia = 0; ia < n; ++ia)
+        for (int x = 0; x < 6; ++x) efg[9 * ia + up[x]] += my[6 * n + 10 + 6 * ia + x] * 2.0 * fpi_V;
   }
   free(acc);
   u = u * 2.0;
@@ -894,10 +941,11 @@ static void ewald_rec(orc_ctx *c, const double *q, const double *mu, es_task tk,
 typedef struct { double u_dir, u_rec, u_self, u_surf, tau_coul[9]; } es_out;
 
 static void multipole_es(orc_ctx *c, const double *s, const double *q, const double *mu, es_task tk,
-                         double *ef, es_out *o) {
+                         double *ef, double *efg, es_out *o) {
   int n = c->n;
   const mdff_desc *d = &c->d;
   double *ef_dir = dalloc(3 * n), *ef_rec = dalloc(3 * n);
+  double *efg_dir = dalloc(9 * (size_t)n), *efg_rec = dalloc(9 * (size_t)n);
   double *f_dir = dalloc(3 * n), *f_rec = dalloc(3 * n);
   double tau_dir[9] = {0}, tau_rec[9] = {0};
   double mutot[3] = {0}, qtot[3] = {0}, musq = 0.0, qsq = 0.0;
@@ -912,9 +960,9 @@ static void multipole_es(orc_ctx *c, const double *s, const double *q, const dou
   double alpha2 = al * al;
   double selfa = al / sqrt(PI);
   double selfa2 = 2.0 * selfa * alpha2 / 3.0;
-  ewald_dir(c, s, q, mu, tk, &o->u_dir, ef_dir, f_dir, tau_dir);
+  ewald_dir(c, s, q, mu, tk, &o->u_dir, ef_dir, efg_dir, f_dir, tau_dir);
   o->u_rec = 0.0;
-  if (!tk.norec) ewald_rec(c, q, mu, tk, &o->u_rec, ef_rec, f_rec, tau_rec);
+  if (!tk.norec) ewald_rec(c, q, mu, tk, &o->u_rec, ef_rec, efg_rec, f_rec, tau_rec);
   double uq = qtot[0] * qtot[0] + qtot[1] * qtot[1] + qtot[2] * qtot[2];
   double ud = mutot[0] * mutot[0] + mutot[1] * mutot[1] + mutot[2] * mutot[2];
   double uqd = 2.0 * (qtot[0] * mutot[0] + qtot[1] * mutot[1] + qtot[2] * mutot[2]);
@@ -927,6 +975,14 @@ static void multipole_es(orc_ctx *c, const double *s, const double *q, const dou
       e += 2.0 * selfa2 * mu[3 * i + a];
       ef[3 * i + a] = e;
     }
+  /* efg = (efg_dir + efg_rec + efg_self) * coul_unit, the self term
+   * -2 selfa2 q on the diagonal whether or not do_efg (src/ewald.f90:214-216,231) */
+  if (efg)
+    for (int i = 0; i < n; ++i)
+      for (int x = 0; x < 9; ++x) {
+        double self = (x % 4 == 0) ? -2.0 * selfa2 * q[i] : 0.0;
+        efg[9 * i + x] = (efg_dir[9 * i + x] + efg_rec[9 * i + x] + self) * d->coul_unit;
+      }
   for (int k = 0; k < 9; ++k) o->tau_coul[k] = (tau_dir[k] + tau_rec[k]) * d->coul_unit / d->press_unit;
   if (tk.forces)
     for (int i = 0; i < n; ++i) {
@@ -934,7 +990,7 @@ static void multipole_es(orc_ctx *c, const double *s, const double *q, const dou
       c->fy[i] = c->fy[i] + (f_rec[3 * i + 1] + f_dir[3 * i + 1]) * d->coul_unit;
       c->fz[i] = c->fz[i] + (f_rec[3 * i + 2] + f_dir[3 * i + 2]) * d->coul_unit;
     }
-  free(ef_dir); free(ef_rec); free(f_dir); free(f_rec);
+  free(ef_dir); free(ef_rec); free(f_dir); free(f_rec); free(efg_dir); free(efg_rec);
 }
 
 int orc_multipole_es(orc_ctx *c, const double *mu, const int32_t task[3], int32_t damp, double *ef,
@@ -949,7 +1005,7 @@ int orc_multipole_es(orc_ctx *c, const double *mu, const int32_t task[3], int32_
   es_out o;
   for (int i = 0; i < c->n; ++i) { c->fx[i] = 0; c->fy[i] = 0; c->fz[i] = 0; }
   double *e = dalloc(3 * c->n);
-  multipole_es(c, s, c->q, mu, tk, e, &o);
+  multipole_es(c, s, c->q, mu, tk, e, NULL, &o);
   if (ef) memcpy(ef, e, sizeof(double) * 3 * c->n);
   if (fx) orc_get_forces(c, fx, fy, fz);
   if (tau_coul) memcpy(tau_coul, o.tau_coul, sizeof o.tau_coul);
@@ -972,7 +1028,7 @@ int orc_dipole_field(orc_ctx *c, const double *mu, double *ef) {
   double *s = frac_coords(c), *zq = dalloc(c->n);
   es_task ti = {0, 0, 1, 0, 1, 0, 0};
   es_out o;
-  multipole_es(c, s, zq, mu, ti, ef, &o);
+  multipole_es(c, s, zq, mu, ti, ef, NULL, &o);
   free(s);
   free(zq);
   return MDFF_OK;
@@ -1060,7 +1116,7 @@ static void scf(orc_ctx *c, const double *s, double *mu_ind, double *u_pol, int 
   for (int i = 0; i < 48; ++i) if (d->dip[i] != 0.0) ldip = 1;
   es_task ts = {1, ldip, ldip, 1, 1, 0, 0};
   es_out o;
-  multipole_es(c, s, c->q, c->dipia, ts, Es, &o);
+  multipole_es(c, s, c->q, c->dipia, ts, Es, NULL, &o);
   memcpy(E, Es, sizeof(double) * N3);
   es_task ti = {0, 0, 1, 0, 1, 0, 0};
   int iscf = 0;
@@ -1075,7 +1131,7 @@ static void scf(orc_ctx *c, const double *s, double *mu_ind, double *u_pol, int 
       for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) upol = upol + mu_ind[3 * i + a] * c->invpol[9 * i + a + 3 * b] * mu_ind[3 * i + b];
     upol = upol * 0.5;
-    multipole_es(c, s, zq, mu_ind, ti, Ei, &o);
+    multipole_es(c, s, zq, mu_ind, ti, Ei, NULL, &o);
     for (size_t x = 0; x < N3; ++x) E[x] = Es[x] + Ei[x];
     if (iscf == 1 && d->calc_md) aspc(c, mu_ind, E, 2);
     rmsd = rmsd_mu(c, E, mu_ind);
@@ -1104,7 +1160,7 @@ static void kO_inner(orc_ctx *c, const double *s, const double *fext, double *mu
   es_out o;
   for (int it = 1; it <= 10 && rmsd_inner > 0.1 * rmsd_ref; ++it) {
     memcpy(mprev, mu, sizeof(double) * N3);
-    multipole_es(c, s, zq, mu, ti, f, &o);
+    multipole_es(c, s, zq, mu, ti, f, NULL, &o);
     for (size_t x = 0; x < N3; ++x) ft[x] = fext[x] + f[x];
     rmsd_inner = rmsd_mu(c, ft, mu);
     induced_moment(c, ft, mu);
@@ -1147,7 +1203,7 @@ static void scf_kO_v2(orc_ctx *c, const double *s, double *mu_ind, double *u_pol
   for (int i = 0; i < 48; ++i) if (d->dip[i] != 0.0) ldip = 1;
   es_task ts = {1, ldip, ldip, 1, 1, 0, 0, 0, 0.0};
   es_out o;
-  multipole_es(c, s, c->q, c->dipia, ts, Es, &o);
+  multipole_es(c, s, c->q, c->dipia, ts, Es, NULL, &o);
   memcpy(E, Es, sizeof(double) * N3);
   memcpy(find, E, sizeof(double) * N3);
   memset(mu_ind, 0, sizeof(double) * N3);
@@ -1167,7 +1223,7 @@ static void scf_kO_v2(orc_ctx *c, const double *s, double *mu_ind, double *u_pol
       for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) upol = upol + mu_ind[3 * i + a] * c->invpol[9 * i + a + 3 * b] * mu_ind[3 * i + b];
     upol = upol * 0.5;
-    multipole_es(c, s, zq, mu_ind, ti, Ei, &o);
+    multipole_es(c, s, zq, mu_ind, ti, Ei, NULL, &o);
     for (size_t x = 0; x < N3; ++x) E[x] = Es[x] + Ei[x];
     if (iscf == 1 && d->calc_md) aspc(c, mu_ind, E, 2);
     for (int i = 0; i < n; ++i) {
@@ -1245,8 +1301,9 @@ int orc_engforce(orc_ctx *c, double *fx, double *fy, double *fz, mdff_terms *t) 
         return fail(c, MDFF_ERR_POLAR_CATASTROPHE, "ERROR : POLARIZATION CATASTROPH");
       }
     es_task tk = {c->task_qq, c->task_qd, c->task_dd, 1, d->doefield, 1, 1};
+    tk.efg = d->doefg;
     es_out o;
-    multipole_es(c, s, c->q, mu, tk, ef, &o);
+    multipole_es(c, s, c->q, mu, tk, ef, c->efg_t, &o);
     tm.u_dir = o.u_dir; tm.u_rec = o.u_rec; tm.u_self = o.u_self;
     tm.u_surf = d->lsurf ? o.u_surf : 0.0;
     tm.u_pol = upol;
@@ -1278,6 +1335,14 @@ int orc_last_terms(orc_ctx *c, mdff_terms *t) {
 int orc_get_dipoles(orc_ctx *c, double *mu, double *ef) {
   if (mu) memcpy(mu, c->mu_t, sizeof(double) * 3 * c->n);
   if (ef) memcpy(ef, c->ef_t, sizeof(double) * 3 * c->n);
+  return MDFF_OK;
+}
+
+/* efg_t (src/coulomb.f90:78) of the last engforce: zeros until a Coulomb
+ * evaluation, then efg as multipole_ES leaves it */
+int orc_get_efg(orc_ctx *c, double *efg) {
+  if (!c || !efg) return MDFF_ERR_BAD_ARG;
+  memcpy(efg, c->efg_t, sizeof(double) * 9 * (size_t)c->n);
   return MDFF_OK;
 }
 

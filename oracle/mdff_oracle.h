@@ -48,6 +48,7 @@ int    orc_vnlistcheck(orc_ctx *c, int32_t *rebuilt);
 int    orc_get_pairs(orc_ctx *c, int32_t which, int32_t *point, int32_t *list, int64_t cap, int64_t *n);
 int    orc_engforce(orc_ctx *c, double *fx, double *fy, double *fz, mdff_terms *t);
 int    orc_get_dipoles(orc_ctx *c, double *mu, double *ef);
+int    orc_get_efg(orc_ctx *c, double *efg);
 int    orc_last_terms(orc_ctx *c, mdff_terms *t);
 int    orc_rescale_velocities(orc_ctx *c, double lambda);
 int    orc_get_cutoffs(orc_ctx *c, double *rc_vdw, double *rc_coul, double *alpha, int32_t *nk);

@@ -275,3 +275,51 @@ def test_driver_lautoES_matches_oracle(tmp_path):
     _, to = o.engforce()
     for k in ("u_coul", "u_dir", "u_rec", "u_self"):
         assert abs(terms[k] - to[k]) <= 1e-10 * abs(to[k]), (k, terms[k], to[k])
+
+
+def _fortran_records(path):
+    """Records of a sequential unformatted file (4-byte length markers)."""
+    raw = path.read_bytes()
+    out, p = [], 0
+    while p < len(raw):
+        n = int(np.frombuffer(raw[p:p + 4], np.int32)[0])
+        out.append(raw[p + 4:p + 4 + n])
+        p += 8 + n
+    return out
+
+
+@pytest.mark.gpu
+def test_driver_efgall_matches_oracle(tmp_path):
+    """doefg: EFGALL as write_EFGALL writes it (src/field.f90:392-448), the
+    formatted file (vxx vyy vzz vxy vxz vyz per ion) and the unformatted one
+    (the whole efg(3,3,natm), whose lower triangle has no reciprocal part),
+    against the oracle's efg_t for the same inputs."""
+    import copy
+
+    import oracle_lib as O
+    from gpu_util import note, rel
+    s = configs.nacl(ncell=2, seed=12)
+    d = copy.deepcopy(s.desc)
+    d.doefg = 1
+    o = O.oracle(d, s.itype)
+    o.set_positions(s.pos)
+    o.engforce()
+    want = o.get_efg()                         # (natm, 3, 3)
+    for fmt in (1, 0):
+        wd = tmp_path / f"fmt{fmt}"
+        wd.mkdir()
+        FI.write_inputs(wd, s, {1: "Na", 2: "Cl"}, npas=0, extra_control=[f" iefgall_format = {fmt}"],
+                        extra_coulomb=[" doefg = .true."])
+        note(f"fortran driver efgall format {fmt}")
+        r = _run(wd)
+        assert r.returncode == 0, r.stdout + r.stderr
+        if fmt:
+            rows = np.loadtxt(wd / "EFGALL", skiprows=9, usecols=range(2, 8))
+            cols = np.stack([want[:, 0, 0], want[:, 1, 1], want[:, 2, 2], want[:, 0, 1], want[:, 0, 2],
+                             want[:, 1, 2]], axis=1)
+            assert rel(rows, cols) < 1e-10
+        else:
+            rec = _fortran_records(wd / "EFGALL")
+            assert int(np.frombuffer(rec[0], np.int32)[0]) == s.natm
+            g = np.frombuffer(rec[-1], np.float64).reshape(s.natm, 3, 3)   # [i, b, a]
+            assert rel(g.transpose(0, 2, 1), want) < 1e-10

@@ -280,6 +280,100 @@ def test_nymand_linse_two_charges_and_two_dipoles():
     assert abs(f[0, 0] - w["f1x"]) < 1e-6
 
 
+def test_efg_nymand_linse_two_charges_and_two_dipoles():
+    """efg_t of multipole_ES (src/ewald.f90:231; src/ewald_dir.f90:543-546,
+    577-582; src/ewald_rec.f90:209-216; self :214-216) against V1,xx and V1,yy
+    quoted in examples/ex04/run_ex4.sh:145,179 (first EFGALL row, columns
+    vxx and vyy)."""
+    g = json.loads((GOLD / "ex04.json").read_text())
+    for name, build in (("two_charges", configs.two_charges), ("two_dipoles", configs.two_dipoles)):
+        s = build()
+        s.desc.doefg = 1
+        e, _, _ = _energy(s)
+        v = e.get_efg()
+        w = g[name]
+        assert abs(v[0, 0, 0] - w["V1xx"]) < 1e-6, (name, v[0, 0, 0])
+        assert abs(v[0, 1, 1] - w["V1yy"]) < 1e-6, (name, v[0, 1, 1])
+
+
+def _efg_cubic(alpha=1.2, kmax=26, seed=11):
+    """NaCl 2x2x2 conventional cells, displaced ions, static dipoles on both
+    types (so every pair enters the real-space mu-mu terms); converged Ewald
+    sums (erfc(alpha rc) ~ 1e-21, exp(-k^2/4alpha^2) ~ 1e-13 at kmax)."""
+    s = configs.nacl(ncell=2, sigma=0.08, alpha=alpha, rcut=5.64, kmax=kmax, seed=seed)
+    d = s.desc
+    d.lbmhft = 0
+    d.dip[0 + 16 * 0], d.dip[0 + 16 * 1], d.dip[0 + 16 * 2] = 0.11, -0.05, 0.07
+    d.dip[1 + 16 * 0], d.dip[1 + 16 * 1], d.dip[1 + 16 * 2] = -0.04, 0.09, 0.13
+    d.doefg = 1
+    d.doefield = 1
+    return s
+
+
+def test_efg_is_the_gradient_of_the_field():
+    """Upper triangle and diagonal of efg = dE_a/dr_b at the ion, E the field
+    ef_t (pinned separately), by central differences in r_i.  The field at i
+    carries no contribution of i's own periodic images (its reciprocal part
+    cancels exactly, src/ewald_rec.f90:196-206), while efg_t does: in a cubic
+    cell it is the background term -(4 pi / 3V) q_i delta (the Ewald
+    reciprocal + self sums of one ion's lattice).  A dropped self term, a wrong
+    reciprocal prefactor or a wrong T3 sign each fail this."""
+    s = _efg_cubic()
+    e, _, _ = _energy(s)
+    v = e.get_efg() / COUL_UNIT
+    L = s.desc.cell[0]
+    own = -4.0 * math.pi / (3.0 * L ** 3)
+    q = np.where(s.itype == 1, 1.0, -1.0)
+    h = 1e-4
+    scale = np.abs(v).max()
+    for i in (0, 5, 37, 60):
+        fd = np.zeros((3, 3))
+        for b in range(3):
+            ef = []
+            for sg in (1.0, -1.0):
+                p = s.pos.copy()
+                p[i, b] += sg * h
+                e.set_positions(p)
+                e.engforce()
+                ef.append(e.get_dipoles()[1][i])
+            fd[:, b] = (ef[0] - ef[1]) / (2 * h)
+        want = fd + own * q[i] * np.eye(3)
+        for a in range(3):
+            for b in range(a, 3):
+                assert abs(v[i, a, b] - want[a, b]) < 2e-7 * scale, (i, a, b, v[i, a, b], want[a, b])
+
+
+def test_efg_alpha_independent_and_traceless():
+    """With both Ewald sums converged the upper triangle and diagonal do not
+    depend on alpha; for a neutral cell the trace vanishes at every ion
+    (Laplace: the other ions and all images are sources elsewhere, the
+    compensating background of a neutral cell is zero)."""
+    v = []
+    for alpha, kmax in ((1.0, 22), (1.3, 30)):
+        e, _, _ = _energy(_efg_cubic(alpha=alpha, kmax=kmax))
+        v.append(e.get_efg())
+    iu = np.triu_indices(3)
+    a, b = v[0][:, iu[0], iu[1]], v[1][:, iu[0], iu[1]]
+    scale = np.abs(a).max()
+    assert np.abs(a - b).max() < 1e-8 * scale
+    tr = np.trace(v[0], axis1=1, axis2=2)
+    assert np.abs(tr).max() < 1e-8 * scale
+
+
+def test_efg_without_doefg_is_the_self_term():
+    """Without doefg multipole_ES still adds efg_self (src/ewald.f90:214-216,
+    231): -2 selfa2 q_i coul_unit on the diagonal, selfa2 = 2 alpha^3 /
+    (3 sqrt(pi))."""
+    s = _efg_cubic()
+    s.desc.doefg = 0
+    e, _, _ = _energy(s)
+    v = e.get_efg()
+    al = s.desc.alphaES
+    q = np.where(s.itype == 1, 1.0, -1.0)
+    want = -2.0 * (2.0 * al ** 3 / (3.0 * math.sqrt(math.pi))) * q * COUL_UNIT
+    assert np.allclose(v, want[:, None, None] * np.eye(3)[None], rtol=1e-14, atol=0)
+
+
 def test_nacl_madelung_constant():
     """Perfect rock salt: U = -N M q^2 / (2 r0) with M = 1.747564594633...,
     forces zero; converged Ewald parameters."""
