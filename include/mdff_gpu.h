@@ -122,7 +122,7 @@ typedef struct mdff_desc {
   int32_t min_scf_pol_iter;        /* default 3                                       */
   int32_t max_scf_pol_iter;        /* default 100                                     */
   int32_t extrapolate_order;       /* ASPC order 0..4, default 0                      */
-  int32_t algo_moment_from_pola;   /* 0: 'scf' (default), 1: 'scf_kO_v2'              */
+  int32_t algo_moment_from_pola;   /* 0: 'scf' (default), 1: 'scf_kO_v2', 2: 'scf_kO_v1' */
   double  omegakO;                 /* mixing of the scf_kO_v2 inner loop, default 0.7 */
 
   /* &coulombtag, continued */

@@ -635,8 +635,6 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   c->task_qd = c->ldip_static && lqch;
   c->task_dd = c->ldip_static;
   if (c->linduced) { c->task_qd |= lqch; c->task_dd = 1; }
-  if (c->linduced && c->ldip_static)
-    return fail(c, MDFF_ERR_UNSUPPORTED, "static dipoles together with polarisable ions are not supported");
   if (d->lnmlj) c->sr_model = 1;
   if (d->lbmhft || d->lbmhftd) c->sr_model = 2;
   if (d->lmorse) c->sr_model = 3;
@@ -795,12 +793,27 @@ static int ewald_field(mdff_ctx* c, bool charges, const double* mu, double* out)
   return MDFF_OK;
 }
 
+// static field of the SCF (src/moment_from_pola.f90:313-330): the charges'
+// field (TT-damped) plus, with static dipoles, theirs (task_static = all:
+// mu-mu real space between two non-zero static dipoles, reciprocal, self,
+// surface), the two added as the sum is linear in the sources
+static int static_field(mdff_ctx* c) {
+  if (!c->ldip_static) return ewald_field(c, true, nullptr, c->e_stat);
+  // e_tot and e_ind are free until the SCF loop starts
+  MDFF_TRY(ewald_field(c, true, nullptr, c->e_tot));
+  MDFF_TRY(ewald_field(c, false, c->dipia, c->e_ind));
+  const size_t n3 = 3 * (size_t)c->n;
+  hipLaunchKernelGGL(k_add3, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, c->e_tot, c->e_ind, c->e_stat);
+  HIP_TRY(c, hipGetLastError());
+  return MDFF_OK;
+}
+
 // moment_from_pola_scf (src/moment_from_pola.f90:240-485)
 static int scf(mdff_ctx* c, mdff_terms* t) {
   const int n = c->n;
   const size_t n3 = 3 * (size_t)n;
   const mdff_desc& d = c->d;
-  MDFF_TRY(ewald_field(c, true, nullptr, c->e_stat));
+  MDFF_TRY(static_field(c));
   HIP_TRY(c, hipMemcpyAsync(c->e_tot, c->e_stat, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
   int iscf = 0;
   double rmsd = 1.79769313486231571e308;
@@ -912,7 +925,7 @@ static int scf_kO_v2(mdff_ctx* c, mdff_terms* t) {
   double* dd = W + 6 * n3;
   double* const kw[4] = {W + 7 * n3, W + 8 * n3, W + 9 * n3, W + 10 * n3};   // kw[3]: zeros
   hipLaunchKernelGGL(k_fill, dim3(div_up(11 * n3, 256)), dim3(256), 0, c->stream, 11 * n3, 0.0, W);
-  MDFF_TRY(ewald_field(c, true, nullptr, c->e_stat));
+  MDFF_TRY(static_field(c));
   HIP_TRY(c, hipMemcpyAsync(c->e_tot, c->e_stat, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
   HIP_TRY(c, hipMemcpyAsync(find, c->e_stat, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
   hipLaunchKernelGGL(k_fill, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, 0.0, c->mu_ind);
@@ -963,6 +976,65 @@ static int scf_kO_v2(mdff_ctx* c, mdff_terms* t) {
     }
     HIP_TRY(c, hipMemcpyAsync(mprev, c->mu_ind, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(fprev, gmin, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIP_TRY(c, hipMemcpyAsync(c->hist, c->mu_ind, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  t->u_pol = upol;
+  t->n_scf = iscf;
+  t->rmsd = rmsd;
+  return MDFF_OK;
+}
+
+// moment_from_pola_scf_kO_v1 (src/moment_from_pola.f90:1042-1320): the inner
+// fixed-point loop on the residual f = E - mu/alpha of the previous
+// iteration, mu += its step, no line minimum; rmsd of (E, mu) each iteration.
+static int scf_kO_v1(mdff_ctx* c, mdff_terms* t) {
+  const int n = c->n;
+  const size_t n3 = 3 * (size_t)n;
+  const mdff_desc& d = c->d;
+  if (!c->kO_ws) HIP_TRY(c, hipMalloc((void**)&c->kO_ws, sizeof(double) * 11 * n3));
+  double* W = c->kO_ws;
+  double* find = W;
+  double* dmu = W + n3;
+  double* const kw[4] = {W + 7 * n3, W + 8 * n3, W + 9 * n3, W + 10 * n3};   // kw[3]: zeros
+  hipLaunchKernelGGL(k_fill, dim3(div_up(11 * n3, 256)), dim3(256), 0, c->stream, 11 * n3, 0.0, W);
+  MDFF_TRY(static_field(c));
+  HIP_TRY(c, hipMemcpyAsync(c->e_tot, c->e_stat, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(find, c->e_stat, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_fill, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, 0.0, c->mu_ind);
+  int ext;
+  if (c->itime >= d.extrapolate_order + 1) ext = d.extrapolate_order;
+  else if (c->itime == 1 || c->itime == 0) ext = 0;
+  else ext = c->itime - 1;
+  double Bc[5] = {0, 0, 0, 0, 0}, w = 0.0;
+  switch (ext) {
+    case 0: Bc[0] = 1.0; w = 0.0; break;
+    case 1: Bc[0] = 2.0; Bc[1] = -1.0; w = 2.0 / 3.0; break;
+    case 2: Bc[0] = 2.5; Bc[1] = -2.0; Bc[2] = 0.5; w = 0.6; break;
+    case 3: Bc[0] = 2.8; Bc[1] = -2.8; Bc[2] = 1.2; Bc[3] = -0.2; w = 4.0 / 7.0; break;
+    default: Bc[0] = 3.0; Bc[1] = -24.0 / 7.0; Bc[2] = 27.0 / 14.0; Bc[3] = -4.0 / 7.0; Bc[4] = 1.0 / 14.0; w = 5.0 / 9.0; break;
+  }
+  int iscf = 0;
+  double rmsd = 1.79769313486231571e308, upol = 0.0;
+  while ((iscf < d.max_scf_pol_iter && rmsd > d.conv_tol_ind) || iscf < d.min_scf_pol_iter) {
+    iscf++;
+    if (iscf != 1 || !d.calc_md) {
+      MDFF_TRY(kO_inner(c, find, dmu, kw));
+      hipLaunchKernelGGL(k_add3, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, c->mu_ind, dmu, c->mu_ind);
+    } else {
+      hipLaunchKernelGGL(k_aspc_predict, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, ext, Bc[0], Bc[1],
+                         Bc[2], Bc[3], Bc[4], c->hist, c->mu_ind);
+    }
+    hipLaunchKernelGGL(k_upol, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->invpol, c->mu_ind, extra(c));
+    double h[1];
+    MDFF_TRY(reduce_cols(c, extra(c), 1, h));
+    upol = 0.5 * h[0];
+    MDFF_TRY(ewald_field(c, false, c->mu_ind, c->e_ind));
+    hipLaunchKernelGGL(k_add3, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, c->e_stat, c->e_ind, c->e_tot);
+    if (iscf == 1 && d.calc_md)
+      hipLaunchKernelGGL(k_aspc_correct, dim3(nblk(n)), dim3(256), 0, c->stream, n, w, c->pol, c->e_tot, c->mu_ind);
+    MDFF_TRY(rmsd_of(c, c->e_tot, c->mu_ind, &rmsd));
+    hipLaunchKernelGGL(k_kO_resid, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->polar, c->pol, c->e_tot,
+                       c->mu_ind, find);
   }
   HIP_TRY(c, hipMemcpyAsync(c->hist, c->mu_ind, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
   t->u_pol = upol;
@@ -1042,6 +1114,7 @@ static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t, bool defer_f
   ev_rec(c, 2 * EVS_SCF);
   if (coul && c->linduced) {
     if (d.algo_moment_from_pola == 1) MDFF_TRY(scf_kO_v2(c, t));
+    else if (d.algo_moment_from_pola == 2) MDFF_TRY(scf_kO_v1(c, t));
     else MDFF_TRY(scf(c, t));
     hipLaunchKernelGGL(k_add3, dim3(div_up(n3, 256)), dim3(256), 0, c->stream, n3, c->dipia, c->mu_ind, c->mu);
   } else if (!c->mu_static) {
@@ -1255,6 +1328,7 @@ int mdff_ctx_create(const mdff_desc* d, const int32_t* itype, mdff_ctx** out) {
     return MDFF_ERR_BAD_ARG;
   if (d->lnmlj && (d->lbmhft || d->lbmhftd)) return MDFF_ERR_UNSUPPORTED;
   if (d->lmorse && (d->lnmlj || d->lbmhft || d->lbmhftd)) return MDFF_ERR_UNSUPPORTED;
+  if (d->algo_moment_from_pola < 0 || d->algo_moment_from_pola > 2) return MDFF_ERR_UNSUPPORTED;
   for (int i = 0; i < d->natm; ++i)
     if (itype[i] < 1 || itype[i] > d->ntype) return MDFF_ERR_BAD_ARG;
   for (int i = 0; i < 4096; ++i)

@@ -55,7 +55,9 @@ SUBROUTINE engforce_gpu_init
   d%pol_damp_k = reshape(pol_damp_k,[4096])
   d%conv_tol_ind = conv_tol_ind ; d%min_scf_pol_iter = min_scf_pol_iter
   d%max_scf_pol_iter = max_scf_pol_iter ; d%extrapolate_order = extrapolate_order
-  d%algo_moment_from_pola = merge(1,0,algo_moment_from_pola .eq. 'scf_kO_v2') ; d%omegakO = omegakO
+  d%algo_moment_from_pola = 0 ; d%omegakO = omegakO
+  if (algo_moment_from_pola .eq. 'scf_kO_v2') d%algo_moment_from_pola = 1
+  if (algo_moment_from_pola .eq. 'scf_kO_v1') d%algo_moment_from_pola = 2
   st = mdff_ctx_create(d, itype, gpu_ctx)
   call mdff_check(st, gpu_ctx, 'engforce_gpu_init')
 END SUBROUTINE engforce_gpu_init

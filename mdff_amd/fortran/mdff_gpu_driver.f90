@@ -350,10 +350,12 @@ contains
       end if
     end do
     if (any(lquad_polar) .or. any(quad /= 0.0_dp) .or. thole_functions .or. lwfc(1) /= 0 .or. &
-        (trim(algo_moment_from_pola) /= 'scf' .and. trim(algo_moment_from_pola) /= 'scf_kO_v2') .or. &
+        (trim(algo_moment_from_pola) /= 'scf' .and. trim(algo_moment_from_pola) /= 'scf_kO_v2' .and. &
+         trim(algo_moment_from_pola) /= 'scf_kO_v1') .or. &
+        (trim(algo_moment_from_pola) /= 'scf' .and. trim(algo_ext_dipole) /= 'aspc') .or. &
         lmorse .and. (lnmlj .or. lbmhft .or. lbmhftd)) then
       write (*, '(a)') 'ERROR option outside the engforce path of this library ' // &
-                       '(quadrupoles, Thole, Wannier centres, SCF variants other than scf and scf_kO_v2)'
+                       '(quadrupoles, Thole, Wannier centres, SCF variants other than scf, scf_kO_v1, scf_kO_v2)'
       stop 1
     end if
     if (lreduced) then
@@ -416,7 +418,9 @@ contains
     d%conv_tol_ind = conv_tol_ind
     d%min_scf_pol_iter = min_scf_pol_iter; d%max_scf_pol_iter = max_scf_pol_iter
     d%extrapolate_order = extrapolate_order
-    d%algo_moment_from_pola = merge(1, 0, trim(algo_moment_from_pola) == 'scf_kO_v2')
+    d%algo_moment_from_pola = 0
+    if (trim(algo_moment_from_pola) == 'scf_kO_v2') d%algo_moment_from_pola = 1
+    if (trim(algo_moment_from_pola) == 'scf_kO_v1') d%algo_moment_from_pola = 2
     d%omegakO = omegakO
   end subroutine fill_desc
 

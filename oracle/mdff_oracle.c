@@ -1247,6 +1247,55 @@ static void scf_kO_v2(orc_ctx *c, const double *s, double *mu_ind, double *u_pol
   free(dmin); free(zero); free(dd);
 }
 
+/* moment_from_pola_scf_kO_v1 (src/moment_from_pola.f90:1042-1320): the inner
+ * loop on the residual f = E - mu/alpha of the last iteration, no line
+ * minimum; rmsd of (E, mu) every iteration (:1236). */
+static void scf_kO_v1(orc_ctx *c, const double *s, double *mu_ind, double *u_pol, int *n_scf, double *rmsd_out) {
+  int n = c->n;
+  size_t N3 = 3 * (size_t)n;
+  const mdff_desc *d = &c->d;
+  double *Es = dalloc(N3), *Ei = dalloc(N3), *E = dalloc(N3), *zq = dalloc(n);
+  double *find = dalloc(N3), *dmu = dalloc(N3);
+  int ldip = 0;
+  for (int i = 0; i < 48; ++i) if (d->dip[i] != 0.0) ldip = 1;
+  es_task ts = {1, ldip, ldip, 1, 1, 0, 0, 0, 0.0};
+  es_out o;
+  multipole_es(c, s, c->q, c->dipia, ts, Es, NULL, &o);
+  memcpy(E, Es, sizeof(double) * N3);
+  memcpy(find, E, sizeof(double) * N3);
+  memset(mu_ind, 0, sizeof(double) * N3);
+  es_task ti = {0, 0, 1, 0, 1, 0, 0, 0, 0.0};
+  int iscf = 0;
+  double rmsd = 1.79769313486231571e308, upol = 0.0;
+  while ((iscf < d->max_scf_pol_iter && rmsd > d->conv_tol_ind) || iscf < d->min_scf_pol_iter) {
+    iscf++;
+    if (iscf != 1 || !d->calc_md) {
+      kO_inner(c, s, find, dmu);
+      for (size_t x = 0; x < N3; ++x) mu_ind[x] = mu_ind[x] + dmu[x];
+    } else {
+      aspc(c, mu_ind, E, 1);
+    }
+    upol = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) upol = upol + mu_ind[3 * i + a] * c->invpol[9 * i + a + 3 * b] * mu_ind[3 * i + b];
+    upol = upol * 0.5;
+    multipole_es(c, s, zq, mu_ind, ti, Ei, NULL, &o);
+    for (size_t x = 0; x < N3; ++x) E[x] = Es[x] + Ei[x];
+    if (iscf == 1 && d->calc_md) aspc(c, mu_ind, E, 2);
+    rmsd = rmsd_mu(c, E, mu_ind);
+    for (int i = 0; i < n; ++i) {
+      if (!d->ldip_polar[c->itype[i] - 1]) continue;
+      for (int a = 0; a < 3; ++a) find[3 * i + a] = E[3 * i + a] - mu_ind[3 * i + a] / c->pol[9 * i + a + 3 * a];
+    }
+  }
+  memcpy(&c->hist[0], mu_ind, sizeof(double) * N3);
+  *u_pol = upol;
+  *n_scf = iscf;
+  *rmsd_out = rmsd;
+  free(Es); free(Ei); free(E); free(zq); free(find); free(dmu);
+}
+
 /* ------------------------------------------------------------------------ */
 /* engforce                                                                  */
 /* ------------------------------------------------------------------------ */
@@ -1289,6 +1338,7 @@ int orc_engforce(orc_ctx *c, double *fx, double *fy, double *fz, mdff_terms *t) 
       double *fsx = dalloc(n), *fsy = dalloc(n), *fsz = dalloc(n);
       memcpy(fsx, c->fx, sizeof(double) * n); memcpy(fsy, c->fy, sizeof(double) * n); memcpy(fsz, c->fz, sizeof(double) * n);
       if (d->algo_moment_from_pola == 1) scf_kO_v2(c, s, mu_ind, &upol, &tm.n_scf, &tm.rmsd);
+      else if (d->algo_moment_from_pola == 2) scf_kO_v1(c, s, mu_ind, &upol, &tm.n_scf, &tm.rmsd);
       else scf(c, s, mu_ind, &upol, &tm.n_scf, &tm.rmsd);
       memcpy(c->fx, fsx, sizeof(double) * n); memcpy(c->fy, fsy, sizeof(double) * n); memcpy(c->fz, fsz, sizeof(double) * n);
       free(fsx); free(fsy); free(fsz);

@@ -224,6 +224,30 @@ def test_pim_scf_kO_v2_fp64_parity(calc_md):
 
 
 @pytest.mark.parametrize("calc_md", [0, 1])
+def test_pim_scf_kO_v1_fp64_parity(calc_md):
+    """moment_from_pola_scf_kO_v1 (src/moment_from_pola.f90:1042-1320): inner
+    loop on the previous residual, no line minimum, rmsd of (E, mu); same
+    iteration count, dipoles, fields, forces and energies as the oracle over
+    three calls (ASPC order 2 in MD)."""
+    s = _pim(tol=1e-8)
+    note(f"pim scf_kO_v1 calc_md={calc_md} n={s.natm}")
+    g, o = pair(s, calc_md=calc_md, doefield=1, algo_moment_from_pola=2, extrapolate_order=2)
+    for it in range(3):
+        g.set_itime(it)
+        o.set_itime(it)
+        fg, tg = g.engforce()
+        fo, to = o.engforce()
+        assert tg["n_scf"] == to["n_scf"]
+        assert rel_force_err(fg, fo) < TOL64
+        bad = compare_terms(tg, to, TOL64)
+        assert not bad, bad
+        mg, eg = g.get_dipoles()
+        mo, eo = o.get_dipoles()
+        assert rel(mg, mo) < TOL64
+        assert rel(eg, eo) < TOL64
+
+
+@pytest.mark.parametrize("calc_md", [0, 1])
 def test_pim_scf_fp64_parity(calc_md):
     """moment_from_pola_scf (src/moment_from_pola.f90:240-485): same
     iteration count, dipoles, fields, forces and energies."""
@@ -231,6 +255,33 @@ def test_pim_scf_fp64_parity(calc_md):
     note(f"pim scf calc_md={calc_md} n={s.natm}")
     g, o = pair(s, calc_md=calc_md, doefield=1)
     for it in range(3):  # successive calls exercise the ASPC history
+        g.set_itime(it)
+        o.set_itime(it)
+        fg, tg = g.engforce()
+        fo, to = o.engforce()
+        assert tg["n_scf"] == to["n_scf"]
+        assert rel_force_err(fg, fo) < TOL64
+        bad = compare_terms(tg, to, TOL64)
+        assert not bad, bad
+        mg, eg = g.get_dipoles()
+        mo, eo = o.get_dipoles()
+        assert rel(mg, mo) < TOL64
+        assert rel(eg, eo) < TOL64
+
+
+@pytest.mark.parametrize("algo", [0, 1, 2])
+def test_pim_with_static_dipoles_fp64_parity(algo):
+    """get_dipole_moments with static dipoles and polarisable ions
+    (src/moment_from_pola.f90:155): the static field of the SCF carries the
+    static dipoles' field (task_static = all, :313-330), mu_t = static +
+    induced; both SCF algorithms, three calls through the ASPC history."""
+    s = _pim(tol=1e-8)
+    d = s.desc
+    d.dip[0 + 16 * 0], d.dip[0 + 16 * 1], d.dip[0 + 16 * 2] = 0.05, -0.02, 0.03
+    d.dip[1 + 16 * 0], d.dip[1 + 16 * 1], d.dip[1 + 16 * 2] = -0.03, 0.04, 0.02
+    note(f"pim + static dipoles algo={algo} n={s.natm}")
+    g, o = pair(s, calc_md=1, doefield=1, algo_moment_from_pola=algo, extrapolate_order=2)
+    for it in range(3):
         g.set_itime(it)
         o.set_itime(it)
         fg, tg = g.engforce()
@@ -260,7 +311,7 @@ def test_pim_extrapolation_order_parity():
 
 # ------------------------------------------------------------------ mixed precision
 
-@pytest.mark.parametrize("case", ["sio2", "nacl", "pim", "pim_large_box", "wide_kxy"])
+@pytest.mark.parametrize("case", ["sio2", "nacl", "pim", "pim_large_box", "wide_kxy", "pim_static"])
 def test_mixed_precision_force_error(case):
     """FP32 pair / field arithmetic with FP64 accumulation (FP64 for the rows'
     inner entries): max_i |dF_i| / RMS_i |F_i| against the FP64 oracle below
@@ -278,6 +329,11 @@ def test_mixed_precision_force_error(case):
         # 1,728 ions in a 25 A box: dipoles of 1e-3..1e-1 e A, so the
         # dipole coefficients mu.b of the structure factor are ~1e-4
         s = configs.mgo_pim(cells=(6, 6, 6), seed=12, sigma=0.1)
+    elif case == "pim_static":
+        s = _pim()
+        d = s.desc
+        d.dip[0 + 16 * 0], d.dip[0 + 16 * 1], d.dip[0 + 16 * 2] = 0.05, -0.02, 0.03
+        d.dip[1 + 16 * 0], d.dip[1 + 16 * 1], d.dip[1 + 16 * 2] = -0.03, 0.04, 0.02
     else:
         s = _pim()
     note(f"mixed {case} n={s.natm}")

@@ -450,6 +450,57 @@ def test_scf_dipoles_solve_the_linear_system():
     assert t["n_scf"] >= 3
 
 
+def test_scf_with_static_dipoles_solves_the_linear_system():
+    """get_dipole_moments with static dipoles on every ion and polarisable
+    anions (src/moment_from_pola.f90:155, static field :313-330 with
+    task_static = all): mu_t = mu_static + mu_ind, and mu_ind solves
+    (alpha^-1 - T) mu_ind = E_q + E_mus, the damped field of the charges plus
+    the field of the static dipoles (the dipole-field operator applied to
+    them), built and solved here independently of the SCF iteration."""
+    s = _pim_small()
+    d = s.desc
+    d.calc_md = 0
+    d.dip[0 + 16 * 0], d.dip[0 + 16 * 1], d.dip[0 + 16 * 2] = 0.05, -0.02, 0.03
+    d.dip[1 + 16 * 0], d.dip[1 + 16 * 1], d.dip[1 + 16 * 2] = -0.03, 0.04, 0.02
+    e, f, t = _energy(s)
+    mu_tot, _ = e.get_dipoles()
+    L = O.lib()
+    dp = C.POINTER(C.c_double)
+    n = s.natm
+    mus = np.array([[d.dip[it - 1 + 16 * a] for a in range(3)] for it in s.itype])
+    pol = np.nonzero(s.itype == 2)[0]
+    ef_q = np.zeros((n, 3))
+    task = (C.c_int32 * 3)(1, 0, 0)
+    zero = np.zeros((n, 3))
+    fx, fy, fz = np.zeros(n), np.zeros(n), np.zeros(n)
+    tau = np.zeros(9)
+    ud, ur, us, usf = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+    L.orc_multipole_es(e.h, zero.ctypes.data_as(dp), task, 1, ef_q.ctypes.data_as(dp),
+                       fx.ctypes.data_as(dp), fy.ctypes.data_as(dp), fz.ctypes.data_as(dp),
+                       tau.ctypes.data_as(dp), C.byref(ud), C.byref(ur), C.byref(us), C.byref(usf))
+    ef_s = np.zeros((n, 3))
+    L.orc_dipole_field(e.h, np.ascontiguousarray(mus).ctypes.data_as(dp), ef_s.ctypes.data_as(dp))
+    # T by differences around a background with every polarisable dipole
+    # non-zero (the real-space mu-mu field joins two non-zero dipoles only)
+    bg = np.zeros((n, 3))
+    bg[pol] = np.random.default_rng(0).normal(0, 1, (len(pol), 3))
+    base = np.zeros((n, 3))
+    L.orc_dipole_field(e.h, bg.ctypes.data_as(dp), base.ctypes.data_as(dp))
+    m = 3 * len(pol)
+    T = np.zeros((m, m))
+    for c, (ia, a) in enumerate([(ia, a) for ia in pol for a in range(3)]):
+        pr = bg.copy()
+        pr[ia, a] += 1.0
+        out = np.zeros((n, 3))
+        L.orc_dipole_field(e.h, pr.ctypes.data_as(dp), out.ctypes.data_as(dp))
+        T[:, c] = (out - base)[pol].reshape(-1)
+    mu_ind = np.linalg.solve(np.eye(m) / 1.2 - T, (ef_q + ef_s)[pol].reshape(-1))
+    assert np.allclose((mu_tot - mus)[pol].reshape(-1), mu_ind, atol=1e-9)
+    nonpol = np.nonzero(s.itype != 2)[0]
+    assert np.array_equal(mu_tot[nonpol], mus[nonpol])
+    assert np.abs(ef_s[pol]).max() > 1e-3      # the static dipoles do move the solution
+
+
 def test_scf_kO_v2_reaches_the_same_fixed_point():
     """moment_from_pola_scf_kO_v2 (src/moment_from_pola.f90:1335-1639, inner
     loop :822-942, line minimum :2415-2455) solves the same equation as the
@@ -469,6 +520,27 @@ def test_scf_kO_v2_reaches_the_same_fixed_point():
     assert np.allclose(f2, f1, atol=1e-8 * np.abs(f1).max())
     assert abs(t2["u_coul"] - t1["u_coul"]) < 1e-9 * abs(t1["u_coul"])
     assert 2 <= t2["n_scf"] < t1["n_scf"]
+
+
+def test_scf_kO_v1_reaches_the_same_fixed_point():
+    """moment_from_pola_scf_kO_v1 (src/moment_from_pola.f90:1042-1320): the
+    inner iteration on the previous residual without a line minimum solves
+    the same equation as the plain SCF; it stops on rmsd(E, mu) like the
+    plain SCF, so the final residual is below the tolerance."""
+    s = _pim_small()
+    s.desc.calc_md = 0
+    e1, f1, t1 = _energy(s)
+    mu1, _ = e1.get_dipoles()
+    s2 = _pim_small()
+    s2.desc.calc_md = 0
+    s2.desc.algo_moment_from_pola = 2
+    e2, f2, t2 = _energy(s2)
+    mu2, _ = e2.get_dipoles()
+    assert np.allclose(mu2, mu1, atol=1e-9)
+    assert np.allclose(f2, f1, atol=1e-8 * np.abs(f1).max())
+    assert abs(t2["u_coul"] - t1["u_coul"]) < 1e-9 * abs(t1["u_coul"])
+    assert t2["rmsd"] <= s2.desc.conv_tol_ind
+    assert 3 <= t2["n_scf"] < t1["n_scf"]
 
 
 # ---------------------------------------------------------------- integrators
