@@ -97,7 +97,8 @@ constexpr int RAW_MOM = 22;      // musq, mutot(3), qtot(3)
 constexpr int RAW_EKIN = 29;     // sum m v^2
 constexpr int RAW_ENVT = 30;     // extended-system energy of a Nose-Hoover chain step
 constexpr int RAW_SCRATCH = 32;  // 7 moments of the SCF field (slot 0 only)
-constexpr int RAW_N = 40;
+constexpr int RAW_LJ = 40;       // 8 sums of the LJ pass beside BMHFT(D): u, vir, stress (6)
+constexpr int RAW_N = 48;
 constexpr int RAW_SLOTS = 1024;  // evaluations held before the sums go to the host
 
 // A deferred fixed-order final sum: out[0..ncol) = sums over nb block
@@ -223,6 +224,14 @@ struct mdff_ctx {
   double rc_vdw, rc_coul, alpha;
   int task_qq = 0, task_qd = 0, task_dd = 0, linduced = 0, ldip_static = 0;
   int sr_model = 0;                       // 0 none, 1 LJ, 2 BMH, 3 Morse
+  // LJ together with BMHFT(D) (src/engforce_driver.f90:130-141): the BMHFT(D)
+  // pass gives the forces (each reference routine zeroes them first), an LJ
+  // pass its energy, virial and stress into peratom_lj (forces into f_lj,
+  // unused), summed into raw[RAW_LJ..]
+  int sr_lj_extra = 0;
+  double* peratom_lj = nullptr;
+  double* f_lj = nullptr;
+  double* lj_part = nullptr;
   double u_tail = 0, p_tail = 0;
   std::vector<PairParam> pp;              // ntype x ntype, (it + ntype*jt)
 
@@ -455,4 +464,5 @@ void recip_free(mdff_ctx* c);
 int realspace_efg(mdff_ctx* c, double* out);
 int efg_alloc(mdff_ctx* c);
 int efg_eval(mdff_ctx* c);
+int realspace_lj_extra(mdff_ctx* c, double* f_scratch, double* peratom);
 void efg_free(mdff_ctx* c);

@@ -637,6 +637,7 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   if (c->linduced) { c->task_qd |= lqch; c->task_dd = 1; }
   if (d->lnmlj) c->sr_model = 1;
   if (d->lbmhft || d->lbmhftd) c->sr_model = 2;
+  c->sr_lj_extra = (d->lnmlj && (d->lbmhft || d->lbmhftd)) ? 1 : 0;
   if (d->lmorse) c->sr_model = 3;
   // per-atom type data (src/read_pos.f90:160-234)
   std::vector<int32_t> it0(n);
@@ -685,6 +686,11 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   DMALLOC(c->ameta, n);
   // per-atom scratch: NPA columns + f_sr, f_dir, ef_dir, ef_rec (3n each) + 8 extra columns
   DMALLOC(c->peratom, (size_t)(NPA + 12 + 8) * n);
+  if (c->sr_lj_extra) {
+    DMALLOC(c->peratom_lj, (size_t)NPA * n);
+    DMALLOC(c->f_lj, 3 * (size_t)n);
+    DMALLOC(c->lj_part, (size_t)8 * colsum_blocks(n));
+  }
   DMALLOC(c->d_pp, c->pp.size());
   HIP_TRY(c, hipMemcpy(c->d_pp, c->pp.data(), sizeof(PairParam) * c->pp.size(), hipMemcpyHostToDevice));
   DMALLOC(c->cell_of, n);
@@ -1089,7 +1095,7 @@ static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t, bool defer_f
   if (recip_first) {
     packed = d.lvnlist && c->lists_valid;
     MDFF_TRY(recip_tables_from_r(c, packed));  // kardir and the phase tables in one launch
-    ov = c->overlap && packed && !dip && !want_field && !c->prof_on;
+    ov = c->overlap && packed && !dip && !want_field && !c->prof_on && !c->sr_lj_extra;
     if (ov) HIP_TRY(c, hipEventRecord(c->ov_tab, c->stream));
     if (!c->mu_static) {
       // no induced dipoles: mu is the static set for the life of the context
@@ -1143,6 +1149,12 @@ static int engforce_launch(mdff_ctx* c, double* raw, mdff_terms* t, bool defer_f
     HIP_TRY(c, hipStreamWaitEvent(c->stream, c->ov_pair, 0));
   } else {
     MDFF_TRY(realspace_force_impl(c, f_sr(c), f_dir(c), ef_dir(c), want_field));
+  }
+  if (c->sr_lj_extra) {
+    int nbl = 0;
+    MDFF_TRY(realspace_lj_extra(c, c->f_lj, c->peratom_lj));
+    MDFF_TRY(reduce_cols_part(c, c->peratom_lj, 8, c->lj_part, &nbl));
+    MDFF_TRY(queue_final(c, c->lj_part, nbl, 8, raw + RAW_LJ));
   }
   ev_rec(c, 2 * EVS_REAL + 1);
   if (!recip_first) MDFF_TRY(recip_stage());
@@ -1213,6 +1225,17 @@ static void engforce_finish(const mdff_ctx* c, const double* raw, mdff_terms* t)
   if (c->sr_model == 2) { t->u_bmhft = usr; t->vir_bmhft = vir; }
   if (c->sr_model == 3) { t->u_morse = usr; t->vir_morse = vir; }
   if (c->sr_model) for (int k = 0; k < 9; ++k) { t->tau_nonb[k] = tnb[k]; t->tau[k] += tnb[k]; }
+  if (c->sr_lj_extra) {
+    // LJ beside BMHFT(D): its energy, virial and stress; tau_nonb stays the
+    // BMHFT(D) one, tau holds both (src/engforce_driver.f90:130-141)
+    const double* s = raw + RAW_LJ;
+    const double v[9] = {s[2], s[5], s[6], s[5], s[3], s[7], s[6], s[7], s[4]};
+    for (int k = 0; k < 9; ++k) t->tau[k] += 0.5 * v[k] * inv_vp;
+    t->u_lj = 0.5 * s[0];
+    t->vir_lj = 0.5 * s[1] / 3.0;
+    t->u_tail = c->u_tail;
+    t->p_tail = c->p_tail;
+  }
   if (d.lcoulomb) {
     static const double zero7[7] = {0, 0, 0, 0, 0, 0, 0};
     const double* mom = has_moments(c) ? raw + RAW_MOM : zero7;
@@ -1326,7 +1349,6 @@ int mdff_ctx_create(const mdff_desc* d, const int32_t* itype, mdff_ctx** out) {
   *out = nullptr;
   if (!d || !itype || d->abi_version != MDFF_ABI_VERSION || d->natm <= 0 || d->ntype <= 0 || d->ntype > 8)
     return MDFF_ERR_BAD_ARG;
-  if (d->lnmlj && (d->lbmhft || d->lbmhftd)) return MDFF_ERR_UNSUPPORTED;
   if (d->lmorse && (d->lnmlj || d->lbmhft || d->lbmhftd)) return MDFF_ERR_UNSUPPORTED;
   if (d->algo_moment_from_pola < 0 || d->algo_moment_from_pola > 2) return MDFF_ERR_UNSUPPORTED;
   for (int i = 0; i < d->natm; ++i)
@@ -1353,7 +1375,7 @@ int mdff_ctx_destroy(mdff_ctx* c) {
                 c->hist, c->frec, c->zeros, c->arec, c->ameta, c->peratom, c->red, c->red_part, c->d_pp, c->cell_of, c->cell_key,
                 c->cell_atoms, c->cell_atoms_in, c->cell_start, c->nl_count, c->nl_start, c->nl_j,
                 c->nl_flag, c->nl_count_in, c->nl_j_tmp, c->nl_flag_tmp, c->sort_tmp, c->d_flag, c->raw,
-                c->cx, c->cy, c->cz, c->fin_part, c->nhc_part, c->nhc_st};
+                c->cx, c->cy, c->cz, c->fin_part, c->nhc_part, c->nhc_st, c->peratom_lj, c->f_lj, c->lj_part};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c->polar) (void)hipFree(c->polar);

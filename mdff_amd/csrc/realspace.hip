@@ -708,6 +708,28 @@ int realspace_field(mdff_ctx* c, int mode, const double* mu_in, double* ef_out, 
   return MDFF_OK;
 }
 
+// The LJ pass of an LJ + BMHFT(D) evaluation: the n-m LJ pair kernel alone
+// (no Coulomb terms), its per-atom energy / virial / stress columns into
+// peratom and its forces into f_scratch, which the caller drops.
+int realspace_lj_extra(mdff_ctx* c, double* f_scratch, double* peratom) {
+  PairArgs a = make_args(c);
+  a.sr_model = 1;
+  a.want_stress = 1;
+  a.fsr = f_scratch;
+  a.peratom = peratom;
+  if (c->d.precision == MDFF_PREC_MIXED) {
+    if (c->rc_inner > 0.0) {
+      a.cnt_in = c->nl_count_in;
+      a.seg = 1;
+    }
+    launch_force<float, false>(c, a);
+  } else {
+    launch_force<double, false>(c, a);
+  }
+  HIP_TRY(c, hipGetLastError());
+  return MDFF_OK;
+}
+
 // Real-space electric field gradient of the engforce task set into out
 // (6 per atom, FP64); the atoms and lists of the current evaluation.
 int realspace_efg(mdff_ctx* c, double* out) {

@@ -19,8 +19,10 @@
  *  - Morse uses the analytic derivative of the reference energy
  *    (src/non_bonded.f90:1322-1325); the reference force (:1335) reads an
  *    uninitialised variable, and engforce never calls it.
- *  - LJ together with BMHFT(D) is rejected (the reference keeps only the second
- *    routine's forces, src/engforce_driver.f90:130-141).
+ *  - LJ together with BMHFT(D) follows the reference: both energies, virials
+ *    and stresses (tau = the sum, tau_nonb = the BMHFT(D) one), the forces of
+ *    BMHFT(D) alone, since each routine zeroes the forces first
+ *    (src/engforce_driver.f90:130-141, src/non_bonded.f90:699-701,1104-1106).
  *
  * How each part is pinned (tests/test_oracle_*.py):
  *  errfc           - bit-exact against the reference's own errfc compiled from
@@ -351,7 +353,6 @@ static void init_nonbonded(orc_ctx *c) {
 int orc_create(const mdff_desc *d, const int32_t *itype, orc_ctx **out) {
   *out = NULL;
   if (!d || !itype || d->natm <= 0 || d->ntype <= 0 || d->ntype > 16) return MDFF_ERR_BAD_ARG;
-  if (d->lnmlj && (d->lbmhft || d->lbmhftd)) return MDFF_ERR_UNSUPPORTED;
   if (d->lmorse && (d->lnmlj || d->lbmhft || d->lbmhftd)) return MDFF_ERR_UNSUPPORTED;
   orc_ctx *c = calloc(1, sizeof *c);
   c->d = *d;

@@ -13,7 +13,7 @@ import oracle_lib as O
 from mdff_amd import capi
 
 TERMS = ["u_lj", "u_bmhft", "u_morse", "u_coul", "u_pol", "u_dir", "u_rec", "u_self", "u_surf",
-         "vir_lj", "vir_bmhft", "vir_morse"]
+         "vir_lj", "vir_bmhft", "vir_morse", "u_tail", "p_tail"]
 
 
 def note(msg: str) -> None:
@@ -84,7 +84,7 @@ def compare_terms(tg, to, tol, keys=TERMS):
         e = abs(a - b) / max(abs(b), 1e-12)
         if e > tol:
             bad[k] = (a, b, e)
-    for k in ("tau_nonb", "tau_coul"):
+    for k in ("tau_nonb", "tau_coul", "tau"):
         if np.abs(to[k]).max() > 0 and rel(tg[k], to[k]) > tol:
             bad[k] = (tg[k], to[k], rel(tg[k], to[k]))
     return bad

@@ -49,7 +49,11 @@ def test_bad_arguments_are_rejected():
     rc = lib.mdff_ctx_create(C.byref(s.desc), it.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(h))
     assert rc == 1
     d = s.desc
-    d.lnmlj = 1
+    d.lmorse = 1
     d.lbmhft = 1
+    rc = lib.mdff_ctx_create(C.byref(d), s.itype.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(h))
+    assert rc == 7
+    d.lmorse = 0
+    d.algo_moment_from_pola = 3
     rc = lib.mdff_ctx_create(C.byref(d), s.itype.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(h))
     assert rc == 7

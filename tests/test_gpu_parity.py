@@ -111,7 +111,7 @@ def _check_engforce(s, tol=TOL64, precision=capi.PREC_FP64, **over):
 
 
 @pytest.mark.parametrize("case", ["nacl_bmh", "sio2_buck", "fcc_lj", "morse", "bmhftd", "triclinic",
-                                  "lsurf", "nolist", "large_kmax", "aniso_kmax"])
+                                  "lsurf", "nolist", "large_kmax", "aniso_kmax", "lj_with_bmh"])
 def test_engforce_fp64_parity(case):
     """engforce (src/engforce_driver.f90:102-190), FP64 form vs the oracle."""
     over = {}
@@ -145,6 +145,14 @@ def test_engforce_fp64_parity(case):
     elif case == "lsurf":
         s = configs.nacl(ncell=2)
         over["lsurf"] = 1
+    elif case == "lj_with_bmh":
+        # both short-range routines (src/engforce_driver.f90:130-141): both
+        # energies and stresses, the BMHFT forces (and u_tail of the LJ)
+        s = configs.nacl(ncell=2, seed=4)
+        over["lnmlj"] = 1
+        for (i, j) in [(1, 1), (1, 2), (2, 2)]:
+            capi.set_pair(s.desc, "epslj", i, j, 0.01 * (i + j))
+            capi.set_pair(s.desc, "sigmalj", i, j, 2.0 + 0.2 * (i + j))
     elif case == "large_kmax":
         # 2 kmax + 1 = 67 nz values: the structure factor runs two nz groups
         s = configs.nacl(ncell=2, alpha=0.9, kmax=33)

@@ -374,6 +374,34 @@ def test_efg_without_doefg_is_the_self_term():
     assert np.allclose(v, want[:, None, None] * np.eye(3)[None], rtol=1e-14, atol=0)
 
 
+def test_lj_with_bmh_keeps_both_energies_and_the_bmh_forces():
+    """engforce with lnmlj and lbmhft (src/engforce_driver.f90:130-141): each
+    routine zeroes the forces first (src/non_bonded.f90:699-701,1104-1106), so
+    the forces are those of BMHFT alone; u_lj, u_bmhft and their virials are
+    those of the separate runs, tau is the sum of both stresses and tau_nonb
+    the BMHFT one."""
+    def run(lj, bmh):
+        s = configs.nacl(ncell=2, seed=4)
+        d = s.desc
+        d.lcoulomb = 0
+        d.lnmlj, d.lbmhft = lj, bmh
+        for (i, j) in [(1, 1), (1, 2), (2, 2)]:
+            set_pair(d, "epslj", i, j, 0.01 * (i + j))
+            set_pair(d, "sigmalj", i, j, 2.0 + 0.2 * (i + j))
+        return _energy(s)
+    _, f_lj, t_lj = run(1, 0)
+    _, f_b, t_b = run(0, 1)
+    _, f, t = run(1, 1)
+    assert np.array_equal(f, f_b)
+    assert np.abs(f_lj).max() > 1e-3            # the LJ forces exist, and are dropped
+    for k in ("u_lj", "vir_lj", "u_tail", "p_tail"):
+        assert t[k] == t_lj[k], k
+    for k in ("u_bmhft", "vir_bmhft"):
+        assert t[k] == t_b[k], k
+    assert np.array_equal(t["tau_nonb"], t_b["tau_nonb"])
+    assert np.allclose(t["tau"], t_lj["tau"] + t_b["tau"], rtol=1e-15, atol=0)
+
+
 def test_nacl_madelung_constant():
     """Perfect rock salt: U = -N M q^2 / (2 r0) with M = 1.747564594633...,
     forces zero; converged Ewald parameters."""
