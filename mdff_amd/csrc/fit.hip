@@ -301,42 +301,82 @@ __device__ void struct_factor(const FitArgs& a, const FitSnap& S, int kb0, int k
   __syncthreads();
 }
 
-// per atom j: im3 = sum_k k Im(C1_k e^{-ik r_j}), re3 = sum_k k Re(C2_k e^{-ik r_j}),
-// re6 = sum_k k k Re(C3_k e^{-ik r_j}) (xx yy zz xy xz yz); flags pick the sums
+// the k data of one gather block staged in LDS (FIT_KC entries): k vector,
+// packed (nx, ny + 512, nz + 512) and the C1 / C2 / C3 pairs the phase needs
+struct GatherStage {
+  double *kx, *ky, *kz, *c1, *c2, *c3;
+  int* n;
+};
+
+__device__ __forceinline__ GatherStage gather_stage_of(double* lds) {
+  GatherStage g;
+  g.kx = lds;
+  g.ky = lds + FIT_KC;
+  g.kz = lds + 2 * FIT_KC;
+  g.c1 = lds + 3 * FIT_KC;
+  g.c2 = lds + 5 * FIT_KC;
+  g.c3 = lds + 7 * FIT_KC;
+  g.n = reinterpret_cast<int*>(lds + 9 * FIT_KC);
+  return g;
+}
+// lanes per atom of the iteration's real-space dipole field (kf_dipreal)
+constexpr int kDipLanes = 4;
+constexpr size_t kGatherLds = sizeof(double) * 9 * FIT_KC + sizeof(int) * FIT_KC;
+
+// the block's k-vectors kb0..kb1 (kb1 - kb0 <= FIT_KC) into LDS, by every
+// thread of the block; c1 / c2i / c3: kws slots (-1: not needed)
+__device__ void gather_stage(const FitArgs& a, const FitSnap& S, int kb0, int kb1, int c1, int c2i, int c3,
+                             const GatherStage& g) {
+  for (int k = kb0 + (int)threadIdx.x; k < kb1; k += blockDim.x) {
+    const int kl = k - kb0, kg = S.k0 + k;
+    g.kx[kl] = a.kv[5 * kg];
+    g.ky[kl] = a.kv[5 * kg + 1];
+    g.kz[kl] = a.kv[5 * kg + 2];
+    g.n[kl] = a.kn[3 * kg] | ((a.kn[3 * kg + 1] + 512) << 10) | ((a.kn[3 * kg + 2] + 512) << 20);
+    const double* K = a.kws + (size_t)FIT_KWS * kg;
+    if (c1 >= 0) { g.c1[2 * kl] = K[c1]; g.c1[2 * kl + 1] = K[c1 + 1]; }
+    if (c2i >= 0) { g.c2[2 * kl] = K[c2i]; g.c2[2 * kl + 1] = K[c2i + 1]; }
+    if (c3 >= 0) { g.c3[2 * kl] = K[c3]; g.c3[2 * kl + 1] = K[c3 + 1]; }
+  }
+  __syncthreads();
+}
+
+// per atom j over the staged k-vectors: im3 = sum_k k Im(C1_k e^{-ik r_j}),
+// re3 = sum_k k Re(C2_k e^{-ik r_j}), re6 = sum_k k k Re(C3_k e^{-ik r_j})
+// (xx yy zz xy xz yz); flags pick the sums
 template <typename T>
-__device__ void gather(const FitArgs& a, const FitSnap& S, const c2<T>* tab, int j, int kb0, int kb1, int c1,
-                       int c2i, int c3, double im3[3], double re3[3], double re6[6]) {
+__device__ void gather(const FitSnap& S, const c2<T>* tab, int j, int nkl, bool do1, bool do2, bool do3,
+                       const GatherStage& g, double im3[3], double re3[3], double re6[6]) {
   for (int c = 0; c < 3; ++c) im3[c] = re3[c] = 0.0;
   for (int c = 0; c < 6; ++c) re6[c] = 0.0;
   // the k list runs (nx, ny) rows with nz innermost: X Y of the atom is
   // formed once per row (the same products as phase())
-  int pnx = -1, pny = 0;
+  int prow = -1;
   c2<T> XY{T(1), T(0)};
-  for (int k = kb0; k < kb1; ++k) {
-    const int kg = S.k0 + k;
-    const double kx = a.kv[5 * kg], ky = a.kv[5 * kg + 1], kz = a.kv[5 * kg + 2];
-    const int nx = a.kn[3 * kg], ny = a.kn[3 * kg + 1], nz = a.kn[3 * kg + 2];
-    if (nx != pnx || ny != pny) {
+  for (int kl = 0; kl < nkl; ++kl) {
+    const double kx = g.kx[kl], ky = g.ky[kl], kz = g.kz[kl];
+    const int pk = g.n[kl];
+    const int nx = pk & 1023, ny = ((pk >> 10) & 1023) - 512, nz = ((pk >> 20) & 1023) - 512;
+    const int row = pk & ((1 << 20) - 1);
+    if (row != prow) {
       XY = cm(tabv(tab, S.n, nx, j, false), tabv(tab, S.n, S.Kx + 1 + (ny < 0 ? -ny : ny), j, ny < 0));
-      pnx = nx;
-      pny = ny;
+      prow = row;
     }
     const c2<T> e = cm(XY, tabv(tab, S.n, S.Kx + S.Ky + 2 + (nz < 0 ? -nz : nz), j, nz < 0));   // e^{+ik r_j}
-    const double* K = a.kws + (size_t)FIT_KWS * kg;
-    if (c1 >= 0) {   // Im(C e^-) = Ci er - Cr ei
-      const double v = (double)((T)K[c1 + 1] * e.re - (T)K[c1] * e.im);
+    if (do1) {   // Im(C e^-) = Ci er - Cr ei
+      const double v = (double)((T)g.c1[2 * kl + 1] * e.re - (T)g.c1[2 * kl] * e.im);
       im3[0] += kx * v;
       im3[1] += ky * v;
       im3[2] += kz * v;
     }
-    if (c2i >= 0) {  // Re(C e^-) = Cr er + Ci ei
-      const double v = (double)((T)K[c2i] * e.re + (T)K[c2i + 1] * e.im);
+    if (do2) {  // Re(C e^-) = Cr er + Ci ei
+      const double v = (double)((T)g.c2[2 * kl] * e.re + (T)g.c2[2 * kl + 1] * e.im);
       re3[0] += kx * v;
       re3[1] += ky * v;
       re3[2] += kz * v;
     }
-    if (c3 >= 0) {
-      const double v = (double)((T)K[c3] * e.re + (T)K[c3 + 1] * e.im);
+    if (do3) {
+      const double v = (double)((T)g.c3[2 * kl] * e.re + (T)g.c3[2 * kl + 1] * e.im);
       re6[0] += kx * kx * v;
       re6[1] += ky * ky * v;
       re6[2] += kz * kz * v;
@@ -401,34 +441,50 @@ __device__ void field_static(const FitArgs& a, const FitSnap& S, int ib0, int ib
 // field of a dipole set v (undamped, real space only), at polarisable atoms,
 // from polarisable atoms (the mu-mu pairs of src/ewald_dir.f90:557-593)
 template <typename T>
-__device__ void field_dip_real(const FitArgs& a, const FitSnap& S, int ib0, int ib1, const double* v, double* out) {
+__device__ void field_dip_real(const FitArgs& a, const FitSnap& S, int ib0, int ib1, const double* v, double* out,
+                               double* red) {
   const double* sx = a.sx + S.a0;
   const double* sy = a.sy + S.a0;
   const double* sz = a.sz + S.a0;
   const int* it = a.itype + S.a0;
   const T alpha = (T)S.alpha;
-  for (int i = ib0 + threadIdx.x; i < ib1; i += blockDim.x) {
-    double e0 = 0, e1 = 0, e2 = 0;
-    if (a.alpha_t[it[i]] != 0.0) {
-      for (int e = a.row[S.row0 + i]; e < a.row[S.row0 + i + 1]; ++e) {
-        const int j = a.nl[S.nl0 + e];
-        if (a.alpha_t[it[j]] == 0.0) continue;
-        const Geo G = min_image(S, sx, sy, sz, i, j);
-        if (G.r2 > S.rc2_c) continue;
-        const T d = sqrt((T)G.r2);
-        const Erf<T> E = erf_terms<T>(d, (T)G.r2, alpha);
-        const T rx = (T)G.rx, ry = (T)G.ry, rz = (T)G.rz;
-        const T vx = (T)v[3 * j], vy = (T)v[3 * j + 1], vz = (T)v[3 * j + 2];
-        const T rv = rx * vx + ry * vy + rz * vz;
-        const T c3 = T(3) * E.F2 * E.dm5 * rv, c1 = E.F1 * E.dm3;
-        e0 += (double)(c3 * rx - c1 * vx);
-        e1 += (double)(c3 * ry - c1 * vy);
-        e2 += (double)(c3 * rz - c1 * vz);
-      }
+  // gridDim.y lanes per atom stride its row; blockIdx.y picks the block's
+  // share of atoms (blockDim / gridDim.y of them)
+  const int L = (int)gridDim.y, sub = (int)threadIdx.x % L, per = (int)blockDim.x / L;
+  const int i = ib0 + (int)blockIdx.y * per + (int)threadIdx.x / L;
+  double e0 = 0, e1 = 0, e2 = 0;
+  if (i < ib1 && a.alpha_t[it[i]] != 0.0) {
+    for (int e = a.row[S.row0 + i] + sub; e < a.row[S.row0 + i + 1]; e += L) {
+      const int j = a.nl[S.nl0 + e];
+      if (a.alpha_t[it[j]] == 0.0) continue;
+      const Geo G = min_image(S, sx, sy, sz, i, j);
+      if (G.r2 > S.rc2_c) continue;
+      const T d = sqrt((T)G.r2);
+      const Erf<T> E = erf_terms<T>(d, (T)G.r2, alpha);
+      const T rx = (T)G.rx, ry = (T)G.ry, rz = (T)G.rz;
+      const T vx = (T)v[3 * j], vy = (T)v[3 * j + 1], vz = (T)v[3 * j + 2];
+      const T rv = rx * vx + ry * vy + rz * vz;
+      const T c3 = T(3) * E.F2 * E.dm5 * rv, c1 = E.F1 * E.dm3;
+      e0 += (double)(c3 * rx - c1 * vx);
+      e1 += (double)(c3 * ry - c1 * vy);
+      e2 += (double)(c3 * rz - c1 * vz);
     }
-    out[3 * i] = e0;
-    out[3 * i + 1] = e1;
-    out[3 * i + 2] = e2;
+  }
+  // the L lane sums of an atom added in lane order
+  red[3 * threadIdx.x] = e0;
+  red[3 * threadIdx.x + 1] = e1;
+  red[3 * threadIdx.x + 2] = e2;
+  __syncthreads();
+  if (sub == 0 && i < ib1) {
+    double f0 = 0, f1 = 0, f2 = 0;
+    for (int l = 0; l < L; ++l) {
+      f0 += red[3 * (threadIdx.x + l)];
+      f1 += red[3 * (threadIdx.x + l) + 1];
+      f2 += red[3 * (threadIdx.x + l) + 2];
+    }
+    out[3 * i] = f0;
+    out[3 * i + 1] = f1;
+    out[3 * i + 2] = f2;
   }
 }
 
@@ -669,10 +725,11 @@ __global__ void __launch_bounds__(256) kf_update(FitArgs a) {
 // real-space dipole field of v = ws[slot] into ws[W_TMP] (iterating snapshots)
 template <typename T>
 __global__ void __launch_bounds__(256) kf_dipreal(FitArgs a, int slot) {
+  extern __shared__ double red[];
   const int4 B = a.ablk[blockIdx.x];
   if (!a.active[B.x]) return;
   const FitSnap S = a.snaps[B.x];
-  field_dip_real<T>(a, S, B.y, B.z, ws_of(a, S, slot), ws_of(a, S, W_TMP));
+  field_dip_real<T>(a, S, B.y, B.z, ws_of(a, S, slot), ws_of(a, S, W_TMP), red);
 }
 
 // structure factors of the block's k-vectors and their per-k follow-ups.
@@ -765,19 +822,22 @@ __global__ void __launch_bounds__(256) kf_sf(FitArgs a, int mode) {
 // polarisable atoms.
 template <typename T, int PH>
 __global__ void __launch_bounds__(256) kf_gpart(FitArgs a) {
+  extern __shared__ double red[];
   const int4 B = a.gblk[blockIdx.x];
   if (PH == 1 && !a.active[B.x]) return;
   const FitSnap S = a.snaps[B.x];
   const c2<T>* tab = reinterpret_cast<const c2<T>*>(a.tab) + S.tab0;
   const int* it = a.itype + S.a0;
+  const int k1 = min(S.nk, B.z + FIT_KC);
+  // PH 0, 1: im3 of C1; 2: + re6 of C1; 3: im3 of C3, re3 of C2, re6 of C1
+  const int c1 = PH == 3 ? K_C3 : K_C1, c2i = PH == 3 ? K_C2 : -1, c3 = PH >= 2 ? K_C1 : -1;
+  const GatherStage g = gather_stage_of(red);
+  gather_stage(a, S, B.z, k1, c1, c2i, c3, g);
   const int j = B.y + threadIdx.x;
   if (j >= min(S.n, B.y + (int)blockDim.x)) return;
   if (PH != 2 && a.alpha_t[it[j]] == 0.0) return;
-  const int k1 = min(S.nk, B.z + FIT_KC);
   double im3[3], re3[3], re6[6];
-  if (PH == 0 || PH == 1) gather<T>(a, S, tab, j, B.z, k1, K_C1, -1, -1, im3, re3, re6);
-  else if (PH == 2) gather<T>(a, S, tab, j, B.z, k1, K_C1, -1, K_C1, im3, re3, re6);
-  else gather<T>(a, S, tab, j, B.z, k1, K_C3, K_C2, K_C1, im3, re3, re6);
+  gather<T>(S, tab, j, k1 - B.z, true, c2i >= 0, c3 >= 0, g, im3, re3, re6);
   double* r = a.gp + ((size_t)a.snap_g0[B.x] + (size_t)B.w * S.n + j) * FIT_GC;
   for (int c = 0; c < 3; ++c) r[c] = im3[c];
   if (PH == 3)
@@ -1251,10 +1311,10 @@ static int fit_run(mdff_fit* f, FitArgs a, size_t tsz) {
       a.ws_dst = vslot;
       a.ws_src = W_ET;
       hipLaunchKernelGGL(kf_update, ga, b, 0, st, a);
-      hipLaunchKernelGGL((kf_dipreal<T>), ga, b, 0, st, a, vslot);
+      hipLaunchKernelGGL((kf_dipreal<T>), dim3(f->nab, kDipLanes), b, 3 * red, st, a, vslot);
       hipLaunchKernelGGL((kf_sfpart<T>), gs, b, shm_sf, st, a, 1, vslot, -1);
       hipLaunchKernelGGL((kf_sf<T>), gk, b, red, st, a, 1);
-      hipLaunchKernelGGL((kf_gpart<T, PH_ITER>), gg, b, 0, st, a);
+      hipLaunchKernelGGL((kf_gpart<T, PH_ITER>), gg, b, kGatherLds, st, a);
       hipLaunchKernelGGL((kf_gather<T, PH_ITER>), ga, b, red, st, a, vslot, bslot);
       hipLaunchKernelGGL(kf_ctl, dim3(1), dim3(64), 0, st, a, what);
       FIT_TRY(f, hipGetLastError());
@@ -1269,7 +1329,7 @@ static int fit_run(mdff_fit* f, FitArgs a, size_t tsz) {
   hipLaunchKernelGGL((kf_static<T>), ga, b, 0, st, a);
   hipLaunchKernelGGL((kf_sfpart<T>), gs, b, shm_sf, st, a, 0, -1, -1);
   hipLaunchKernelGGL((kf_sf<T>), gk, b, red, st, a, 0);
-  hipLaunchKernelGGL((kf_gpart<T, PH_STATIC>), gg, b, 0, st, a);
+  hipLaunchKernelGGL((kf_gpart<T, PH_STATIC>), gg, b, kGatherLds, st, a);
   hipLaunchKernelGGL((kf_gather<T, PH_STATIC>), ga, b, red, st, a, -1, -1);
   FIT_TRY(f, hipGetLastError());
   FIT_RC(iterate(W_MU, W_ES, C_SCF));
@@ -1277,7 +1337,7 @@ static int fit_run(mdff_fit* f, FitArgs a, size_t tsz) {
   hipLaunchKernelGGL((kf_pair<T, false>), ga, b, red, st, a);
   hipLaunchKernelGGL((kf_sfpart<T>), gs, b, shm_sf, st, a, 2, (int)W_MU, -1);
   hipLaunchKernelGGL((kf_sf<T>), gk, b, red, st, a, 2);
-  hipLaunchKernelGGL((kf_gpart<T, PH_FORCE>), gg, b, 0, st, a);
+  hipLaunchKernelGGL((kf_gpart<T, PH_FORCE>), gg, b, kGatherLds, st, a);
   hipLaunchKernelGGL((kf_gather<T, PH_FORCE>), ga, b, red, st, a, -1, -1);
   hipLaunchKernelGGL(kf_ctl, dim3(1), dim3(64), 0, st, a, (int)C_FORCE);
   FIT_TRY(f, hipGetLastError());
@@ -1287,7 +1347,7 @@ static int fit_run(mdff_fit* f, FitArgs a, size_t tsz) {
   hipLaunchKernelGGL(kf_ctl, dim3(1), dim3(64), 0, st, a, (int)C_PAIRG);
   hipLaunchKernelGGL((kf_sfpart<T>), gs, b, shm_sf, st, a, 3, (int)W_MU, (int)W_G);
   hipLaunchKernelGGL((kf_sf<T>), gk, b, red, st, a, 3);
-  hipLaunchKernelGGL((kf_gpart<T, PH_H>), gg, b, 0, st, a);
+  hipLaunchKernelGGL((kf_gpart<T, PH_H>), gg, b, kGatherLds, st, a);
   hipLaunchKernelGGL((kf_gather<T, PH_H>), ga, b, red, st, a, -1, -1);
   hipLaunchKernelGGL(kf_ctl, dim3(1), dim3(64), 0, st, a, (int)C_HN);
   FIT_TRY(f, hipGetLastError());

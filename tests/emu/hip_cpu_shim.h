@@ -85,8 +85,9 @@ void emu_launch_f(F fn, dim3 g, dim3 b) {
     grp.done.resize(b.x);
     g_grp = &grp;
     g_shared = lds.data();
-    for (unsigned bx; (bx = next.fetch_add(1)) < g.x;) {
-      blockIdx = dim3(bx);
+    const unsigned nblocks = g.x * g.y * g.z;
+    for (unsigned bx; (bx = next.fetch_add(1)) < nblocks;) {
+      blockIdx = dim3(bx % g.x, (bx / g.x) % g.y, bx / (g.x * g.y));
       const uintptr_t p = reinterpret_cast<uintptr_t>(&fn);
       for (unsigned t = 0; t < b.x; ++t) {
         getcontext(&grp.ctx[t]);
@@ -102,7 +103,7 @@ void emu_launch_f(F fn, dim3 g, dim3 b) {
         for (unsigned t = 0; t < b.x; ++t) {
           if (grp.done[t]) continue;
           grp.cur = (int)t;
-          threadIdx = dim3(t);
+          threadIdx = dim3(t, 0, 0);
           swapcontext(&grp.sched, &grp.ctx[t]);
           if (!grp.done[t]) ++left;
         }
