@@ -265,6 +265,10 @@ __device__ void struct_factor(const FitArgs& a, const FitSnap& S, int kb0, int k
       }
       __syncthreads();
       if (kok) {
+        // FP32 form: the tile's (at most FIT_CA) terms summed in T, the tile
+        // sums in FP64; the iteration's k.v in T as well.  FP64 form: every
+        // term added in FP64.
+        T tr = T(0), ti = T(0);
         for (int jj = 0; jj < nc; ++jj) {
           const c2<T>* row = lt + jj * ncol;
           const c2<T> X = row[nx];
@@ -274,22 +278,38 @@ __device__ void struct_factor(const FitArgs& a, const FitSnap& S, int kb0, int k
           Z.im *= sz;
           const c2<T> e = cm(cm(X, Y), Z);
           const double* A = la + 7 * jj;
-          double ar = 0.0, ai = 0.0;
-          if (mode == 0) {
-            ar = A[0];
-          } else if (mode == 1) {
-            ai = kx * A[1] + ky * A[2] + kz * A[3];
+          T art = T(0), ait = T(0);
+          if (mode == 1 && sizeof(T) == 4) {
+            ait = (T)kx * (T)A[1] + (T)ky * (T)A[2] + (T)kz * (T)A[3];
           } else {
-            ar = A[0];
-            ai = kx * A[1] + ky * A[2] + kz * A[3];
-            if (mode == 3) {
-              const double kg3 = kx * A[4] + ky * A[5] + kz * A[6];
-              ar *= kg3;
-              ai *= kg3;
+            double ar = 0.0, ai = 0.0;
+            if (mode == 0) {
+              ar = A[0];
+            } else if (mode == 1) {
+              ai = kx * A[1] + ky * A[2] + kz * A[3];
+            } else {
+              ar = A[0];
+              ai = kx * A[1] + ky * A[2] + kz * A[3];
+              if (mode == 3) {
+                const double kg3 = kx * A[4] + ky * A[5] + kz * A[6];
+                ar *= kg3;
+                ai *= kg3;
+              }
             }
+            art = (T)ar;
+            ait = (T)ai;
           }
-          sr += (double)((T)ar * e.re - (T)ai * e.im);
-          si += (double)((T)ar * e.im + (T)ai * e.re);
+          if constexpr (sizeof(T) == 4) {
+            tr += art * e.re - ait * e.im;
+            ti += art * e.im + ait * e.re;
+          } else {
+            sr += (double)(art * e.re - ait * e.im);
+            si += (double)(art * e.im + ait * e.re);
+          }
+        }
+        if constexpr (sizeof(T) == 4) {
+          sr += (double)tr;
+          si += (double)ti;
         }
       }
     }
@@ -353,36 +373,84 @@ __device__ void gather(const FitSnap& S, const c2<T>* tab, int j, int nkl, bool 
   // formed once per row (the same products as phase())
   int prow = -1;
   c2<T> XY{T(1), T(0)};
-  for (int kl = 0; kl < nkl; ++kl) {
-    const double kx = g.kx[kl], ky = g.ky[kl], kz = g.kz[kl];
-    const int pk = g.n[kl];
-    const int nx = pk & 1023, ny = ((pk >> 10) & 1023) - 512, nz = ((pk >> 20) & 1023) - 512;
-    const int row = pk & ((1 << 20) - 1);
-    if (row != prow) {
-      XY = cm(tabv(tab, S.n, nx, j, false), tabv(tab, S.n, S.Kx + 1 + (ny < 0 ? -ny : ny), j, ny < 0));
-      prow = row;
+  // Z of four k-vectors loaded ahead of their use (independent loads in
+  // flight); the k-vectors themselves in order.  FP32 form: the terms of the
+  // four summed in T (P), those sums in FP64; FP64 form: each term in FP64.
+  constexpr int U = 4;
+  constexpr bool F32 = sizeof(T) == 4;
+  for (int kb = 0; kb < nkl; kb += U) {
+    c2<T> Zs[U];
+    int pks[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      pks[u] = g.n[min(kb + u, nkl - 1)];
+      const int nz = ((pks[u] >> 20) & 1023) - 512;
+      Zs[u] = tabv(tab, S.n, S.Kx + S.Ky + 2 + (nz < 0 ? -nz : nz), j, nz < 0);
     }
-    const c2<T> e = cm(XY, tabv(tab, S.n, S.Kx + S.Ky + 2 + (nz < 0 ? -nz : nz), j, nz < 0));   // e^{+ik r_j}
-    if (do1) {   // Im(C e^-) = Ci er - Cr ei
-      const double v = (double)((T)g.c1[2 * kl + 1] * e.re - (T)g.c1[2 * kl] * e.im);
-      im3[0] += kx * v;
-      im3[1] += ky * v;
-      im3[2] += kz * v;
+    T P[12] = {};
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int kl = kb + u;
+      if (kl >= nkl) break;
+      const double kx = g.kx[kl], ky = g.ky[kl], kz = g.kz[kl];
+      const int pk = pks[u];
+      const int nx = pk & 1023, ny = ((pk >> 10) & 1023) - 512;
+      const int row = pk & ((1 << 20) - 1);
+      if (row != prow) {
+        XY = cm(tabv(tab, S.n, nx, j, false), tabv(tab, S.n, S.Kx + 1 + (ny < 0 ? -ny : ny), j, ny < 0));
+        prow = row;
+      }
+      const c2<T> e = cm(XY, Zs[u]);   // e^{+ik r_j}
+      const T tx = (T)kx, ty = (T)ky, tz = (T)kz;
+      if (do1) {   // Im(C e^-) = Ci er - Cr ei
+        const T vt = (T)g.c1[2 * kl + 1] * e.re - (T)g.c1[2 * kl] * e.im;
+        if constexpr (F32) {
+          P[0] += tx * vt;
+          P[1] += ty * vt;
+          P[2] += tz * vt;
+        } else {
+          im3[0] += kx * vt;
+          im3[1] += ky * vt;
+          im3[2] += kz * vt;
+        }
+      }
+      if (do2) {  // Re(C e^-) = Cr er + Ci ei
+        const T vt = (T)g.c2[2 * kl] * e.re + (T)g.c2[2 * kl + 1] * e.im;
+        if constexpr (F32) {
+          P[3] += tx * vt;
+          P[4] += ty * vt;
+          P[5] += tz * vt;
+        } else {
+          re3[0] += kx * vt;
+          re3[1] += ky * vt;
+          re3[2] += kz * vt;
+        }
+      }
+      if (do3) {
+        const T vt = (T)g.c3[2 * kl] * e.re + (T)g.c3[2 * kl + 1] * e.im;
+        if constexpr (F32) {
+          P[6] += tx * tx * vt;
+          P[7] += ty * ty * vt;
+          P[8] += tz * tz * vt;
+          P[9] += tx * ty * vt;
+          P[10] += tx * tz * vt;
+          P[11] += ty * tz * vt;
+        } else {
+          re6[0] += kx * kx * vt;
+          re6[1] += ky * ky * vt;
+          re6[2] += kz * kz * vt;
+          re6[3] += kx * ky * vt;
+          re6[4] += kx * kz * vt;
+          re6[5] += ky * kz * vt;
+        }
+      }
     }
-    if (do2) {  // Re(C e^-) = Cr er + Ci ei
-      const double v = (double)((T)g.c2[2 * kl] * e.re + (T)g.c2[2 * kl + 1] * e.im);
-      re3[0] += kx * v;
-      re3[1] += ky * v;
-      re3[2] += kz * v;
-    }
-    if (do3) {
-      const double v = (double)((T)g.c3[2 * kl] * e.re + (T)g.c3[2 * kl + 1] * e.im);
-      re6[0] += kx * kx * v;
-      re6[1] += ky * ky * v;
-      re6[2] += kz * kz * v;
-      re6[3] += kx * ky * v;
-      re6[4] += kx * kz * v;
-      re6[5] += ky * kz * v;
+    if constexpr (F32) {
+      for (int c = 0; c < 3; ++c) {
+        im3[c] += (double)P[c];
+        re3[c] += (double)P[3 + c];
+      }
+      for (int c = 0; c < 6; ++c) re6[c] += (double)P[6 + c];
     }
   }
 }
