@@ -224,6 +224,12 @@ struct mdff_ctx {
   double rc_vdw, rc_coul, alpha;
   int task_qq = 0, task_qd = 0, task_dd = 0, linduced = 0, ldip_static = 0;
   int sr_model = 0;                       // 0 none, 1 LJ, 2 BMH, 3 Morse
+  // reciprocal kernels in the FP32-class form (split-half / FP32 tables): the
+  // mixed form without dipoles.  With dipoles (static or induced) the mixed
+  // form keeps FP32 real space and runs the reciprocal sums in FP64: the
+  // FP32-class dipole structure factors set its force error above 1e-5
+  // (MDFF_RECIP_DIP_F32=1 restores them, for comparison)
+  int rec_f32 = 0;
   // LJ together with BMHFT(D) (src/engforce_driver.f90:130-141): the BMHFT(D)
   // pass gives the forces (each reference routine zeroes them first), an LJ
   // pass its energy, virial and stress into peratom_lj (forces into f_lj,

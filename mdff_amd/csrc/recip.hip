@@ -2132,7 +2132,7 @@ static double env_pos(const char* name, double def) {
 // the gather's target block count (768).
 static int recip_inner_setup(mdff_ctx* c, const std::vector<double>& Amain) {
   c->kin_on = 0;
-  const bool fp32 = c->d.precision == MDFF_PREC_MIXED;
+  const bool fp32 = c->rec_f32 != 0;
   if (!fp32 || !c->task_qq || c->task_qd || c->task_dd) return MDFF_OK;
   const double kcut = env_pos("MDFF_KIN_ALPHA", 2.5) * c->alpha;
   int kin[3];
@@ -2251,7 +2251,7 @@ int recip_setup(mdff_ctx* c) {
   HIP_TRY(c, hipMalloc((void**)&c->kvec, sizeof(double) * 4 * nb));
   HIP_TRY(c, hipMemcpy(c->kA, A.data(), sizeof(double) * nb, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->kvec, kv.data(), sizeof(double) * 4 * nb, hipMemcpyHostToDevice));
-  const size_t tsz = c->d.precision == MDFF_PREC_MIXED ? sizeof(cplx<float>) : sizeof(cplx<double>);
+  const size_t tsz = c->rec_f32 ? sizeof(cplx<float>) : sizeof(cplx<double>);
   const int ncol = Kx + Ky + Kz + 3;
   HIP_TRY(c, hipMalloc(&c->tab, tsz * ncol * (size_t)c->n));
   // structure-factor chunks: enough work items to fill the GPU
@@ -2269,7 +2269,7 @@ int recip_setup(mdff_ctx* c) {
   const int M = c->NXk * c->NYk;
   int splits = std::max(1, std::min(M, div_up((int64_t)262144, c->n)));
   // matrix-core gather: A operand and its own split count
-  const bool fp32 = c->d.precision == MDFF_PREC_MIXED;
+  const bool fp32 = c->rec_f32 != 0;
   c->mma_nks = mma_bucket(Kz + 1);
   c->mma_mtiles = div_up(M, 4);
   c->mma_ct = mma_ct(fp32, c->mma_nks);
@@ -2416,14 +2416,14 @@ static void tables_r_t(mdff_ctx* c, bool pack) {
 }
 
 int recip_tables_from_r(mdff_ctx* c, bool pack) {
-  if (c->d.precision == MDFF_PREC_MIXED) tables_r_t<float>(c, pack);
+  if (c->rec_f32) tables_r_t<float>(c, pack);
   else tables_r_t<double>(c, pack);
   HIP_TRY(c, hipGetLastError());
   return MDFF_OK;
 }
 
 int recip_tables(mdff_ctx* c) {
-  if (c->d.precision == MDFF_PREC_MIXED) tables_t<float>(c);
+  if (c->rec_f32) tables_t<float>(c);
   else tables_t<double>(c);
   HIP_TRY(c, hipGetLastError());
   return MDFF_OK;
@@ -2734,7 +2734,6 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
 
 int recip_eval(mdff_ctx* c, const double* q, const double* mu, bool use_q, bool use_mu, bool forces,
                bool field, double* ef_out, double* f_out, double* e_out) {
-  if (c->d.precision == MDFF_PREC_MIXED)
-    return eval_t<float>(c, q, mu, use_q, use_mu, forces, field, ef_out, f_out, e_out);
+  if (c->rec_f32) return eval_t<float>(c, q, mu, use_q, use_mu, forces, field, ef_out, f_out, e_out);
   return eval_t<double>(c, q, mu, use_q, use_mu, forces, field, ef_out, f_out, e_out);
 }

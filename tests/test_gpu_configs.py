@@ -170,3 +170,23 @@ def test_mgo_pim_fp64_matches_oracle():
     mg, _ = g.get_dipoles()
     mo, _ = o.get_dipoles()
     assert rel(mg, mo) < TOL64
+
+
+def test_mgo_pim_12k_mixed_meets_the_force_bar():
+    """cfg4 at its full 12,000 ions: the mixed form (FP32 real space, the
+    reciprocal sums of a dipole system in FP64) against the FP64 form, both
+    on the GPU: max_i |dF_i| / RMS |F| within 1e-5, the same SCF iteration
+    count."""
+    s = configs.mgo_pim()
+    note(f"mgo pim 12k mixed vs fp64 n={s.natm}")
+    out = {}
+    for prec in (capi.PREC_FP64, capi.PREC_MIXED):
+        d = copy.deepcopy(s.desc)
+        d.precision = prec
+        e = capi.gpu_engine(d, s.itype)
+        e.set_positions(s.pos)
+        out[prec] = e.engforce()
+        e.close()
+    (f64, t64), (fm, tm) = out[capi.PREC_FP64], out[capi.PREC_MIXED]
+    assert tm["n_scf"] == t64["n_scf"]
+    assert rms_force_err(fm, f64) < 1e-5, rms_force_err(fm, f64)
