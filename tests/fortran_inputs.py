@@ -19,8 +19,10 @@ def _r(x) -> str:
 
 def write_inputs(workdir: Path, s, names: dict[int, str], npas: int = 0, dt_ps: float = 0.001,
                  extra_control: list[str] | None = None, extra_md: list[str] | None = None,
-                 extra_coulomb: list[str] | None = None) -> Path:
-    """control.F and POSFF (Cartesian, rvn) for a Buckingham/BMH + Ewald system."""
+                 extra_coulomb: list[str] | None = None, pim: list[str] | None = None) -> Path:
+    """control.F and POSFF (Cartesian, rvn) for a Buckingham/BMH + Ewald system;
+    pim: extra &pimtag lines added to the polarisabilities and TT damping the
+    descriptor holds (written only when some type is polarisable)."""
     d = s.desc
     ntype = int(d.ntype)
     L = [
@@ -61,6 +63,28 @@ def write_inputs(workdir: Path, s, names: dict[int, str], npas: int = 0, dt_ps: 
         " qch = " + ", ".join(_r(d.qch[k]) for k in range(ntype)),
         "&end",
     ]
+    if any(d.ldip_polar[k] for k in range(ntype)):
+        L.append("&pimtag")
+        for it in range(1, ntype + 1):
+            if not d.ldip_polar[it - 1]:
+                continue
+            L.append(f" ldip_polar({it}) = .true.")
+            for a in range(1, 4):
+                for b in range(1, 4):
+                    v = d.poldip[(it - 1) + 16 * (a - 1) + 48 * (b - 1)]
+                    if v != 0.0:
+                        L.append(f" poldip({it},{a},{b}) = {_r(v)}")
+        for k in range(1, ntype + 1):
+            for i in range(1, ntype + 1):
+                for j in range(1, ntype + 1):
+                    x = capi.t3(k, i, j)
+                    if d.ldip_damping[x]:
+                        L += [f" ldip_damping({k},{i},{j}) = .true.", f" pol_damp_b({k},{i},{j}) = {_r(d.pol_damp_b[x])}",
+                              f" pol_damp_c({k},{i},{j}) = {_r(d.pol_damp_c[x])}",
+                              f" pol_damp_k({k},{i},{j}) = {d.pol_damp_k[x]}"]
+        L += [f" conv_tol_ind = {_r(d.conv_tol_ind)}", f" min_scf_pol_iter = {d.min_scf_pol_iter}",
+              f" max_scf_pol_iter = {d.max_scf_pol_iter}", f" extrapolate_order = {d.extrapolate_order}",
+              *(pim or []), "&end"]
     workdir.mkdir(parents=True, exist_ok=True)
     (workdir / "control.F").write_text("\n".join(L) + "\n")
     cell = np.asarray(d.cell[:], float).reshape(3, 3)   # rows = lattice vectors

@@ -323,3 +323,35 @@ def test_driver_efgall_matches_oracle(tmp_path):
             assert int(np.frombuffer(rec[0], np.int32)[0]) == s.natm
             g = np.frombuffer(rec[-1], np.float64).reshape(s.natm, 3, 3)   # [i, b, a]
             assert rel(g.transpose(0, 2, 1), want) < 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("algo", ["scf", "scf_kO_v1", "scf_kO_v2"])
+def test_driver_pim_matches_oracle(tmp_path, algo):
+    """A polarisable-ion input through the driver's &pimtag (ldip_polar,
+    poldip, TT damping tables, SCF controls; src/pim.f90:107-140) with each
+    SCF algorithm the library runs, scf_kO_v1 as examples/ex12 uses it: the
+    t = 0 energies, u_pol, SCF iteration count and forces against the oracle
+    run on the same descriptor."""
+    import copy
+
+    import oracle_lib as O
+    from gpu_util import note, rel
+    s = configs.mgo_pim(cells=(2, 2, 3), a=4.212, seed=9, sigma=0.1, alpha=0.8, rcut=4.2, kmax=8, tol=1e-8)
+    FI.write_inputs(tmp_path, s, {1: "Mg", 2: "O"}, npas=0,
+                    pim=[f" algo_moment_from_pola = '{algo}'", " omegakO = 0.7"])
+    note(f"fortran driver pim {algo}")
+    r = _run(tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    terms = FI.parse_terms(r.stdout)
+    fdrv = np.loadtxt(tmp_path / "FORCES")
+    d = copy.deepcopy(s.desc)
+    d.calc_md = 1
+    d.algo_moment_from_pola = {"scf": 0, "scf_kO_v2": 1, "scf_kO_v1": 2}[algo]
+    o = O.oracle(d, s.itype)
+    o.set_positions(s.pos)
+    fo, to = o.engforce()
+    assert int(terms["n_scf"]) == to["n_scf"]
+    for k in ("u_bmhft", "u_coul", "u_pol"):
+        assert abs(terms[k] - to[k]) <= 1e-10 * abs(to[k]), (k, terms[k], to[k])
+    assert rel(fdrv, fo) < 1e-10
