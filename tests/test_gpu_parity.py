@@ -170,6 +170,28 @@ def test_engforce_fp64_parity(case):
     assert not bad, bad
 
 
+@pytest.mark.parametrize("natm", [1, 3])
+def test_tiny_systems(natm):
+    """One ion (no pairs: the Ewald self and reciprocal terms alone, zero
+    force) and three ions (ragged rows) in a cubic box, FP64 and mixed,
+    against the oracle."""
+    s = configs.nacl(ncell=1, seed=6)
+    keep = [0, 4, 1][:natm]
+    s.pos = s.pos[keep]
+    s.itype = s.itype[keep]
+    s.vel = s.vel[keep]
+    s.desc.natm = natm
+    note(f"tiny system n={natm}")
+    for prec in (capi.PREC_FP64, capi.PREC_MIXED):
+        g, o = pair(s, precision=prec)
+        fg, tg = g.engforce()
+        fo, to = o.engforce()
+        assert np.abs(fg - fo).max() < 1e-9 * max(1.0, np.abs(fo).max()), (prec, fg, fo)
+        assert abs(tg["u_coul"] - to["u_coul"]) < 1e-9 * abs(to["u_coul"]), (prec, tg["u_coul"], to["u_coul"])
+        if natm == 1:
+            assert np.abs(fg).max() < 1e-12
+
+
 def test_nymand_linse_on_gpu():
     """examples/ex04 values reproduced by the GPU path itself."""
     w = json.loads((GOLD / "ex04.json").read_text())
