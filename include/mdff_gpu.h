@@ -66,8 +66,8 @@ typedef enum mdff_precision {
   MDFF_PREC_MIXED = 1   /* FP32 pair / dipole-field arithmetic, FP64 force and field
                            accumulation (pair energies / stresses: FP32 within one
                            lane's row share, FP64 beyond), FP64 minimum image,
-                           FP64 k.r phases, FP64 SCF residual; with dipoles (static or
-                           induced) the reciprocal sums run in FP64                       */
+                           FP64 k.r phases, FP64 SCF residual, the reciprocal box
+                           |k| <= 2.5 alpha of the force evaluation in FP64          */
 } mdff_precision;
 
 typedef enum mdff_trunc {  /* src/non_bonded.f90:54-56 ctrunc */

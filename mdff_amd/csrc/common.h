@@ -225,10 +225,10 @@ struct mdff_ctx {
   int task_qq = 0, task_qd = 0, task_dd = 0, linduced = 0, ldip_static = 0;
   int sr_model = 0;                       // 0 none, 1 LJ, 2 BMH, 3 Morse
   // reciprocal kernels in the FP32-class form (split-half / FP32 tables): the
-  // mixed form without dipoles.  With dipoles (static or induced) the mixed
-  // form keeps FP32 real space and runs the reciprocal sums in FP64: the
-  // FP32-class dipole structure factors set its force error above 1e-5
-  // (MDFF_RECIP_DIP_F32=1 restores them, for comparison)
+  // mixed form.  A dipole system keeps them only where the FP64 inner k-box
+  // of its force evaluation exists (else, or with MDFF_DIP_RECIP_FP64=1, its
+  // reciprocal sums run in FP64): without that box the FP32-class dipole
+  // structure factors set the force error above 1e-5
   int rec_f32 = 0;
   // LJ together with BMHFT(D) (src/engforce_driver.f90:130-141): the BMHFT(D)
   // pass gives the forces (each reference routine zeroes them first), an LJ
@@ -376,6 +376,7 @@ struct mdff_ctx {
   double* kvec_in = nullptr;              // kx, ky, kz, kcoe of the inner box (4 x nb_in)
   double* sk_in_part = nullptr;           // [chunk][2 nb_in] FP64 partials
   double* wk_in = nullptr;                // W = A conj(S) of the inner box (2 x nb_in)
+  double* dip_part = nullptr;             // dipole systems: [q, mu_x, mu_y, mu_z][chunk][2 nb_in]
   void* amat_h3 = nullptr;                // gather A operand, split halves
   float* ascale_h3 = nullptr;             // inverse scale per gather m-tile
   double* kO_ws = nullptr;                // scf_kO_v2 work vectors (11 x 3n)

@@ -761,9 +761,9 @@ static int ctx_init(mdff_ctx* c, const mdff_desc* d, const int32_t* itype) {
   // exceed 256 VGPRs with the unrolled atom tile)
   c->sk_ctm = 1;
   {
-    const char* ev = getenv("MDFF_RECIP_DIP_F32");
-    const bool dip_f32 = ev && ev[0] == '1';
-    c->rec_f32 = (d->precision == MDFF_PREC_MIXED && (dip_f32 || !(c->task_qd || c->task_dd))) ? 1 : 0;
+    const char* ev = getenv("MDFF_DIP_RECIP_FP64");
+    const bool dip64 = ev && ev[0] == '1' && (c->task_qd || c->task_dd);
+    c->rec_f32 = (d->precision == MDFF_PREC_MIXED && !dip64) ? 1 : 0;
   }
   if (d->lcoulomb) MDFF_TRY(recip_setup(c));
   return MDFF_OK;

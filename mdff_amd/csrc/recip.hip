@@ -1468,6 +1468,7 @@ k_gather_h3(Gh3Args a) {
 struct InArgs {
   int n, Kx, Ky, Kz, NY, NZ, nb, chunk, tile, msplit, ts;
   const double *sx, *sy, *sz, *q;
+  int qstride;                    // the per-atom weight is q[i * qstride] (3: a dipole component)
   const double* kA;               // A of the inner box (nb)
   const double* kvec;             // kx, ky, kz, kcoe (4 nb)
   double* part;                   // [chunk][2 nb]
@@ -1624,7 +1625,7 @@ __global__ void __launch_bounds__(256) k_in_phase(InArgs a) {
   const double sv = d == 0 ? a.sx[i] : (d == 1 ? a.sy[i] : a.sz[i]);
   const int K = d == 0 ? a.Kx : (d == 1 ? a.Ky : a.Kz);
   const int col = d == 0 ? 0 : (d == 1 ? a.Kx + 1 : a.Kx + a.Ky + 2);
-  phase_row(sv, K, a.tab + (size_t)i * ncol + col, 1, d == 2 ? a.q[i] : 1.0);
+  phase_row(sv, K, a.tab + (size_t)i * ncol + col, 1, d == 2 ? a.q[(size_t)i * a.qstride] : 1.0);
 }
 
 // grid: one block per chunk of atoms; block: SG subgroups of a.ts threads
@@ -1903,7 +1904,7 @@ __global__ void __launch_bounds__(256) k_in_gather(InArgs a) {
 //   s0 = W(0) + sum_j [(Sr c - Di s) + i (Dr s + Si c)]
 //   s1 = sum_j [(Dr jc - Si js) + i (Sr js + Di jc)]
 // with S = W(+j) + W(-j), D = W(+j) - W(-j), z_j = c + i s, (jc, js) = j z_j
-template <int KZ>
+template <int KZ, int MOM>
 __global__ void __launch_bounds__(256) k_in_gather_s(InArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool act = i < a.n;
@@ -1945,10 +1946,13 @@ __global__ void __launch_bounds__(256) k_in_gather_s(InArgs a) {
   }
   const double* __restrict__ wt = a.wt;
   double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+  // MOM 9: the second moments n_a n_b Re(W e^{ik.r}) as well (xx yy zz xy
+  // xz yz, the dipole force terms), with the nz^2-weighted sum s2
+  double mxx = 0.0, myy = 0.0, mzz = 0.0, mxy = 0.0, mxz = 0.0, myz = 0.0;
   for (int m = mb; m < me; ++m) {
     const int nx = m / a.NY, ny = m - (m / a.NY) * a.NY - a.Ky;
     const double* e = wt + (size_t)m * (KZ + 1) * 4;
-    double s0r = e[0], s0i = e[1], s1r = 0.0, s1i = 0.0;
+    double s0r = e[0], s0i = e[1], s1r = 0.0, s1i = 0.0, s2r = 0.0, s2i = 0.0;
 #pragma unroll
     for (int j = 1; j <= KZ; ++j) {
       const double sr = e[4 * j], si = e[4 * j + 1], dr = e[4 * j + 2], di = e[4 * j + 3];
@@ -1960,6 +1964,11 @@ __global__ void __launch_bounds__(256) k_in_gather_s(InArgs a) {
       s1r = fma(-si, js[j], s1r);
       s1i = fma(sr, js[j], s1i);
       s1i = fma(di, jc[j], s1i);
+      if (MOM == 9) {
+        // j^2 (W(+j) z + W(-j) conj z) = j (sigma j z terms)
+        s2r = fma((double)j, fma(sr, jc[j], -di * js[j]), s2r);
+        s2i = fma((double)j, fma(dr, js[j], si * jc[j]), s2i);
+      }
     }
     const cplx<double> XY = cmul(X, Y);
     const double i0 = XY.re * s0i + XY.im * s0r;
@@ -1967,6 +1976,18 @@ __global__ void __launch_bounds__(256) k_in_gather_s(InArgs a) {
     u0 += (double)nx * i0;
     u1 += (double)ny * i0;
     u2 += i1;
+    if (MOM == 9) {
+      const double r0 = XY.re * s0r - XY.im * s0i;
+      const double r1 = XY.re * s1r - XY.im * s1i;
+      const double r2 = XY.re * s2r - XY.im * s2i;
+      const double fx = (double)nx, fy = (double)ny;
+      mxx += fx * fx * r0;
+      myy += fy * fy * r0;
+      mxy += fx * fy * r0;
+      mxz += fx * r1;
+      myz += fy * r1;
+      mzz += r2;
+    }
     if (ny == a.Ky) {
       X = cmul(X, x1);
       Y = y0;
@@ -1975,10 +1996,38 @@ __global__ void __launch_bounds__(256) k_in_gather_s(InArgs a) {
     }
   }
   if (!act) return;
-  double* o = a.out + (size_t)blockIdx.y * 3 * a.n;
+  double* o = a.out + (size_t)blockIdx.y * MOM * a.n;
   o[i] = u0;
   o[(size_t)a.n + i] = u1;
   o[(size_t)2 * a.n + i] = u2;
+  if (MOM == 9) {
+    o[(size_t)3 * a.n + i] = mxx;
+    o[(size_t)4 * a.n + i] = myy;
+    o[(size_t)5 * a.n + i] = mzz;
+    o[(size_t)6 * a.n + i] = mxy;
+    o[(size_t)7 * a.n + i] = mxz;
+    o[(size_t)8 * a.n + i] = myz;
+  }
+}
+
+// The inner box's structure factor of a charge + dipole set from the chunk
+// partials of its four weighted sums (weights q, mu_x, mu_y, mu_z, each
+// through k_in_phase / k_in_sk_s): S_c(k) = Sq + i (kx Sx + ky Sy + kz Sz)
+// per chunk c, as (q + i k.mu) e^{ik.r} summed over the chunk's atoms
+// (src/ewald_rec.f90:116-148); written where k_in_reduce_r reads it.
+__global__ void k_in_dip_combine(InArgs a, int nchunk, int use_q, const double* __restrict__ pq,
+                                 const double* __restrict__ px, const double* __restrict__ py,
+                                 const double* __restrict__ pz) {
+  const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= (size_t)nchunk * a.nb) return;
+  const int k = (int)(x % a.nb);
+  const double kx = a.kvec[4 * k], ky = a.kvec[4 * k + 1], kz = a.kvec[4 * k + 2];
+  const size_t o = 2 * x;
+  const double tr = kx * px[o] + ky * py[o] + kz * pz[o];
+  const double ti = kx * px[o + 1] + ky * py[o + 1] + kz * pz[o + 1];
+  const double qr = use_q ? pq[o] : 0.0, qi = use_q ? pq[o + 1] : 0.0;
+  a.part[o] = qr - ti;
+  a.part[o + 1] = qi + tr;
 }
 
 using InKernel = void (*)(InArgs);
@@ -1988,13 +2037,16 @@ InKernel pick_in_sk(int kz, std::integer_sequence<int, I...>) {
   return t[kz - 1];
 }
 template <int... I>
-InKernel pick_in_gather(int kz, std::integer_sequence<int, I...>) {
-  static const InKernel t[] = {k_in_gather_s<I + 1>...};
-  return t[kz - 1];
+InKernel pick_in_gather(int kz, int mom, std::integer_sequence<int, I...>) {
+  static const InKernel t3[] = {k_in_gather_s<I + 1, 3>...};
+  static const InKernel t9[] = {k_in_gather_s<I + 1, 9>...};
+  return mom == 9 ? t9[kz - 1] : t3[kz - 1];
 }
 // the register-blocked kernels for Kz' = kz (1 <= kz <= kInKZ)
 InKernel in_sk_kernel(int kz) { return pick_in_sk(kz, std::make_integer_sequence<int, kInKZ>{}); }
-InKernel in_gather_kernel(int kz) { return pick_in_gather(kz, std::make_integer_sequence<int, kInKZ>{}); }
+InKernel in_gather_kernel(int kz, int mom = 3) {
+  return pick_in_gather(kz, mom, std::make_integer_sequence<int, kInKZ>{});
+}
 
 }  // namespace
 
@@ -2133,7 +2185,7 @@ static double env_pos(const char* name, double def) {
 static int recip_inner_setup(mdff_ctx* c, const std::vector<double>& Amain) {
   c->kin_on = 0;
   const bool fp32 = c->rec_f32 != 0;
-  if (!fp32 || !c->task_qq || c->task_qd || c->task_dd) return MDFF_OK;
+  if (!fp32 || !c->task_qq) return MDFF_OK;
   const double kcut = env_pos("MDFF_KIN_ALPHA", 2.5) * c->alpha;
   int kin[3];
   for (int d = 0; d < 3; ++d) {
@@ -2217,12 +2269,36 @@ static int recip_inner_setup(mdff_ctx* c, const std::vector<double>& Amain) {
     c->kin_splits = div_up(M, c->kin_msplit);
   }
   HIP_TRY(c, hipMalloc((void**)&c->sk_in_part, sizeof(double) * 2 * (size_t)nbi * c->in_nch));
+  // dipole systems: the chunk partials of the four weighted structure
+  // factors (q, mu_x, mu_y, mu_z) of the force evaluation
+  if ((c->task_qd || c->task_dd) && c->kin_reg)
+    HIP_TRY(c, hipMalloc((void**)&c->dip_part, sizeof(double) * 8 * (size_t)nbi * c->in_nch));
   c->kin_on = 1;
   return MDFF_OK;
 }
 
+// whether the register-blocked inner box exists for this cell and kmax (the
+// condition recip_inner_setup applies): dipole systems need it to keep the
+// FP32-class reciprocal kernels
+static bool inner_reg_possible(const mdff_ctx* c) {
+  if (!c->task_qq) return false;
+  const double kcut = env_pos("MDFF_KIN_ALPHA", 2.5) * c->alpha;
+  int kin[3];
+  for (int d = 0; d < 3; ++d) {
+    const double* a = c->A + 3 * d;
+    const double len = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    kin[d] = std::min(c->d.kES[d], (int)ceil(kcut * len / (2.0 * MDFF_PI)));
+    if (kin[d] < 1) return false;
+  }
+  const int M = (kin[0] + 1) * (2 * kin[1] + 1);
+  return kin[2] <= kInKZ && div_up(M, 64) * 64 <= 1024;
+}
+
 int recip_setup(mdff_ctx* c) {
   const int Kx = c->d.kES[0], Ky = c->d.kES[1], Kz = c->d.kES[2];
+  // a dipole system keeps the FP32-class reciprocal kernels only with the
+  // FP64 inner box of its force evaluation; otherwise its sums run in FP64
+  if (c->rec_f32 && (c->task_qd || c->task_dd) && !inner_reg_possible(c)) c->rec_f32 = 0;
   c->K[0] = Kx; c->K[1] = Ky; c->K[2] = Kz;
   c->NXk = Kx + 1; c->NYk = 2 * Ky + 1; c->NZk = 2 * Kz + 1;
   c->nk_box = c->NXk * c->NYk * c->NZk;
@@ -2369,14 +2445,14 @@ int recip_setup(mdff_ctx* c) {
   if (c->h3) splits = std::max(splits, c->h3_splits);
   c->rec_splits = splits;
   // moments of every split, plus (3 moments each) the inner box's
-  HIP_TRY(c, hipMalloc((void**)&c->rec_part, sizeof(double) * (9 * (size_t)splits + 3 * (size_t)c->kin_splits) * c->n));
+  HIP_TRY(c, hipMalloc((void**)&c->rec_part, sizeof(double) * (9 * (size_t)splits + 9 * (size_t)c->kin_splits) * c->n));
   return MDFF_OK;
 }
 
 void recip_free(mdff_ctx* c) {
   void* ps[] = {c->kA, c->kvec, c->tab, c->sk_part, c->sk, c->wk, c->wk_t, c->rec_part, c->amat, c->rec_blk,
                 c->amat_h3, c->ascale_h3, c->kA_mask, c->kA_in, c->kvec_in, c->sk_in_part, c->wk_in, c->tab_in, c->wt_in,
-                c->mu_pscale};
+                c->mu_pscale, c->dip_part};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -2564,7 +2640,11 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
   }
   // inner k-box in FP64 (mixed form, charges only): its structure factor
   // here; the main kernels below see A = 0 on the box
-  const bool inner = sizeof(T) == 4 && c->kin_on && use_q && !use_mu;
+  // charges: every evaluation; dipoles: the force evaluation (its error is
+  // the mixed form's; the SCF fields stay on the FP32-class kernels)
+  const bool inner_q = sizeof(T) == 4 && c->kin_on && use_q && !use_mu;
+  const bool inner_d = sizeof(T) == 4 && c->kin_on && c->kin_reg && c->dip_part && forces && use_mu;
+  const bool inner = inner_q || inner_d;
   const double* kA = inner ? c->kA_mask : c->kA;
   InArgs ia{};
   if (inner) {
@@ -2576,12 +2656,33 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     ia.tile = c->kin_tile;
     ia.sx = c->sx; ia.sy = c->sy; ia.sz = c->sz;
     ia.q = q;
+    ia.qstride = 1;
     ia.kA = c->kA_in;
     ia.kvec = c->kvec_in;
     ia.part = c->sk_in_part;
     ia.wk = c->wk_in;
     const size_t lds = c->kin_lds;
-    if (c->kin_reg) {
+    if (inner_d) {
+      // four weighted structure factors (q, mu_x, mu_y, mu_z), then
+      // Sq + i k.(Sx, Sy, Sz) per chunk into sk_in_part
+      ia.ts = c->kin_ts;
+      ia.tab = reinterpret_cast<cplx<double>*>(c->tab_in);
+      ia.wt = c->wt_in;
+      const size_t pstride = 2 * (size_t)ia.nb * c->in_nch;
+      for (int w = use_q ? 0 : 1; w < 4; ++w) {
+        InArgs iw = ia;
+        iw.q = w == 0 ? q : mu + (w - 1);
+        iw.qstride = w == 0 ? 1 : 3;
+        iw.part = c->dip_part + (size_t)w * pstride;
+        PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(k_in_phase, dim3(div_up(3 * c->n, 256)), dim3(256), 0, c->stream, iw));
+        PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(in_sk_kernel(iw.Kz), dim3(c->in_nch), dim3(c->kin_ts * c->kin_sg), lds,
+                                                 c->stream, iw));
+      }
+      const size_t tot = (size_t)c->in_nch * ia.nb;
+      PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(k_in_dip_combine, dim3(div_up(tot, 256)), dim3(256), 0, c->stream, ia,
+                                               c->in_nch, use_q ? 1 : 0, c->dip_part, c->dip_part + pstride,
+                                               c->dip_part + 2 * pstride, c->dip_part + 3 * pstride));
+    } else if (c->kin_reg) {
       ia.ts = c->kin_ts;
       ia.tab = reinterpret_cast<cplx<double>*>(c->tab_in);
       ia.wt = c->wt_in;
@@ -2696,12 +2797,12 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
   }
   if (inner) {
     // the inner box's moments: kin_splits more splits after the main ones
-    ia.out = c->rec_part + (size_t)nsplit * 3 * c->n;
+    ia.out = c->rec_part + (size_t)nsplit * moments * c->n;
     ia.msplit = c->kin_msplit;
     const size_t lds = (size_t)kInMT * ia.NZ * 16;
     const dim3 gg(div_up(c->n, 256), c->kin_splits);
     if (c->kin_reg) {
-      PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(in_gather_kernel(ia.Kz), gg, dim3(256), 0, c->stream, ia));
+      PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(in_gather_kernel(ia.Kz, moments), gg, dim3(256), 0, c->stream, ia));
     } else {
       PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(k_in_gather, gg, dim3(256), lds, c->stream, ia));
     }

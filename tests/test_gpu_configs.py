@@ -173,10 +173,10 @@ def test_mgo_pim_fp64_matches_oracle():
 
 
 def test_mgo_pim_12k_mixed_meets_the_force_bar():
-    """cfg4 at its full 12,000 ions: the mixed form (FP32 real space, the
-    reciprocal sums of a dipole system in FP64) against the FP64 form, both
-    on the GPU: max_i |dF_i| / RMS |F| within 1e-5, the same SCF iteration
-    count."""
+    """cfg4 at its full 12,000 ions: the mixed form (FP32 real space,
+    FP32-class reciprocal kernels with the FP64 inner k-box in the force
+    evaluation) against the FP64 form, both on the GPU: max_i |dF_i| / RMS
+    |F| within 1e-5, the same SCF iteration count."""
     s = configs.mgo_pim()
     note(f"mgo pim 12k mixed vs fp64 n={s.natm}")
     out = {}

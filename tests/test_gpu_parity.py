@@ -342,7 +342,7 @@ def test_pim_extrapolation_order_parity():
 
 # ------------------------------------------------------------------ mixed precision
 
-@pytest.mark.parametrize("case", ["sio2", "nacl", "pim", "pim_large_box", "wide_kxy", "pim_static"])
+@pytest.mark.parametrize("case", ["sio2", "nacl", "pim", "pim_large_box", "wide_kxy", "pim_static", "pim_fp64_rec"])
 def test_mixed_precision_force_error(case, monkeypatch):
     """FP32 pair / field arithmetic with FP64 accumulation (FP64 for the rows'
     inner entries): max_i |dF_i| / RMS_i |F_i| against the FP64 oracle below
@@ -358,10 +358,12 @@ def test_mixed_precision_force_error(case, monkeypatch):
         s.desc.kES[2] = 8
     elif case == "pim_large_box":
         # 1,728 ions in a 25 A box: dipoles of 1e-3..1e-1 e A, so the
-        # dipole coefficients mu.b of the structure factor are ~1e-4; run on
-        # the FP32-class dipole structure factors (MDFF_RECIP_DIP_F32=1)
+        # dipole coefficients mu.b of the structure factor are ~1e-4
         s = configs.mgo_pim(cells=(6, 6, 6), seed=12, sigma=0.1)
-        monkeypatch.setenv("MDFF_RECIP_DIP_F32", "1")
+    elif case == "pim_fp64_rec":
+        # the fallback of dipole systems: FP64 reciprocal sums
+        s = _pim()
+        monkeypatch.setenv("MDFF_DIP_RECIP_FP64", "1")
     elif case == "pim_static":
         s = _pim()
         d = s.desc
@@ -487,9 +489,6 @@ def test_mixed_reciprocal_split_half_vs_fp32_matrix_core(case, monkeypatch):
         s = _sio2(natm=900, seed=27, kmax=9)
     else:
         s = _pim()
-        # the FP32-class dipole structure factors (the mixed form runs the
-        # reciprocal sums of dipole systems in FP64 unless asked)
-        monkeypatch.setenv("MDFF_RECIP_DIP_F32", "1")
     note(f"mixed reciprocal kernels {case} n={s.natm}")
     _, o = pair(s)
     fo, to = o.engforce()
