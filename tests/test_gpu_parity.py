@@ -342,7 +342,8 @@ def test_pim_extrapolation_order_parity():
 
 # ------------------------------------------------------------------ mixed precision
 
-@pytest.mark.parametrize("case", ["sio2", "nacl", "pim", "pim_large_box", "wide_kxy", "pim_static", "pim_fp64_rec"])
+@pytest.mark.parametrize("case", ["sio2", "nacl", "pim", "pim_large_box", "wide_kxy", "pim_static", "pim_fp64_rec",
+                                  "static_only"])
 def test_mixed_precision_force_error(case, monkeypatch):
     """FP32 pair / field arithmetic with FP64 accumulation (FP64 for the rows'
     inner entries): max_i |dF_i| / RMS_i |F_i| against the FP64 oracle below
@@ -360,6 +361,13 @@ def test_mixed_precision_force_error(case, monkeypatch):
         # 1,728 ions in a 25 A box: dipoles of 1e-3..1e-1 e A, so the
         # dipole coefficients mu.b of the structure factor are ~1e-4
         s = configs.mgo_pim(cells=(6, 6, 6), seed=12, sigma=0.1)
+    elif case == "static_only":
+        # static dipoles, no polarisable ion: the force evaluation's inner
+        # k-box with dipole weights, no SCF
+        s = configs.nacl(ncell=3)
+        d = s.desc
+        d.dip[0 + 16 * 0], d.dip[0 + 16 * 1], d.dip[0 + 16 * 2] = 0.05, -0.02, 0.03
+        d.dip[1 + 16 * 0], d.dip[1 + 16 * 1], d.dip[1 + 16 * 2] = -0.03, 0.04, 0.02
     elif case == "pim_fp64_rec":
         # the fallback of dipole systems: FP64 reciprocal sums
         s = _pim()
