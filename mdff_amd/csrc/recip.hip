@@ -1631,6 +1631,51 @@ __global__ void __launch_bounds__(256) k_in_phase(InArgs a) {
 // grid: one block per chunk of atoms; block: SG subgroups of a.ts threads
 // (a.ts a multiple of 64, >= M); subgroup g takes atoms g, g + SG, ... of
 // each tile
+// one atom's operands of k_in_sk_s: q Z(0..KZ) (wave-uniform: scalar loads)
+// and X(nx), Y(|ny|) from the LDS tile
+template <int KZ>
+__device__ __forceinline__ void in_sk_load(const cplx<double>* __restrict__ tab, size_t zrow,
+                                           const cplx<double>* xy, int xrow, int nx, int cy,
+                                           cplx<double> (&z)[KZ + 1], cplx<double>& X, cplx<double>& Y) {
+#pragma unroll
+  for (int j = 0; j <= KZ; ++j) z[j] = tab[zrow + j];
+  X = xy[xrow + nx];
+  Y = xy[xrow + cy];
+}
+
+// one atom's terms: P = X Y (Y conjugated for ny < 0) and S(m, 0) += P q Z(0)
+template <int KZ>
+__device__ __forceinline__ cplx<double> in_sk_first(const cplx<double>& X, cplx<double> Y, double sy,
+                                                    const cplx<double> (&z)[KZ + 1], double (&pr)[KZ + 1],
+                                                    double (&pi)[KZ + 1]) {
+  Y.im *= sy;
+  const cplx<double> P = cmul(X, Y);
+  pr[0] = fma(P.re, z[0].re, pr[0]);
+  pr[0] = fma(-P.im, z[0].im, pr[0]);
+  pi[0] = fma(P.re, z[0].im, pi[0]);
+  pi[0] = fma(P.im, z[0].re, pi[0]);
+  return P;
+}
+
+// then S(m, +j) and S(m, -j) += P q Z(+-j), j = 1..KZ
+template <int KZ>
+__device__ __forceinline__ void in_sk_rest(const cplx<double>& P, const cplx<double> (&z)[KZ + 1],
+                                           double (&pr)[KZ + 1], double (&pi)[KZ + 1], double (&qr)[KZ + 1],
+                                           double (&qi)[KZ + 1]) {
+#pragma unroll
+  for (int j = 1; j <= KZ; ++j) {
+    const double zr = z[j].re, zi = z[j].im;
+    pr[j] = fma(P.re, zr, pr[j]);
+    pr[j] = fma(-P.im, zi, pr[j]);
+    pi[j] = fma(P.re, zi, pi[j]);
+    pi[j] = fma(P.im, zr, pi[j]);
+    qr[j] = fma(P.re, zr, qr[j]);
+    qr[j] = fma(P.im, zi, qr[j]);
+    qi[j] = fma(P.im, zr, qi[j]);
+    qi[j] = fma(-P.re, zi, qi[j]);
+  }
+}
+
 template <int KZ>
 __global__ void __launch_bounds__(1024) k_in_sk_s(InArgs a) {
   extern __shared__ unsigned char smem_raw[];
@@ -1658,42 +1703,34 @@ __global__ void __launch_bounds__(1024) k_in_sk_s(InArgs a) {
       xy[x] = tab[(size_t)(t0 + at) * ncol + (x - at * nxy)];
     }
     __syncthreads();
-    // q Z of the next atom loaded while the current one is summed (the
-    // last atom's row is loaded again at the end: every address in range)
-    cplx<double> zn[KZ + 1];
-    if (g < na) {
-      const cplx<double>* zq = tab + (size_t)(t0 + g) * ncol + nxy;
-#pragma unroll
-      for (int j = 0; j <= KZ; ++j) zn[j] = zq[j];
-    }
-    for (int at = g; at < na; at += SG) {
-      cplx<double> z[KZ + 1];
-#pragma unroll
-      for (int j = 0; j <= KZ; ++j) z[j] = zn[j];
+    // two operand sets, A and B, alternate between atoms.  Each atom's
+    // first terms (P = X Y and nz = 0) come before the next atom's loads
+    // (q Z by scalar loads, X and Y from LDS) issue, so the wait for its own
+    // operands, which were issued one atom earlier, falls before them; the
+    // next atom's operands then land while the rest of its terms are summed.
+    // (Scalar loads return out of order: a wait on them is a full wait.)  An
+    // atom index past the tile loads the last atom's row again (in range,
+    // unused).
+    cplx<double> zA[KZ + 1], zB[KZ + 1];
+    cplx<double> XA{0.0, 0.0}, YA{0.0, 0.0}, XB{0.0, 0.0}, YB{0.0, 0.0};
+    if (g < na) in_sk_load<KZ>(tab, (size_t)(t0 + g) * ncol + nxy, xy, g * nxy, nx, cy, zA, XA, YA);
+    for (int at = g; at < na; at += 2 * SG) {
+      const int b = at + SG;
       {
-        const cplx<double>* zq = tab + (size_t)(t0 + min(at + SG, na - 1)) * ncol + nxy;
-#pragma unroll
-        for (int j = 0; j <= KZ; ++j) zn[j] = zq[j];
+        const cplx<double> P = in_sk_first<KZ>(XA, YA, sy, zA, pr, pi);
+        __builtin_amdgcn_sched_barrier(0);
+        const int an = __builtin_amdgcn_readfirstlane(min(b, na - 1));
+        in_sk_load<KZ>(tab, (size_t)(t0 + an) * ncol + nxy, xy, an * nxy, nx, cy, zB, XB, YB);
+        __builtin_amdgcn_sched_barrier(0);
+        in_sk_rest<KZ>(P, zA, pr, pi, qr, qi);
       }
-      const cplx<double> X = xy[at * nxy + nx];
-      cplx<double> Y = xy[at * nxy + cy];
-      Y.im *= sy;
-      const cplx<double> P = cmul(X, Y);
-      pr[0] = fma(P.re, z[0].re, pr[0]);
-      pr[0] = fma(-P.im, z[0].im, pr[0]);
-      pi[0] = fma(P.re, z[0].im, pi[0]);
-      pi[0] = fma(P.im, z[0].re, pi[0]);
-#pragma unroll
-      for (int j = 1; j <= KZ; ++j) {
-        const double zr = z[j].re, zi = z[j].im;
-        pr[j] = fma(P.re, zr, pr[j]);
-        pr[j] = fma(-P.im, zi, pr[j]);
-        pi[j] = fma(P.re, zi, pi[j]);
-        pi[j] = fma(P.im, zr, pi[j]);
-        qr[j] = fma(P.re, zr, qr[j]);
-        qr[j] = fma(P.im, zi, qr[j]);
-        qi[j] = fma(P.im, zr, qi[j]);
-        qi[j] = fma(-P.re, zi, qi[j]);
+      if (b < na) {
+        const cplx<double> P = in_sk_first<KZ>(XB, YB, sy, zB, pr, pi);
+        __builtin_amdgcn_sched_barrier(0);
+        const int an = __builtin_amdgcn_readfirstlane(min(b + SG, na - 1));
+        in_sk_load<KZ>(tab, (size_t)(t0 + an) * ncol + nxy, xy, an * nxy, nx, cy, zA, XA, YA);
+        __builtin_amdgcn_sched_barrier(0);
+        in_sk_rest<KZ>(P, zB, pr, pi, qr, qi);
       }
     }
   }
@@ -2230,13 +2267,19 @@ static int recip_inner_setup(mdff_ctx* c, const std::vector<double>& Amain) {
   const int ts = div_up(M, 64) * 64;
   c->kin_reg = (kin[2] <= kInKZ && ts <= 1024) ? 1 : 0;
   if (c->kin_reg) {
-    // k_in_sk_s: SG subgroups of ts threads (at most 4, at most 1024
-    // threads), one block per chunk, about 256 chunks (one per CU); the
-    // chunk's X / Y rows through LDS in tiles within 48 KiB.
+    // k_in_sk_s: SG subgroups of ts threads (at most 1024 threads), one
+    // block per chunk; the chunk's X / Y rows through LDS in tiles within
+    // 48 KiB.  Charge-only systems, whose pair kernel runs beside the
+    // reciprocal kernels on the second stream: one subgroup and about 384
+    // chunks (1.5 blocks per CU), which leaves room on the SIMDs for the
+    // pair kernel's waves (24k a-SiO2: 0.298 against 0.307 ms per MD step,
+    // although the kernel alone takes longer); otherwise 4 subgroups and
+    // about 256 chunks (one per CU).
     // k_in_gather_s: m ranges for about 768 blocks of 256 atoms.
-    const int sg = std::min((int)env_pos("MDFF_KIN_SG", 4), 1024 / ts);
+    const bool beside = c->overlap && !(c->task_qd || c->task_dd);
+    const int sg = std::min((int)env_pos("MDFF_KIN_SG", beside ? 1 : 4), 1024 / ts);
     const int nxy = kin[0] + kin[1] + 2;
-    int chunk = std::max(div_up(c->n, (int)env_pos("MDFF_KIN_CHUNKS", 256)), 4 * sg);
+    int chunk = std::max(div_up(c->n, (int)env_pos("MDFF_KIN_CHUNKS", beside ? 384 : 256)), 4 * sg);
     const int tile = std::min(chunk, 49152 / (16 * nxy));
     chunk = div_up(chunk, tile) * tile;
     c->kin_ts = ts;
