@@ -368,6 +368,8 @@ struct mdff_ctx {
   int nb_in = 0, in_chunk = 0, in_nch = 0, kin_tile = 0, kin_msplit = 0, kin_splits = 0;
   int kin_reg = 0;                        // 1: the scalar-operand forms (Kz' small enough)
   int kin_ts = 0, kin_sg = 0;             // k_in_sk_s: threads per subgroup, subgroups
+  int kin_jb = 1, kin_kb = 0;             // scalar-operand forms: blocks of j = |nz|, KZ (j per block - 1)
+  int kin_zpad = 0;                       // j columns of the phase rows and the W table: kin_jb (kin_kb + 1)
   size_t kin_lds = 0;                     // LDS bytes of the structure-factor kernel
   void* tab_in = nullptr;                 // per atom X, Y, q Z rows (FP64 complex)
   double* wt_in = nullptr;                // per (m, j) sum / difference table of W

@@ -1457,7 +1457,7 @@ k_gather_h3(Gh3Args a) {
 // same terms as the main path, so the split changes which precision sums
 // a k-vector, not the sum.
 // Generic forms (any Kz'; the scalar-operand forms below replace them when
-// Kz' <= kInKZ):
+// Kz' <= kInKZmax and the (nx, ny) columns fit one block):
 //   k_in_sk      S(k) partials per chunk of atoms, one thread per k, FP64
 //                phase tables of the chunk's atoms staged in LDS
 //   k_in_reduce  S(k) over the chunks in order; W = A conj(S); the energy
@@ -1467,6 +1467,7 @@ k_gather_h3(Gh3Args a) {
 // ---------------------------------------------------------------------------
 struct InArgs {
   int n, Kx, Ky, Kz, NY, NZ, nb, chunk, tile, msplit, ts;
+  int zpad;                       // scalar-operand forms: Z columns of a phase row, j entries of a W table row
   const double *sx, *sy, *sz, *q;
   int qstride;                    // the per-atom weight is q[i * qstride] (3: a dipole component)
   const double* kA;               // A of the inner box (nb)
@@ -1603,8 +1604,12 @@ __global__ void __launch_bounds__(256) k_in_reduce(InArgs a, int nchunk) {
   if (threadIdx.x < 7) a.blk[blockIdx.x * 7 + threadIdx.x] = r;
 }
 
-// Forms for boxes with Kz' <= kInKZ (KZ = Kz' a template argument), where
-// the operands shared by a wave come through scalar loads rather than LDS:
+// Forms for boxes with Kz' <= kInKZmax, where the operands shared by a wave
+// come through scalar loads rather than LDS.  The nz range is taken in
+// blocks of KZ + 1 values of j = |nz| (KZ <= kInKZ a template argument;
+// block b holds j = b (KZ + 1) + 0..KZ, the last block's surplus j masked),
+// one grid row per block (k_in_sk_s: blockIdx.y, k_in_gather_s:
+// blockIdx.z); boxes with Kz' <= kInKZ are one block with KZ = Kz':
 //   k_in_phase     per atom the FP64 rows X(0..Kx'), Y(0..Ky'), q Z(0..Kz')
 //   k_in_sk_s      one thread per m = (nx, ny) and subgroup of atoms, the
 //                  S(m, +-nz) sums in registers; q Z of the wave's atom is
@@ -1615,13 +1620,14 @@ __global__ void __launch_bounds__(256) k_in_reduce(InArgs a, int nchunk) {
 //                  differences of W(m, +j) and W(m, -j) that the gather reads
 //   k_in_gather_s  one thread per atom, Z(0..Kz') in registers, the (m, j)
 //                  table through scalar loads
-constexpr int kInKZ = 12;
+constexpr int kInKZ = 8;       // largest KZ: the two operand sets' q Z in scalar registers
+constexpr int kInKZmax = 31;   // NZ = 2 Kz' + 1 <= 64: one row m per k_in_reduce_r wave
 
 __global__ void __launch_bounds__(256) k_in_phase(InArgs a) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= 3 * a.n) return;
   const int i = r / 3, d = r - 3 * (r / 3);
-  const int ncol = a.Kx + a.Ky + a.Kz + 3;
+  const int ncol = a.Kx + a.Ky + 2 + a.zpad;
   const double sv = d == 0 ? a.sx[i] : (d == 1 ? a.sy[i] : a.sz[i]);
   const int K = d == 0 ? a.Kx : (d == 1 ? a.Ky : a.Kz);
   const int col = d == 0 ? 0 : (d == 1 ? a.Kx + 1 : a.Kx + a.Ky + 2);
@@ -1643,17 +1649,25 @@ __device__ __forceinline__ void in_sk_load(const cplx<double>* __restrict__ tab,
   Y = xy[xrow + cy];
 }
 
-// one atom's terms: P = X Y (Y conjugated for ny < 0) and S(m, 0) += P q Z(0)
+// one atom's terms: P = X Y (Y conjugated for ny < 0) and the block's first
+// j: S(m, +j0) += P q Z(j0), and S(m, -j0) += P q conj Z(j0) unless j0 = 0
 template <int KZ>
 __device__ __forceinline__ cplx<double> in_sk_first(const cplx<double>& X, cplx<double> Y, double sy,
-                                                    const cplx<double> (&z)[KZ + 1], double (&pr)[KZ + 1],
-                                                    double (&pi)[KZ + 1]) {
+                                                    const cplx<double> (&z)[KZ + 1], bool j0pos,
+                                                    double (&pr)[KZ + 1], double (&pi)[KZ + 1],
+                                                    double (&qr)[KZ + 1], double (&qi)[KZ + 1]) {
   Y.im *= sy;
   const cplx<double> P = cmul(X, Y);
   pr[0] = fma(P.re, z[0].re, pr[0]);
   pr[0] = fma(-P.im, z[0].im, pr[0]);
   pi[0] = fma(P.re, z[0].im, pi[0]);
   pi[0] = fma(P.im, z[0].re, pi[0]);
+  if (j0pos) {
+    qr[0] = fma(P.re, z[0].re, qr[0]);
+    qr[0] = fma(P.im, z[0].im, qr[0]);
+    qi[0] = fma(P.im, z[0].re, qi[0]);
+    qi[0] = fma(-P.re, z[0].im, qi[0]);
+  }
   return P;
 }
 
@@ -1676,11 +1690,13 @@ __device__ __forceinline__ void in_sk_rest(const cplx<double>& P, const cplx<dou
   }
 }
 
-template <int KZ>
+template <int KZ, bool MB>
 __global__ void __launch_bounds__(1024) k_in_sk_s(InArgs a) {
   extern __shared__ unsigned char smem_raw[];
   const int nxy = a.Kx + a.Ky + 2;
-  const int ncol = nxy + KZ + 1;
+  const int ncol = nxy + a.zpad;                                 // Z(0..Kz') and zero padding
+  const int j0 = MB ? blockIdx.y * (KZ + 1) : 0;                 // this block's first j
+  const bool j0pos = MB && j0 > 0;
   cplx<double>* xy = reinterpret_cast<cplx<double>*>(smem_raw);   // [tile][nxy]
   const int M = (a.Kx + 1) * a.NY;
   const int SG = blockDim.x / a.ts;
@@ -1713,22 +1729,22 @@ __global__ void __launch_bounds__(1024) k_in_sk_s(InArgs a) {
     // unused).
     cplx<double> zA[KZ + 1], zB[KZ + 1];
     cplx<double> XA{0.0, 0.0}, YA{0.0, 0.0}, XB{0.0, 0.0}, YB{0.0, 0.0};
-    if (g < na) in_sk_load<KZ>(tab, (size_t)(t0 + g) * ncol + nxy, xy, g * nxy, nx, cy, zA, XA, YA);
+    if (g < na) in_sk_load<KZ>(tab, (size_t)(t0 + g) * ncol + nxy + j0, xy, g * nxy, nx, cy, zA, XA, YA);
     for (int at = g; at < na; at += 2 * SG) {
       const int b = at + SG;
       {
-        const cplx<double> P = in_sk_first<KZ>(XA, YA, sy, zA, pr, pi);
+        const cplx<double> P = in_sk_first<KZ>(XA, YA, sy, zA, j0pos, pr, pi, qr, qi);
         __builtin_amdgcn_sched_barrier(0);
         const int an = __builtin_amdgcn_readfirstlane(min(b, na - 1));
-        in_sk_load<KZ>(tab, (size_t)(t0 + an) * ncol + nxy, xy, an * nxy, nx, cy, zB, XB, YB);
+        in_sk_load<KZ>(tab, (size_t)(t0 + an) * ncol + nxy + j0, xy, an * nxy, nx, cy, zB, XB, YB);
         __builtin_amdgcn_sched_barrier(0);
         in_sk_rest<KZ>(P, zA, pr, pi, qr, qi);
       }
       if (b < na) {
-        const cplx<double> P = in_sk_first<KZ>(XB, YB, sy, zB, pr, pi);
+        const cplx<double> P = in_sk_first<KZ>(XB, YB, sy, zB, j0pos, pr, pi, qr, qi);
         __builtin_amdgcn_sched_barrier(0);
         const int an = __builtin_amdgcn_readfirstlane(min(b + SG, na - 1));
-        in_sk_load<KZ>(tab, (size_t)(t0 + an) * ncol + nxy, xy, an * nxy, nx, cy, zA, XA, YA);
+        in_sk_load<KZ>(tab, (size_t)(t0 + an) * ncol + nxy + j0, xy, an * nxy, nx, cy, zA, XA, YA);
         __builtin_amdgcn_sched_barrier(0);
         in_sk_rest<KZ>(P, zB, pr, pi, qr, qi);
       }
@@ -1754,12 +1770,14 @@ __global__ void __launch_bounds__(1024) k_in_sk_s(InArgs a) {
   if (g != 0 || m >= M) return;
   double* out = a.part + (size_t)blockIdx.x * 2 * a.nb + 2 * (size_t)m * a.NZ;
 #pragma unroll
-  for (int j = 0; j <= KZ; ++j) {
-    out[2 * (KZ + j)] = pr[j];
-    out[2 * (KZ + j) + 1] = pi[j];
+  for (int jj = 0; jj <= KZ; ++jj) {
+    const int j = j0 + jj;
+    if (j > a.Kz) break;
+    out[2 * (a.Kz + j)] = pr[jj];
+    out[2 * (a.Kz + j) + 1] = pi[jj];
     if (j > 0) {
-      out[2 * (KZ - j)] = qr[j];
-      out[2 * (KZ - j) + 1] = qi[j];
+      out[2 * (a.Kz - j)] = qr[jj];
+      out[2 * (a.Kz - j) + 1] = qi[jj];
     }
   }
 }
@@ -1845,7 +1863,7 @@ __global__ void __launch_bounds__(1024) k_in_reduce_r(InArgs a, int nchunk) {
   const int t = threadIdx.x;
   if (t < rows * nj) {
     const int r = t / nj, j = t - (t / nj) * nj;
-    double* o = a.wt + ((size_t)(m0 + r) * nj + j) * 4;
+    double* o = a.wt + ((size_t)(m0 + r) * a.zpad + j) * 4;
     const double* p = wl[r * a.NZ + a.Kz + j];
     const double* n = wl[r * a.NZ + a.Kz - j];
     if (j == 0) {
@@ -1941,11 +1959,14 @@ __global__ void __launch_bounds__(256) k_in_gather(InArgs a) {
 //   s0 = W(0) + sum_j [(Sr c - Di s) + i (Dr s + Si c)]
 //   s1 = sum_j [(Dr jc - Si js) + i (Sr js + Di jc)]
 // with S = W(+j) + W(-j), D = W(+j) - W(-j), z_j = c + i s, (jc, js) = j z_j
-template <int KZ, int MOM>
+template <int KZ, int MOM, bool MB>
 __global__ void __launch_bounds__(256) k_in_gather_s(InArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool act = i < a.n;
   const int ii = act ? i : 0;
+  // this block's j = j0 + 0..KZ (blockIdx.z); the W table's entries of the
+  // surplus j beyond Kz' are zero padding
+  const int j0 = MB ? blockIdx.z * (KZ + 1) : 0;
   double x1r, x1i, y1r, y1i;
   double zc[KZ + 1], zs[KZ + 1];
   {
@@ -1956,8 +1977,12 @@ __global__ void __launch_bounds__(256) k_in_gather_s(InArgs a) {
     double s1, c1;
     t = a.sz[ii] - rint(a.sz[ii]);
     sincospi(2.0 * t, &s1, &c1);
-    zc[0] = 1.0;
-    zs[0] = 0.0;
+    if (j0 == 0) {
+      zc[0] = 1.0;
+      zs[0] = 0.0;
+    } else {
+      sincospi(2.0 * (double)j0 * t, &zs[0], &zc[0]);
+    }
 #pragma unroll
     for (int k = 1; k <= KZ; ++k) {
       zc[k] = zc[k - 1] * c1 - zs[k - 1] * s1;
@@ -1967,8 +1992,8 @@ __global__ void __launch_bounds__(256) k_in_gather_s(InArgs a) {
   double jc[KZ + 1], js[KZ + 1];          // j z_j
 #pragma unroll
   for (int k = 0; k <= KZ; ++k) {
-    jc[k] = (double)k * zc[k];
-    js[k] = (double)k * zs[k];
+    jc[k] = (double)(j0 + k) * zc[k];
+    js[k] = (double)(j0 + k) * zs[k];
   }
   const cplx<double> x1{x1r, x1i}, y1{y1r, y1i};
   cplx<double> y0{1.0, 0.0};
@@ -1988,10 +2013,16 @@ __global__ void __launch_bounds__(256) k_in_gather_s(InArgs a) {
   double mxx = 0.0, myy = 0.0, mzz = 0.0, mxy = 0.0, mxz = 0.0, myz = 0.0;
   for (int m = mb; m < me; ++m) {
     const int nx = m / a.NY, ny = m - (m / a.NY) * a.NY - a.Ky;
-    const double* e = wt + (size_t)m * (KZ + 1) * 4;
-    double s0r = e[0], s0i = e[1], s1r = 0.0, s1i = 0.0, s2r = 0.0, s2i = 0.0;
+    const double* e = wt + ((size_t)m * a.zpad + j0) * 4;
+    double s0r = 0.0, s0i = 0.0, s1r = 0.0, s1i = 0.0, s2r = 0.0, s2i = 0.0;
+    if (j0 == 0) {
+      // W(0)
+      s0r = e[0];
+      s0i = e[1];
+    }
 #pragma unroll
-    for (int j = 1; j <= KZ; ++j) {
+    for (int j = 0; j <= KZ; ++j) {
+      if (j == 0 && j0 == 0) continue;
       const double sr = e[4 * j], si = e[4 * j + 1], dr = e[4 * j + 2], di = e[4 * j + 3];
       s0r = fma(sr, zc[j], s0r);
       s0r = fma(-di, zs[j], s0r);
@@ -2003,8 +2034,9 @@ __global__ void __launch_bounds__(256) k_in_gather_s(InArgs a) {
       s1i = fma(di, jc[j], s1i);
       if (MOM == 9) {
         // j^2 (W(+j) z + W(-j) conj z) = j (sigma j z terms)
-        s2r = fma((double)j, fma(sr, jc[j], -di * js[j]), s2r);
-        s2i = fma((double)j, fma(dr, js[j], si * jc[j]), s2i);
+        const double fj = (double)(j0 + j);
+        s2r = fma(fj, fma(sr, jc[j], -di * js[j]), s2r);
+        s2i = fma(fj, fma(dr, js[j], si * jc[j]), s2i);
       }
     }
     const cplx<double> XY = cmul(X, Y);
@@ -2033,7 +2065,7 @@ __global__ void __launch_bounds__(256) k_in_gather_s(InArgs a) {
     }
   }
   if (!act) return;
-  double* o = a.out + (size_t)blockIdx.y * MOM * a.n;
+  double* o = a.out + ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * MOM * a.n;
   o[i] = u0;
   o[(size_t)a.n + i] = u1;
   o[(size_t)2 * a.n + i] = u2;
@@ -2069,20 +2101,26 @@ __global__ void k_in_dip_combine(InArgs a, int nchunk, int use_q, const double* 
 
 using InKernel = void (*)(InArgs);
 template <int... I>
-InKernel pick_in_sk(int kz, std::integer_sequence<int, I...>) {
-  static const InKernel t[] = {k_in_sk_s<I + 1>...};
-  return t[kz - 1];
+InKernel pick_in_sk(int kz, bool mb, std::integer_sequence<int, I...>) {
+  static const InKernel t[] = {k_in_sk_s<I + 1, false>...};
+  static const InKernel tb[] = {k_in_sk_s<I + 1, true>...};
+  return mb ? tb[kz - 1] : t[kz - 1];
 }
 template <int... I>
-InKernel pick_in_gather(int kz, int mom, std::integer_sequence<int, I...>) {
-  static const InKernel t3[] = {k_in_gather_s<I + 1, 3>...};
-  static const InKernel t9[] = {k_in_gather_s<I + 1, 9>...};
-  return mom == 9 ? t9[kz - 1] : t3[kz - 1];
+InKernel pick_in_gather(int kz, int mom, bool mb, std::integer_sequence<int, I...>) {
+  static const InKernel t3[] = {k_in_gather_s<I + 1, 3, false>...};
+  static const InKernel t9[] = {k_in_gather_s<I + 1, 9, false>...};
+  static const InKernel t3b[] = {k_in_gather_s<I + 1, 3, true>...};
+  static const InKernel t9b[] = {k_in_gather_s<I + 1, 9, true>...};
+  return mom == 9 ? (mb ? t9b[kz - 1] : t9[kz - 1]) : (mb ? t3b[kz - 1] : t3[kz - 1]);
 }
-// the register-blocked kernels for Kz' = kz (1 <= kz <= kInKZ)
-InKernel in_sk_kernel(int kz) { return pick_in_sk(kz, std::make_integer_sequence<int, kInKZ>{}); }
-InKernel in_gather_kernel(int kz, int mom = 3) {
-  return pick_in_gather(kz, mom, std::make_integer_sequence<int, kInKZ>{});
+// the register-blocked kernels for KZ = kz (1 <= kz <= kInKZ); mb: more
+// than one block of j (the block offset taken from the grid)
+InKernel in_sk_kernel(int kz, bool mb) {
+  return pick_in_sk(kz, mb, std::make_integer_sequence<int, kInKZ>{});
+}
+InKernel in_gather_kernel(int kz, int mom, bool mb) {
+  return pick_in_gather(kz, mom, mb, std::make_integer_sequence<int, kInKZ>{});
 }
 
 }  // namespace
@@ -2265,7 +2303,10 @@ static int recip_inner_setup(mdff_ctx* c, const std::vector<double>& Amain) {
   const int ncol = kin[0] + kin[1] + kin[2] + 3;
   const int M = (kin[0] + 1) * NY;
   const int ts = div_up(M, 64) * 64;
-  c->kin_reg = (kin[2] <= kInKZ && ts <= 1024) ? 1 : 0;
+  c->kin_reg = (kin[2] <= kInKZmax && ts <= 1024) ? 1 : 0;
+  // blocks of at most kInKZ + 1 values of j = 0..Kz'
+  c->kin_jb = c->kin_reg ? div_up(kin[2] + 1, kInKZ + 1) : 1;
+  c->kin_kb = c->kin_reg ? div_up(kin[2] + 1, c->kin_jb) - 1 : 0;
   if (c->kin_reg) {
     // k_in_sk_s: SG subgroups of ts threads (at most 1024 threads), one
     // block per chunk; the chunk's X / Y rows through LDS in tiles within
@@ -2291,8 +2332,14 @@ static int recip_inner_setup(mdff_ctx* c, const std::vector<double>& Amain) {
     const int sp = std::max(1, std::min(M, div_up((int)env_pos("MDFF_KIN_GBLOCKS", 768), div_up(c->n, 256))));
     c->kin_msplit = div_up(M, sp);
     c->kin_splits = div_up(M, c->kin_msplit);
-    HIP_TRY(c, hipMalloc((void**)&c->tab_in, 16 * (size_t)ncol * c->n));
-    HIP_TRY(c, hipMalloc((void**)&c->wt_in, sizeof(double) * 4 * (size_t)M * (kin[2] + 1)));
+    // phase rows and W table rows padded to whole blocks of j; the padding
+    // is zero from here on (no kernel writes it)
+    c->kin_zpad = c->kin_jb * (c->kin_kb + 1);
+    const size_t ncolp = (size_t)nxy + c->kin_zpad;
+    HIP_TRY(c, hipMalloc((void**)&c->tab_in, 16 * ncolp * c->n));
+    HIP_TRY(c, hipMalloc((void**)&c->wt_in, sizeof(double) * 4 * (size_t)M * c->kin_zpad));
+    HIP_TRY(c, hipMemset(c->tab_in, 0, 16 * ncolp * c->n));
+    HIP_TRY(c, hipMemset(c->wt_in, 0, sizeof(double) * 4 * (size_t)M * c->kin_zpad));
   } else {
     // k_in_sk: about 2048 blocks; tiles of atoms within 48 KiB of LDS.
     // k_in_gather: m ranges of whole kInMT tiles, about 2048 blocks
@@ -2334,7 +2381,7 @@ static bool inner_reg_possible(const mdff_ctx* c) {
     if (kin[d] < 1) return false;
   }
   const int M = (kin[0] + 1) * (2 * kin[1] + 1);
-  return kin[2] <= kInKZ && div_up(M, 64) * 64 <= 1024;
+  return kin[2] <= kInKZmax && div_up(M, 64) * 64 <= 1024;
 }
 
 int recip_setup(mdff_ctx* c) {
@@ -2488,7 +2535,8 @@ int recip_setup(mdff_ctx* c) {
   if (c->h3) splits = std::max(splits, c->h3_splits);
   c->rec_splits = splits;
   // moments of every split, plus (3 moments each) the inner box's
-  HIP_TRY(c, hipMalloc((void**)&c->rec_part, sizeof(double) * (9 * (size_t)splits + 9 * (size_t)c->kin_splits) * c->n));
+  HIP_TRY(c, hipMalloc((void**)&c->rec_part,
+                       sizeof(double) * (9 * (size_t)splits + 9 * (size_t)c->kin_splits * c->kin_jb) * c->n));
   return MDFF_OK;
 }
 
@@ -2695,6 +2743,7 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     ia.Kx = c->kin[0]; ia.Ky = c->kin[1]; ia.Kz = c->kin[2];
     ia.NY = 2 * c->kin[1] + 1; ia.NZ = 2 * c->kin[2] + 1;
     ia.nb = c->nb_in;
+    ia.zpad = c->kin_zpad;
     ia.chunk = c->in_chunk;
     ia.tile = c->kin_tile;
     ia.sx = c->sx; ia.sy = c->sy; ia.sz = c->sz;
@@ -2718,8 +2767,8 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
         iw.qstride = w == 0 ? 1 : 3;
         iw.part = c->dip_part + (size_t)w * pstride;
         PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(k_in_phase, dim3(div_up(3 * c->n, 256)), dim3(256), 0, c->stream, iw));
-        PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(in_sk_kernel(iw.Kz), dim3(c->in_nch), dim3(c->kin_ts * c->kin_sg), lds,
-                                                 c->stream, iw));
+        PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(in_sk_kernel(c->kin_kb, c->kin_jb > 1), dim3(c->in_nch, c->kin_jb),
+                                                 dim3(c->kin_ts * c->kin_sg), lds, c->stream, iw));
       }
       const size_t tot = (size_t)c->in_nch * ia.nb;
       PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(k_in_dip_combine, dim3(div_up(tot, 256)), dim3(256), 0, c->stream, ia,
@@ -2730,8 +2779,8 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
       ia.tab = reinterpret_cast<cplx<double>*>(c->tab_in);
       ia.wt = c->wt_in;
       PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(k_in_phase, dim3(div_up(3 * c->n, 256)), dim3(256), 0, c->stream, ia));
-      PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(in_sk_kernel(ia.Kz), dim3(c->in_nch), dim3(c->kin_ts * c->kin_sg), lds,
-                                               c->stream, ia));
+      PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(in_sk_kernel(c->kin_kb, c->kin_jb > 1), dim3(c->in_nch, c->kin_jb),
+                                               dim3(c->kin_ts * c->kin_sg), lds, c->stream, ia));
     } else {
       PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(k_in_sk, dim3(div_up(ia.nb, 256), c->in_nch), dim3(256), lds,
                                                c->stream, ia));
@@ -2843,13 +2892,14 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
     ia.out = c->rec_part + (size_t)nsplit * moments * c->n;
     ia.msplit = c->kin_msplit;
     const size_t lds = (size_t)kInMT * ia.NZ * 16;
-    const dim3 gg(div_up(c->n, 256), c->kin_splits);
+    const dim3 gg(div_up(c->n, 256), c->kin_splits, c->kin_jb);
     if (c->kin_reg) {
-      PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(in_gather_kernel(ia.Kz, moments), gg, dim3(256), 0, c->stream, ia));
+      PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(in_gather_kernel(c->kin_kb, moments, c->kin_jb > 1), gg, dim3(256),
+                                               0, c->stream, ia));
     } else {
       PROF_LAUNCH(c, PK_INNER, hipLaunchKernelGGL(k_in_gather, gg, dim3(256), lds, c->stream, ia));
     }
-    nsplit += c->kin_splits;
+    nsplit += c->kin_splits * c->kin_jb;
   }
   RecFinal rf{};
   rf.n = c->n;

@@ -343,7 +343,7 @@ def test_pim_extrapolation_order_parity():
 # ------------------------------------------------------------------ mixed precision
 
 @pytest.mark.parametrize("case", ["sio2", "nacl", "pim", "pim_large_box", "wide_kxy", "pim_static", "pim_fp64_rec",
-                                  "static_only"])
+                                  "static_only", "inner_two_jblocks", "inner_three_jblocks", "pim_inner_jblocks"])
 def test_mixed_precision_force_error(case, monkeypatch):
     """FP32 pair / field arithmetic with FP64 accumulation (FP64 for the rows'
     inner entries): max_i |dF_i| / RMS_i |F_i| against the FP64 oracle below
@@ -368,6 +368,16 @@ def test_mixed_precision_force_error(case, monkeypatch):
         d = s.desc
         d.dip[0 + 16 * 0], d.dip[0 + 16 * 1], d.dip[0 + 16 * 2] = 0.05, -0.02, 0.03
         d.dip[1 + 16 * 0], d.dip[1 + 16 * 1], d.dip[1 + 16 * 2] = -0.03, 0.04, 0.02
+    elif case == "inner_two_jblocks":
+        # inner k-box |n| <= 11: its nz range in two blocks of j = |nz|
+        s = configs.nacl(ncell=3, alpha=1.5, kmax=16)
+    elif case == "inner_three_jblocks":
+        # inner k-box |n| <= 19, 780 (nx, ny) columns: three blocks of j
+        s = configs.nacl(ncell=3, alpha=2.7, kmax=22)
+    elif case == "pim_inner_jblocks":
+        # dipole weights and the nine-moment gather with two blocks of j
+        # (inner k-box 8 x 8 x 12)
+        s = configs.mgo_pim(cells=(2, 2, 3), a=4.212, seed=9, sigma=0.1, alpha=2.2, rcut=4.2, kmax=14, tol=1e-9)
     elif case == "pim_fp64_rec":
         # the fallback of dipole systems: FP64 reciprocal sums
         s = _pim()
