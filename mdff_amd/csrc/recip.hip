@@ -2311,13 +2311,15 @@ static int recip_inner_setup(mdff_ctx* c, const std::vector<double>& Amain) {
     // k_in_sk_s: SG subgroups of ts threads (at most 1024 threads), one
     // block per chunk; the chunk's X / Y rows through LDS in tiles within
     // 48 KiB.  Charge-only systems, whose pair kernel runs beside the
-    // reciprocal kernels on the second stream: one subgroup and about 384
-    // chunks (1.5 blocks per CU), which leaves room on the SIMDs for the
-    // pair kernel's waves (24k a-SiO2: 0.298 against 0.307 ms per MD step,
-    // although the kernel alone takes longer); otherwise 4 subgroups and
-    // about 256 chunks (one per CU).
+    // reciprocal kernels on the second stream in MD steps that reuse the
+    // lists: one subgroup and about 384 chunks (1.5 blocks per CU), which
+    // leaves room on the SIMDs for the pair kernel's waves (24k a-SiO2:
+    // 0.298 against 0.307 ms per MD step, although the kernel alone takes
+    // longer); dipole systems: 4 subgroups and about 256 chunks (one per
+    // CU).  The shape depends on the system only, so that a run with every
+    // kernel on one stream sums in the same order.
     // k_in_gather_s: m ranges for about 768 blocks of 256 atoms.
-    const bool beside = c->overlap && !(c->task_qd || c->task_dd);
+    const bool beside = !(c->task_qd || c->task_dd);
     const int sg = std::min((int)env_pos("MDFF_KIN_SG", beside ? 1 : 4), 1024 / ts);
     const int nxy = kin[0] + kin[1] + 2;
     int chunk = std::max(div_up(c->n, (int)env_pos("MDFF_KIN_CHUNKS", beside ? 384 : 256)), 4 * sg);
