@@ -2209,6 +2209,13 @@ static size_t sk_lds(size_t tsz, int ncol, int ta) {
 // split-half kernels: staged atoms per tile, K steps (of 16) compiled for
 // the gather, LDS bytes of the structure factor
 constexpr int kH3TA = 32;
+constexpr int kH3MaxChunk = 2048;   // atoms per structure-factor chunk (FP32 chunk totals)
+// largest system whose structure factor takes the split-half kernel: on a
+// relaxed 200,000-ion LiF-like melt (kmax 30) it left a force error of
+// 3.4e-5 (above the 1e-5 bar) at the ions within a few A of the cell's
+// corner, wherever the origin is put, against 2.2e-6 with the FP32
+// matrix-core structure factor; 24,000 ions are within the bar (2.5e-6)
+constexpr int kH3SkMaxAtoms = 65536;
 static const int kH3Nsh[] = {4, 7, 10, 15};
 static int h3_bucket(int nsh) {
   for (int b : kH3Nsh)
@@ -2499,7 +2506,8 @@ int recip_setup(mdff_ctx* c) {
       // the gather for any kz its K steps cover; the structure factor where
       // its staged columns fit (else the FP32 matrix-core one feeds it)
       c->h3 = 1;
-      c->h3_sk = !sk_off && nxy <= 8 * (256 / kH3TA) && sk_h3_lds(c->K[0], c->K[1], kH3TA) <= 65536;
+      c->h3_sk = !sk_off && c->n <= kH3SkMaxAtoms && nxy <= 8 * (256 / kH3TA) &&
+                 sk_h3_lds(c->K[0], c->K[1], kH3TA) <= 65536;
       c->h3_nsh = nsh;
       c->h3_ct = nsh <= 7 ? 2 : 1;
       c->h3_mt8 = div_up(M, 8);
@@ -2518,8 +2526,12 @@ int recip_setup(mdff_ctx* c) {
         const int64_t cap =
             (int64_t)cus * blocks_per_cu(reinterpret_cast<const void*>(&k_sk_h3<kH3TA, 0>), lds);
         const int64_t mblocks = div_up(div_up(M, 32), 4) * (int64_t)div_up(c->NZk, 64);
+        // at most kH3MaxChunk atoms per chunk: a chunk's FP32 total adds one
+        // tile sum per 32 atoms, so its rounding grows with the chunk (the
+        // chunks are summed in FP64)
+        const int ch0 = div_up(c->n, kH3MaxChunk);
         int64_t best = -1;
-        for (int ch = 1; ch <= 128 && (ch == 1 || div_up(c->n, ch) >= kH3TA); ++ch) {
+        for (int ch = ch0; ch <= std::max(128, ch0) && (ch == ch0 || div_up(c->n, ch) >= kH3TA); ++ch) {
           const int per = div_up(c->n, ch);
           const int nch = div_up(c->n, per);
           const int64_t cost = ((mblocks * nch + cap - 1) / cap) * div_up(per, kH3TA);
