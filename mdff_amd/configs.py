@@ -351,18 +351,22 @@ def relax_sd(engine, pos: np.ndarray, iters: int = 150, max_step: float = 0.05,
     """Steepest descent on the engine's engforce: every iteration moves all
     ions along F by min(max_step, gain*max|F|) scaled by F/max|F|.  Used to
     take a random packing off its overlapping contacts before dynamics (a raw
-    packing releases eV per ion and heats to ~1e5 K).  Returns the relaxed
-    positions and the final max |F|."""
+    packing releases eV per ion and heats to ~1e5 K).  After every move the
+    engine's Verlet lists are checked (vnlistcheck, as the MD loop does each
+    step): engforce itself reuses the lists.  Returns the relaxed positions
+    and the final max |F|."""
     p = np.array(pos, dtype=np.float64, copy=True)
     fmax = float("inf")
     for _ in range(iters):
         engine.set_positions(p)
+        engine.vnlistcheck()
         f, _ = engine.engforce()
         fmax = float(np.abs(f).max())
         if fmax == 0.0:
             break
         p += min(max_step, gain * fmax) * f / fmax
     engine.set_positions(p)
+    engine.vnlistcheck()
     return p, fmax
 
 

@@ -153,6 +153,28 @@ def test_lif_200k_invariants_and_nhc3():
     assert abs(H[-1] - H[0]) < 1e-2 * abs(ek[0])
 
 
+def test_lif_200k_relaxed_mixed_force_error():
+    """A relaxed 200,000-ion LiF-like melt at kmax 30 (small forces, so the
+    relative error is demanding): the mixed form within the bar of the FP64
+    form.  The split-half structure factor left 3.4e-5 here, at the ions
+    near the cell's corner; systems of this size take the FP32 matrix-core
+    structure factor."""
+    s = configs.lif(kmax=30)
+    note(f"lif 200k relaxed n={s.natm} kmax={s.desc.kES[0]}")
+    g64 = capi.gpu_engine(copy.deepcopy(s.desc), s.itype)
+    pos, _ = configs.relax_sd(g64, s.pos, iters=200)
+    g64.set_positions(pos)
+    f64, _ = g64.engforce()
+    g64.close()
+    d = copy.deepcopy(s.desc)
+    d.precision = capi.PREC_MIXED
+    g32 = capi.gpu_engine(d, s.itype)
+    g32.set_positions(pos)
+    f32, _ = g32.engforce()
+    g32.close()
+    assert rms_force_err(f32, f64) < TOL_MIXED, rms_force_err(f32, f64)
+
+
 def test_mgo_pim_fp64_matches_oracle():
     """cfg4's force field and SCF (tolerance 1e-7, TT damping, ASPC off) on
     8 x 8 x 9 cells, 4,608 ions (the oracle's SCF at 12,000 ions takes
