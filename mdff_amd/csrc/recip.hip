@@ -1015,6 +1015,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   for (int t0 = abeg; t0 < aend; t0 += TA) {
     const int nt = min(TA, aend - t0);
     const bool sok = sia < nt;
+    // every other tile stages -Z and its sums are subtracted: an offset that
+    // the matrix cores' FP32 accumulation adds whatever the sign of the terms
+    // (it grew with the atom count and acted as a charge at the phase origin:
+    // 3.3e-5 force error on a relaxed 200k LiF-like melt, 2.1e-6 with the
+    // alternation) cancels between tiles
+    const float tsg = (((t0 - abeg) / TA) & 1) ? -1.f : 1.f;
     __syncthreads();
     // Z rows: part 0 (Zr, -Zi), part 1 (Zi, Zr), Z(-n) = conj Z(n); atoms
     // beyond the chunk get zeros (their B is zero too)
@@ -1022,7 +1028,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     for (int u = 0; u < ZU; ++u) {
       const int nzl = scg + CG * u;
       const int nzi = zoff + nzl;
-      cplx<float> z = zn[u];
+      cplx<float> z{tsg * zn[u].re, tsg * zn[u].im};
       if (nzi - a.Kz < 0) z.im = -z.im;
       if (!sok || nzi >= a.NZ) z = cplx<float>{0.f, 0.f};
       _Float16 rh, rl, ih, il;
@@ -1117,7 +1123,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       }
     }
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt) tot[rt] += acc[rt];
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) tot[rt][r] = fmaf(tsg, acc[rt][r], tot[rt][r]);
   }
   // D: lane holds rows 8*(r/4) + 4*hl + r%4 of its column m; row R of tile
   // rt -> nz index zoff + (32 rt + R) / 2, part (32 rt + R) % 2
@@ -2210,12 +2218,6 @@ static size_t sk_lds(size_t tsz, int ncol, int ta) {
 // the gather, LDS bytes of the structure factor
 constexpr int kH3TA = 32;
 constexpr int kH3MaxChunk = 2048;   // atoms per structure-factor chunk (FP32 chunk totals)
-// largest system whose structure factor takes the split-half kernel: on a
-// relaxed 200,000-ion LiF-like melt (kmax 30) it left a force error of
-// 3.4e-5 (above the 1e-5 bar) at the ions within a few A of the cell's
-// corner, wherever the origin is put, against 2.2e-6 with the FP32
-// matrix-core structure factor; 24,000 ions are within the bar (2.5e-6)
-constexpr int kH3SkMaxAtoms = 65536;
 static const int kH3Nsh[] = {4, 7, 10, 15};
 static int h3_bucket(int nsh) {
   for (int b : kH3Nsh)
@@ -2506,8 +2508,7 @@ int recip_setup(mdff_ctx* c) {
       // the gather for any kz its K steps cover; the structure factor where
       // its staged columns fit (else the FP32 matrix-core one feeds it)
       c->h3 = 1;
-      c->h3_sk = !sk_off && c->n <= kH3SkMaxAtoms && nxy <= 8 * (256 / kH3TA) &&
-                 sk_h3_lds(c->K[0], c->K[1], kH3TA) <= 65536;
+      c->h3_sk = !sk_off && nxy <= 8 * (256 / kH3TA) && sk_h3_lds(c->K[0], c->K[1], kH3TA) <= 65536;
       c->h3_nsh = nsh;
       c->h3_ct = nsh <= 7 ? 2 : 1;
       c->h3_mt8 = div_up(M, 8);

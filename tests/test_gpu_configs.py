@@ -156,9 +156,10 @@ def test_lif_200k_invariants_and_nhc3():
 def test_lif_200k_relaxed_mixed_force_error():
     """A relaxed 200,000-ion LiF-like melt at kmax 30 (small forces, so the
     relative error is demanding): the mixed form within the bar of the FP64
-    form.  The split-half structure factor left 3.4e-5 here, at the ions
-    near the cell's corner; systems of this size take the FP32 matrix-core
-    structure factor."""
+    form.  Before the split-half structure factor alternated the sign of its
+    atom tiles, an offset of the matrix cores' FP32 accumulation that grows
+    with the atom count left 3.3e-5 here, at the ions near the cell's
+    corner."""
     s = configs.lif(kmax=30)
     note(f"lif 200k relaxed n={s.natm} kmax={s.desc.kES[0]}")
     g64 = capi.gpu_engine(copy.deepcopy(s.desc), s.itype)
