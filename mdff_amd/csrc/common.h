@@ -354,6 +354,7 @@ struct mdff_ctx {
   // mixed form: split-half (FP16 hi + lo) matrix-core reciprocal kernels
   int h3 = 0;                             // split-half gather (and its A operand from the reduction)
   int h3_sk = 0;                          // split-half structure factor as well
+  int h3_wide = 0;                        // ... in its wide-mesh form (more than 64 staged X / Y columns)
   int h3_ch = 1, h3_nsh = 0, h3_ct = 1, h3_splits = 1, h3_mt8 = 0;
   float h3_pscale = 1.f;                  // charge scale of the structure factor's B operand
   float* mu_pscale = nullptr;             // device: scales of the two dipole passes (k_mu_pscale)

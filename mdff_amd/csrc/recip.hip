@@ -934,7 +934,10 @@ struct Sh3Args {
   float* part;                   // [chunk][2*nk_box]
 };
 
-template <int TA, int MODE>
+// WIDE: up to 128 staged X / Y columns (16 per staging thread) and Y staged
+// for ny >= 0 only, conjugated on read for ny < 0 (the LDS of a wide mesh,
+// e.g. Kx = Ky = 56, stays within two blocks per CU)
+template <int TA, int MODE, bool WIDE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_sk_h3(Sh3Args a) {
   constexpr bool CQ = MODE != 1;
   const float psc = a.pscale_dev ? a.pscale_dev[MODE == 2 ? 1 : 0] : a.pscale;
@@ -949,8 +952,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   uint32_t* zs = reinterpret_cast<uint32_t*>(smem_raw);                 // [plane hi/lo][128 rows][RW]
   const int nxy = a.Kx + a.Ky + 2;                                       // X and Y columns
   // staged columns: X(nx) (times c when CQ), nx = 0..Kx, then Y(ny),
-  // ny = -Ky..Ky (Y(-n) = conj Y(n))
-  const int nst = a.Kx + 1 + 2 * a.Ky + 1;
+  // ny = -Ky..Ky (Y(-n) = conj Y(n)); WIDE: ny = 0..Ky
+  const int nst = WIDE ? nxy : a.Kx + 1 + 2 * a.Ky + 1;
   cplx<float>* st = reinterpret_cast<cplx<float>*>(zs + 2 * 128 * RW);  // [nst][TAP]
   float* sq = reinterpret_cast<float*>(st + (size_t)nst * TAP);         // [3][TAP]
   const cplx<float>* tab = reinterpret_cast<const cplx<float>*>(a.tab);
@@ -968,7 +971,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   const bool bok = m < a.M;
   const int nx = bok ? m / a.NY : 0, ny = bok ? m % a.NY - a.Ky : 0;
   const float4* xp = reinterpret_cast<const float4*>(st + (size_t)nx * TAP + 4 * hl);
-  const float4* yp = reinterpret_cast<const float4*>(st + (size_t)(a.Kx + 1 + a.Ky + ny) * TAP + 4 * hl);
+  const int ycol = WIDE ? a.Kx + 1 + (ny < 0 ? -ny : ny) : a.Kx + 1 + a.Ky + ny;
+  const float4* yp = reinterpret_cast<const float4*>(st + (size_t)ycol * TAP + 4 * hl);
+  const float ysg = (WIDE && ny < 0) ? -1.f : 1.f;          // conj Y(|ny|) for ny < 0
   // columns m >= M (bok false) read valid columns; their D columns are not
   // stored, and every D column depends on its own B column only
   const float fnx = (float)nx, fny = (float)ny;
@@ -987,7 +992,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   // issued before the MFMAs of tile t and written to LDS after them
   constexpr int CG = 256 / TA;
   constexpr int ZU = RT * 16 / CG;
-  constexpr int XU = 8;
+  constexpr int XU = WIDE ? 16 : 8;
   const int sia = threadIdx.x % TA, scg = threadIdx.x / TA;
   cplx<float> zn[ZU], xn[XU];
   double qd = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;        // raw a-coefficient inputs (used at store time)
@@ -1050,6 +1055,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         if (cidx <= a.Kx) {
           if (CQ) v = cplx<float>{cs * v.re, cs * v.im};
           st[(size_t)cidx * TAP + sia] = v;
+        } else if (WIDE) {
+          st[(size_t)cidx * TAP + sia] = v;
         } else {
           const int nyc = cidx - a.Kx - 1;
           st[(size_t)(a.Kx + 1 + a.Ky + nyc) * TAP + sia] = v;
@@ -1085,7 +1092,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       const float4 x01 = xp[4 * s], x23 = xp[4 * s + 1];
       const float4 y01 = yp[4 * s], y23 = yp[4 * s + 1];
       const float xr[4] = {x01.x, x01.z, x23.x, x23.z}, xi[4] = {x01.y, x01.w, x23.y, x23.w};
-      const float yr[4] = {y01.x, y01.z, y23.x, y23.z}, yi[4] = {y01.y, y01.w, y23.y, y23.w};
+      const float yr[4] = {y01.x, y01.z, y23.x, y23.z};
+      const float yi[4] = {ysg * y01.y, ysg * y01.w, ysg * y23.y, ysg * y23.w};
       float cq[4] = {0.f, 0.f, 0.f, 0.f}, cx[4] = {0.f, 0.f, 0.f, 0.f}, cy[4] = {0.f, 0.f, 0.f, 0.f};
       if (!CQ) {
         const float4 c0v = qp[2 * s], c1v = qp[TAP / 4 + 2 * s], c2v = qp[TAP / 2 + 2 * s];
@@ -2224,8 +2232,8 @@ static int h3_bucket(int nsh) {
     if (nsh <= b) return b;
   return 0;
 }
-static size_t sk_h3_lds(int kx, int ky, int ta) {
-  const int nst = kx + 1 + 2 * ky + 1;
+static size_t sk_h3_lds(int kx, int ky, int ta, bool wide = false) {
+  const int nst = wide ? kx + ky + 2 : kx + 1 + 2 * ky + 1;
   return (size_t)2 * 128 * (ta + 4) * 4 + (size_t)nst * (ta + 4) * 8 + (size_t)3 * (ta + 4) * 4;
 }
 static const void* gather_h3_kernel(int ct, int nsh) {
@@ -2509,6 +2517,21 @@ int recip_setup(mdff_ctx* c) {
       // its staged columns fit (else the FP32 matrix-core one feeds it)
       c->h3 = 1;
       c->h3_sk = !sk_off && nxy <= 8 * (256 / kH3TA) && sk_h3_lds(c->K[0], c->K[1], kH3TA) <= 65536;
+      if (!sk_off && !c->h3_sk && nxy <= 16 * (256 / kH3TA)) {
+        // wide mesh: the wide form where its LDS fits a block on this device
+        int maxlds = 0;
+        const size_t lw = sk_h3_lds(c->K[0], c->K[1], kH3TA, true);
+        if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess &&
+            lw <= (size_t)maxlds) {
+          bool ok = true;
+          for (const void* fn : {reinterpret_cast<const void*>(&k_sk_h3<kH3TA, 0, true>),
+                                 reinterpret_cast<const void*>(&k_sk_h3<kH3TA, 1, true>),
+                                 reinterpret_cast<const void*>(&k_sk_h3<kH3TA, 2, true>)})
+            ok = ok && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lw) == hipSuccess;
+          (void)hipGetLastError();
+          c->h3_sk = c->h3_wide = ok ? 1 : 0;
+        }
+      }
       c->h3_nsh = nsh;
       c->h3_ct = nsh <= 7 ? 2 : 1;
       c->h3_mt8 = div_up(M, 8);
@@ -2523,9 +2546,10 @@ int recip_setup(mdff_ctx* c) {
       c->h3_splits = bs;
       // structure-factor chunks from the kernel's occupancy
       if (c->h3_sk) {
-        const size_t lds = sk_h3_lds(c->K[0], c->K[1], kH3TA);
-        const int64_t cap =
-            (int64_t)cus * blocks_per_cu(reinterpret_cast<const void*>(&k_sk_h3<kH3TA, 0>), lds);
+        const size_t lds = sk_h3_lds(c->K[0], c->K[1], kH3TA, c->h3_wide);
+        const void* fn = c->h3_wide ? reinterpret_cast<const void*>(&k_sk_h3<kH3TA, 0, true>)
+                                    : reinterpret_cast<const void*>(&k_sk_h3<kH3TA, 0, false>);
+        const int64_t cap = (int64_t)cus * blocks_per_cu(fn, lds);
         const int64_t mblocks = div_up(div_up(M, 32), 4) * (int64_t)div_up(c->NZk, 64);
         // at most kH3MaxChunk atoms per chunk: a chunk's FP32 total adds one
         // tile sum per 32 atoms, so its rounding grows with the chunk (the
@@ -2686,16 +2710,27 @@ static int eval_t(mdff_ctx* c, const double* q, const double* mu, bool use_q, bo
                          b[1], b[2], b[3], b[4], b[5], b[6], b[7], b[8], c->mu_pscale);
       sh.pscale_dev = c->mu_pscale;
     }
-    const size_t lds = sk_h3_lds(c->K[0], c->K[1], kH3TA);
+    const bool wide = c->h3_wide != 0;
+    const size_t lds = sk_h3_lds(c->K[0], c->K[1], kH3TA, wide);
     dim3 gs(div_up(div_up(s.M, 32), 4), ch, div_up(c->NZk, 64));
-    if (use_mu) PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, 1>), gs, dim3(256), lds, c->stream, sh));
-    else PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, 0>), gs, dim3(256), lds, c->stream, sh));
+    if (use_mu && wide)
+      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, 1, true>), gs, dim3(256), lds, c->stream, sh));
+    else if (use_mu)
+      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, 1, false>), gs, dim3(256), lds, c->stream, sh));
+    else if (wide)
+      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, 0, true>), gs, dim3(256), lds, c->stream, sh));
+    else
+      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, 0, false>), gs, dim3(256), lds, c->stream, sh));
     if (use_mu) {
       Sh3Args s2 = sh;
       s2.mode = 2;
       s2.part = p1 + (size_t)2 * ch * c->nk_box;
       const size_t tot = (size_t)ch * c->nk_box;
-      PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, 2>), gs, dim3(256), lds, c->stream, s2);
+      if (wide)
+        PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, 2, true>), gs, dim3(256), lds, c->stream, s2));
+      else
+        PROF_LAUNCH(c, PK_SK, hipLaunchKernelGGL((k_sk_h3<kH3TA, 2, false>), gs, dim3(256), lds, c->stream, s2));
+      PROF_LAUNCH(c, PK_SK,
                   hipLaunchKernelGGL(k_sk_combine<float>, dim3(div_up(tot, 256)), dim3(256), 0, c->stream,
                                      c->nk_box, c->NZk, c->K[2], ch, p1, s2.part));
     }
