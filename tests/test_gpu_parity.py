@@ -342,8 +342,9 @@ def test_pim_extrapolation_order_parity():
 
 # ------------------------------------------------------------------ mixed precision
 
-@pytest.mark.parametrize("case", ["sio2", "nacl", "pim", "pim_large_box", "wide_kxy", "pim_static", "pim_fp64_rec",
-                                  "static_only", "inner_two_jblocks", "inner_three_jblocks", "pim_inner_jblocks"])
+@pytest.mark.parametrize("case", ["sio2", "nacl", "pim", "pim_large_box", "wide_kxy", "wide_kxy_f32_sk", "pim_static",
+                                  "pim_fp64_rec", "static_only", "inner_two_jblocks", "inner_three_jblocks",
+                                  "pim_inner_jblocks"])
 def test_mixed_precision_force_error(case, monkeypatch):
     """FP32 pair / field arithmetic with FP64 accumulation (FP64 for the rows'
     inner entries): max_i |dF_i| / RMS_i |F_i| against the FP64 oracle below
@@ -352,11 +353,14 @@ def test_mixed_precision_force_error(case, monkeypatch):
         s = _sio2(natm=900, seed=26)
     elif case == "nacl":
         s = configs.nacl(ncell=3)
-    elif case == "wide_kxy":
-        # kmax (34, 34, 8): more X/Y columns than the split-half structure
-        # factor stages, so the FP32 matrix-core one feeds the split-half gather
+    elif case in ("wide_kxy", "wide_kxy_f32_sk"):
+        # kmax (34, 34, 8): 70 X/Y columns, the wide form of the split-half
+        # structure factor; or (MDFF_RECIP_F32MMA=2) the FP32 matrix-core
+        # structure factor feeding the split-half gather
         s = configs.nacl(ncell=2, alpha=0.9, kmax=34)
         s.desc.kES[2] = 8
+        if case == "wide_kxy_f32_sk":
+            monkeypatch.setenv("MDFF_RECIP_F32MMA", "2")
     elif case == "pim_large_box":
         # 1,728 ions in a 25 A box: dipoles of 1e-3..1e-1 e A, so the
         # dipole coefficients mu.b of the structure factor are ~1e-4
